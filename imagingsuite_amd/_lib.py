@@ -1,0 +1,120 @@
+"""ctypes binding of the C ABI in include/muhrec_hip.h (libmuhrec_hip.so).
+
+The library is built in-tree by ``imagingsuite_amd.build.build()``.  There is
+no fallback: if the shared object is missing, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libmuhrec_hip.so")
+
+MR_OK = 0
+MR_ERR_INVALID = 1
+MR_ERR_HIP = 2
+MR_ERR_UNSUPPORTED = 3
+MR_ERR_INTERNAL = 4
+
+FILTER_TYPES = {"ram-lak": 0, "shepp-logan": 1, "hanning": 2, "hamming": 3, "butterworth": 4}
+
+# Every symbol include/muhrec_hip.h declares.
+EXPORTED_SYMBOLS = (
+    "mr_last_error", "mr_version", "mr_device_count",
+    "mr_profile_enable", "mr_profile_reset", "mr_profile_query",
+    "mr_filter_plan_create", "mr_filter_plan_destroy", "mr_filter_plan_info", "mr_filter_rows",
+    "mr_bp_plan_create", "mr_bp_plan_destroy", "mr_bp_plan_mask", "mr_bp_parallel",
+    "mr_fbp_parallel",
+    "mr_fdk_plan_create", "mr_fdk_plan_destroy", "mr_fdk_plan_window", "mr_fdk_bp",
+)
+
+
+class MrError(RuntimeError):
+    """A non-zero mr_status from the library (the reference throws ReconException)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[mr_status {code}] {msg}")
+        self.code = code
+
+
+class BpParams(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_size_t), ("slices", ctypes.c_size_t),
+                ("center", ctypes.c_float), ("direction", ctypes.c_int),
+                ("interp", ctypes.c_int), ("buffer_size", ctypes.c_int),
+                ("multiprojbp", ctypes.c_int), ("use_roi", ctypes.c_int),
+                ("roi", ctypes.c_size_t * 4)]
+
+
+class FdkParams(ctypes.Structure):
+    _fields_ = [("sod", ctypes.c_float), ("sdd", ctypes.c_float),
+                ("pixel", ctypes.c_float * 2), ("voxel", ctypes.c_float * 3),
+                ("ppoint", ctypes.c_float * 2), ("center", ctypes.c_float),
+                ("direction", ctypes.c_int), ("roi", ctypes.c_size_t * 4),
+                ("projection_roi3", ctypes.c_size_t), ("use_roi", ctypes.c_int),
+                ("matrix_roi", ctypes.c_size_t * 4)]
+
+
+_lib = None
+
+
+def load():
+    """Load libmuhrec_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with imagingsuite_amd.build.build() "
+            "(there is no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, sz, f32, i32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_float, ctypes.c_int
+    fp = ctypes.POINTER(ctypes.c_float)
+    szp = ctypes.POINTER(ctypes.c_size_t)
+    L.mr_last_error.restype = ctypes.c_char_p
+    L.mr_version.restype = i32
+    L.mr_device_count.restype = i32
+    L.mr_profile_enable.argtypes = [i32]
+    L.mr_profile_reset.argtypes = []
+    L.mr_profile_query.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
+                                   ctypes.POINTER(ctypes.c_long)]
+    L.mr_filter_plan_create.argtypes =[ctypes.POINTER(vp), sz, i32, f32, f32, i32, f32, sz]
+    L.mr_filter_plan_destroy.argtypes = [vp]
+    L.mr_filter_plan_info.argtypes = [vp, szp, szp, fp]
+    L.mr_filter_rows.argtypes = [vp, vp, sz, vp]
+    L.mr_bp_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(BpParams)]
+    L.mr_bp_plan_destroy.argtypes = [vp]
+    L.mr_bp_plan_mask.argtypes = [vp, szp, szp]
+    L.mr_bp_parallel.argtypes = [vp, vp, sz, fp, fp, vp, i32, vp]
+    L.mr_fbp_parallel.argtypes = [vp, vp, vp, sz, fp, fp, vp, vp]
+    L.mr_fdk_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(FdkParams)]
+    L.mr_fdk_plan_destroy.argtypes = [vp]
+    L.mr_fdk_plan_window.argtypes = [vp, szp]
+    L.mr_fdk_bp.argtypes = [vp, vp, sz, sz, fp, fp, vp, vp]
+    for name in EXPORTED_SYMBOLS:
+        if name not in ("mr_last_error", "mr_version", "mr_device_count"):
+            getattr(L, name).restype = i32
+    _lib = L
+    return L
+
+
+def profile(enable: bool) -> None:
+    """Bracket every kernel launch with HIP events (see mr_profile_enable)."""
+    check(load().mr_profile_enable(int(bool(enable))))
+
+
+def profile_reset() -> None:
+    check(load().mr_profile_reset())
+
+
+def profile_query(kernel: str = ""):
+    """(total milliseconds, launches) of kernels whose name contains ``kernel``."""
+    ms, n = ctypes.c_double(), ctypes.c_long()
+    check(load().mr_profile_query(kernel.encode(), ctypes.byref(ms), ctypes.byref(n)))
+    return ms.value, n.value
+
+
+def check(rc: int) -> None:
+    if rc != MR_OK:
+        msg = load().mr_last_error().decode(errors="replace")
+        raise MrError(rc, msg)

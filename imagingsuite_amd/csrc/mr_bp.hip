@@ -1,0 +1,313 @@
+// Parallel-beam back-projection on gfx950 (MultiProjBPparallel, MultiProjBP,
+// NearestNeighborBP).
+//
+// Data layout in HBM
+//   d_proj  [n_proj][slices][width]    the reference's TImage<float,3> stack
+//   sino    [n_used][width][vpad]      weighted, transposed copy (slice fastest),
+//                                      vpad = slices rounded up to 4, zero padded:
+//                                      exactly the reference's projection buffer
+//                                      (StdBackProjectorBase.cpp:169-172)
+//   volume  [width(y)][width(x)][slices]  MatrixZXY, slice fastest
+//
+// bp_kernel: one thread owns one (x, y) voxel column and ZT consecutive
+// slices held in registers; it walks every used projection in order, so each
+// voxel sees the same sequence of float additions as the reference's z-pillar
+// loop (MultiProjBPparallel.cpp:133-163) and the result is bit-identical.
+// Geometry tables are read with scalar loads; detector columns are read as
+// float4 vectors along the slice axis.
+#include "mr_common.h"
+#include "mr_host.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace mr {
+
+// sino[k][u][v] = proj[idx[k]][v][u] * w[idx[k]]  (v < slices), 0 for v >= slices.
+__global__ __launch_bounds__(256) void bp_prep_kernel(const float *__restrict__ proj,
+                                                      float *__restrict__ sino,
+                                                      const int *__restrict__ idx,
+                                                      const float *__restrict__ wts, int nu, int nv,
+                                                      int vpad)
+{
+    __shared__ float tile[32][33];
+    const int k = blockIdx.z;
+    const int p = idx[k];
+    const float w = wts[p];
+    const int u0 = blockIdx.x * 32, v0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    const float *src = proj + (size_t)p * nu * nv;
+    for (int r = ty; r < 32; r += 8) {
+        const int v = v0 + r, u = u0 + tx;
+        tile[r][tx] = (v < nv && u < nu) ? src[(size_t)v * nu + u] * w : 0.0f;
+    }
+    __syncthreads();
+    float *dst = sino + (size_t)k * nu * vpad;
+    for (int r = ty; r < 32; r += 8) {
+        const int u = u0 + r, v = v0 + tx;
+        if (u < nu && v < vpad) dst[(size_t)u * vpad + v] = tile[tx][r];
+    }
+}
+
+struct BpArgs {
+    const float *sino;
+    float *vol;
+    const float4 *tab;   // (fStartU, fCos, fSin, 0) per used projection
+    const int2 *mask;    // [first, second) per y, 0-based x
+    int nu, nv, vpad, nk, nz4, bstride, accumulate;
+};
+
+template <int ZT, int INTERP>
+__global__ __launch_bounds__(256) void bp_kernel(BpArgs a)
+{
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int z0 = blockIdx.z * ZT;
+    if (x >= a.nu || y >= a.nu) return;
+    const int zn = min(ZT, a.nv - z0);          // slices stored by this thread
+    const int zl = max(0, min(ZT, a.nz4 - z0));  // slices updated (multiple of 4)
+    float *out = a.vol + ((size_t)y * a.nu + x) * a.nv + z0;
+    float acc[ZT];
+#pragma unroll
+    for (int z = 0; z < ZT; ++z) acc[z] = (a.accumulate && z < zn) ? out[z] : 0.0f;
+    const int2 m = a.mask[y];
+    if (x >= m.x && x < m.y && zl > 0) {
+        const float yf = (float)(y + 1);
+        const float xf = (float)(x + 1);
+        const int num2 = a.nu - 2;
+        const float *base = a.sino + z0;
+        for (int k = 0; k < a.nk; ++k) {
+            const float4 t = a.tab[k];
+            const float lsu = t.x + t.y * yf;
+            const float u = lsu - t.z * xf;
+            const int n = (int)u;
+            if (n < 0 || n > num2) continue;
+            const float *A = base + ((size_t)k * a.nu + n) * a.vpad;
+            if (INTERP == 0) {
+                const float wb = fabsf(u - (float)n);
+                const float wa = 1.0f - wb;
+                const float *B = A + a.bstride;
+#pragma unroll
+                for (int g = 0; g < ZT / 4; ++g) {
+                    if (4 * g < zl) {
+                        const float4 va = *reinterpret_cast<const float4 *>(A + 4 * g);
+                        const float4 vb = *reinterpret_cast<const float4 *>(B + 4 * g);
+                        acc[4 * g + 0] = acc[4 * g + 0] + (va.x * wa + vb.x * wb);
+                        acc[4 * g + 1] = acc[4 * g + 1] + (va.y * wa + vb.y * wb);
+                        acc[4 * g + 2] = acc[4 * g + 2] + (va.z * wa + vb.z * wb);
+                        acc[4 * g + 3] = acc[4 * g + 3] + (va.w * wa + vb.w * wb);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int g = 0; g < ZT / 4; ++g) {
+                    if (4 * g < zl) {
+                        const float4 va = *reinterpret_cast<const float4 *>(A + 4 * g);
+                        acc[4 * g + 0] = acc[4 * g + 0] + va.x;
+                        acc[4 * g + 1] = acc[4 * g + 1] + va.y;
+                        acc[4 * g + 2] = acc[4 * g + 2] + va.z;
+                        acc[4 * g + 3] = acc[4 * g + 3] + va.w;
+                    }
+                }
+            }
+        }
+    }
+    if ((a.nv & 3) == 0 && zn == ZT) {
+#pragma unroll
+        for (int g = 0; g < ZT / 4; ++g)
+            *reinterpret_cast<float4 *>(out + 4 * g) =
+                make_float4(acc[4 * g], acc[4 * g + 1], acc[4 * g + 2], acc[4 * g + 3]);
+    } else {
+#pragma unroll
+        for (int z = 0; z < ZT; ++z)
+            if (z < zn) out[z] = acc[z];
+    }
+}
+
+template <int ZT, int INTERP>
+static int launch_bp(const BpArgs &a, hipStream_t s)
+{
+    dim3 grid((unsigned)((a.nu + 15) / 16), (unsigned)((a.nu + 15) / 16),
+              (unsigned)((a.nv + ZT - 1) / ZT));
+    ProfScope prof("bp_parallel", s);
+    hipLaunchKernelGGL((bp_kernel<ZT, INTERP>), grid, dim3(256), 0, s, a);
+    MR_HIP(hipGetLastError());
+    return MR_OK;
+}
+
+int run_bp(const BpArgs &a, int interp, hipStream_t s)
+{
+    if (a.nv >= 16) return interp == 0 ? launch_bp<16, 0>(a, s) : launch_bp<16, 1>(a, s);
+    if (a.nv >= 8) return interp == 0 ? launch_bp<8, 0>(a, s) : launch_bp<8, 1>(a, s);
+    return interp == 0 ? launch_bp<4, 0>(a, s) : launch_bp<4, 1>(a, s);
+}
+
+}  // namespace mr
+
+struct mr_bp_plan {
+    mr_bp_params p{};
+    int device = 0;
+    size_t vpad = 0;
+    std::vector<size_t> first, second;
+    int2 *d_mask = nullptr;
+    float *d_sino = nullptr;
+    size_t sino_cap = 0;   // floats
+    float4 *d_tab = nullptr;
+    size_t tab_cap = 0;    // entries
+    int *d_idx = nullptr;
+    float *d_w = nullptr;
+    size_t idx_cap = 0;
+};
+
+namespace {
+
+int ensure_capacity(mr_bp_plan *p, size_t n_proj, size_t n_used)
+{
+    const size_t need = std::max<size_t>(n_used, 1) * p->p.width * p->vpad;
+    if (need > p->sino_cap) {
+        if (p->d_sino) (void)hipFree(p->d_sino);
+        p->d_sino = nullptr;
+        p->sino_cap = 0;
+        MR_HIP(hipMalloc(&p->d_sino, need * sizeof(float)));
+        p->sino_cap = need;
+    }
+    const size_t nt = std::max<size_t>(n_used, 1);
+    if (nt > p->tab_cap) {
+        if (p->d_tab) (void)hipFree(p->d_tab);
+        p->d_tab = nullptr;
+        p->tab_cap = 0;
+        MR_HIP(hipMalloc(&p->d_tab, nt * sizeof(float4)));
+        p->tab_cap = nt;
+    }
+    const size_t ni = std::max<size_t>(n_proj, 1);
+    if (ni > p->idx_cap) {
+        if (p->d_idx) (void)hipFree(p->d_idx);
+        if (p->d_w) (void)hipFree(p->d_w);
+        p->d_idx = nullptr;
+        p->d_w = nullptr;
+        p->idx_cap = 0;
+        MR_HIP(hipMalloc(&p->d_idx, ni * sizeof(int)));
+        MR_HIP(hipMalloc(&p->d_w, ni * sizeof(float)));
+        p->idx_cap = ni;
+    }
+    return MR_OK;
+}
+
+}  // namespace
+
+extern "C" int mr_bp_plan_create(mr_bp_plan **plan, const mr_bp_params *params)
+{
+    MR_GUARD_BEGIN
+    mr::clear_error();
+    MR_REQUIRE(plan != nullptr && params != nullptr, "NULL argument");
+    *plan = nullptr;
+    const mr_bp_params &q = *params;
+    MR_REQUIRE(q.width >= 2 && q.width <= 65536, "width must be in [2, 65536]");
+    MR_REQUIRE(q.slices >= 1 && q.slices <= 65536, "slices must be in [1, 65536]");
+    MR_REQUIRE(q.interp == MR_INTERP_LINEAR || q.interp == MR_INTERP_NEAREST, "unknown interpolation");
+    MR_REQUIRE(q.direction == 0 || q.direction == 1, "direction must be 0 (CW) or 1 (CCW)");
+    MR_REQUIRE(q.buffer_size >= 0, "buffer_size must be >= 0");
+    if (q.use_roi) MR_REQUIRE(q.roi[0] <= q.roi[2] && q.roi[1] <= q.roi[3], "matrix ROI x1<x0 or y1<y0");
+    auto *p = new mr_bp_plan();
+    p->p = q;
+    if (hipGetDevice(&p->device) != hipSuccess) p->device = 0;
+    p->vpad = (q.slices + 3) & ~(size_t)3;
+    mr::circle_mask(q.width, q.width, q.use_roi != 0, q.roi, false, p->first, p->second);
+    std::vector<int2> m(q.width);
+    for (size_t y = 0; y < q.width; ++y) {
+        // reference loop x = first+1 .. second (1-based) == [first, second) 0-based
+        m[y].x = (int)p->first[y];
+        m[y].y = (int)std::max(p->first[y], p->second[y]);
+    }
+    hipError_t e = hipMalloc(&p->d_mask, sizeof(int2) * q.width);
+    if (e == hipSuccess) e = hipMemcpy(p->d_mask, m.data(), sizeof(int2) * q.width, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        mr_bp_plan_destroy(p);
+        return mr::hip_fail(e, "mr_bp_plan_create", __FILE__, __LINE__);
+    }
+    *plan = p;
+    return MR_OK;
+    MR_GUARD_END
+}
+
+extern "C" int mr_bp_plan_destroy(mr_bp_plan *plan)
+{
+    MR_GUARD_BEGIN
+    if (!plan) return MR_OK;
+    if (plan->d_mask) (void)hipFree(plan->d_mask);
+    if (plan->d_sino) (void)hipFree(plan->d_sino);
+    if (plan->d_tab) (void)hipFree(plan->d_tab);
+    if (plan->d_idx) (void)hipFree(plan->d_idx);
+    if (plan->d_w) (void)hipFree(plan->d_w);
+    delete plan;
+    return MR_OK;
+    MR_GUARD_END
+}
+
+extern "C" int mr_bp_plan_mask(const mr_bp_plan *plan, size_t *first, size_t *second)
+{
+    MR_GUARD_BEGIN
+    mr::clear_error();
+    MR_REQUIRE(plan != nullptr && first != nullptr && second != nullptr, "NULL argument");
+    std::copy(plan->first.begin(), plan->first.end(), first);
+    std::copy(plan->second.begin(), plan->second.end(), second);
+    return MR_OK;
+    MR_GUARD_END
+}
+
+extern "C" int mr_bp_parallel(mr_bp_plan *plan, const float *d_proj, size_t n_proj,
+                              const float *angles, const float *weights, float *d_volume,
+                              int accumulate, void *stream)
+{
+    MR_GUARD_BEGIN
+    mr::clear_error();
+    MR_REQUIRE(plan != nullptr, "plan is NULL");
+    MR_REQUIRE(d_volume != nullptr, "volume pointer is NULL");
+    MR_REQUIRE(n_proj == 0 || (d_proj && angles && weights), "NULL projection/angle/weight pointer");
+    hipStream_t s = (hipStream_t)stream;
+    const mr_bp_params &q = plan->p;
+    std::vector<size_t> used = mr::used_projections(n_proj, (size_t)q.buffer_size);
+    const size_t nk = used.size();
+    MR_REQUIRE(nk <= 65535, "at most 65535 projections per call");
+    int rc = ensure_capacity(plan, n_proj, nk);
+    if (rc != MR_OK) return rc;
+    std::vector<float> tab;
+    mr::bp_tables(angles, used.data(), nk, q.direction, q.center, q.width / 2, tab);
+    std::vector<int> idx(nk);
+    for (size_t k = 0; k < nk; ++k) idx[k] = (int)used[k];
+    if (nk > 0) {
+        MR_HIP(hipMemcpyAsync(plan->d_tab, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice, s));
+        MR_HIP(hipMemcpyAsync(plan->d_idx, idx.data(), sizeof(int) * nk, hipMemcpyHostToDevice, s));
+        MR_HIP(hipMemcpyAsync(plan->d_w, weights, sizeof(float) * n_proj, hipMemcpyHostToDevice, s));
+        dim3 g((unsigned)((q.width + 31) / 32), (unsigned)((plan->vpad + 31) / 32), (unsigned)nk);
+        mr::ProfScope prof("bp_prep", s);
+        hipLaunchKernelGGL(mr::bp_prep_kernel, g, dim3(256), 0, s, d_proj, plan->d_sino,
+                           plan->d_idx, plan->d_w, (int)q.width, (int)q.slices, (int)plan->vpad);
+        MR_HIP(hipGetLastError());
+    }
+    mr::BpArgs a{};
+    a.sino = plan->d_sino; a.vol = d_volume; a.tab = plan->d_tab; a.mask = plan->d_mask;
+    a.nu = (int)q.width; a.nv = (int)q.slices; a.vpad = (int)plan->vpad; a.nk = (int)nk;
+    a.nz4 = (int)((q.slices / 4) * 4);
+    a.bstride = q.multiprojbp ? a.nz4 : (int)plan->vpad;
+    a.accumulate = accumulate ? 1 : 0;
+    rc = mr::run_bp(a, q.interp, s);
+    if (rc != MR_OK) return rc;
+    // Host buffers (tables) must stay alive until the copies have been consumed.
+    MR_HIP(hipStreamSynchronize(s));
+    return MR_OK;
+    MR_GUARD_END
+}
+
+extern "C" int mr_fbp_parallel(mr_filter_plan *fplan, mr_bp_plan *bplan, float *d_proj,
+                               size_t n_proj, const float *angles, const float *weights,
+                               float *d_volume, void *stream)
+{
+    MR_GUARD_BEGIN
+    mr::clear_error();
+    MR_REQUIRE(fplan != nullptr && bplan != nullptr, "plan is NULL");
+    int rc = mr_filter_rows(fplan, d_proj, n_proj * bplan->p.slices, stream);
+    if (rc != MR_OK) return rc;
+    return mr_bp_parallel(bplan, d_proj, n_proj, angles, weights, d_volume, 0, stream);
+    MR_GUARD_END
+}

@@ -1,0 +1,270 @@
+// FDK cone-beam back-projection on gfx950 (FDKbp_single).
+//
+// Data layout in HBM
+//   d_proj  [n_proj][win_rows][width]  window projections; weighted, filtered and
+//                                      scaled in place (fdkbp_single.cpp:376-389)
+//   volume  [nz][ny][nx]               MatrixXYZ, written already mirrored in x and
+//                                      flipped in z (fdkreconbase.cpp:259-270)
+//
+// fdk_kernel: one thread owns one (x, y) voxel column and KT consecutive
+// slices in registers.  With the reference geometry (detector up = z, no
+// tilt) the homogeneous depth and the detector column do not depend on z, so
+// 1/depth, its square and the column index are computed once per projection
+// and column; only the detector row changes along z.  The per-projection
+// coefficients are the reference's xip/yip/zip products
+// (fdkbp_single.cpp:420-456) evaluated on the host in float, and the
+// accumulation order per voxel is the reference's projection order, so the
+// result is bit-identical to FDKbp_single.
+#include "mr_common.h"
+#include "mr_host.h"
+#include "mr_filter.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace mr {
+
+struct FdkArgs {
+    const float *img;
+    float *vol;
+    const float *ptab;  // 8 floats per projection: cx0 cx1 cx2 cy0 cy1 cy2 zc0 zc2
+    const float *zt1;   // [np][nz] zip row term per projection and slice
+    const int2 *mask;   // (first, second]: reference loop i = first+1 .. second
+    int nx, ny, nz, w, h, np;
+    float ox, oy, sx, sy;
+};
+
+template <int KT>
+__global__ __launch_bounds__(256) void fdk_kernel(FdkArgs a)
+{
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int k0 = blockIdx.z * KT;
+    if (i >= a.nx || j >= a.ny) return;
+    const int kn = min(KT, a.nz - k0);
+    float acc[KT];
+#pragma unroll
+    for (int kk = 0; kk < KT; ++kk) acc[kk] = 0.0f;
+    const int2 m = a.mask[j];
+    if (i > m.x && i <= m.y) {
+        const float x = a.ox + (float)i * a.sx;
+        const float y = a.oy + (float)j * a.sy;
+        const float fw = (float)a.w, fh = (float)a.h;
+        for (int p = 0; p < a.np; ++p) {
+            const float *t = a.ptab + 8 * p;
+            const float xip0 = x * t[0], xip1 = x * t[1], xip2 = x * t[2];
+            const float yip0 = y * t[3], yip1 = y * t[4], yip2 = y * t[5];
+            const float acc20 = t[6] + yip0;
+            const float acc22 = t[7] + yip2;
+            const float dw = 1.0f / (acc22 + xip2);
+            float c = dw * (acc20 + xip0);
+            c = c + 0.5f;
+            if (c < 0.0f || c >= fw) continue;
+            const int cc = (int)c;
+            const float dw2 = dw * dw;
+            const float *col = a.img + (size_t)p * a.h * a.w + cc;
+            const float *zt = a.zt1 + (size_t)p * a.nz + k0;
+#pragma unroll
+            for (int kk = 0; kk < KT; ++kk) {
+                if (kk < kn) {
+                    const float acc21 = zt[kk] + yip1;
+                    float r = dw * (acc21 + xip1);
+                    r = r + 0.5f;
+                    float val = 0.0f;
+                    if (!(r < 0.0f) && r < fh) val = col[(size_t)(int)r * a.w];
+                    acc[kk] = acc[kk] + dw2 * val;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int kk = 0; kk < KT; ++kk)
+        if (kk < kn)
+            a.vol[((size_t)(a.nz - 1 - (k0 + kk)) * a.ny + j) * a.nx + (a.nx - 1 - i)] = acc[kk];
+}
+
+}  // namespace mr
+
+struct mr_fdk_plan {
+    mr_fdk_params p{};
+    mr::FdkGeometry g{};
+    int device = 0;
+    size_t nx = 0, nz = 0, P = 0;
+    size_t cbct[4] = {0, 0, 0, 0};
+    float origin[3] = {0, 0, 0};
+    float ic[2] = {0, 0};
+    int2 *d_mask = nullptr;
+    float *d_hs = nullptr;
+    float2 *d_tw = nullptr;
+    float *d_ptab = nullptr;
+    float *d_zt1 = nullptr;
+    float *d_w = nullptr;
+    size_t cap_np = 0;
+};
+
+extern "C" int mr_fdk_plan_create(mr_fdk_plan **plan, const mr_fdk_params *params)
+{
+    MR_GUARD_BEGIN
+    mr::clear_error();
+    MR_REQUIRE(plan != nullptr && params != nullptr, "NULL argument");
+    *plan = nullptr;
+    const mr_fdk_params &q = *params;
+    MR_REQUIRE(q.roi[2] > q.roi[0] && q.roi[3] > q.roi[1], "ROI x1<=x0 or y1<=y0");
+    MR_REQUIRE(q.sdd > 0 && q.sod > 0 && q.pixel[0] > 0 && q.pixel[1] > 0, "invalid distances/pixel size");
+    MR_REQUIRE(q.direction == 0 || q.direction == 1, "direction must be 0 (CW) or 1 (CCW)");
+    auto *p = new mr_fdk_plan();
+    p->p = q;
+    mr::FdkGeometry &g = p->g;
+    g.sod = q.sod; g.sdd = q.sdd; g.pixel[0] = q.pixel[0]; g.pixel[1] = q.pixel[1];
+    for (int i = 0; i < 3; ++i) g.voxel[i] = q.voxel[i];
+    g.ppoint[0] = q.ppoint[0]; g.ppoint[1] = q.ppoint[1]; g.center = q.center;
+    g.direction = q.direction;
+    for (int i = 0; i < 4; ++i) g.roi[i] = q.roi[i];
+    g.projection_roi3 = q.projection_roi3;
+    if (hipGetDevice(&p->device) != hipSuccess) p->device = 0;
+    p->nx = q.roi[2] - q.roi[0];
+    p->nz = q.roi[3] - q.roi[1];
+    p->P = mr::fdk_padwidth(p->nx);
+    mr::fdk_cbct_roi(g, p->nx, p->cbct);
+    const float U = (float)(q.roi[2] - q.roi[0]);
+    const float V = (float)(q.roi[3] - q.roi[1]);
+    p->origin[0] = -(U - q.center) * q.voxel[0] - q.voxel[0] / 2;
+    p->origin[1] = -(U - q.center) * q.voxel[1] - q.voxel[1] / 2;
+    p->origin[2] = -(V - (q.ppoint[1] - (float)q.roi[1])) * q.voxel[2] - q.voxel[2] / 2;
+    p->ic[0] = q.ppoint[0] - (float)p->cbct[0];
+    p->ic[1] = q.ppoint[1] - (float)p->cbct[1];
+    std::vector<size_t> first, second;
+    mr::circle_mask(p->nx, p->nx, q.use_roi != 0, q.matrix_roi, true, first, second);
+    std::vector<int2> m(p->nx);
+    for (size_t y = 0; y < p->nx; ++y) { m[y].x = (int)first[y]; m[y].y = (int)second[y]; }
+    // ramp_filter_tuned's spectral ramp with its roll-off (fdkbp_single.cpp:906-919)
+    const size_t P = p->P;
+    std::vector<float> ramp(P);
+    const float DEGTORADF = 3.1415926535897932384626f / 180.0f;
+    for (size_t i = 0; i < P; ++i) {
+        ramp[i] = i < P / 2 ? (float)(unsigned)i : (float)(unsigned)(P - i);
+        ramp[i] *= (0.54 + 0.46 * (std::cos((float)(unsigned)i * DEGTORADF * 360 / (float)(unsigned)P) + 1));
+        ramp[i] /= (double)(unsigned)P * 0.5;
+    }
+    std::vector<float> tw = mr::make_twiddles(P);
+    hipError_t e = hipMalloc(&p->d_mask, sizeof(int2) * p->nx);
+    if (e == hipSuccess) e = hipMalloc(&p->d_hs, sizeof(float) * P);
+    if (e == hipSuccess) e = hipMalloc(&p->d_tw, sizeof(float2) * P);
+    if (e == hipSuccess) e = hipMemcpy(p->d_mask, m.data(), sizeof(int2) * p->nx, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_hs, ramp.data(), sizeof(float) * P, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_tw, tw.data(), sizeof(float) * 2 * P, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        mr_fdk_plan_destroy(p);
+        return mr::hip_fail(e, "mr_fdk_plan_create", __FILE__, __LINE__);
+    }
+    *plan = p;
+    return MR_OK;
+    MR_GUARD_END
+}
+
+extern "C" int mr_fdk_plan_destroy(mr_fdk_plan *plan)
+{
+    MR_GUARD_BEGIN
+    if (!plan) return MR_OK;
+    for (void *ptr : {(void *)plan->d_mask, (void *)plan->d_hs, (void *)plan->d_tw,
+                      (void *)plan->d_ptab, (void *)plan->d_zt1, (void *)plan->d_w})
+        if (ptr) (void)hipFree(ptr);
+    delete plan;
+    return MR_OK;
+    MR_GUARD_END
+}
+
+extern "C" int mr_fdk_plan_window(const mr_fdk_plan *plan, size_t *cbct_roi)
+{
+    MR_GUARD_BEGIN
+    mr::clear_error();
+    MR_REQUIRE(plan != nullptr && cbct_roi != nullptr, "NULL argument");
+    for (int i = 0; i < 4; ++i) cbct_roi[i] = plan->cbct[i];
+    return MR_OK;
+    MR_GUARD_END
+}
+
+extern "C" int mr_fdk_bp(mr_fdk_plan *plan, float *d_proj, size_t n_proj, size_t win_rows,
+                         const float *angles, const float *weights, float *d_volume, void *stream)
+{
+    MR_GUARD_BEGIN
+    mr::clear_error();
+    MR_REQUIRE(plan != nullptr, "plan is NULL");
+    MR_REQUIRE(d_volume != nullptr, "volume pointer is NULL");
+    MR_REQUIRE(n_proj == 0 || (d_proj && angles && weights), "NULL projection/angle/weight pointer");
+    MR_REQUIRE(win_rows >= 11, "FDK window needs at least 11 detector rows (5-row margins)");
+    MR_REQUIRE(n_proj <= 65535, "too many projections for one call");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t nx = plan->nx, nz = plan->nz, w = nx, h = win_rows;
+    if (n_proj > plan->cap_np) {
+        for (float **ptr : {&plan->d_ptab, &plan->d_zt1, &plan->d_w})
+            if (*ptr) { (void)hipFree(*ptr); *ptr = nullptr; }
+        plan->cap_np = 0;
+        MR_HIP(hipMalloc(&plan->d_ptab, sizeof(float) * 8 * n_proj));
+        MR_HIP(hipMalloc(&plan->d_zt1, sizeof(float) * nz * n_proj));
+        MR_HIP(hipMalloc(&plan->d_w, sizeof(float) * n_proj));
+        plan->cap_np = n_proj;
+    }
+    // per-projection coefficients (fdkbp_single.cpp:420-456)
+    std::vector<float> ptab(8 * std::max<size_t>(n_proj, 1)), zt1(nz * std::max<size_t>(n_proj, 1));
+    const float *ic = plan->ic;
+    for (size_t ip = 0; ip < n_proj; ++ip) {
+        float pm[12];
+        mr::fdk_proj_matrix(plan->g, angles[ip], pm);
+        if (pm[2] != 0.0f || pm[10] != 0.0f) {
+            mr::set_error("projection matrix depends on z in x/depth (tilted geometry) - unsupported");
+            return MR_ERR_UNSUPPORTED;
+        }
+        float *t = &ptab[8 * ip];
+        t[0] = pm[0] + ic[0] * pm[8];
+        t[1] = pm[4] + ic[1] * pm[8];
+        t[2] = pm[8];
+        t[3] = pm[1] + ic[0] * pm[9];
+        t[4] = pm[5] + ic[1] * pm[9];
+        t[5] = pm[9];
+        t[6] = ic[0] * pm[11] + pm[3];
+        t[7] = pm[11];
+        const float a1 = pm[6] + ic[1] * pm[10];
+        const float b1 = ic[1] * pm[11];
+        for (size_t k = 0; k < nz; ++k) {
+            const float z = plan->origin[2] + (float)(long)k * plan->p.voxel[2];
+            zt1[ip * nz + k] = z * a1 + b1 + pm[7];
+        }
+    }
+    if (n_proj == 0) {
+        // nothing back-projected: the reference leaves the zeroed volume
+        int rz = mr::fill_zero(d_volume, nx * nx * nz, s);
+        if (rz != MR_OK) return rz;
+        MR_HIP(hipStreamSynchronize(s));
+        return MR_OK;
+    }
+    MR_HIP(hipMemcpyAsync(plan->d_ptab, ptab.data(), sizeof(float) * 8 * n_proj, hipMemcpyHostToDevice, s));
+    MR_HIP(hipMemcpyAsync(plan->d_zt1, zt1.data(), sizeof(float) * nz * n_proj, hipMemcpyHostToDevice, s));
+    MR_HIP(hipMemcpyAsync(plan->d_w, weights, sizeof(float) * n_proj, hipMemcpyHostToDevice, s));
+    // weight, ramp filter, 1/P, (SOD/SDD)^2, SOD/SDD  (fdkreconbase.cpp:247, fdkbp_single.cpp:376-389)
+    const float sod = plan->p.sod, sdd = plan->p.sdd;
+    mr::RowFilterArgs fa{};
+    fa.rows = d_proj; fa.n_rows = (long)(n_proj * h); fa.n = (int)w; fa.mode = 1;
+    fa.insert = (int)((plan->P - w) / 2); fa.taper = nullptr; fa.hs = plan->d_hs; fa.tw = plan->d_tw;
+    fa.post0 = 1.0f / (float)plan->P; fa.post1 = (sod * sod) / (sdd * sdd); fa.post2 = sod / sdd;
+    fa.pre_w = plan->d_w; fa.rows_per_proj = (int)h;
+    int rc = mr::run_row_filter(fa, (int)plan->P, s);
+    if (rc != MR_OK) return rc;
+    rc = mr::run_fdk_margin_rows(d_proj, (int)w, (int)h, (long)n_proj, s);
+    if (rc != MR_OK) return rc;
+    mr::FdkArgs a{};
+    a.img = d_proj; a.vol = d_volume; a.ptab = plan->d_ptab; a.zt1 = plan->d_zt1; a.mask = plan->d_mask;
+    a.nx = (int)nx; a.ny = (int)nx; a.nz = (int)nz; a.w = (int)w; a.h = (int)h; a.np = (int)n_proj;
+    a.ox = plan->origin[0]; a.oy = plan->origin[1]; a.sx = plan->p.voxel[0]; a.sy = plan->p.voxel[1];
+    constexpr int KT = 16;
+    dim3 grid((unsigned)((nx + 63) / 64), (unsigned)((nx + 3) / 4), (unsigned)((nz + KT - 1) / KT));
+    {
+        mr::ProfScope prof("fdk_bp", s);
+        hipLaunchKernelGGL(mr::fdk_kernel<KT>, grid, dim3(256), 0, s, a);
+    }
+    MR_HIP(hipGetLastError());
+    MR_HIP(hipStreamSynchronize(s));
+    return MR_OK;
+    MR_GUARD_END
+}

@@ -1,0 +1,31 @@
+// Row ramp-filter launch interface shared by the parallel-beam and FDK paths.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <vector>
+
+namespace mr {
+
+struct RowFilterArgs {
+    float *rows;                // n_rows x n, filtered in place
+    long n_rows;
+    int n;                      // row width
+    int mode;                   // 0 ProjectionFilterSingle, 1 FDK ramp_filter_tuned
+    int insert;                 // mode 0: nInsert; mode 1: (P - n) / 2
+    const float *taper;         // mode 0: nInsert sigmoid weights
+    const float *hs;            // P real spectral multipliers
+    const float2 *tw;           // P twiddles exp(-2 pi i k / P)
+    float post0, post1, post2;  // sequential output multipliers
+    const float *pre_w;         // mode 1: per-projection input weight (nullable)
+    int rows_per_proj;          // rows of one projection (mode 1)
+};
+
+int run_row_filter(const RowFilterArgs &a, int P, hipStream_t s);
+int run_fdk_margin_rows(float *img, int n, int h, long n_proj, hipStream_t s);
+std::vector<float> make_twiddles(size_t P);
+// Zero-fills n floats with a kernel (no hipMemset, so callers may capture it in a graph).
+int fill_zero(float *d, size_t n, hipStream_t s);
+
+}  // namespace mr

@@ -1,0 +1,268 @@
+// Ramp filters on gfx950: ProjectionFilterSingle (parallel beam) and
+// FDKbp_single::ramp_filter_tuned (cone beam).
+//
+// One workgroup filters two detector rows at once: row a goes into the real
+// part and row b into the imaginary part of one complex line of length P held
+// in LDS.  Both filters multiply the spectrum by a real, even sequence, so
+// IFFT(Hs * FFT(a + i b)) = filt(a) + i filt(b) exactly and no split/merge
+// step is needed.  Padding, tapering and scaling are fused into the load and
+// store of the line; the rows are filtered in place in HBM.
+#include "mr_common.h"
+#include "mr_fft.h"
+#include "mr_host.h"
+#include "mr_filter.h"
+
+#include <cmath>
+#include <vector>
+
+namespace mr {
+
+
+
+// ProjectionFilterSingle::Pad (ProjectionFilter.cpp:346-361) evaluated at
+// position i < P of the P+16 long line.
+__device__ __forceinline__ float pb_pad(const float *row, int n, int i, int ins, int P,
+                                        const float *__restrict__ taper, float first, float last)
+{
+    if (i < ins) return first * taper[i];
+    const int d = i - ins;
+    if (d < n) return row[d];
+    const int t = P + 15 - i;
+    return (t < ins) ? last * taper[t] : 0.0f;
+}
+
+// ramp_filter_tuned zero padding with the 5-column edge copies
+// (fdkbp_single.cpp:855-897); the row itself is pre-multiplied by the
+// projection weight (FdkReconBase::Process, fdkreconbase.cpp:243-249).
+__device__ __forceinline__ float fdk_pad(const float *row, int n, int c, int ins, int P, float w)
+{
+    const int cc = c < 5 ? 5 : (c >= P - 5 ? P - 6 : c);
+    const int d = cc - ins;
+    return (d >= 0 && d < n) ? row[d] * w : 0.0f;
+}
+
+template <int P, int NT>
+__global__ __launch_bounds__(NT) void row_filter_kernel(RowFilterArgs a)
+{
+    extern __shared__ float2 line[];
+    const long r0 = 2 * (long)blockIdx.x;
+    const bool has1 = r0 + 1 < a.n_rows;
+    float *row0 = a.rows + r0 * a.n;
+    float *row1 = has1 ? row0 + a.n : row0;
+    const int t = threadIdx.x;
+
+    if (a.mode == 0) {
+        const float f0 = row0[0], l0 = row0[a.n - 1];
+        const float f1 = row1[0], l1 = row1[a.n - 1];
+        for (int i = t; i < P; i += NT) {
+            const float x0 = pb_pad(row0, a.n, i, a.insert, P, a.taper, f0, l0);
+            const float x1 = has1 ? pb_pad(row1, a.n, i, a.insert, P, a.taper, f1, l1) : 0.0f;
+            line[i] = make_float2(x0, x1);
+        }
+    } else {
+        const float w0 = a.pre_w ? a.pre_w[r0 / a.rows_per_proj] : 1.0f;
+        const float w1 = (a.pre_w && has1) ? a.pre_w[(r0 + 1) / a.rows_per_proj] : 1.0f;
+        for (int i = t; i < P; i += NT) {
+            const float x0 = fdk_pad(row0, a.n, i, a.insert, P, w0);
+            const float x1 = has1 ? fdk_pad(row1, a.n, i, a.insert, P, w1) : 0.0f;
+            line[i] = make_float2(x0, x1);
+        }
+    }
+    __syncthreads();
+    fft_lds<P, NT, false>(line, a.tw);
+    for (int i = t; i < P; i += NT) {
+        const float h = a.hs[i];
+        float2 v = line[i];
+        line[i] = make_float2(v.x * h, v.y * h);
+    }
+    __syncthreads();
+    fft_lds<P, NT, true>(line, a.tw);
+    if (a.mode == 0) {
+        for (int i = t; i < a.n; i += NT) {
+            const float2 v = line[i + a.insert];
+            row0[i] = v.x * a.post0;
+            if (has1) row1[i] = v.y * a.post0;
+        }
+    } else {
+        for (int i = t; i < a.n; i += NT) {
+            const float2 v = line[i + a.insert];
+            row0[i] = ((v.x * a.post0) * a.post1) * a.post2;
+            if (has1) row1[i] = ((v.y * a.post0) * a.post1) * a.post2;
+        }
+    }
+}
+
+// FDK margin rows: rows 0..4 take the filtered row 5, rows h-5..h-1 the
+// filtered row h-6 (fdkbp_single.cpp:880-886).  Filtering is row-local, so
+// copying the filtered rows equals filtering the copied raw rows.
+__global__ void fdk_margin_rows_kernel(float *img, int n, int h, long n_proj)
+{
+    const long p = blockIdx.y;
+    const int rr = blockIdx.z;  // 0..9
+    const int dst = rr < 5 ? rr : h - 10 + rr;
+    const int src = rr < 5 ? 5 : h - 6;
+    float *base = img + p * (long)n * h;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
+        base[(long)dst * n + c] = base[(long)src * n + c];
+}
+
+template <int P>
+static int launch_row_filter(const RowFilterArgs &a, hipStream_t s)
+{
+    constexpr int NT = (P / 16) < 64 ? 64 : ((P / 16) > 512 ? 512 : (P / 16));
+    const size_t lds = sizeof(float2) * P;
+    if (lds > 65536)
+        MR_HIP(hipFuncSetAttribute((const void *)row_filter_kernel<P, NT>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const long pairs = (a.n_rows + 1) / 2;
+    if (pairs == 0) return MR_OK;
+    if (pairs > 0x7fffffffL) {
+        set_error("too many rows for one launch");
+        return MR_ERR_INVALID;
+    }
+    ProfScope prof(a.mode == 0 ? "row_filter_pb" : "row_filter_fdk", s);
+    hipLaunchKernelGGL((row_filter_kernel<P, NT>), dim3((unsigned)pairs), dim3(NT), lds, s, a);
+    MR_HIP(hipGetLastError());
+    return MR_OK;
+}
+
+int run_row_filter(const RowFilterArgs &a, int P, hipStream_t s)
+{
+    switch (P) {
+    case 64: return launch_row_filter<64>(a, s);
+    case 128: return launch_row_filter<128>(a, s);
+    case 256: return launch_row_filter<256>(a, s);
+    case 512: return launch_row_filter<512>(a, s);
+    case 1024: return launch_row_filter<1024>(a, s);
+    case 2048: return launch_row_filter<2048>(a, s);
+    case 4096: return launch_row_filter<4096>(a, s);
+    case 8192: return launch_row_filter<8192>(a, s);
+    case 16384: return launch_row_filter<16384>(a, s);
+    default:
+        set_error("FFT length " + std::to_string(P) + " outside the supported 64..16384");
+        return MR_ERR_UNSUPPORTED;
+    }
+}
+
+int run_fdk_margin_rows(float *img, int n, int h, long n_proj, hipStream_t s)
+{
+    if (n_proj == 0) return MR_OK;
+    dim3 grid((unsigned)((n + 255) / 256), (unsigned)n_proj, 10);
+    ProfScope prof("fdk_margin_rows", s);
+    hipLaunchKernelGGL(fdk_margin_rows_kernel, grid, dim3(256), 0, s, img, n, h, n_proj);
+    MR_HIP(hipGetLastError());
+    return MR_OK;
+}
+
+// P twiddles exp(-2 pi i k / P) in double, rounded to float.
+std::vector<float> make_twiddles(size_t P)
+{
+    std::vector<float> tw(2 * P);
+    const double pi = 3.1415926535897932384626433832795;
+    for (size_t k = 0; k < P; ++k) {
+        const double ang = -2.0 * pi * (double)k / (double)P;
+        tw[2 * k] = (float)std::cos(ang);
+        tw[2 * k + 1] = (float)std::sin(ang);
+    }
+    return tw;
+}
+
+}  // namespace mr
+
+// --------------------------------------------------------------------------
+// ProjectionFilterSingle plan (C ABI)
+// --------------------------------------------------------------------------
+struct mr_filter_plan {
+    int device = 0;
+    size_t width = 0, P = 0, n_insert = 0;
+    float scale = 0.0f;
+    std::vector<float> H;       // P/2 (reference layout)
+    float *d_hs = nullptr;      // P
+    float *d_taper = nullptr;   // n_insert (>= 1)
+    float2 *d_tw = nullptr;     // P
+};
+
+extern "C" int mr_filter_plan_create(mr_filter_plan **plan, size_t width, int filter_type,
+                                     float cutoff, float order, int use_bias, float bias_weight,
+                                     size_t padding_doubler)
+{
+    MR_GUARD_BEGIN
+    mr::clear_error();
+    MR_REQUIRE(plan != nullptr, "plan is NULL");
+    *plan = nullptr;
+    MR_REQUIRE(width >= 2, "filter width must be at least 2");
+    MR_REQUIRE(filter_type >= 0 && filter_type <= 4, "unknown filter type");
+    MR_REQUIRE(padding_doubler <= 8, "paddingdoubler out of range");
+    std::vector<float> H, taper;
+    size_t P = 0, n_insert = 0;
+    int rc = mr::build_pb_filter(width, filter_type, cutoff, order, use_bias != 0, bias_weight,
+                                 padding_doubler, P, n_insert, H, taper);
+    if (rc != MR_OK) return rc;
+    if (P < 64 || P > 16384) {
+        mr::set_error("FFT length " + std::to_string(P) + " outside the supported 64..16384");
+        return MR_ERR_UNSUPPORTED;
+    }
+    auto *p = new mr_filter_plan();
+    p->width = width; p->P = P; p->n_insert = n_insert; p->H = H;
+    p->scale = 3.1415926535897932384626f / (4.0f * (float)(P / 2));
+    if (hipGetDevice(&p->device) != hipSuccess) p->device = 0;
+    std::vector<float> hs(P);
+    for (size_t k = 0; k < P; ++k)
+        hs[k] = k < P / 2 ? H[k] : (k == P / 2 ? 1.0f : H[P - k]);
+    std::vector<float> tw = mr::make_twiddles(P);
+    if (taper.empty()) taper.push_back(0.0f);
+    hipError_t e = hipMalloc(&p->d_hs, sizeof(float) * P);
+    if (e == hipSuccess) e = hipMalloc(&p->d_taper, sizeof(float) * taper.size());
+    if (e == hipSuccess) e = hipMalloc(&p->d_tw, sizeof(float2) * P);
+    if (e == hipSuccess) e = hipMemcpy(p->d_hs, hs.data(), sizeof(float) * P, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(p->d_taper, taper.data(), sizeof(float) * taper.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_tw, tw.data(), sizeof(float) * 2 * P, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        mr_filter_plan_destroy(p);
+        return mr::hip_fail(e, "mr_filter_plan_create upload", __FILE__, __LINE__);
+    }
+    *plan = p;
+    return MR_OK;
+    MR_GUARD_END
+}
+
+extern "C" int mr_filter_plan_destroy(mr_filter_plan *plan)
+{
+    MR_GUARD_BEGIN
+    if (!plan) return MR_OK;
+    if (plan->d_hs) (void)hipFree(plan->d_hs);
+    if (plan->d_taper) (void)hipFree(plan->d_taper);
+    if (plan->d_tw) (void)hipFree(plan->d_tw);
+    delete plan;
+    return MR_OK;
+    MR_GUARD_END
+}
+
+extern "C" int mr_filter_plan_info(const mr_filter_plan *plan, size_t *fft_size, size_t *n_insert,
+                                   float *h_filter)
+{
+    MR_GUARD_BEGIN
+    mr::clear_error();
+    MR_REQUIRE(plan != nullptr, "plan is NULL");
+    if (fft_size) *fft_size = plan->P;
+    if (n_insert) *n_insert = plan->n_insert;
+    if (h_filter) std::copy(plan->H.begin(), plan->H.end(), h_filter);
+    return MR_OK;
+    MR_GUARD_END
+}
+
+extern "C" int mr_filter_rows(mr_filter_plan *plan, float *d_rows, size_t n_rows, void *stream)
+{
+    MR_GUARD_BEGIN
+    mr::clear_error();
+    MR_REQUIRE(plan != nullptr, "plan is NULL");
+    MR_REQUIRE(d_rows != nullptr || n_rows == 0, "rows pointer is NULL");
+    if (n_rows == 0) return MR_OK;
+    mr::RowFilterArgs a{};
+    a.rows = d_rows; a.n_rows = (long)n_rows; a.n = (int)plan->width; a.mode = 0;
+    a.insert = (int)plan->n_insert; a.taper = plan->d_taper; a.hs = plan->d_hs; a.tw = plan->d_tw;
+    a.post0 = plan->scale; a.post1 = 1.0f; a.post2 = 1.0f; a.pre_w = nullptr; a.rows_per_proj = 1;
+    return mr::run_row_filter(a, (int)plan->P, (hipStream_t)stream);
+    MR_GUARD_END
+}

@@ -1,0 +1,237 @@
+"""Host-side mirror of MuhRec's hot-path modules over the HIP C ABI.
+
+The classes keep the reference modules' names, parameter keys and argument
+meanings:
+
+* ``ProjectionFilterSingle`` -- frameworks/tomography/Preprocessing/
+  StdPreprocModules/src/ProjectionFilter.cpp:78-118 (Configure, ProcessCore).
+* ``BackProjector`` for ``MultiProjBPparallel`` / ``MultiProjBP`` /
+  ``NearestNeighborBP`` -- frameworks/tomography/Backprojectors/
+  StdBackProjectors/src/StdBackProjectorBase.cpp:120-201 (SetROI, Process(3D)).
+* ``FDKbp_single`` -- frameworks/tomography/Backprojectors/FDKBackProjectors/
+  src/fdkreconbase.cpp:91-277.
+
+Data stays on the GPU: inputs and outputs are torch CUDA tensors (torch is
+only the device-memory/stream plumbing).  Errors from the library raise
+``MrError`` (the reference throws ReconException).
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import BpParams, FdkParams, MrError, check
+
+BP_MODULES = {
+    "MultiProjBPparallel": dict(interp=0, multiprojbp=0),
+    "MultiProjBP": dict(interp=0, multiprojbp=1),
+    "NearestNeighborBP": dict(interp=1, multiprojbp=0),
+}
+
+
+def _ptr(t) -> ctypes.c_void_p:
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream) -> ctypes.c_void_p:
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def _f32arr(x, n: int) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+    if a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+def _require_cuda_f32(t, ndim: int, name: str):
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError(f"{name} must be a CUDA tensor")
+    if t.dtype != torch.float32 or not t.is_contiguous() or t.dim() != ndim:
+        raise TypeError(f"{name} must be a contiguous float32 tensor with {ndim} dims")
+
+
+class ProjectionFilterSingle:
+    """Ramp filter along detector rows (the reference's default pre-BP module)."""
+
+    def __init__(self, width: int, filtertype: str = "Hamming", cutoff: float = 0.5,
+                 order: float = 1.0, usebias: bool = True, biasweight: float = 0.1,
+                 paddingdoubler: int = 2):
+        L = _lib.load()
+        ft = _lib.FILTER_TYPES.get(filtertype.lower())
+        if ft is None:
+            raise MrError(_lib.MR_ERR_INVALID, "Could not convert filter string to enum.")
+        self.width = int(width)
+        self._plan = ctypes.c_void_p()
+        check(L.mr_filter_plan_create(ctypes.byref(self._plan), self.width, ft, cutoff, order,
+                                      int(bool(usebias)), biasweight, paddingdoubler))
+        fs, ni = ctypes.c_size_t(), ctypes.c_size_t()
+        check(L.mr_filter_plan_info(self._plan, ctypes.byref(fs), ctypes.byref(ni), None))
+        self.fft_size, self.n_insert = fs.value, ni.value
+        H = np.zeros(self.fft_size // 2, np.float32)
+        check(L.mr_filter_plan_info(self._plan, None, None,
+                                    H.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        self.filter = H
+
+    def process(self, proj, stream=None):
+        """Filters ``proj`` ([..., width] CUDA float32) in place and returns it."""
+        import torch
+        if not isinstance(proj, torch.Tensor) or not proj.is_cuda or proj.dtype != torch.float32 \
+                or not proj.is_contiguous():
+            raise TypeError("proj must be a contiguous float32 CUDA tensor")
+        if proj.numel() == 0:
+            raise MrError(_lib.MR_ERR_INVALID, "Empty projection image")
+        if proj.shape[-1] != self.width:
+            raise MrError(_lib.MR_ERR_INVALID, "row width does not match the configured ROI")
+        check(_lib.load().mr_filter_rows(self._plan, _ptr(proj), proj.numel() // self.width,
+                                         _stream(stream)))
+        return proj
+
+    def close(self):
+        if getattr(self, "_plan", None) and self._plan.value:
+            _lib.load().mr_filter_plan_destroy(self._plan)
+            self._plan = ctypes.c_void_p()
+
+    __del__ = close
+
+
+class BackProjector:
+    """Parallel-beam back-projector (MultiProjBPparallel, MultiProjBP, NearestNeighborBP).
+
+    ``width``/``slices`` are the block's SizeU/SizeV, ``center`` is fCenter
+    (ROI-relative), ``direction`` 0 = CW, 1 = CCW, ``buffer_size`` is
+    ProjectionBufferSize (16 reproduces the reference exactly, including its
+    dropped projections; 0 back-projects every projection)."""
+
+    def __init__(self, name: str, width: int, slices: int, center: float, direction: int = 0,
+                 buffer_size: int = 16, roi: Optional[Sequence[int]] = None):
+        if name not in BP_MODULES:
+            raise MrError(_lib.MR_ERR_INVALID, f"unknown back-projector {name}")
+        L = _lib.load()
+        p = BpParams()
+        p.width, p.slices, p.center, p.direction = int(width), int(slices), float(center), int(direction)
+        p.interp = BP_MODULES[name]["interp"]
+        p.multiprojbp = BP_MODULES[name]["multiprojbp"]
+        p.buffer_size = int(buffer_size)
+        p.use_roi = 0 if roi is None else 1
+        if roi is not None:
+            for i in range(4):
+                p.roi[i] = int(roi[i])
+        self.name, self.width, self.slices = name, int(width), int(slices)
+        self._plan = ctypes.c_void_p()
+        check(L.mr_bp_plan_create(ctypes.byref(self._plan), ctypes.byref(p)))
+
+    @property
+    def mask(self):
+        f = np.zeros(self.width, np.uintp)
+        s = np.zeros(self.width, np.uintp)
+        szp = ctypes.POINTER(ctypes.c_size_t)
+        check(_lib.load().mr_bp_plan_mask(self._plan, f.ctypes.data_as(szp), s.ctypes.data_as(szp)))
+        return f.astype(np.int64), s.astype(np.int64)
+
+    def process(self, proj, angles, weights, volume=None, accumulate: bool = False, stream=None):
+        """Back-projects ``proj`` [n_proj, slices, width] (CUDA) into a
+        [width, width, slices] ZXY volume (allocated when not given)."""
+        import torch
+        _require_cuda_f32(proj, 3, "proj")
+        n_proj = proj.shape[0]
+        if proj.shape[1] != self.slices or proj.shape[2] != self.width:
+            raise MrError(_lib.MR_ERR_INVALID, "projection size does not match the ROI")
+        a = _f32arr(angles, n_proj)
+        w = _f32arr(weights, n_proj)
+        if volume is None:
+            volume = torch.empty((self.width, self.width, self.slices), dtype=torch.float32,
+                                 device=proj.device)
+            accumulate = False
+        _require_cuda_f32(volume, 3, "volume")
+        fp = ctypes.POINTER(ctypes.c_float)
+        check(_lib.load().mr_bp_parallel(self._plan, _ptr(proj), n_proj, a.ctypes.data_as(fp),
+                                         w.ctypes.data_as(fp), _ptr(volume), int(bool(accumulate)),
+                                         _stream(stream)))
+        return volume
+
+    def close(self):
+        if getattr(self, "_plan", None) and self._plan.value:
+            _lib.load().mr_bp_plan_destroy(self._plan)
+            self._plan = ctypes.c_void_p()
+
+    __del__ = close
+
+
+def fbp_parallel(filt: ProjectionFilterSingle, bp: BackProjector, proj, angles, weights,
+                 volume, stream=None):
+    """Filter ``proj`` in place and back-project it (one engine block)."""
+    _require_cuda_f32(proj, 3, "proj")
+    _require_cuda_f32(volume, 3, "volume")
+    n_proj = proj.shape[0]
+    a = _f32arr(angles, n_proj)
+    w = _f32arr(weights, n_proj)
+    fp = ctypes.POINTER(ctypes.c_float)
+    check(_lib.load().mr_fbp_parallel(filt._plan, bp._plan, _ptr(proj), n_proj,
+                                      a.ctypes.data_as(fp), w.ctypes.data_as(fp), _ptr(volume),
+                                      _stream(stream)))
+    return volume
+
+
+class FDKbp_single:
+    """FDK cone-beam back-projector for one slab (SetROI(roi) + Process(3D))."""
+
+    def __init__(self, sod: float, sdd: float, pixel, voxel, ppoint, center: float,
+                 roi: Sequence[int], projection_roi3: int, direction: int = 0,
+                 matrix_roi: Optional[Sequence[int]] = None):
+        L = _lib.load()
+        p = FdkParams()
+        p.sod, p.sdd = sod, sdd
+        p.pixel[0], p.pixel[1] = pixel
+        p.voxel[0], p.voxel[1], p.voxel[2] = voxel
+        p.ppoint[0], p.ppoint[1] = ppoint
+        p.center, p.direction = center, direction
+        for i in range(4):
+            p.roi[i] = int(roi[i])
+        p.projection_roi3 = int(projection_roi3)
+        p.use_roi = 0 if matrix_roi is None else 1
+        if matrix_roi is not None:
+            for i in range(4):
+                p.matrix_roi[i] = int(matrix_roi[i])
+        self.nx = int(roi[2] - roi[0])
+        self.nz = int(roi[3] - roi[1])
+        self._plan = ctypes.c_void_p()
+        check(L.mr_fdk_plan_create(ctypes.byref(self._plan), ctypes.byref(p)))
+
+    @property
+    def window(self):
+        w = (ctypes.c_size_t * 4)()
+        check(_lib.load().mr_fdk_plan_window(self._plan, w))
+        return [int(v) for v in w]
+
+    def process(self, proj, angles, weights, volume=None, stream=None):
+        """``proj`` [n_proj, win_rows, width] CUDA (consumed as scratch) ->
+        [nz, nx, nx] XYZ volume."""
+        import torch
+        _require_cuda_f32(proj, 3, "proj")
+        n_proj, rows, width = proj.shape
+        if width != self.nx:
+            raise MrError(_lib.MR_ERR_INVALID, "projection width does not match the ROI")
+        a = _f32arr(angles, n_proj)
+        w = _f32arr(weights, n_proj)
+        if volume is None:
+            volume = torch.empty((self.nz, self.nx, self.nx), dtype=torch.float32, device=proj.device)
+        _require_cuda_f32(volume, 3, "volume")
+        fp = ctypes.POINTER(ctypes.c_float)
+        check(_lib.load().mr_fdk_bp(self._plan, _ptr(proj), n_proj, rows, a.ctypes.data_as(fp),
+                                    w.ctypes.data_as(fp), _ptr(volume), _stream(stream)))
+        return volume
+
+    def close(self):
+        if getattr(self, "_plan", None) and self._plan.value:
+            _lib.load().mr_fdk_plan_destroy(self._plan)
+            self._plan = ctypes.c_void_p()
+
+    __del__ = close

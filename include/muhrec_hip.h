@@ -1,0 +1,162 @@
+/*
+ * muhrec_hip.h -- C ABI of the MI355X (gfx950) reconstruction kernels.
+ *
+ * Plain pointers and sizes only.  Every function returns an mr_status code;
+ * on failure mr_last_error() returns a thread-local message.  No C++
+ * exception crosses this boundary.  Pointers named d_* are device pointers
+ * (hipMalloc'd, or a torch CUDA tensor's data_ptr); everything else is host
+ * memory.  `stream` is a hipStream_t (NULL = the default stream).  Plans are
+ * bound to the device that is current when they are created.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the DTasev/imagingsuite root).
+ */
+#ifndef MUHREC_HIP_H
+#define MUHREC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum mr_status {
+    MR_OK = 0,
+    MR_ERR_INVALID = 1,     /* bad argument (the reference throws ReconException) */
+    MR_ERR_HIP = 2,         /* a HIP runtime call failed */
+    MR_ERR_UNSUPPORTED = 3, /* configuration outside what the kernels implement */
+    MR_ERR_INTERNAL = 4     /* unexpected host-side exception */
+};
+
+/* ProjectionFilterBase::FilterType,
+ * frameworks/tomography/Preprocessing/StdPreprocModules/include/ProjectionFilter.h:27-33 */
+enum mr_filter_type {
+    MR_FILTER_RAMLAK = 0,
+    MR_FILTER_SHEPPLOGAN = 1,
+    MR_FILTER_HANNING = 2,
+    MR_FILTER_HAMMING = 3,
+    MR_FILTER_BUTTERWORTH = 4
+};
+
+enum mr_interp {
+    MR_INTERP_LINEAR = 0,  /* MultiProjBP / MultiProjBPparallel */
+    MR_INTERP_NEAREST = 1  /* NearestNeighborBP */
+};
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *mr_last_error(void);
+/* ABI version of this library. */
+int mr_version(void);
+/* Number of visible HIP devices (0 when none); never fails. */
+int mr_device_count(void);
+
+/* Kernel timing (no reference counterpart; measurement hook for benchmarks).
+ * While enabled, every kernel launch of this library is bracketed by HIP
+ * events on the stream it is launched on.  mr_profile_query synchronises on
+ * the recorded events and returns the summed duration and launch count of
+ * the kernels whose name contains `kernel` ("" = all). */
+int mr_profile_enable(int on);
+int mr_profile_reset(void);
+int mr_profile_query(const char *kernel, double *total_ms, long *launches);
+
+/* ------------------------------------------------------------------------
+ * ProjectionFilterSingle ramp filter.
+ * Replaces ProjectionFilterBase::Configure + ProjectionFilterSingle::BuildFilter
+ * (frameworks/tomography/Preprocessing/StdPreprocModules/src/ProjectionFilter.cpp:78-92,254-302)
+ * and FilterProjection / ProcessCore(3D) (:103-118, :304-344).
+ * ------------------------------------------------------------------------ */
+typedef struct mr_filter_plan mr_filter_plan;
+
+/* width = ROI width N; the FFT length is 2^(ceil(log2 N)+padding_doubler).
+ * Fails with MR_ERR_INVALID for a cut-off outside (0, 0.5] (reference :265-266). */
+int mr_filter_plan_create(mr_filter_plan **plan, size_t width, int filter_type, float cutoff,
+                          float order, int use_bias, float bias_weight, size_t padding_doubler);
+int mr_filter_plan_destroy(mr_filter_plan *plan);
+/* fft_size, n_insert and (optionally) the fft_size/2 filter coefficients H. */
+int mr_filter_plan_info(const mr_filter_plan *plan, size_t *fft_size, size_t *n_insert,
+                        float *h_filter);
+/* Filters n_rows rows of `width` samples in place (device memory).  A projection
+ * stack [n_proj][rows][width] is n_proj*rows rows. */
+int mr_filter_rows(mr_filter_plan *plan, float *d_rows, size_t n_rows, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Parallel-beam back-projection.
+ * Replaces StdBackProjectorBase::SetROI/Process(2D/3D)
+ * (frameworks/tomography/Backprojectors/StdBackProjectors/src/StdBackProjectorBase.cpp:64-201),
+ * MultiProjectionBPparallel::BackProject (.../src/MultiProjBPparallel.cpp:39-169),
+ * MultiProjectionBP::BackProject (.../src/MultiProjBP.cpp:36-144) and
+ * NearestNeighborBP::BackProject (.../src/NNMultiProjBP.cpp:36-119),
+ * with BackProjectorModuleBase::BuildCircleMask
+ * (frameworks/tomography/Framework/ReconFramework/src/BackProjectorModuleBase.cpp:153-218).
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    size_t width;        /* SizeU = roi[2]-roi[0]; volume is width x width x slices */
+    size_t slices;       /* SizeV = detector rows of the block */
+    float center;        /* ProjectionInfo.fCenter (ROI-relative) */
+    int direction;       /* kipl::base::eRotationDirection: 0 CW, 1 CCW */
+    int interp;          /* mr_interp */
+    int buffer_size;     /* ProjectionBufferSize; >0 reproduces the reference's
+                            drop of every buffer_size-th projection, 0 uses all */
+    int multiprojbp;     /* 1: MultiProjBP column-B offset (differs only when slices%4 != 0) */
+    int use_roi;         /* MatrixInfo.bUseROI */
+    size_t roi[4];       /* MatrixInfo.roi x0,y0,x1,y1 (inclusive) */
+} mr_bp_params;
+
+typedef struct mr_bp_plan mr_bp_plan;
+
+int mr_bp_plan_create(mr_bp_plan **plan, const mr_bp_params *params);
+int mr_bp_plan_destroy(mr_bp_plan *plan);
+/* Host copy of the circle mask: for row y the voxels first[y] <= x < second[y]
+ * are reconstructed (BuildCircleMask's pair, with `second` exclusive). */
+int mr_bp_plan_mask(const mr_bp_plan *plan, size_t *first, size_t *second);
+/* Back-projects n_proj filtered projections d_proj [n_proj][slices][width]
+ * (the reference's TImage<float,3> layout) with host angles (degrees) and
+ * weights into d_volume [width(y)][width(x)][slices] (MatrixZXY).
+ * accumulate = 0 overwrites the volume (SetROI zeroes it); 1 adds to it. */
+int mr_bp_parallel(mr_bp_plan *plan, const float *d_proj, size_t n_proj, const float *angles,
+                   const float *weights, float *d_volume, int accumulate, void *stream);
+/* Filter (in place on d_proj) followed by back-projection: one engine block of
+ * ReconEngine::Process3D (frameworks/tomography/Framework/ReconFramework/src/ReconEngine.cpp:1499-1511,1663-1714). */
+int mr_fbp_parallel(mr_filter_plan *fplan, mr_bp_plan *bplan, float *d_proj, size_t n_proj,
+                    const float *angles, const float *weights, float *d_volume, void *stream);
+
+/* ------------------------------------------------------------------------
+ * FDK cone-beam back-projection (FDKbp_single).
+ * Replaces FdkReconBase::SetROI/Process(3D)
+ * (frameworks/tomography/Backprojectors/FDKBackProjectors/src/fdkreconbase.cpp:91-277),
+ * FDKbp_single::getProjMatrix / ramp_filter_tuned / project_volume_onto_image_c
+ * (.../src/fdkbp_single.cpp:83-255, 832-958, 293-531, 641-683).
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    float sod, sdd;         /* ProjectionInfo.fSOD, fSDD (mm) */
+    float pixel[2];         /* ProjectionInfo.fResolution (mm) */
+    float voxel[3];         /* MatrixInfo.fVoxelSize (mm) */
+    float ppoint[2];        /* ProjectionInfo.fpPoint (piercing point, detector px) */
+    float center;           /* ProjectionInfo.fCenter */
+    int direction;          /* 0 CW, 1 CCW */
+    size_t roi[4];          /* ProjectionInfo.roi of this slab (x0,y0,x1,y1) */
+    size_t projection_roi3; /* ProjectionInfo.projection_roi[3] */
+    int use_roi;            /* MatrixInfo.bUseROI */
+    size_t matrix_roi[4];   /* MatrixInfo.roi */
+} mr_fdk_params;
+
+typedef struct mr_fdk_plan mr_fdk_plan;
+
+int mr_fdk_plan_create(mr_fdk_plan **plan, const mr_fdk_params *params);
+int mr_fdk_plan_destroy(mr_fdk_plan *plan);
+/* Detector window (CBCT_roi x0,y0,x1,y1) whose rows the caller hands to
+ * mr_fdk_bp, as ReconEngine::Run3DFull computes it (ReconEngine.cpp:954-1008). */
+int mr_fdk_plan_window(const mr_fdk_plan *plan, size_t *cbct_roi);
+/* d_proj: [n_proj][win_rows][width] raw projections of the window (device,
+ * used as scratch: it is weighted and filtered in place).  d_volume:
+ * [nz][nx][nx] (MatrixXYZ, x fastest) with nx = roi[2]-roi[0], nz = roi[3]-roi[1];
+ * overwritten with the mirrored/flipped result (fdkreconbase.cpp:259-270). */
+int mr_fdk_bp(mr_fdk_plan *plan, float *d_proj, size_t n_proj, size_t win_rows,
+              const float *angles, const float *weights, float *d_volume, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MUHREC_HIP_H */

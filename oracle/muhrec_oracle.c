@@ -1,0 +1,789 @@
+/*
+ * CPU oracle for the MuhRec reconstruction hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is a plain-C restatement of the
+ * reference algorithms, used by tests/, by __graft_entry__.smoke() and by the
+ * cpu_baseline leg of bench.py as the checker and as the timed CPU baseline.
+ * No product code path links, loads or calls it.
+ *
+ * Two flavours of every function:
+ *   *_f32 : "MuhRec-faithful" single precision.  Same float operation order,
+ *           same truncations and the same quirks as the reference C++.  The
+ *           only deviation is the FFT, which the reference takes from FFTW
+ *           (float); here it is an exact double-precision DFT (radix-2),
+ *           rounded to float where FFTW hands back floats.
+ *   *_f64 : the same algorithm evaluated in double precision everywhere.
+ *
+ * Reference files restated (paths relative to the reference root):
+ *   frameworks/tomography/Preprocessing/StdPreprocModules/src/ProjectionFilter.cpp
+ *   core/kipl/kipl/src/fft/fftbasef.cpp
+ *   core/kipl/kipl/include/math/core/mathfunctions.hpp
+ *   frameworks/tomography/Framework/ReconFramework/src/BackProjectorModuleBase.cpp
+ *   frameworks/tomography/Backprojectors/StdBackProjectors/src/StdBackProjectorBase.cpp
+ *   frameworks/tomography/Backprojectors/StdBackProjectors/src/MultiProjBPparallel.cpp
+ *   frameworks/tomography/Backprojectors/StdBackProjectors/src/MultiProjBP.cpp
+ *   frameworks/tomography/Backprojectors/StdBackProjectors/src/NNMultiProjBP.cpp
+ *   frameworks/tomography/Backprojectors/FDKBackProjectors/src/fdkreconbase.cpp
+ *   frameworks/tomography/Backprojectors/FDKBackProjectors/src/fdkbp_single.cpp
+ *   frameworks/tomography/Framework/ReconAlgorithms/ReconAlgorithms/linearforwardprojector.cpp
+ *
+ * Parity pinning of each function is documented beside its tests
+ * (tests/test_oracle_*.py).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_FPI 3.1415926535897932384626f
+#define ORC_DPI 3.1415926535897932384626433832795028841971693993751
+
+/* ------------------------------------------------------------------------ */
+/* double-precision complex FFT (radix-2, in place, power-of-two length)     */
+/* ------------------------------------------------------------------------ */
+
+typedef struct { double re, im; } orc_cplx;
+
+static void orc_fft(orc_cplx *a, size_t n, int sign, const orc_cplx *tw /* n/2 */)
+{
+    /* bit reversal */
+    for (size_t i = 1, j = 0; i < n; ++i) {
+        size_t bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) { orc_cplx t = a[i]; a[i] = a[j]; a[j] = t; }
+    }
+    for (size_t len = 2; len <= n; len <<= 1) {
+        size_t half = len >> 1, step = n / len;
+        for (size_t i = 0; i < n; i += len) {
+            for (size_t k = 0; k < half; ++k) {
+                orc_cplx w = tw[k * step];
+                if (sign > 0) w.im = -w.im;
+                orc_cplx u = a[i + k], v = a[i + k + half];
+                orc_cplx t = { v.re * w.re - v.im * w.im, v.re * w.im + v.im * w.re };
+                a[i + k].re = u.re + t.re; a[i + k].im = u.im + t.im;
+                a[i + k + half].re = u.re - t.re; a[i + k + half].im = u.im - t.im;
+            }
+        }
+    }
+}
+
+/* twiddles exp(-2 pi i k / n), k < n/2 */
+static orc_cplx *orc_twiddles(size_t n)
+{
+    orc_cplx *tw = (orc_cplx *)malloc(sizeof(orc_cplx) * (n / 2 + 1));
+    for (size_t k = 0; k < n / 2 + 1; ++k) {
+        double ang = -2.0 * ORC_DPI * (double)k / (double)n;
+        tw[k].re = cos(ang); tw[k].im = sin(ang);
+    }
+    return tw;
+}
+
+/* exposed for the oracle's own self-test against a brute-force DFT */
+int orc_fft_c2c(double *re, double *im, size_t n, int sign)
+{
+    if (n == 0 || (n & (n - 1))) return -1;
+    orc_cplx *a = (orc_cplx *)malloc(sizeof(orc_cplx) * n);
+    orc_cplx *tw = orc_twiddles(n);
+    for (size_t i = 0; i < n; ++i) { a[i].re = re[i]; a[i].im = im[i]; }
+    orc_fft(a, n, sign, tw);
+    for (size_t i = 0; i < n; ++i) { re[i] = a[i].re; im[i] = a[i].im; }
+    free(a); free(tw);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* ProjectionFilterSingle (SPP/src/ProjectionFilter.cpp)                     */
+/* ------------------------------------------------------------------------ */
+
+/* ComputeFilterSize, ProjectionFilter.cpp:120-125 */
+size_t orc_filter_size(size_t len, size_t paddingdoubler)
+{
+    double e = log((double)len) / log(2.0);
+    return (size_t)1 << ((size_t)ceil(e) + paddingdoubler);
+}
+
+enum { ORC_RAMLAK = 0, ORC_SHEPPLOGAN, ORC_HANNING, ORC_HAMMING, ORC_BUTTERWORTH };
+
+/* ProjectionFilterSingle::BuildFilter, ProjectionFilter.cpp:254-302.
+ * Returns 0, or -1 when the cut-off is outside (0, 0.5] (the reference throws). */
+int orc_filter_build_f32(int ftype, float cutoff, float order, int usebias, float biasweight,
+                         size_t P, float *H)
+{
+    const size_t N2 = P / 2;
+    if (!((0 < cutoff) && (cutoff <= 0.5f))) return -1;
+    float FilterOrder = 0.0f;
+    if (ftype == ORC_BUTTERWORTH) FilterOrder = order;
+    const size_t c = (size_t)(2 * cutoff * (float)N2);
+    float rampstep = 1.0f / (((float)N2 - 1.0f));
+    for (size_t i = 0; i < N2; ++i) H[i] = (float)i * rampstep;
+    float fstep = 1.0f / (((float)c - 1.0f));
+    for (size_t i = 0; i < c && i < N2; ++i) {
+        switch (ftype) {
+        case ORC_RAMLAK: break;
+        case ORC_SHEPPLOGAN:
+            H[i] = (float)(sin(ORC_DPI * (double)i * (double)fstep * (double)cutoff)) / (float)N2;
+            break;
+        case ORC_HANNING:
+            H[i] *= (float)(0.5f + 0.5f * cosf((float)i * fstep * ORC_FPI * cutoff));
+            break;
+        case ORC_HAMMING:
+            H[i] *= (float)(0.54 + 0.46 * cos((double)((float)i * fstep) * ORC_DPI * (double)cutoff));
+            break;
+        case ORC_BUTTERWORTH:
+            H[i] /= (float)(1 + powf((float)i * fstep * cutoff, FilterOrder));
+            break;
+        default: break;
+        }
+    }
+    for (size_t i = c; i < N2; ++i) H[i] = 0.0f;
+    if (usebias) H[0] = biasweight * H[1];
+    return 0;
+}
+
+/* Same filter, every quantity in double. */
+int orc_filter_build_f64(int ftype, double cutoff, double order, int usebias, double biasweight,
+                         size_t P, double *H)
+{
+    const size_t N2 = P / 2;
+    if (!((0 < cutoff) && (cutoff <= 0.5))) return -1;
+    double FilterOrder = (ftype == ORC_BUTTERWORTH) ? order : 0.0;
+    const size_t c = (size_t)(2 * cutoff * (double)N2);
+    double rampstep = 1.0 / ((double)N2 - 1.0);
+    for (size_t i = 0; i < N2; ++i) H[i] = (double)i * rampstep;
+    double fstep = 1.0 / ((double)c - 1.0);
+    for (size_t i = 0; i < c && i < N2; ++i) {
+        double t = (double)i * fstep * cutoff;
+        switch (ftype) {
+        case ORC_SHEPPLOGAN: H[i] = sin(ORC_DPI * t) / (double)N2; break;
+        case ORC_HANNING: H[i] *= 0.5 + 0.5 * cos(ORC_DPI * t); break;
+        case ORC_HAMMING: H[i] *= 0.54 + 0.46 * cos(ORC_DPI * t); break;
+        case ORC_BUTTERWORTH: H[i] /= 1 + pow(t, FilterOrder); break;
+        default: break;
+        }
+    }
+    for (size_t i = c; i < N2; ++i) H[i] = 0.0;
+    if (usebias) H[0] = biasweight * H[1];
+    return 0;
+}
+
+/* PreparePadding, ProjectionFilter.cpp:363-375 with Sigmoid from
+ * core/kipl/kipl/include/math/core/mathfunctions.hpp:60-63 (float instance:
+ * exp takes a float argument).  Returns nInsert. */
+size_t orc_filter_taper_f32(size_t nImage, size_t P, float *taper)
+{
+    size_t nInsert = (P / 2) - (nImage / 2);
+    for (size_t i = 0; i < nInsert; ++i) {
+        float x = (float)i / (float)nInsert - 0.5f;
+        taper[i] = (float)(1.0 / (1.0 + expf(-(x - 0.0f) / 0.07f)));
+    }
+    return nInsert;
+}
+
+size_t orc_filter_taper_f64(size_t nImage, size_t P, double *taper)
+{
+    size_t nInsert = (P / 2) - (nImage / 2);
+    for (size_t i = 0; i < nInsert; ++i) {
+        double x = (double)i / (double)nInsert - 0.5;
+        taper[i] = 1.0 / (1.0 + exp(-x / 0.07));
+    }
+    return nInsert;
+}
+
+/* Pad, ProjectionFilter.cpp:346-361: destination of length P+16; only the
+ * first P samples enter the FFT. */
+static void orc_pad_f64(const float *src, size_t n, double *dst, size_t P,
+                        const double *tap, size_t nInsert, int f32)
+{
+    const size_t nDest = P + 16;
+    for (size_t i = 0; i < P; ++i) dst[i] = 0.0;
+    for (size_t i = 0; i < n; ++i) dst[nInsert + i] = src[i];
+    for (size_t i = 0; i < nInsert; ++i) {
+        double l, r;
+        if (f32) { l = (double)(src[0] * (float)tap[i]); r = (double)(src[n - 1] * (float)tap[i]); }
+        else     { l = (double)src[0] * tap[i];           r = (double)src[n - 1] * tap[i]; }
+        dst[i] = l;
+        size_t j = nDest - 1 - i;
+        if (j < P) dst[j] = r;
+    }
+}
+
+/* FilterProjection, ProjectionFilter.cpp:304-344.  rows: nrows x n, row-major,
+ * filtered in place.  Bins k < P/2 are scaled by H[k], the Nyquist bin P/2 is
+ * passed through, output sample i = c2r(...)[i+nInsert] * pi/(2P). */
+void orc_filter_rows_f32(float *rows, size_t nrows, size_t n, size_t P,
+                         const float *H, const float *taper, size_t nInsert)
+{
+    const size_t N2 = P / 2;
+    orc_cplx *tw = orc_twiddles(P);
+    double *tapd = (double *)malloc(sizeof(double) * (nInsert + 1));
+    for (size_t i = 0; i < nInsert; ++i) tapd[i] = taper[i];
+    const float scale = ORC_FPI / (4.0f * (float)N2);
+#pragma omp parallel
+    {
+        orc_cplx *a = (orc_cplx *)malloc(sizeof(orc_cplx) * P);
+        double *pad = (double *)malloc(sizeof(double) * P);
+#pragma omp for schedule(static)
+        for (long r = 0; r < (long)nrows; ++r) {
+            float *row = rows + (size_t)r * n;
+            orc_pad_f64(row, n, pad, P, tapd, nInsert, 1);
+            for (size_t i = 0; i < P; ++i) { a[i].re = pad[i]; a[i].im = 0.0; }
+            orc_fft(a, P, -1, tw);
+            /* FFTW hands back float spectra; the filter multiplies in float. */
+            for (size_t k = 0; k <= N2; ++k) {
+                float re = (float)a[k].re, im = (float)a[k].im;
+                if (k < N2) { re *= H[k]; im *= H[k]; }
+                a[k].re = re; a[k].im = (k == 0 || k == N2) ? 0.0 : im;
+            }
+            for (size_t k = 1; k < N2; ++k) { a[P - k].re = a[k].re; a[P - k].im = -a[k].im; }
+            orc_fft(a, P, +1, tw);
+            for (size_t i = 0; i < n; ++i) row[i] = (float)a[i + nInsert].re * scale;
+        }
+        free(a); free(pad);
+    }
+    free(tapd); free(tw);
+}
+
+/* Same filter in double precision: in (float rows) -> out (double rows). */
+void orc_filter_rows_f64(const float *in, double *out, size_t nrows, size_t n, size_t P,
+                         const double *H, const double *taper, size_t nInsert)
+{
+    const size_t N2 = P / 2;
+    orc_cplx *tw = orc_twiddles(P);
+    const double scale = ORC_DPI / (2.0 * (double)P);
+#pragma omp parallel
+    {
+        orc_cplx *a = (orc_cplx *)malloc(sizeof(orc_cplx) * P);
+        double *pad = (double *)malloc(sizeof(double) * P);
+#pragma omp for schedule(static)
+        for (long r = 0; r < (long)nrows; ++r) {
+            orc_pad_f64(in + (size_t)r * n, n, pad, P, taper, nInsert, 0);
+            for (size_t i = 0; i < P; ++i) { a[i].re = pad[i]; a[i].im = 0.0; }
+            orc_fft(a, P, -1, tw);
+            for (size_t k = 0; k <= N2; ++k) {
+                double h = (k < N2) ? H[k] : 1.0;
+                a[k].re *= h; a[k].im = (k == 0 || k == N2) ? 0.0 : a[k].im * h;
+            }
+            for (size_t k = 1; k < N2; ++k) { a[P - k].re = a[k].re; a[P - k].im = -a[k].im; }
+            orc_fft(a, P, +1, tw);
+            for (size_t i = 0; i < n; ++i) out[(size_t)r * n + i] = a[i + nInsert].re * scale;
+        }
+        free(a); free(pad);
+    }
+    free(tw);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Circle mask, RF/src/BackProjectorModuleBase.cpp:153-218 (no tilt).        */
+/* roi == NULL: no MatrixInfo ROI.  roi = {x0,y0,x1,y1} inclusive,           */
+/* cone != 0 selects the cone-beam ROI clipping branch.                      */
+/* ------------------------------------------------------------------------ */
+void orc_circle_mask(size_t nSizeX, size_t nSizeY, const size_t *roi, int cone,
+                     size_t *first, size_t *second)
+{
+    const float cx = (float)(nSizeX >> 1);
+    const float cy = (float)(nSizeY >> 1);
+    const float R = cx - 1;
+    const float R2 = R * R;
+    for (size_t i = 0; i < nSizeY; ++i) {
+        float y = (float)i - cy;
+        float y2 = y * y;
+        if (y2 <= R2) {
+            float x = sqrtf(R2 - y2);
+            first[i] = (size_t)ceilf(cx - x);
+            size_t s = (size_t)floorf(cx + x);
+            second[i] = (nSizeX - 1u) < s ? (nSizeX - 1u) : s;
+        } else {
+            first[i] = 0u; second[i] = 0u;
+        }
+        if (roi) {
+            if (roi[1] <= i && i <= roi[3]) {
+                if (!cone) {
+                    if (first[i] < roi[0]) first[i] = roi[0];
+                    if (second[i] > roi[2]) second[i] = roi[2];
+                } else {
+                    size_t a = nSizeX - roi[2], b = nSizeX - roi[0];
+                    if (first[i] < a) first[i] = a;
+                    if (second[i] > b) second[i] = b;
+                }
+            } else {
+                first[i] = 0u; second[i] = 0u;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Parallel-beam back-projection                                             */
+/* StdBackProjectorBase::Process(2D/3D), SBP/src/StdBackProjectorBase.cpp:64-149 */
+/* MultiProjectionBPparallel::BackProject, SBP/src/MultiProjBPparallel.cpp:115-167 */
+/* MultiProjectionBP::BackProject, SBP/src/MultiProjBP.cpp:95-143            */
+/* NearestNeighborBP::BackProject, SBP/src/NNMultiProjBP.cpp:80-118          */
+/* ------------------------------------------------------------------------ */
+
+/* Indices of the projections that the reference back-projects, in order.
+ * bufsize = ProjectionBufferSize; 0 disables the buffer drop (every
+ * projection used).  Returns the count. */
+size_t orc_used_projections(size_t np, size_t bufsize, size_t *used)
+{
+    size_t cnt = 0, c = 0;
+    size_t slot[4096];
+    for (size_t i = 0; i < np; ++i) {
+        if (bufsize == 0) { used[cnt++] = i; continue; }
+        slot[c] = i; c++;
+        int last = (i == np - 1);
+        if (last || bufsize <= c) {
+            if (bufsize <= c) c--;
+            for (size_t k = 0; k < c; ++k) used[cnt++] = slot[k];
+            c = 0;
+        }
+    }
+    return cnt;
+}
+
+/* Per-projection tables, StdBackProjectorBase.cpp:73-80 (float sin/cos of a
+ * float argument). */
+void orc_bp_tables_f32(const float *angles, size_t np, int direction, float center,
+                       size_t matrixCenterX, float *fsin, float *fcos, float *fstartu)
+{
+    float dirWeight = 2.0f * ((float)direction - 0.5f);
+    for (size_t i = 0; i < np; ++i) {
+        fsin[i] = sinf(dirWeight * angles[i] * ORC_FPI / 180.0f);
+        fcos[i] = cosf(dirWeight * angles[i] * ORC_FPI / 180.0f);
+        fstartu[i] = (float)matrixCenterX * (fsin[i] - fcos[i]) + center;
+    }
+}
+
+/* proj: np x nv x nu (u fastest, one projection after the other, already
+ *       filtered, NOT yet weighted).
+ * vol : nu(y) x nu(x) x nv(z), z fastest (MatrixZXY), overwritten.
+ * interp: 0 linear (MultiProjBPparallel / MultiProjBP), 1 nearest (NearestNeighborBP)
+ * multiprojbp: 1 reproduces MultiProjBP's B-column offset 4*(nv/4) instead of
+ *       the padded column length (identical when nv % 4 == 0).
+ * Only slices z < 4*(nv/4) are updated (SizeZ = volume.Size(0)/4). */
+void orc_bp_parallel_f32(const float *proj, size_t nu, size_t nv, size_t np,
+                         const float *angles, const float *weights, float center, int direction,
+                         int interp, size_t bufsize, int multiprojbp,
+                         const size_t *mfirst, const size_t *msecond, float *vol)
+{
+    const size_t vpad = (nv + 3) & ~(size_t)3;
+    const size_t sizeZ4 = (nv / 4) * 4;
+    const size_t bstride = multiprojbp ? (nv / 4) * 4 : vpad;
+    const int sizeUm2 = (int)nu - 2;
+    size_t *used = (size_t *)malloc(sizeof(size_t) * (np + 1));
+    size_t nused = orc_used_projections(np, bufsize, used);
+    float *fsin = (float *)malloc(sizeof(float) * np);
+    float *fcos = (float *)malloc(sizeof(float) * np);
+    float *fsu = (float *)malloc(sizeof(float) * np);
+    orc_bp_tables_f32(angles, np, direction, center, nu / 2, fsin, fcos, fsu);
+    /* weighted, transposed projection buffer: [nused][nu][vpad] (+1 column guard) */
+    float *buf = (float *)calloc((nused * nu + 1) * vpad + 8, sizeof(float));
+#pragma omp parallel for schedule(static)
+    for (long k = 0; k < (long)nused; ++k) {
+        const float *p = proj + used[k] * nu * nv;
+        float w = weights[used[k]];
+        for (size_t v = 0; v < nv; ++v)
+            for (size_t u = 0; u < nu; ++u)
+                buf[((size_t)k * nu + u) * vpad + v] = p[v * nu + u] * w;
+    }
+    memset(vol, 0, sizeof(float) * nu * nu * nv);
+#pragma omp parallel
+    {
+        float *lsu = (float *)malloc(sizeof(float) * (nused + 1));
+#pragma omp for schedule(dynamic, 4)
+        for (long y = 1; y <= (long)nu; ++y) {
+            const size_t x0 = mfirst[y - 1], x1 = msecond[y - 1];
+            for (size_t k = 0; k < nused; ++k) lsu[k] = fsu[used[k]] + fcos[used[k]] * (float)y;
+            for (size_t x = x0 + 1; x <= x1; ++x) {
+                float *col = vol + ((size_t)(y - 1) * nu + (x - 1)) * nv;
+                for (size_t k = 0; k < nused; ++k) {
+                    float u = lsu[k] - fsin[used[k]] * (float)x;
+                    int n = (int)u;
+                    if ((n < 0) || (sizeUm2 < n)) continue;
+                    const float *A = buf + ((size_t)k * nu + (size_t)n) * vpad;
+                    if (interp == 1) {
+                        for (size_t z = 0; z < sizeZ4; ++z) col[z] = col[z] + A[z];
+                    } else {
+                        const float *B = A + bstride;
+                        const float wb = fabsf(u - (float)n);
+                        const float wa = 1.0f - wb;
+                        for (size_t z = 0; z < sizeZ4; ++z) {
+                            float a = A[z] * wa;
+                            float b = B[z] * wb;
+                            float s = a + b;
+                            col[z] = col[z] + s;
+                        }
+                    }
+                }
+            }
+        }
+        free(lsu);
+    }
+    free(buf); free(used); free(fsin); free(fcos); free(fsu);
+}
+
+/* The same back-projection in double precision (tables, positions,
+ * interpolation and accumulation). */
+void orc_bp_parallel_f64(const double *proj, size_t nu, size_t nv, size_t np,
+                         const double *angles, const double *weights, double center, int direction,
+                         int interp, size_t bufsize,
+                         const size_t *mfirst, const size_t *msecond, double *vol)
+{
+    const size_t sizeZ4 = (nv / 4) * 4;
+    const int sizeUm2 = (int)nu - 2;
+    size_t *used = (size_t *)malloc(sizeof(size_t) * (np + 1));
+    size_t nused = orc_used_projections(np, bufsize, used);
+    double *fsin = (double *)malloc(sizeof(double) * np);
+    double *fcos = (double *)malloc(sizeof(double) * np);
+    double *fsu = (double *)malloc(sizeof(double) * np);
+    double dirWeight = 2.0 * ((double)direction - 0.5);
+    double mcx = (double)(nu / 2);
+    for (size_t i = 0; i < np; ++i) {
+        fsin[i] = sin(dirWeight * angles[i] * ORC_DPI / 180.0);
+        fcos[i] = cos(dirWeight * angles[i] * ORC_DPI / 180.0);
+        fsu[i] = mcx * (fsin[i] - fcos[i]) + center;
+    }
+    memset(vol, 0, sizeof(double) * nu * nu * nv);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long y = 1; y <= (long)nu; ++y) {
+        const size_t x0 = mfirst[y - 1], x1 = msecond[y - 1];
+        for (size_t x = x0 + 1; x <= x1; ++x) {
+            double *col = vol + ((size_t)(y - 1) * nu + (x - 1)) * nv;
+            for (size_t k = 0; k < nused; ++k) {
+                size_t i = used[k];
+                double u = fsu[i] + fcos[i] * (double)y - fsin[i] * (double)x;
+                int n = (int)u;
+                if ((n < 0) || (sizeUm2 < n)) continue;
+                const double *p = proj + i * nu * nv;
+                double w = weights[i];
+                if (interp == 1) {
+                    for (size_t z = 0; z < sizeZ4; ++z) col[z] += w * p[z * nu + n];
+                } else {
+                    double wb = fabs(u - (double)n), wa = 1.0 - wb;
+                    for (size_t z = 0; z < sizeZ4; ++z)
+                        col[z] += w * (p[z * nu + n] * wa + p[z * nu + n + 1] * wb);
+                }
+            }
+        }
+    }
+    free(used); free(fsin); free(fcos); free(fsu);
+}
+
+/* ------------------------------------------------------------------------ */
+/* FDK cone-beam (FDKbp_single), FDK/fdkreconbase.cpp:217-277 and           */
+/* FDK/fdkbp_single.cpp:83-531, 641-683, 832-982                             */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    float sod, sdd;          /* ProjectionInfo.fSOD, fSDD */
+    float pix[2];            /* ProjectionInfo.fResolution */
+    float vox[3];            /* MatrixInfo.fVoxelSize */
+    float ppoint[2];         /* ProjectionInfo.fpPoint */
+    float center;            /* ProjectionInfo.fCenter */
+    int direction;           /* 0 CW, 1 CCW */
+    uint64_t roi[4];         /* ProjectionInfo.roi at back-projection time (slab) */
+    uint64_t proj_roi3;      /* ProjectionInfo.projection_roi[3] */
+} orc_fdk_geom;
+
+/* getProjMatrix, fdkbp_single.cpp:83-255 (tgt = 0, vup = z). */
+void orc_fdk_projmatrix_f32(const orc_fdk_geom *g, float angle, float *pm /*12*/)
+{
+    float ext[16] = {0}, intr[12] = {0};
+    float sad = g->sod, sid = g->sdd;
+    float offCenter = ((g->ppoint[0] - (float)g->roi[0]) - g->center) * g->vox[0];
+    float cam[3] = {0.0f, 0.0f, 0.0f};
+    if (g->direction == 0) {
+        cam[0] = sad * cosf(-angle * ORC_FPI / 180.0f);
+        cam[1] = sad * sinf(-angle * ORC_FPI / 180.0f);
+    } else {
+        cam[0] = sad * cosf(angle * ORC_FPI / 180.0f);
+        cam[1] = sad * sinf(angle * ORC_FPI / 180.0f);
+    }
+    float vup[3] = {0.0f, 0.0f, 1.0f}, tgt[3] = {0.0f, 0.0f, 0.0f};
+    float nrm[3], plt[3], pup[3];
+    nrm[0] = cam[0] - tgt[0]; nrm[1] = cam[1] - tgt[1]; nrm[2] = cam[2] - tgt[2];
+    float nn = sqrtf(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+    nrm[0] /= nn; nrm[1] /= nn; nrm[2] /= nn;
+    plt[0] = nrm[1] * vup[2] - nrm[2] * vup[1];
+    plt[1] = nrm[2] * vup[0] - nrm[0] * vup[2];
+    plt[2] = nrm[0] * vup[1] - nrm[1] * vup[0];
+    float np_ = sqrtf(plt[0] * plt[0] + plt[1] * plt[1] + plt[2] * plt[2]);
+    plt[0] /= np_; plt[1] /= np_; plt[2] /= np_;
+    pup[0] = plt[1] * nrm[2] - plt[2] * nrm[1];
+    pup[1] = plt[2] * nrm[0] - plt[0] * nrm[2];
+    pup[2] = plt[0] * nrm[1] - plt[1] * nrm[0];
+    float nu_ = sqrtf(pup[0] * pup[0] + pup[1] * pup[1] + pup[2] * pup[2]);
+    pup[0] /= nu_; pup[1] /= nu_; pup[2] /= nu_;
+    ext[0] = -1.0f * plt[0]; ext[1] = -1.0f * plt[1]; ext[2] = -1.0f * plt[2];
+    ext[4] = -1.0f * pup[0]; ext[5] = -1.0f * pup[1]; ext[6] = -1.0f * pup[2];
+    ext[8] = -1.0f * nrm[0]; ext[9] = -1.0f * nrm[1]; ext[10] = -1.0f * nrm[2];
+    ext[15] = 1.0f;
+    ext[3] = plt[0] * tgt[0] + plt[1] * tgt[1] + plt[2] * tgt[2] - offCenter;
+    ext[7] = pup[0] * tgt[0] + pup[1] * tgt[1] + pup[2] * tgt[2];
+    ext[11] = nrm[0] * tgt[0] + nrm[1] * tgt[1] + nrm[2] * tgt[2] + sad;
+    intr[0] = 1.0f / g->pix[0];
+    intr[5] = 1.0f / g->pix[1];
+    intr[10] = 1.0f / sid;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 4; ++j) {
+            float acc = 0.0f;
+            for (int k = 0; k < 4; ++k) acc += intr[i * 4 + k] * ext[k * 4 + j];
+            pm[i * 4 + j] = acc;
+        }
+}
+
+/* Detector-row window, fdkbp_single.cpp:322-367 (identical to the engine's
+ * CBCT_roi, RF/src/ReconEngine.cpp:954-1008).  nvol = volume.Size(1). */
+void orc_fdk_cbct_roi(const orc_fdk_geom *g, size_t nvol, uint64_t *cb)
+{
+    const float radius = (float)nvol * g->vox[0] / 2;
+    const float pp1 = g->ppoint[1];
+    const float r1 = (float)g->roi[1], r3 = (float)g->roi[3];
+    const float vs = g->vox[0], sdd = g->sdd, sod = g->sod, res = g->pix[0];
+    cb[0] = g->roi[0]; cb[2] = g->roi[2]; cb[1] = 0; cb[3] = 0;
+    if (pp1 >= r1 && pp1 >= r3) {
+        cb[3] = (uint64_t)(pp1 - ((pp1 - r3) * vs * sdd / (sod + radius)) / res);
+        float value = pp1 - ((pp1 - r1) * vs * sdd / (sod - radius)) / res;
+        if (value <= 0) cb[1] = 0;
+        else cb[1] = (uint64_t)(pp1 - ((pp1 - r1) * vs * sdd / (sod - radius)) / res);
+    }
+    if (pp1 < r1 && pp1 < r3) {
+        float value = pp1 + ((r1 - pp1) * vs * sdd / (sod + radius)) / res;
+        cb[1] = (uint64_t)value;
+        float value2 = pp1 + ((r3 - pp1) * vs * sdd / (sod - radius)) / res;
+        if (value2 >= (float)g->proj_roi3) cb[3] = g->proj_roi3;
+        else cb[3] = (uint64_t)value2;
+    }
+    if (pp1 >= r1 && pp1 < r3) {
+        float value = pp1 - ((pp1 - r1) * vs * sdd / (sod - radius)) / res;
+        if (value <= 0) cb[1] = 0;
+        else cb[1] = (uint64_t)(pp1 - ((pp1 - r1) * vs * sdd / (sod - radius)) / res);
+        float value2 = pp1 + ((r3 - pp1) * vs * sdd / (sod - radius)) / res;
+        if (value2 >= (float)g->proj_roi3) cb[3] = g->proj_roi3;
+        else cb[3] = (uint64_t)value2;
+    }
+    if (cb[1] != 0) cb[1] -= 8;
+    if (cb[3] + 8 <= g->proj_roi3) cb[3] += 8;
+}
+
+/* prepareFFT, fdkbp_single.cpp:960-963 */
+size_t orc_fdk_padwidth(size_t width)
+{
+    return (size_t)(unsigned int)(2 * pow(2, ceil(log((double)(unsigned int)width) / log(2))));
+}
+
+/* ramp_filter_tuned, fdkbp_single.cpp:832-958, on one w x h image in place.
+ * f64 != 0: evaluate in double (window and scale in double); the image stays
+ * float only for the f32 flavour. */
+static void orc_fdk_filter_impl(float *imgf, double *imgd, size_t w, size_t h)
+{
+    const unsigned int MARGINF = 5;
+    const size_t P = orc_fdk_padwidth(w);
+    const size_t padf = (P - w) / 2;
+    const int f64 = imgd != NULL;
+    double *pad = (double *)calloc(P * h, sizeof(double));
+    for (size_t r = 0; r < h; ++r)
+        for (size_t c = 0; c < w; ++c)
+            pad[r * P + padf + c] = f64 ? imgd[r * w + c] : (double)imgf[r * w + c];
+    for (size_t r = 0; r < MARGINF; ++r) memcpy(pad + r * P, pad + MARGINF * P, P * sizeof(double));
+    for (size_t r = h - MARGINF; r < h; ++r)
+        memcpy(pad + r * P, pad + (h - MARGINF - 1) * P, P * sizeof(double));
+    for (size_t r = 0; r < h; ++r) {
+        for (size_t c = 0; c < MARGINF; ++c) pad[r * P + c] = pad[r * P + MARGINF];
+        for (size_t c = P - MARGINF; c < P; ++c) pad[r * P + c] = pad[r * P + P - MARGINF - 1];
+    }
+    float *rampf = (float *)malloc(sizeof(float) * P);
+    double *rampd = (double *)malloc(sizeof(double) * P);
+    const float DEGTORADF = ORC_FPI / 180.0f;
+    for (size_t i = 0; i < P; ++i) {
+        rampf[i] = (i < P / 2) ? (float)i : (float)(P - i);
+        rampd[i] = rampf[i];
+        rampf[i] = (float)(rampf[i] * (0.54 + 0.46 * (cosf((float)(unsigned)i * DEGTORADF * 360 / (float)(unsigned)P) + 1)));
+        rampf[i] = (float)(rampf[i] / ((double)P * 0.5));
+        rampd[i] = rampd[i] * (0.54 + 0.46 * (cos(2.0 * ORC_DPI * (double)i / (double)P) + 1)) / ((double)P * 0.5);
+    }
+    const float rpad = 1.0f / (float)P;
+    orc_cplx *tw = orc_twiddles(P);
+#pragma omp parallel
+    {
+        orc_cplx *a = (orc_cplx *)malloc(sizeof(orc_cplx) * P);
+#pragma omp for schedule(static)
+        for (long r = 0; r < (long)h; ++r) {
+            for (size_t i = 0; i < P; ++i) {
+                a[i].re = f64 ? pad[r * P + i] : (double)(float)pad[r * P + i];
+                a[i].im = 0.0;
+            }
+            orc_fft(a, P, -1, tw);
+            for (size_t c = 0; c < P; ++c) {
+                if (f64) { a[c].re *= rampd[c]; a[c].im *= rampd[c]; }
+                else {
+                    a[c].re = (double)((float)a[c].re * rampf[c]);
+                    a[c].im = (double)((float)a[c].im * rampf[c]);
+                }
+            }
+            orc_fft(a, P, +1, tw);
+            for (size_t c = 0; c < w; ++c) {
+                size_t i = padf + c;
+                if (f64) imgd[r * w + c] = a[i].re / (double)P;
+                else imgf[r * w + c] = (float)a[i].re * rpad;
+            }
+        }
+        free(a);
+    }
+    free(tw); free(rampf); free(rampd); free(pad);
+}
+
+void orc_fdk_filter_f32(float *img, size_t w, size_t h) { orc_fdk_filter_impl(img, NULL, w, h); }
+void orc_fdk_filter_f64(double *img, size_t w, size_t h) { orc_fdk_filter_impl(NULL, img, w, h); }
+
+/* Full FDKbp_single::Process(3D) on one slab.
+ * proj : np x h x w window projections (the rows the engine hands over),
+ * vol  : nz x ny x nx (x fastest, MatrixXYZ) with nx = ny = roi[2]-roi[0] and
+ *        nz = roi[3]-roi[1]; receives the mirrored/flipped final volume.
+ * mfirst/msecond: circle mask over y (BuildCircleMask, XYZ variant). */
+void orc_fdk_bp_f32(const float *proj, size_t w, size_t h, size_t np,
+                    const float *angles, const float *weights, const orc_fdk_geom *g,
+                    const size_t *mfirst, const size_t *msecond, float *vol)
+{
+    const size_t nx = (size_t)(g->roi[2] - g->roi[0]), ny = nx;
+    const size_t nz = (size_t)(g->roi[3] - g->roi[1]);
+    float *cv = (float *)calloc(nx * ny * nz, sizeof(float));
+    float *img = (float *)malloc(sizeof(float) * w * h);
+    const float scale = g->sod / g->sdd;
+    const float sad_sid_2 = (g->sod * g->sod) / (g->sdd * g->sdd);
+    const float sp[3] = {g->vox[0], g->vox[1], g->vox[2]};
+    const float U = (float)(g->roi[2] - g->roi[0]);
+    const float V = (float)(g->roi[3] - g->roi[1]);
+    float origin[3];
+    origin[0] = -(U - g->center) * sp[0] - sp[0] / 2;
+    origin[1] = -(U - g->center) * sp[1] - sp[1] / 2;
+    origin[2] = -(V - (g->ppoint[1] - (float)g->roi[1])) * sp[2] - sp[2] / 2;
+    uint64_t cb[4];
+    orc_fdk_cbct_roi(g, ny, cb);
+    const float ic[2] = {g->ppoint[0] - (float)cb[0], g->ppoint[1] - (float)cb[1]};
+    float *xip = (float *)malloc(sizeof(float) * 3 * nx);
+    float *yip = (float *)malloc(sizeof(float) * 3 * ny);
+    float *zip = (float *)malloc(sizeof(float) * 3 * nz);
+    for (size_t ip = 0; ip < np; ++ip) {
+        const float *src = proj + ip * w * h;
+        for (size_t i = 0; i < w * h; ++i) img[i] = src[i] * weights[ip];
+        float pm[12];
+        orc_fdk_projmatrix_f32(g, angles[ip], pm);
+        orc_fdk_filter_f32(img, w, h);
+        for (size_t i = 0; i < w * h; ++i) { img[i] *= sad_sid_2; img[i] *= scale; }
+        for (size_t i = 0; i < nx; ++i) {
+            float x = (float)(origin[0] + (float)i * sp[0]);
+            xip[i * 3 + 0] = x * (pm[0] + ic[0] * pm[8]);
+            xip[i * 3 + 1] = x * (pm[4] + ic[1] * pm[8]);
+            xip[i * 3 + 2] = x * pm[8];
+        }
+        for (size_t j = 0; j < ny; ++j) {
+            float y = (float)(origin[1] + (float)j * sp[1]);
+            yip[j * 3 + 0] = y * (pm[1] + ic[0] * pm[9]);
+            yip[j * 3 + 1] = y * (pm[5] + ic[1] * pm[9]);
+            yip[j * 3 + 2] = y * pm[9];
+        }
+        for (size_t k = 0; k < nz; ++k) {
+            float z = (float)(origin[2] + (float)k * sp[2]);
+            zip[k * 3 + 0] = z * (pm[2] + ic[0] * pm[10]) + ic[0] * pm[11] + pm[3];
+            zip[k * 3 + 1] = z * (pm[6] + ic[1] * pm[10]) + ic[1] * pm[11] + pm[7];
+            zip[k * 3 + 2] = z * pm[10] + pm[11];
+        }
+#pragma omp parallel for schedule(static)
+        for (long k = 0; k < (long)nz; ++k) {
+            for (size_t j = 0; j < ny; ++j) {
+                float acc2[3];
+                acc2[0] = zip[3 * k] + yip[3 * j];
+                acc2[1] = zip[3 * k + 1] + yip[3 * j + 1];
+                acc2[2] = zip[3 * k + 2] + yip[3 * j + 2];
+                float *row = cv + ((size_t)k * ny + j) * nx;
+                for (size_t i = mfirst[j] + 1; i <= msecond[j]; ++i) {
+                    float dw = 1.0f / (acc2[2] + xip[3 * i + 2]);
+                    float r = dw * (acc2[1] + xip[3 * i + 1]);
+                    float c = dw * (acc2[0] + xip[3 * i]);
+                    float val = 0.0f;
+                    r += 0.5f;
+                    c += 0.5f;
+                    if (!(r < 0) && !(r >= (float)h) && !(c < 0) && !(c >= (float)w))
+                        val = img[(size_t)(int)r * w + (size_t)(int)c];
+                    row[i] += dw * dw * val;
+                }
+            }
+        }
+    }
+    /* mirror x, flip z: fdkreconbase.cpp:259-270 */
+    for (size_t k = 0; k < nz; ++k)
+        for (size_t j = 0; j < ny; ++j)
+            for (size_t i = 0; i < nx; ++i)
+                vol[((nz - 1 - k) * ny + j) * nx + i] = cv[(k * ny + j) * nx + (nx - 1 - i)];
+    free(cv); free(img); free(xip); free(yip); free(zip);
+}
+
+/* FDK in double precision: geometry evaluated in double from the same
+ * configuration, FDK filter in double, sampling/accumulation in double. */
+void orc_fdk_bp_f64(const float *proj, size_t w, size_t h, size_t np,
+                    const double *angles, const double *weights, const orc_fdk_geom *g,
+                    const size_t *mfirst, const size_t *msecond, double *vol)
+{
+    const size_t nx = (size_t)(g->roi[2] - g->roi[0]), ny = nx;
+    const size_t nz = (size_t)(g->roi[3] - g->roi[1]);
+    double *cv = (double *)calloc(nx * ny * nz, sizeof(double));
+    double *img = (double *)malloc(sizeof(double) * w * h);
+    const double sod = g->sod, sdd = g->sdd;
+    const double scale = sod / sdd, sad_sid_2 = (sod * sod) / (sdd * sdd);
+    const double sp[3] = {g->vox[0], g->vox[1], g->vox[2]};
+    const double U = (double)(g->roi[2] - g->roi[0]);
+    const double V = (double)(g->roi[3] - g->roi[1]);
+    double origin[3];
+    origin[0] = -(U - g->center) * sp[0] - sp[0] / 2;
+    origin[1] = -(U - g->center) * sp[1] - sp[1] / 2;
+    origin[2] = -(V - ((double)g->ppoint[1] - (double)g->roi[1])) * sp[2] - sp[2] / 2;
+    uint64_t cb[4];
+    orc_fdk_cbct_roi(g, ny, cb);
+    const double ic[2] = {(double)g->ppoint[0] - (double)cb[0], (double)g->ppoint[1] - (double)cb[1]};
+    const double offCenter = ((double)g->ppoint[0] - (double)g->roi[0] - g->center) * sp[0];
+    for (size_t ip = 0; ip < np; ++ip) {
+        const float *src = proj + ip * w * h;
+        for (size_t i = 0; i < w * h; ++i) img[i] = (double)src[i] * weights[ip];
+        orc_fdk_filter_f64(img, w, h);
+        for (size_t i = 0; i < w * h; ++i) img[i] *= sad_sid_2 * scale;
+        /* projection matrix in closed form: rows -plt/pix_u, -pup/pix_v, -nrm/sdd */
+        double a = (g->direction == 0 ? -angles[ip] : angles[ip]) * ORC_DPI / 180.0;
+        double nr[3] = {cos(a), sin(a), 0.0};
+        double pl[3] = {nr[1], -nr[0], 0.0};
+        double pu[3] = {pl[1] * nr[2] - pl[2] * nr[1], pl[2] * nr[0] - pl[0] * nr[2], pl[0] * nr[1] - pl[1] * nr[0]};
+        double pm[12] = {-pl[0] / g->pix[0], -pl[1] / g->pix[0], -pl[2] / g->pix[0], -offCenter / g->pix[0],
+                         -pu[0] / g->pix[1], -pu[1] / g->pix[1], -pu[2] / g->pix[1], 0.0,
+                         -nr[0] / sdd, -nr[1] / sdd, -nr[2] / sdd, sod / sdd};
+#pragma omp parallel for schedule(static)
+        for (long k = 0; k < (long)nz; ++k) {
+            double z = origin[2] + (double)k * sp[2];
+            for (size_t j = 0; j < ny; ++j) {
+                double y = origin[1] + (double)j * sp[1];
+                double *row = cv + ((size_t)k * ny + j) * nx;
+                for (size_t i = mfirst[j] + 1; i <= msecond[j]; ++i) {
+                    double x = origin[0] + (double)i * sp[0];
+                    double p0 = pm[0] * x + pm[1] * y + pm[2] * z + pm[3];
+                    double p1 = pm[4] * x + pm[5] * y + pm[6] * z + pm[7];
+                    double p2 = pm[8] * x + pm[9] * y + pm[10] * z + pm[11];
+                    double dw = 1.0 / p2;
+                    double c = dw * (p0 + ic[0] * p2) + 0.5;
+                    double r = dw * (p1 + ic[1] * p2) + 0.5;
+                    double val = 0.0;
+                    if (!(r < 0) && !(r >= (double)h) && !(c < 0) && !(c >= (double)w))
+                        val = img[(size_t)(int)r * w + (size_t)(int)c];
+                    row[i] += dw * dw * val;
+                }
+            }
+        }
+    }
+    for (size_t k = 0; k < nz; ++k)
+        for (size_t j = 0; j < ny; ++j)
+            for (size_t i = 0; i < nx; ++i)
+                vol[((nz - 1 - k) * ny + j) * nx + i] = cv[(k * ny + j) * nx + (nx - 1 - i)];
+    free(cv); free(img);
+}

@@ -1,0 +1,69 @@
+"""C ABI of libmuhrec_hip.so: loads, exports every declared symbol, and
+rejects invalid configurations before touching a device (CPU only)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(mr_\w+)\s*\(", txt, re.M)))
+
+
+def test_header_symbols_exported():
+    import imagingsuite_amd._lib as L
+    lib = L.load()
+    declared = _declared("muhrec_hip.h")
+    assert len(declared) >= 15
+    assert set(declared) == set(L.EXPORTED_SYMBOLS)
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+def test_last_error_and_version():
+    import imagingsuite_amd._lib as L
+    lib = L.load()
+    assert lib.mr_version() >= 1
+    assert isinstance(lib.mr_last_error(), bytes)
+
+
+def test_invalid_cutoff_rejected_without_device():
+    import imagingsuite_amd._lib as L
+    lib = L.load()
+    plan = ctypes.c_void_p()
+    rc = lib.mr_filter_plan_create(ctypes.byref(plan), 256, 3, 0.7, 1.0, 1, 0.1, 2)
+    assert rc == L.MR_ERR_INVALID
+    assert b"Cut off" in lib.mr_last_error()
+    assert not plan.value
+
+
+def test_bp_plan_rejects_bad_params():
+    import imagingsuite_amd._lib as L
+    lib = L.load()
+    p = L.BpParams()
+    p.width, p.slices, p.interp, p.direction = 1, 4, 0, 0
+    plan = ctypes.c_void_p()
+    assert lib.mr_bp_plan_create(ctypes.byref(plan), ctypes.byref(p)) == L.MR_ERR_INVALID
+    p.width, p.use_roi = 64, 1
+    p.roi[0], p.roi[1], p.roi[2], p.roi[3] = 10, 0, 5, 63   # x1 < x0
+    assert lib.mr_bp_plan_create(ctypes.byref(plan), ctypes.byref(p)) == L.MR_ERR_INVALID
+    assert b"ROI" in lib.mr_last_error()
+
+
+def test_null_arguments_rejected():
+    import imagingsuite_amd._lib as L
+    lib = L.load()
+    assert lib.mr_filter_rows(None, None, 0, None) == L.MR_ERR_INVALID
+    assert lib.mr_bp_parallel(None, None, 0, None, None, None, 0, None) == L.MR_ERR_INVALID
+    assert lib.mr_fdk_bp(None, None, 0, 0, None, None, None, None) == L.MR_ERR_INVALID
+
+
+def test_filter_type_string_rejected():
+    from imagingsuite_amd import MrError
+    from imagingsuite_amd.recon import ProjectionFilterSingle
+    with pytest.raises(MrError):
+        ProjectionFilterSingle(256, filtertype="gaussian")
