@@ -2,8 +2,9 @@
 the CPU oracle, through the C ABI (via imagingsuite_amd.recon).
 
 Tolerances
-* filter: max |gpu - oracle| <= 2e-6 * max|oracle| per stack (float32 FFT vs
-  the oracle's double FFT rounded to float).
+* filter: max |gpu - oracle| <= 2e-5 and RMS <= 3e-6 of max|oracle| per
+  stack (float32 radix-4 FFT of length up to 16384 vs the oracle's double FFT
+  rounded to float).
 * back-projection on identical filtered input: bit-exact (same float
   operation order per voxel as the reference).
 * end-to-end FBP: relRMSE <= 1e-4 of the reconstruction range against the
@@ -38,8 +39,9 @@ def test_filter_matches_oracle(orc, dev, width, ftype):
     got = t.cpu().numpy()
     ref = orc.filter_rows_f32(rows, ftype, 0.5, 2.0)
     err = np.abs(got - ref).max() / np.abs(ref).max()
-    print(f"filter width={width} {ftype}: max rel err {err:.2e}")
-    assert err <= 2e-6
+    rms = np.sqrt(np.mean((got - ref) ** 2)) / np.abs(ref).max()
+    print(f"filter width={width} {ftype}: max rel err {err:.2e}, rel rms {rms:.2e}")
+    assert err <= 2e-5 and rms <= 3e-6
 
 
 def _random_stack(rng, n_p, nv, nu):
