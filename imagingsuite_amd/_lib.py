@@ -26,7 +26,7 @@ EXPORTED_SYMBOLS = (
     "mr_filter_plan_create", "mr_filter_plan_destroy", "mr_filter_plan_info", "mr_filter_rows",
     "mr_bp_plan_create", "mr_bp_plan_destroy", "mr_bp_plan_mask", "mr_bp_parallel",
     "mr_fbp_parallel",
-    "mr_fdk_plan_create", "mr_fdk_plan_destroy", "mr_fdk_plan_window", "mr_fdk_bp",
+    "mr_fdk_window", "mr_fdk_plan_create", "mr_fdk_plan_destroy", "mr_fdk_plan_window", "mr_fdk_bp",
 )
 
 
@@ -43,7 +43,7 @@ class BpParams(ctypes.Structure):
                 ("center", ctypes.c_float), ("direction", ctypes.c_int),
                 ("interp", ctypes.c_int), ("buffer_size", ctypes.c_int),
                 ("multiprojbp", ctypes.c_int), ("use_roi", ctypes.c_int),
-                ("roi", ctypes.c_size_t * 4)]
+                ("roi", ctypes.c_size_t * 4), ("fused", ctypes.c_int)]
 
 
 class FdkParams(ctypes.Structure):
@@ -87,6 +87,7 @@ def load():
     L.mr_bp_plan_mask.argtypes = [vp, szp, szp]
     L.mr_bp_parallel.argtypes = [vp, vp, sz, fp, fp, vp, i32, vp]
     L.mr_fbp_parallel.argtypes = [vp, vp, vp, sz, fp, fp, vp, vp]
+    L.mr_fdk_window.argtypes = [ctypes.POINTER(FdkParams), szp]
     L.mr_fdk_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(FdkParams)]
     L.mr_fdk_plan_destroy.argtypes = [vp]
     L.mr_fdk_plan_window.argtypes = [vp, szp]

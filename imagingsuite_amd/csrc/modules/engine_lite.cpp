@@ -1,0 +1,255 @@
+// Block driver over the MuhRec plugin boundary: loads modules with
+// dlopen/dlsym exactly as the reference loader does
+// (frameworks/tomography/Framework/ReconFramework/src/ModuleItem.cpp:82-168)
+// and runs the parallel-beam and cone-beam block loops of
+// ReconEngine::Run3DFull / Process3D / BackProject3D / TransferMatrix
+// (frameworks/tomography/Framework/ReconFramework/src/ReconEngine.cpp:531-574, 838-1185, 1314-1714).
+#include <dlfcn.h>
+
+#include <cstring>
+#include <sstream>
+#include <string>
+
+#include "mr/module_api.h"
+#include "muhrec_hip.h"
+#include "muhrec_modules.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+typedef void *(*GetModule_t)(const char *, const char *, void *);
+typedef int (*Destroy_t)(const char *, void *);
+typedef int (*GetModuleList_t)(const char *, void *);
+
+std::map<std::string, std::string> parse_kv(const char *s)
+{
+    std::map<std::string, std::string> m;
+    if (!s) return m;
+    std::string str(s);
+    size_t pos = 0;
+    while (pos <= str.size()) {
+        size_t end = str.find(';', pos);
+        if (end == std::string::npos) end = str.size();
+        std::string item = str.substr(pos, end - pos);
+        size_t eq = item.find('=');
+        if (eq != std::string::npos) {
+            auto trim = [](std::string t) {
+                size_t a = t.find_first_not_of(" \t"), b = t.find_last_not_of(" \t");
+                return a == std::string::npos ? std::string() : t.substr(a, b - a + 1);
+            };
+            m[trim(item.substr(0, eq))] = trim(item.substr(eq + 1));
+        }
+        pos = end + 1;
+    }
+    return m;
+}
+
+template <typename T>
+void parse_n(const std::string &s, T *out, int n)
+{
+    std::istringstream is(s);
+    for (int i = 0; i < n; ++i) is >> out[i];
+}
+
+ReconConfig make_config(const std::map<std::string, std::string> &kv)
+{
+    ReconConfig c;
+    auto get = [&](const char *k) -> const std::string * {
+        auto it = kv.find(k);
+        return it == kv.end() ? nullptr : &it->second;
+    };
+    if (auto v = get("geometry"))
+        c.ProjectionInfo.beamgeometry = (*v == "cone") ? ReconConfig::cProjections::BeamGeometry_Cone
+                                                       : ReconConfig::cProjections::BeamGeometry_Parallel;
+    if (auto v = get("center")) c.ProjectionInfo.fCenter = std::stof(*v);
+    if (auto v = get("direction"))
+        c.ProjectionInfo.eDirection = (*v == "ccw") ? kipl::base::RotationDirCCW : kipl::base::RotationDirCW;
+    if (auto v = get("resolution")) parse_n(*v, c.ProjectionInfo.fResolution, 2);
+    if (auto v = get("voxelsize")) parse_n(*v, c.MatrixInfo.fVoxelSize, 3);
+    if (auto v = get("sod")) c.ProjectionInfo.fSOD = std::stof(*v);
+    if (auto v = get("sdd")) c.ProjectionInfo.fSDD = std::stof(*v);
+    if (auto v = get("ppoint")) parse_n(*v, c.ProjectionInfo.fpPoint, 2);
+    if (auto v = get("roi")) parse_n(*v, c.ProjectionInfo.roi, 4);
+    if (auto v = get("projection_roi3")) c.ProjectionInfo.projection_roi[3] = std::stoul(*v);
+    if (auto v = get("usematrixroi")) c.MatrixInfo.bUseROI = (*v == "true");
+    if (auto v = get("matrixroi")) parse_n(*v, c.MatrixInfo.roi, 4);
+    return c;
+}
+
+// ProjectionReader::Read sends angles and weights as ostream text (6 digits).
+std::string list_text(const float *v, size_t n)
+{
+    std::ostringstream s;
+    for (size_t i = 0; i < n; ++i) s << v[i] << " ";
+    return s.str();
+}
+
+}  // namespace
+
+struct mre_engine {
+    void *so = nullptr;
+    GetModule_t get = nullptr;
+    Destroy_t destroy = nullptr;
+    PreprocModuleBase *pre = nullptr;
+    BackProjectorModuleBase *bp = nullptr;
+    std::map<std::string, std::string> pre_params, bp_params;
+    ReconConfig cfg;
+    size_t slice_block = 32;
+};
+
+extern "C" const char *mre_last_error(void) { return g_err.c_str(); }
+
+extern "C" int mre_engine_destroy(mre_engine *e)
+{
+    if (!e) return 0;
+    if (e->destroy) {
+        if (e->pre) e->destroy("muhrec", e->pre);
+        if (e->bp) e->destroy("muhrecbp", e->bp);
+    }
+    if (e->so) dlclose(e->so);
+    delete e;
+    return 0;
+}
+
+extern "C" int mre_engine_create(mre_engine **engine, const char *module_so, const char *preproc_module,
+                                 const char *preproc_params, const char *bp_module, const char *bp_params,
+                                 const char *config)
+{
+    g_err.clear();
+    if (!engine || !module_so || !bp_module) {
+        g_err = "NULL argument";
+        return 1;
+    }
+    *engine = nullptr;
+    auto *e = new mre_engine();
+    try {
+        e->so = dlopen(module_so, RTLD_LAZY);
+        if (!e->so) throw ReconException(std::string("Could not load module library: ") + dlerror(), __FILE__, __LINE__);
+        e->get = reinterpret_cast<GetModule_t>(dlsym(e->so, "GetModule"));
+        if (!e->get) throw ReconException("Could not find symbol GetModule", __FILE__, __LINE__);
+        e->destroy = reinterpret_cast<Destroy_t>(dlsym(e->so, "Destroy"));
+        if (!e->destroy) throw ReconException("Could not find symbol Destroy", __FILE__, __LINE__);
+        e->cfg = make_config(parse_kv(config));
+        e->pre_params = parse_kv(preproc_params);
+        e->bp_params = parse_kv(bp_params);
+        e->cfg.backprojector.m_sModule = bp_module;
+        e->cfg.backprojector.parameters = e->bp_params;
+        if (preproc_module && *preproc_module) {
+            e->pre = static_cast<PreprocModuleBase *>(e->get("muhrec", preproc_module, nullptr));
+            if (!e->pre) throw ReconException(std::string("Failed to create module ") + preproc_module, __FILE__, __LINE__);
+            e->pre->Configure(e->cfg, e->pre_params);
+        }
+        e->bp = static_cast<BackProjectorModuleBase *>(e->get("muhrecbp", bp_module, nullptr));
+        if (!e->bp) throw ReconException(std::string("Failed to create module ") + bp_module, __FILE__, __LINE__);
+        e->bp->Configure(e->cfg, e->bp_params);
+        e->slice_block = (size_t)GetIntParameter(e->bp_params, "SliceBlock");
+        if (e->slice_block == 0) throw ReconException("SliceBlock must be positive", __FILE__, __LINE__);
+    } catch (const std::exception &ex) {
+        g_err = ex.what();
+        mre_engine_destroy(e);
+        return 2;
+    }
+    *engine = e;
+    return 0;
+}
+
+extern "C" int mre_engine_run(mre_engine *e, const float *projections, size_t n_proj, size_t rows, size_t width,
+                              const float *angles, const float *weights, float *volume)
+{
+    g_err.clear();
+    if (!e || !projections || !angles || !weights || !volume) {
+        g_err = "NULL argument";
+        return 1;
+    }
+    try {
+        const auto &P = e->cfg.ProjectionInfo;
+        const size_t *roi = P.roi;
+        const bool cone = P.beamgeometry == ReconConfig::cProjections::BeamGeometry_Cone;
+        if (roi[2] <= roi[0] || roi[3] <= roi[1]) throw ReconException("Invalid projection ROI", __FILE__, __LINE__);
+        if (width != roi[2] - roi[0]) throw ReconException("width does not match roi", __FILE__, __LINE__);
+        if (!cone && rows != roi[3] - roi[1]) throw ReconException("rows do not match roi", __FILE__, __LINE__);
+        const size_t total = roi[3] - roi[1];
+        const size_t sb = e->slice_block;
+        const size_t nblocks = total / sb + (total % sb ? 1 : 0);
+        std::map<std::string, std::string> params;
+        params["angles"] = list_text(angles, n_proj);
+        params["weights"] = list_text(weights, n_proj);
+        std::string dose;
+        for (size_t i = 0; i < n_proj; ++i) dose += "1 ";
+        params["dose"] = dose;
+        for (size_t b = 0; b < nblocks; ++b) {
+            size_t block[4] = {roi[0], roi[1] + b * sb, roi[2], std::min(roi[1] + (b + 1) * sb, roi[3])};
+            size_t r0 = block[1] - roi[1], r1 = block[3] - roi[1];   // rows of the stack used
+            if (cone) {
+                mr_fdk_params q{};
+                q.sod = P.fSOD; q.sdd = P.fSDD;
+                q.pixel[0] = P.fResolution[0]; q.pixel[1] = P.fResolution[1];
+                for (int i = 0; i < 3; ++i) q.voxel[i] = e->cfg.MatrixInfo.fVoxelSize[i];
+                q.ppoint[0] = P.fpPoint[0]; q.ppoint[1] = P.fpPoint[1];
+                q.center = P.fCenter; q.direction = (int)P.eDirection;
+                for (int i = 0; i < 4; ++i) q.roi[i] = block[i];
+                q.projection_roi3 = P.projection_roi[3];
+                size_t cb[4];
+                if (mr_fdk_window(&q, cb) != MR_OK) throw ReconException(mr_last_error(), __FILE__, __LINE__);
+                r0 = cb[1];
+                r1 = std::min(cb[3], rows);
+                if (r1 <= r0) throw ReconException("empty detector window", __FILE__, __LINE__);
+            }
+            const size_t nr = r1 - r0;
+            size_t dims[3] = {width, nr, n_proj};
+            kipl::base::TImage<float, 3> img(dims);
+            for (size_t p = 0; p < n_proj; ++p)
+                std::memcpy(img.GetLinePtr(0, p), projections + (p * rows + r0) * width, sizeof(float) * nr * width);
+            if (e->pre) e->pre->Process(img, params);
+            e->bp->SetROI(block);
+            e->bp->Process(img, params);
+            const size_t ns = block[3] - block[1];
+            for (size_t i = 0; i < ns; ++i) {
+                kipl::base::TImage<float, 2> slice = e->bp->GetSlice(i);
+                std::memcpy(volume + (block[1] - roi[1] + i) * width * width, slice.GetDataPtr(),
+                            sizeof(float) * std::min(slice.Size(), width * width));
+            }
+        }
+    } catch (const std::exception &ex) {
+        g_err = ex.what();
+        return 2;
+    }
+    return 0;
+}
+
+// Serialises GetModuleList of a module library: "name:key=value,key=value;..."
+extern "C" int mre_module_list(const char *module_so, const char *application, char *buf, size_t len)
+{
+    g_err.clear();
+    void *so = dlopen(module_so, RTLD_LAZY);
+    if (!so) {
+        g_err = std::string("Could not load module library: ") + dlerror();
+        return 2;
+    }
+    auto f = reinterpret_cast<GetModuleList_t>(dlsym(so, "GetModuleList"));
+    if (!f) {
+        g_err = "Could not find symbol GetModuleList";
+        dlclose(so);
+        return 2;
+    }
+    std::map<std::string, std::map<std::string, std::string>> list;
+    int rc = f(application, &list);
+    std::string out;
+    for (auto &m : list) {
+        out += m.first + ":";
+        bool first = true;
+        for (auto &kv : m.second) {
+            if (!first) out += ",";
+            out += kv.first + "=" + kv.second;
+            first = false;
+        }
+        out += ";";
+    }
+    dlclose(so);
+    if (buf && len) {
+        std::strncpy(buf, out.c_str(), len - 1);
+        buf[len - 1] = '\0';
+    }
+    return rc;
+}

@@ -1,0 +1,584 @@
+// MuhRec plugin modules backed by the gfx950 kernels of libmuhrec_hip.so.
+//
+// Module names, parameter keys and the plugin quartet follow the reference
+// libraries they replace:
+//   ProjectionFilterSingle  frameworks/tomography/Preprocessing/StdPreprocModules/src/ProjectionFilter.cpp:78-375
+//   MultiProjBPparallel, MultiProjBP, NearestNeighborBP
+//                           frameworks/tomography/Backprojectors/StdBackProjectors/src/StdBackProjectorBase.cpp:21-342
+//   FDKbp_single            frameworks/tomography/Backprojectors/FDKBackProjectors/src/fdkreconbase.cpp:20-330
+//   GetModule/Destroy/LibVersion/GetModuleList
+//                           .../StdBackProjectors/src/StdBackProjectors.cpp:25-88,
+//                           .../StdPreprocModules/src/StdPreprocModules.cpp:40-170
+//
+// Each module keeps its data in host TImages (the API contract) and moves it
+// through device buffers it owns; the arithmetic runs in the HIP kernels.
+// Kernel/HIP failures surface as ReconException, as the reference's do.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <limits>
+#include <sstream>
+
+#include "mr/module_api.h"
+#include "muhrec_hip.h"
+#include "muhrec_modules.h"
+
+namespace {
+
+void check_mr(int rc, const char *what, const char *file, int line)
+{
+    if (rc != MR_OK) throw ReconException(std::string(what) + ": " + mr_last_error(), file, line);
+}
+void check_hip(hipError_t e, const char *what, const char *file, int line)
+{
+    if (e != hipSuccess) throw ReconException(std::string(what) + ": " + hipGetErrorString(e), file, line);
+}
+#define MRM_CHECK(x) check_mr((x), #x, __FILE__, __LINE__)
+#define MRM_HIP(x) check_hip((x), #x, __FILE__, __LINE__)
+
+template <typename T>
+std::string value2string(T v)
+{
+    std::ostringstream s;
+    s << v;
+    return s.str();
+}
+
+std::string lower(std::string s)
+{
+    for (auto &c : s) c = (char)std::tolower((unsigned char)c);
+    return s;
+}
+
+bool string2bool(const std::string &str)
+{
+    const std::string s = lower(str);
+    if (s == "true" || s == "yes") return true;
+    if (s == "false" || s == "no") return false;
+    throw ReconException("string2bool could not parse string", __FILE__, __LINE__);
+}
+
+bool optional_bool(std::map<std::string, std::string> &p, const std::string &key, bool dflt)
+{
+    auto it = p.find(key);
+    return (it == p.end() || it->second.empty()) ? dflt : string2bool(it->second);
+}
+
+// Growable device buffer of floats.
+struct DevBuf {
+    float *p = nullptr;
+    size_t n = 0;
+    void reserve(size_t m)
+    {
+        if (m <= n) return;
+        release();
+        MRM_HIP(hipMalloc(&p, m * sizeof(float)));
+        n = m;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+const char *kFilterNames[] = {"Ram-Lak", "Shepp-Logan", "Hanning", "Hamming", "Butterworth"};
+
+int filter_enum(const std::string &str)
+{
+    const std::string s = lower(str);
+    const char *keys[] = {"ram-lak", "shepp-logan", "hanning", "hamming", "butterworth"};
+    for (int i = 0; i < 5; ++i)
+        if (s == keys[i]) return i;
+    throw ReconException("Could not convert filter string to enum.", __FILE__, __LINE__);
+}
+
+// -------------------------------------------------------------------------
+// ProjectionFilterSingle
+// -------------------------------------------------------------------------
+class HipProjectionFilterSingle : public PreprocModuleBase {
+public:
+    explicit HipProjectionFilterSingle(kipl::interactors::InteractionBase *interactor)
+        : PreprocModuleBase("ProjectionFilterSingle", interactor)
+    {
+    }
+    ~HipProjectionFilterSingle() override { drop_plan(); }
+
+    int Configure(ReconConfig config, std::map<std::string, std::string> parameters) override
+    {
+        const size_t N = config.ProjectionInfo.roi[2] - config.ProjectionInfo.roi[0];
+        m_FilterType = filter_enum(GetStringParameter(parameters, "filtertype"));
+        m_fCutOff = GetFloatParameter(parameters, "cutoff");
+        m_fOrder = GetFloatParameter(parameters, "order");
+        m_bUseBias = string2bool(GetStringParameter(parameters, "usebias"));
+        m_fBiasWeight = GetFloatParameter(parameters, "biasweight");
+        m_nPaddingDoubler = (size_t)GetIntParameter(parameters, "paddingdoubler");
+        nImageSize = N;
+        if (!((0 < m_fCutOff) && (m_fCutOff <= 0.5f)))
+            throw ReconException("Cut off frequency is not in valid range", __FILE__, __LINE__);
+        drop_plan();
+        return 0;
+    }
+
+    std::map<std::string, std::string> GetParameters() override
+    {
+        std::map<std::string, std::string> p;
+        p["filtertype"] = kFilterNames[m_FilterType];
+        p["cutoff"] = value2string(m_fCutOff);
+        p["order"] = value2string(m_fOrder);
+        p["usebias"] = m_bUseBias ? "true" : "false";
+        p["biasweight"] = value2string(m_fBiasWeight);
+        p["paddingdoubler"] = value2string(m_nPaddingDoubler);
+        return p;
+    }
+
+protected:
+    int ProcessCore(kipl::base::TImage<float, 2> &img, std::map<std::string, std::string> &) override
+    {
+        if (img.Size() == 0) throw ReconException("Empty projection image", __FILE__, __LINE__);
+        filter_rows(img.GetDataPtr(), img.Size(1), img.Size(0));
+        return 0;
+    }
+
+    int ProcessCore(kipl::base::TImage<float, 3> &img, std::map<std::string, std::string> &) override
+    {
+        if (img.Size() == 0) throw ReconException("Empty projection image", __FILE__, __LINE__);
+        if (UpdateStatus(0.0f, m_sModuleName)) return 0;
+        filter_rows(img.GetDataPtr(), img.Size(1) * img.Size(2), img.Size(0));
+        UpdateStatus(1.0f, m_sModuleName);
+        return 0;
+    }
+
+private:
+    void drop_plan()
+    {
+        if (plan_) mr_filter_plan_destroy(plan_);
+        plan_ = nullptr;
+    }
+
+    void filter_rows(float *rows, size_t n_rows, size_t width)
+    {
+        if (width != nImageSize)
+            throw ReconException("Projection width does not match the configured ROI", __FILE__, __LINE__);
+        if (!plan_)
+            MRM_CHECK(mr_filter_plan_create(&plan_, nImageSize, m_FilterType, m_fCutOff, m_fOrder,
+                                            m_bUseBias ? 1 : 0, m_fBiasWeight, m_nPaddingDoubler));
+        const size_t n = n_rows * width;
+        buf_.reserve(n);
+        MRM_HIP(hipMemcpy(buf_.p, rows, n * sizeof(float), hipMemcpyHostToDevice));
+        MRM_CHECK(mr_filter_rows(plan_, buf_.p, n_rows, nullptr));
+        MRM_HIP(hipMemcpy(rows, buf_.p, n * sizeof(float), hipMemcpyDeviceToHost));
+    }
+
+    int m_FilterType = 3;  // Hamming
+    float m_fCutOff = 0.5f, m_fOrder = 1.0f;
+    bool m_bUseBias = true;
+    float m_fBiasWeight = 0.1f;
+    size_t m_nPaddingDoubler = 2, nImageSize = 0;
+    mr_filter_plan *plan_ = nullptr;
+    DevBuf buf_;
+};
+
+// -------------------------------------------------------------------------
+// MultiProjBPparallel / MultiProjBP / NearestNeighborBP
+// -------------------------------------------------------------------------
+class HipStdBackProjector : public BackProjectorModuleBase {
+public:
+    HipStdBackProjector(const std::string &name, int interp, int multiprojbp,
+                        kipl::interactors::InteractionBase *interactor)
+        : BackProjectorModuleBase("muhrec", name, MatrixZXY, interactor), interp_(interp),
+          multiprojbp_(multiprojbp)
+    {
+        nSubVolume[0] = nSubVolume[1] = 1;
+    }
+    ~HipStdBackProjector() override
+    {
+        if (plan_) mr_bp_plan_destroy(plan_);
+    }
+
+    int Configure(ReconConfig config, std::map<std::string, std::string> parameters) override
+    {
+        mConfig = config;
+        nProjectionBufferSize = (size_t)GetIntParameter(parameters, "ProjectionBufferSize");
+        nSliceBlock = (size_t)GetIntParameter(parameters, "SliceBlock");
+        GetUIntParameterVector(parameters, "SubVolume", nSubVolume, 2);
+        bufferDrop_ = optional_bool(parameters, "BufferDrop", true);
+        fused_ = optional_bool(parameters, "fused", false);
+        return 0;
+    }
+    int Initialize() override { return 0; }
+
+    std::map<std::string, std::string> GetParameters() override
+    {
+        std::map<std::string, std::string> p;
+        p["ProjectionBufferSize"] = value2string(nProjectionBufferSize);
+        p["SliceBlock"] = value2string(nSliceBlock);
+        p["SubVolume"] = value2string(nSubVolume[0]) + " " + value2string(nSubVolume[1]);
+        p["BufferDrop"] = bufferDrop_ ? "true" : "false";
+        p["fused"] = fused_ ? "true" : "false";
+        return p;
+    }
+
+    void SetROI(size_t *roi) override
+    {
+        ClearAll();
+        if (mConfig.ProjectionInfo.bCorrectTilt)
+            throw ReconException("Tilt correction is not supported by the HIP back-projector", __FILE__,
+                                 __LINE__);
+        SizeU = roi[2] - roi[0];
+        SizeV = mConfig.ProjectionInfo.imagetype == ReconConfig::cProjections::ImageType_Proj_RepeatSinogram
+                    ? roi[3]
+                    : roi[3] - roi[1];
+        for (int i = 0; i < 4; ++i) mConfig.ProjectionInfo.roi[i] = roi[i];
+        MatrixDims[0] = SizeV;
+        MatrixDims[1] = SizeU;
+        MatrixDims[2] = SizeU;
+        volume.Resize(MatrixDims);
+        volume = 0.0f;
+        BuildCircleMask();
+        if (plan_) mr_bp_plan_destroy(plan_);
+        plan_ = nullptr;
+        mr_bp_params p{};
+        p.width = SizeU;
+        p.slices = SizeV;
+        p.center = mConfig.ProjectionInfo.fCenter;
+        p.direction = (int)mConfig.ProjectionInfo.eDirection;
+        p.interp = interp_;
+        p.buffer_size = 0;  // the module applies ProjectionBufferSize itself (see flush)
+        p.multiprojbp = multiprojbp_;
+        p.use_roi = mConfig.MatrixInfo.bUseROI ? 1 : 0;
+        for (int i = 0; i < 4; ++i) p.roi[i] = mConfig.MatrixInfo.roi[i];
+        p.fused = fused_ ? 1 : 0;
+        MRM_CHECK(mr_bp_plan_create(&plan_, &p));
+        dvol_.reserve(volume.Size());
+        MRM_HIP(hipMemset(dvol_.p, 0, volume.Size() * sizeof(float)));
+        pending_.clear();
+        pending_angle_.clear();
+        pending_weight_.clear();
+        nProjCounter = 0;
+    }
+
+    // StdBackProjectorBase::Process(2D): buffer the projection; when the
+    // buffer is full the newest one is dropped (reference :104-114) unless
+    // BufferDrop is false; the buffered ones are back-projected.
+    size_t Process(kipl::base::TImage<float, 2> proj, float angle, float weight, bool bLast) override
+    {
+        if (volume.Size() == 0) throw ReconException("The target matrix is not allocated.", __FILE__, __LINE__);
+        if (proj.Size(0) != SizeU || proj.Size(1) != SizeV)
+            throw ReconException("Projection size does not match the ROI", __FILE__, __LINE__);
+        pending_.insert(pending_.end(), proj.GetDataPtr(), proj.GetDataPtr() + proj.Size());
+        pending_angle_.push_back(angle);
+        pending_weight_.push_back(weight);
+        ++nProjCounter;
+        if (bLast || nProjectionBufferSize <= nProjCounter) {
+            if (nProjectionBufferSize <= nProjCounter && bufferDrop_) {
+                pending_.resize(pending_.size() - proj.Size());
+                pending_angle_.pop_back();
+                pending_weight_.pop_back();
+            }
+            flush();
+        }
+        return nProjCounter;
+    }
+
+    // StdBackProjectorBase::Process(3D) (reference :120-149).
+    size_t Process(kipl::base::TImage<float, 3> projections, std::map<std::string, std::string> parameters) override
+    {
+        if (volume.Size() == 0) throw ReconException("The target matrix is not allocated.", __FILE__, __LINE__);
+        const size_t nProj = projections.Size(2);
+        if (projections.Size(0) != SizeU || projections.Size(1) != SizeV)
+            throw ReconException("Projection size does not match the ROI", __FILE__, __LINE__);
+        std::vector<float> weights(nProj + 16), angles(nProj + 16);
+        GetFloatParameterVector(parameters, "weights", weights.data(), (int)nProj);
+        GetFloatParameterVector(parameters, "angles", angles.data(), (int)nProj);
+        const size_t psz = SizeU * SizeV;
+        // The reference back-projects each full buffer in turn; the order of
+        // additions per voxel is the projection order either way, so the used
+        // projections are collected and back-projected in one launch.
+        for (size_t i = 0; i < nProj; ++i) {
+            if (UpdateStatus((float)i / nProj, "Back-projecting")) {
+                discard();
+                return 0;
+            }
+            pending_.insert(pending_.end(), projections.GetLinePtr(0, i), projections.GetLinePtr(0, i) + psz);
+            pending_angle_.push_back(angles[i]);
+            pending_weight_.push_back(weights[i]);
+            ++nProjCounter;
+            if (i == nProj - 1 || nProjectionBufferSize <= nProjCounter) {
+                if (nProjectionBufferSize <= nProjCounter && bufferDrop_) {
+                    pending_.resize(pending_.size() - psz);
+                    pending_angle_.pop_back();
+                    pending_weight_.pop_back();
+                }
+                nProjCounter = 0;
+            }
+        }
+        flush();
+        return 0;
+    }
+
+    void GetHistogram(float *axis, size_t *hist, size_t nBins) override
+    {
+        // Min()/Max() of the reference read the first slice of each masked
+        // column (StdBackProjectorBase.cpp:282-320).
+        float mn = std::numeric_limits<float>::max(), mx = -std::numeric_limits<float>::max();
+        for (size_t y = 0; y < mask.size(); ++y)
+            for (size_t x = mask[y].first; x < mask[y].second; ++x) {
+                const float v = volume.GetLinePtr(x, y)[0];
+                mn = std::min(mn, v);
+                mx = std::max(mx, v);
+            }
+        std::memset(axis, 0, sizeof(float) * nBins);
+        std::memset(hist, 0, sizeof(size_t) * nBins);
+        const float scale = (mx - mn) / (nBins + 1);
+        const float inv = 1.0f / scale;
+        for (size_t y = 0; y < mask.size(); ++y)
+            for (size_t x = mask[y].first; x < mask[y].second; ++x) {
+                const float *p = volume.GetLinePtr(x, y);
+                for (size_t z = 0; z < volume.Size(0); ++z) {
+                    size_t idx = static_cast<size_t>(inv * (p[z] - mn));
+                    if (p[z] < mn) idx = 0;
+                    else if (nBins <= idx) idx = nBins - 1;
+                    hist[idx]++;
+                }
+            }
+        axis[0] = mn + scale / 2.0f;
+        for (size_t i = 1; i < nBins; ++i) axis[i] = axis[i - 1] + scale;
+    }
+
+    void GetMatrixDims(size_t *dims) override
+    {
+        dims[0] = volume.Size(1);
+        dims[1] = volume.Size(2);
+        dims[2] = volume.Size(0);
+    }
+
+private:
+    // Back-project every buffered projection (one kernel launch) into the
+    // device volume and refresh the host volume.
+    void flush()
+    {
+        const size_t n = pending_angle_.size();
+        if (n > 0) {
+            dproj_.reserve(pending_.size());
+            MRM_HIP(hipMemcpy(dproj_.p, pending_.data(), pending_.size() * sizeof(float), hipMemcpyHostToDevice));
+            MRM_CHECK(mr_bp_parallel(plan_, dproj_.p, n, pending_angle_.data(), pending_weight_.data(), dvol_.p,
+                                     1, nullptr));
+            MRM_HIP(hipMemcpy(volume.GetDataPtr(), dvol_.p, volume.Size() * sizeof(float), hipMemcpyDeviceToHost));
+        }
+        discard();
+    }
+
+    void discard()
+    {
+        pending_.clear();
+        pending_angle_.clear();
+        pending_weight_.clear();
+        nProjCounter = 0;
+    }
+
+    int interp_, multiprojbp_;
+    size_t SizeU = 0, SizeV = 0, nProjCounter = 0;
+    size_t nProjectionBufferSize = 16, nSliceBlock = 32, nSubVolume[2];
+    bool bufferDrop_ = true, fused_ = false;
+    std::vector<float> pending_, pending_angle_, pending_weight_;
+    mr_bp_plan *plan_ = nullptr;
+    DevBuf dproj_, dvol_;
+};
+
+// -------------------------------------------------------------------------
+// FDKbp_single
+// -------------------------------------------------------------------------
+class HipFdkSingle : public BackProjectorModuleBase {
+public:
+    explicit HipFdkSingle(kipl::interactors::InteractionBase *interactor)
+        : BackProjectorModuleBase("muhrec", "FDKbp_single", MatrixXYZ, interactor)
+    {
+        nSubVolume[0] = nSubVolume[1] = 1;
+    }
+    ~HipFdkSingle() override
+    {
+        if (plan_) mr_fdk_plan_destroy(plan_);
+    }
+
+    int Configure(ReconConfig config, std::map<std::string, std::string> parameters) override
+    {
+        mConfig = config;
+        nProjectionBufferSize = (size_t)GetIntParameter(parameters, "ProjectionBufferSize");
+        nSliceBlock = (size_t)GetIntParameter(parameters, "SliceBlock");
+        GetUIntParameterVector(parameters, "SubVolume", nSubVolume, 2);
+        return 0;
+    }
+    int Initialize() override { return 0; }
+
+    std::map<std::string, std::string> GetParameters() override
+    {
+        std::map<std::string, std::string> p;
+        p["ProjectionBufferSize"] = value2string(nProjectionBufferSize);
+        p["SliceBlock"] = value2string(nSliceBlock);
+        p["SubVolume"] = value2string(nSubVolume[0]) + " " + value2string(nSubVolume[1]);
+        return p;
+    }
+
+    void SetROI(size_t *roi) override
+    {
+        ClearAll();
+        if (mConfig.ProjectionInfo.bCorrectTilt)
+            throw ReconException("Tilt correction is not supported by the HIP FDK back-projector", __FILE__,
+                                 __LINE__);
+        SizeU = roi[2] - roi[0];
+        SizeV = mConfig.ProjectionInfo.imagetype == ReconConfig::cProjections::ImageType_Proj_RepeatSinogram
+                    ? roi[3]
+                    : roi[3] - roi[1];
+        for (int i = 0; i < 4; ++i) mConfig.ProjectionInfo.roi[i] = roi[i];
+        MatrixDims[0] = SizeU;
+        MatrixDims[1] = SizeU;
+        MatrixDims[2] = SizeV;
+        volume.Resize(MatrixDims);
+        volume = 0.0f;
+        BuildCircleMask();
+        if (plan_) mr_fdk_plan_destroy(plan_);
+        plan_ = nullptr;
+        const auto &P = mConfig.ProjectionInfo;
+        mr_fdk_params q{};
+        q.sod = P.fSOD;
+        q.sdd = P.fSDD;
+        q.pixel[0] = P.fResolution[0];
+        q.pixel[1] = P.fResolution[1];
+        for (int i = 0; i < 3; ++i) q.voxel[i] = mConfig.MatrixInfo.fVoxelSize[i];
+        q.ppoint[0] = P.fpPoint[0];
+        q.ppoint[1] = P.fpPoint[1];
+        q.center = P.fCenter;
+        q.direction = (int)P.eDirection;
+        for (int i = 0; i < 4; ++i) q.roi[i] = roi[i];
+        q.projection_roi3 = P.projection_roi[3];
+        q.use_roi = mConfig.MatrixInfo.bUseROI ? 1 : 0;
+        for (int i = 0; i < 4; ++i) q.matrix_roi[i] = mConfig.MatrixInfo.roi[i];
+        MRM_CHECK(mr_fdk_plan_create(&plan_, &q));
+    }
+
+    // FdkReconBase::Process(3D) (reference :217-277): every projection is
+    // weighted, filtered and back-projected; the volume is mirrored in x and
+    // flipped in z.
+    size_t Process(kipl::base::TImage<float, 3> projections, std::map<std::string, std::string> parameters) override
+    {
+        if (volume.Size() == 0) throw ReconException("The target matrix is not allocated.", __FILE__, __LINE__);
+        const size_t nProj = projections.Size(2);
+        if (projections.Size(0) != SizeU)
+            throw ReconException("Projection width does not match the ROI", __FILE__, __LINE__);
+        std::vector<float> weights(nProj + 16), angles(nProj + 16);
+        GetFloatParameterVector(parameters, "weights", weights.data(), (int)nProj);
+        GetFloatParameterVector(parameters, "angles", angles.data(), (int)nProj);
+        if (UpdateStatus(0.0f, "FDK back-projection")) return 0;
+        dproj_.reserve(projections.Size());
+        dvol_.reserve(volume.Size());
+        MRM_HIP(hipMemcpy(dproj_.p, projections.GetDataPtr(), projections.Size() * sizeof(float),
+                          hipMemcpyHostToDevice));
+        MRM_CHECK(mr_fdk_bp(plan_, dproj_.p, nProj, projections.Size(1), angles.data(), weights.data(), dvol_.p,
+                            nullptr));
+        MRM_HIP(hipMemcpy(volume.GetDataPtr(), dvol_.p, volume.Size() * sizeof(float), hipMemcpyDeviceToHost));
+        UpdateStatus(1.0f, "FDK back-projection");
+        return 0;
+    }
+
+    void GetHistogram(float *axis, size_t *hist, size_t nBins) override
+    {
+        float mn = std::numeric_limits<float>::max(), mx = -std::numeric_limits<float>::max();
+        const float *d = volume.GetDataPtr();
+        for (size_t i = 0; i < volume.Size(); ++i) {
+            mn = std::min(mn, d[i]);
+            mx = std::max(mx, d[i]);
+        }
+        std::memset(axis, 0, sizeof(float) * nBins);
+        std::memset(hist, 0, sizeof(size_t) * nBins);
+        const float scale = (mx - mn) / (nBins + 1);
+        const float inv = 1.0f / scale;
+        for (size_t i = 0; i < volume.Size(); ++i) {
+            size_t idx = static_cast<size_t>(inv * (d[i] - mn));
+            if (d[i] < mn) idx = 0;
+            else if (nBins <= idx) idx = nBins - 1;
+            hist[idx]++;
+        }
+        axis[0] = mn + scale / 2.0f;
+        for (size_t i = 1; i < nBins; ++i) axis[i] = axis[i - 1] + scale;
+    }
+
+    float GetVoxelSize() override { return mConfig.MatrixInfo.fVoxelSize[0]; }
+
+private:
+    size_t SizeU = 0, SizeV = 0;
+    size_t nProjectionBufferSize = 2000, nSliceBlock = 32, nSubVolume[2];
+    mr_fdk_plan *plan_ = nullptr;
+    DevBuf dproj_, dvol_;
+};
+
+const char *kBpNames[] = {"MultiProjBP", "MultiProjBPparallel", "NearestNeighborBP", "FDKbp_single"};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Plugin quartet
+// ---------------------------------------------------------------------------
+extern "C" void *GetModule(const char *application, const char *name, void *vinteractor)
+{
+    if (application == nullptr || name == nullptr) return nullptr;
+    auto *interactor = reinterpret_cast<kipl::interactors::InteractionBase *>(vinteractor);
+    const std::string app = application, n = name;
+    try {
+        if (app == "muhrec") {
+            if (n == "ProjectionFilterSingle") return static_cast<PreprocModuleBase *>(new HipProjectionFilterSingle(interactor));
+            return nullptr;
+        }
+        if (app == "muhrecbp") {
+            if (n == "MultiProjBP") return static_cast<BackProjectorModuleBase *>(new HipStdBackProjector(n, 0, 1, interactor));
+            if (n == "MultiProjBPparallel") return static_cast<BackProjectorModuleBase *>(new HipStdBackProjector(n, 0, 0, interactor));
+            if (n == "NearestNeighborBP") return static_cast<BackProjectorModuleBase *>(new HipStdBackProjector(n, 1, 0, interactor));
+            if (n == "FDKbp_single") return static_cast<BackProjectorModuleBase *>(new HipFdkSingle(interactor));
+        }
+    } catch (...) {
+        return nullptr;
+    }
+    return nullptr;
+}
+
+extern "C" int Destroy(const char *application, void *obj)
+{
+    if (application == nullptr) return -1;
+    const std::string app = application;
+    if (app == "muhrec") {
+        delete reinterpret_cast<PreprocModuleBase *>(obj);
+        return 0;
+    }
+    if (app == "muhrecbp") {
+        delete reinterpret_cast<BackProjectorModuleBase *>(obj);
+        return 0;
+    }
+    return -1;
+}
+
+extern "C" int LibVersion() { return -1; }
+
+extern "C" int GetModuleList(const char *application, void *listptr)
+{
+    if (application == nullptr || listptr == nullptr) return -1;
+    auto *list = reinterpret_cast<std::map<std::string, std::map<std::string, std::string>> *>(listptr);
+    const std::string app = application;
+    if (app == "muhrec") {
+        HipProjectionFilterSingle f(nullptr);
+        (*list)["ProjectionFilterSingle"] = f.GetParameters();
+        return 0;
+    }
+    if (app == "muhrecbp") {
+        for (const char *n : kBpNames) {
+            BackProjectorModuleBase *m = reinterpret_cast<BackProjectorModuleBase *>(GetModule("muhrecbp", n, nullptr));
+            if (!m) continue;
+            (*list)[n] = m->GetParameters();
+            delete m;
+        }
+        return 0;
+    }
+    return -1;
+}

@@ -19,6 +19,7 @@
 #include "mr_host.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 namespace mr {
@@ -124,6 +125,129 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a)
     }
 }
 
+// LDS-staged variant.  A block owns a 16 x 16 voxel tile and ZT slices.  For
+// each chunk of KC projections it stages, per projection, the SPAN detector
+// columns the tile can reach (the tile diagonal spans at most 15*sqrt(2)+2
+// columns) x ZT slices into LDS, then every thread gathers its two columns
+// from LDS with 16-byte reads.  Rows of the staged tile are ZT+4 floats apart,
+// which spreads 16 distinct columns over distinct LDS banks.
+// EXACT = true keeps the reference's float operation order per voxel
+// (bit-identical); EXACT = false folds the interpolation into two FMAs.
+template <int ZT, int INTERP, bool EXACT, int KC>
+__global__ __launch_bounds__(256) void bp_lds_kernel(BpArgs a)
+{
+    constexpr int T = 16;
+    constexpr int SPAN = 26;
+    constexpr int RS = ZT + 4;
+    constexpr int G = ZT / 4;
+    __shared__ float sm[KC][SPAN][RS];
+    __shared__ int sb0[KC];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int x0 = blockIdx.x * T, y0 = blockIdx.y * T;
+    const int x = x0 + tx, y = y0 + ty;
+    const int z0 = blockIdx.z * ZT;
+    const bool in_vol = x < a.nu && y < a.nu;
+    const int zn = min(ZT, a.nv - z0);
+    const int zl = max(0, min(ZT, a.nz4 - z0));
+    float *out = a.vol + ((size_t)min(y, a.nu - 1) * a.nu + min(x, a.nu - 1)) * a.nv + z0;
+    float acc[ZT];
+#pragma unroll
+    for (int z = 0; z < ZT; ++z) acc[z] = (in_vol && a.accumulate && z < zn) ? out[z] : 0.0f;
+    bool inside = false;
+    if (in_vol) {
+        const int2 m = a.mask[y];
+        inside = x >= m.x && x < m.y;
+    }
+    const bool any_inside = __syncthreads_or(inside ? 1 : 0) != 0;
+    if (any_inside && zl > 0) {
+        const float yf = (float)(y + 1), xf = (float)(x + 1);
+        const int num2 = a.nu - 2;
+        for (int kb = 0; kb < a.nk; kb += KC) {
+            const int kc = min(KC, a.nk - kb);
+            if (threadIdx.x < kc) {
+                // lowest detector column any voxel of the tile reaches
+                const float4 t = a.tab[kb + threadIdx.x];
+                const float ya = (float)(y0 + 1), yb = (float)(y0 + T);
+                const float xa = (float)(x0 + 1), xb = (float)(x0 + T);
+                const float la = t.x + t.y * ya, lb = t.x + t.y * yb;
+                int n0 = min(min((int)(la - t.z * xa), (int)(la - t.z * xb)),
+                             min((int)(lb - t.z * xa), (int)(lb - t.z * xb)));
+                sb0[threadIdx.x] = max(n0, 0);
+            }
+            __syncthreads();
+            for (int e = threadIdx.x; e < kc * SPAN * G; e += 256) {
+                const int kk = e / (SPAN * G);
+                const int r = e - kk * SPAN * G;
+                const int b = r / G, q = r - b * G;
+                const int col = sb0[kk] + b;
+                float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (col < a.nu && z0 + 4 * q < a.vpad)
+                    v = *reinterpret_cast<const float4 *>(
+                        a.sino + ((size_t)(kb + kk) * a.nu + col) * a.vpad + z0 + 4 * q);
+                *reinterpret_cast<float4 *>(&sm[kk][b][4 * q]) = v;
+            }
+            __syncthreads();
+            if (inside) {
+                for (int kk = 0; kk < kc; ++kk) {
+                    const float4 t = a.tab[kb + kk];
+                    const float lsu = t.x + t.y * yf;
+                    const float u = lsu - t.z * xf;
+                    const int n = (int)u;
+                    if (n < 0 || n > num2) continue;
+                    const int loc = min(max(n - sb0[kk], 0), SPAN - 2);
+                    const float *A = &sm[kk][loc][0];
+                    if (INTERP == 0) {
+                        const float wb = fabsf(u - (float)n);
+                        const float wa = 1.0f - wb;
+                        const float *B = A + RS;
+#pragma unroll
+                        for (int g = 0; g < G; ++g) {
+                            if (4 * g < zl) {
+                                const float4 va = *reinterpret_cast<const float4 *>(A + 4 * g);
+                                const float4 vb = *reinterpret_cast<const float4 *>(B + 4 * g);
+                                if (EXACT) {
+                                    acc[4 * g + 0] = acc[4 * g + 0] + (va.x * wa + vb.x * wb);
+                                    acc[4 * g + 1] = acc[4 * g + 1] + (va.y * wa + vb.y * wb);
+                                    acc[4 * g + 2] = acc[4 * g + 2] + (va.z * wa + vb.z * wb);
+                                    acc[4 * g + 3] = acc[4 * g + 3] + (va.w * wa + vb.w * wb);
+                                } else {
+                                    acc[4 * g + 0] = __builtin_fmaf(vb.x, wb, __builtin_fmaf(va.x, wa, acc[4 * g + 0]));
+                                    acc[4 * g + 1] = __builtin_fmaf(vb.y, wb, __builtin_fmaf(va.y, wa, acc[4 * g + 1]));
+                                    acc[4 * g + 2] = __builtin_fmaf(vb.z, wb, __builtin_fmaf(va.z, wa, acc[4 * g + 2]));
+                                    acc[4 * g + 3] = __builtin_fmaf(vb.w, wb, __builtin_fmaf(va.w, wa, acc[4 * g + 3]));
+                                }
+                            }
+                        }
+                    } else {
+#pragma unroll
+                        for (int g = 0; g < G; ++g) {
+                            if (4 * g < zl) {
+                                const float4 va = *reinterpret_cast<const float4 *>(A + 4 * g);
+                                acc[4 * g + 0] = acc[4 * g + 0] + va.x;
+                                acc[4 * g + 1] = acc[4 * g + 1] + va.y;
+                                acc[4 * g + 2] = acc[4 * g + 2] + va.z;
+                                acc[4 * g + 3] = acc[4 * g + 3] + va.w;
+                            }
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (!in_vol) return;
+    if ((a.nv & 3) == 0 && zn == ZT) {
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+            *reinterpret_cast<float4 *>(out + 4 * g) =
+                make_float4(acc[4 * g], acc[4 * g + 1], acc[4 * g + 2], acc[4 * g + 3]);
+    } else {
+#pragma unroll
+        for (int z = 0; z < ZT; ++z)
+            if (z < zn) out[z] = acc[z];
+    }
+}
+
 template <int ZT, int INTERP>
 static int launch_bp(const BpArgs &a, hipStream_t s)
 {
@@ -135,8 +259,30 @@ static int launch_bp(const BpArgs &a, hipStream_t s)
     return MR_OK;
 }
 
-int run_bp(const BpArgs &a, int interp, hipStream_t s)
+template <int ZT, int INTERP, bool EXACT>
+static int launch_bp_lds(const BpArgs &a, hipStream_t s)
 {
+    constexpr int KC = 16;
+    dim3 grid((unsigned)((a.nu + 15) / 16), (unsigned)((a.nu + 15) / 16),
+              (unsigned)((a.nv + ZT - 1) / ZT));
+    ProfScope prof("bp_parallel", s);
+    hipLaunchKernelGGL((bp_lds_kernel<ZT, INTERP, EXACT, KC>), grid, dim3(256), 0, s, a);
+    MR_HIP(hipGetLastError());
+    return MR_OK;
+}
+
+// impl: 0 = direct global gathers, 1 = LDS-staged tiles.  exact: bit-faithful
+// float order (otherwise fused multiply-adds).
+int run_bp(const BpArgs &a, int interp, int impl, int exact, hipStream_t s)
+{
+    if (impl == 1) {
+        if (a.nv >= 16) {
+            if (interp == 1) return launch_bp_lds<16, 1, true>(a, s);
+            return exact ? launch_bp_lds<16, 0, true>(a, s) : launch_bp_lds<16, 0, false>(a, s);
+        }
+        if (interp == 1) return launch_bp_lds<8, 1, true>(a, s);
+        return exact ? launch_bp_lds<8, 0, true>(a, s) : launch_bp_lds<8, 0, false>(a, s);
+    }
     if (a.nv >= 16) return interp == 0 ? launch_bp<16, 0>(a, s) : launch_bp<16, 1>(a, s);
     if (a.nv >= 8) return interp == 0 ? launch_bp<8, 0>(a, s) : launch_bp<8, 1>(a, s);
     return interp == 0 ? launch_bp<4, 0>(a, s) : launch_bp<4, 1>(a, s);
@@ -157,6 +303,7 @@ struct mr_bp_plan {
     int *d_idx = nullptr;
     float *d_w = nullptr;
     size_t idx_cap = 0;
+    int impl = 1;  // 1 LDS-staged kernel, 0 direct-gather kernel
 };
 
 namespace {
@@ -212,6 +359,10 @@ extern "C" int mr_bp_plan_create(mr_bp_plan **plan, const mr_bp_params *params)
     p->p = q;
     if (hipGetDevice(&p->device) != hipSuccess) p->device = 0;
     p->vpad = (q.slices + 3) & ~(size_t)3;
+    // MultiProjBP's B-column offset only differs from the padded column when
+    // slices % 4 != 0; that case stays on the direct-gather kernel.
+    p->impl = (q.multiprojbp && (q.slices & 3)) ? 0 : 1;
+    if (const char *e = std::getenv("MR_BP_IMPL")) p->impl = std::atoi(e) == 0 ? 0 : p->impl;
     mr::circle_mask(q.width, q.width, q.use_roi != 0, q.roi, false, p->first, p->second);
     std::vector<int2> m(q.width);
     for (size_t y = 0; y < q.width; ++y) {
@@ -291,7 +442,7 @@ extern "C" int mr_bp_parallel(mr_bp_plan *plan, const float *d_proj, size_t n_pr
     a.nz4 = (int)((q.slices / 4) * 4);
     a.bstride = q.multiprojbp ? a.nz4 : (int)plan->vpad;
     a.accumulate = accumulate ? 1 : 0;
-    rc = mr::run_bp(a, q.interp, s);
+    rc = mr::run_bp(a, q.interp, plan->impl, q.fused ? 0 : 1, s);
     if (rc != MR_OK) return rc;
     // Host buffers (tables) must stay alive until the copies have been consumed.
     MR_HIP(hipStreamSynchronize(s));

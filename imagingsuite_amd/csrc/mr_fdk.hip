@@ -103,6 +103,25 @@ struct mr_fdk_plan {
     size_t cap_np = 0;
 };
 
+extern "C" int mr_fdk_window(const mr_fdk_params *params, size_t *cbct_roi)
+{
+    MR_GUARD_BEGIN
+    mr::clear_error();
+    MR_REQUIRE(params != nullptr && cbct_roi != nullptr, "NULL argument");
+    const mr_fdk_params &q = *params;
+    MR_REQUIRE(q.roi[2] > q.roi[0], "ROI x1<=x0");
+    mr::FdkGeometry g{};
+    g.sod = q.sod; g.sdd = q.sdd; g.pixel[0] = q.pixel[0]; g.pixel[1] = q.pixel[1];
+    for (int i = 0; i < 3; ++i) g.voxel[i] = q.voxel[i];
+    g.ppoint[0] = q.ppoint[0]; g.ppoint[1] = q.ppoint[1]; g.center = q.center;
+    g.direction = q.direction;
+    for (int i = 0; i < 4; ++i) g.roi[i] = q.roi[i];
+    g.projection_roi3 = q.projection_roi3;
+    mr::fdk_cbct_roi(g, q.roi[2] - q.roi[0], cbct_roi);
+    return MR_OK;
+    MR_GUARD_END
+}
+
 extern "C" int mr_fdk_plan_create(mr_fdk_plan **plan, const mr_fdk_params *params)
 {
     MR_GUARD_BEGIN

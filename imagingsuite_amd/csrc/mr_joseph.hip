@@ -1,0 +1,227 @@
+// Matched Joseph projector pair for parallel beam on gfx950: forward A and
+// its exact adjoint A^T.  No reference counterpart exists (the reference's
+// LinearForwardProjector, frameworks/tomography/Framework/ReconAlgorithms/
+// ReconAlgorithms/linearforwardprojector.cpp:15-143, is 2-D only and is not
+// the adjoint of any back-projector); the geometry is the one of the
+// reference's parallel-beam back-projector (StdBackProjectorBase.cpp:73-80):
+//   u(x, y) = center + cos(phi) (y + 1 - c) - sin(phi) (x + 1 - c),  c = nx / 2.
+//
+// With m = max(|sin phi|, |cos phi|) both operators use the same kernel
+//   K(t) = max(0, 1 - |t| / m) / m,
+// A[phi][n][z]  = sum_{x,y} K(u(x,y) - n) vol[y][x][z]   (ray-driven Joseph
+//                 stepping along the axis the ray is closest to: linear
+//                 interpolation between the two nearest voxels, step 1/m)
+// AT[y][x][z]   = sum_{phi,n} K(u(x,y) - n) sino[phi][n][z]   (voxel-driven:
+//                 the two detector bins around u with weights K)
+// so A^T is the transpose of A in exact arithmetic.
+//
+// Layout: volume [nx(y)][nx(x)][nz] (MatrixZXY), sinogram [n_ang][n_det][nz]
+// (slice fastest).  Each thread owns ZT slices held as float4 vectors.
+#include "mr_common.h"
+#include "mr_filter.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace mr {
+
+struct JosephArgs {
+    const float *src;
+    float *dst;
+    const float4 *tab;  // per angle: (su, cos, sin, 1/m)   su = c (sin - cos) + center
+    int nx, nz, nd, na;
+};
+
+// Forward: one thread per (angle, bin) and ZT slices.
+template <int ZT>
+__global__ __launch_bounds__(256) void joseph_forward_kernel(JosephArgs a)
+{
+    const int n = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int ang = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z0 = blockIdx.z * ZT;
+    if (n >= a.nd || ang >= a.na) return;
+    const float4 t = a.tab[ang];
+    const float su = t.x, c = t.y, s = t.z, inv_m = t.w;
+    const int N = a.nx;
+    float acc[ZT];
+#pragma unroll
+    for (int z = 0; z < ZT; ++z) acc[z] = 0.0f;
+    const float fn = (float)n;
+    if (fabsf(s) >= fabsf(c)) {
+        // step over rows y: x(y) solves u(x, y) = n
+        const float inv_s = 1.0f / s;
+        for (int y = 0; y < N; ++y) {
+            const float xf = (su + c * (float)(y + 1) - fn) * inv_s - 1.0f;
+            const float fl = floorf(xf);
+            const int x0 = (int)fl;
+            const float f = xf - fl;
+            if (x0 < -1 || x0 >= N) continue;
+            const float *row = a.src + ((size_t)y * N) * a.nz + z0;
+#pragma unroll
+            for (int g = 0; g < ZT / 4; ++g) {
+                float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;
+                if (x0 >= 0) v0 = *reinterpret_cast<const float4 *>(row + (size_t)x0 * a.nz + 4 * g);
+                if (x0 + 1 < N) v1 = *reinterpret_cast<const float4 *>(row + (size_t)(x0 + 1) * a.nz + 4 * g);
+                acc[4 * g + 0] += (1.0f - f) * v0.x + f * v1.x;
+                acc[4 * g + 1] += (1.0f - f) * v0.y + f * v1.y;
+                acc[4 * g + 2] += (1.0f - f) * v0.z + f * v1.z;
+                acc[4 * g + 3] += (1.0f - f) * v0.w + f * v1.w;
+            }
+        }
+    } else {
+        // step over columns x: y(x) solves u(x, y) = n
+        const float inv_c = 1.0f / c;
+        for (int x = 0; x < N; ++x) {
+            const float yf = (fn - su + s * (float)(x + 1)) * inv_c - 1.0f;
+            const float fl = floorf(yf);
+            const int y0 = (int)fl;
+            const float f = yf - fl;
+            if (y0 < -1 || y0 >= N) continue;
+            const float *col = a.src + (size_t)x * a.nz + z0;
+#pragma unroll
+            for (int g = 0; g < ZT / 4; ++g) {
+                float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;
+                if (y0 >= 0) v0 = *reinterpret_cast<const float4 *>(col + (size_t)y0 * N * a.nz + 4 * g);
+                if (y0 + 1 < N) v1 = *reinterpret_cast<const float4 *>(col + (size_t)(y0 + 1) * N * a.nz + 4 * g);
+                acc[4 * g + 0] += (1.0f - f) * v0.x + f * v1.x;
+                acc[4 * g + 1] += (1.0f - f) * v0.y + f * v1.y;
+                acc[4 * g + 2] += (1.0f - f) * v0.z + f * v1.z;
+                acc[4 * g + 3] += (1.0f - f) * v0.w + f * v1.w;
+            }
+        }
+    }
+    float *out = a.dst + ((size_t)ang * a.nd + n) * a.nz + z0;
+#pragma unroll
+    for (int g = 0; g < ZT / 4; ++g)
+        *reinterpret_cast<float4 *>(out + 4 * g) =
+            make_float4(acc[4 * g] * inv_m, acc[4 * g + 1] * inv_m, acc[4 * g + 2] * inv_m, acc[4 * g + 3] * inv_m);
+}
+
+// Adjoint: one thread per (x, y) voxel column and ZT slices.
+template <int ZT>
+__global__ __launch_bounds__(256) void joseph_adjoint_kernel(JosephArgs a)
+{
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int z0 = blockIdx.z * ZT;
+    const int N = a.nx;
+    if (x >= N || y >= N) return;
+    float acc[ZT];
+#pragma unroll
+    for (int z = 0; z < ZT; ++z) acc[z] = 0.0f;
+    const float xf = (float)(x + 1), yf = (float)(y + 1);
+    for (int ang = 0; ang < a.na; ++ang) {
+        const float4 t = a.tab[ang];
+        const float u = (t.x + t.y * yf) - t.z * xf;
+        const float fl = floorf(u);
+        const int n1 = (int)fl;
+        const float f = u - fl;
+        const float inv_m = t.w;
+        const float w1 = fmaxf(0.0f, 1.0f - f * inv_m) * inv_m;
+        const float w2 = fmaxf(0.0f, 1.0f - (1.0f - f) * inv_m) * inv_m;
+        const float *base = a.src + (size_t)ang * a.nd * a.nz + z0;
+        const bool ok1 = n1 >= 0 && n1 < a.nd, ok2 = n1 + 1 >= 0 && n1 + 1 < a.nd;
+#pragma unroll
+        for (int g = 0; g < ZT / 4; ++g) {
+            float4 v1 = make_float4(0.f, 0.f, 0.f, 0.f), v2 = v1;
+            if (ok1) v1 = *reinterpret_cast<const float4 *>(base + (size_t)n1 * a.nz + 4 * g);
+            if (ok2) v2 = *reinterpret_cast<const float4 *>(base + (size_t)(n1 + 1) * a.nz + 4 * g);
+            acc[4 * g + 0] += w1 * v1.x + w2 * v2.x;
+            acc[4 * g + 1] += w1 * v1.y + w2 * v2.y;
+            acc[4 * g + 2] += w1 * v1.z + w2 * v2.z;
+            acc[4 * g + 3] += w1 * v1.w + w2 * v2.w;
+        }
+    }
+    float *out = a.dst + ((size_t)y * N + x) * a.nz + z0;
+#pragma unroll
+    for (int g = 0; g < ZT / 4; ++g)
+        *reinterpret_cast<float4 *>(out + 4 * g) =
+            make_float4(acc[4 * g], acc[4 * g + 1], acc[4 * g + 2], acc[4 * g + 3]);
+}
+
+std::vector<float> joseph_tables(const mr_joseph_geom &g, const float *angles, size_t n_ang)
+{
+    std::vector<float> tab(4 * n_ang);
+    const double dirw = g.direction == 0 ? -1.0 : 1.0;
+    const double cx = (double)(g.nx / 2);
+    for (size_t i = 0; i < n_ang; ++i) {
+        const double phi = dirw * (double)angles[i] * 3.14159265358979323846 / 180.0;
+        const double s = std::sin(phi), c = std::cos(phi);
+        tab[4 * i + 0] = (float)(cx * (s - c) + (double)g.center);
+        tab[4 * i + 1] = (float)c;
+        tab[4 * i + 2] = (float)s;
+        tab[4 * i + 3] = (float)(1.0 / std::max(std::fabs(s), std::fabs(c)));
+    }
+    return tab;
+}
+
+}  // namespace mr
+
+namespace {
+
+int joseph_common(const mr_joseph_geom *g, const float *src, const float *angles, size_t n_ang, float *dst,
+                  void *stream, bool forward)
+{
+    mr::clear_error();
+    MR_REQUIRE(g != nullptr, "geometry is NULL");
+    MR_REQUIRE(src != nullptr && dst != nullptr && angles != nullptr, "NULL pointer");
+    MR_REQUIRE(g->nx >= 2 && g->n_det >= 2 && g->nz >= 4, "sizes too small");
+    MR_REQUIRE(g->nz % 4 == 0, "nz must be a multiple of 4");
+    MR_REQUIRE(n_ang >= 1 && n_ang <= 65535 * 4, "angle count out of range");
+    MR_REQUIRE(g->direction == 0 || g->direction == 1, "direction must be 0 (CW) or 1 (CCW)");
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<float> tab = mr::joseph_tables(*g, angles, n_ang);
+    float4 *d_tab = nullptr;
+    MR_HIP(hipMallocAsync((void **)&d_tab, sizeof(float4) * n_ang, s));
+    hipError_t e = hipMemcpyAsync(d_tab, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) {
+        (void)hipFreeAsync(d_tab, s);
+        return mr::hip_fail(e, "hipMemcpyAsync", __FILE__, __LINE__);
+    }
+    mr::JosephArgs a{src, dst, d_tab, (int)g->nx, (int)g->nz, (int)g->n_det, (int)n_ang};
+    constexpr int ZT = 4;
+    // ZT = 16 when the slice count allows it
+    if (forward) {
+        mr::ProfScope prof("joseph_forward", s);
+        if (g->nz % 16 == 0) {
+            dim3 grid((unsigned)((g->n_det + 63) / 64), (unsigned)((n_ang + 3) / 4), (unsigned)(g->nz / 16));
+            hipLaunchKernelGGL(mr::joseph_forward_kernel<16>, grid, dim3(256), 0, s, a);
+        } else {
+            dim3 grid((unsigned)((g->n_det + 63) / 64), (unsigned)((n_ang + 3) / 4), (unsigned)(g->nz / ZT));
+            hipLaunchKernelGGL(mr::joseph_forward_kernel<ZT>, grid, dim3(256), 0, s, a);
+        }
+    } else {
+        mr::ProfScope prof("joseph_adjoint", s);
+        if (g->nz % 16 == 0) {
+            dim3 grid((unsigned)((g->nx + 15) / 16), (unsigned)((g->nx + 15) / 16), (unsigned)(g->nz / 16));
+            hipLaunchKernelGGL(mr::joseph_adjoint_kernel<16>, grid, dim3(256), 0, s, a);
+        } else {
+            dim3 grid((unsigned)((g->nx + 15) / 16), (unsigned)((g->nx + 15) / 16), (unsigned)(g->nz / ZT));
+            hipLaunchKernelGGL(mr::joseph_adjoint_kernel<ZT>, grid, dim3(256), 0, s, a);
+        }
+    }
+    e = hipGetLastError();
+    hipError_t e2 = hipFreeAsync(d_tab, s);
+    if (e != hipSuccess) return mr::hip_fail(e, "joseph kernel launch", __FILE__, __LINE__);
+    if (e2 != hipSuccess) return mr::hip_fail(e2, "hipFreeAsync", __FILE__, __LINE__);
+    return MR_OK;
+}
+
+}  // namespace
+
+extern "C" int mr_joseph_forward(const mr_joseph_geom *geom, const float *d_volume, const float *angles,
+                                 size_t n_ang, float *d_sino, void *stream)
+{
+    MR_GUARD_BEGIN
+    return joseph_common(geom, d_volume, angles, n_ang, d_sino, stream, true);
+    MR_GUARD_END
+}
+
+extern "C" int mr_joseph_adjoint(const mr_joseph_geom *geom, const float *d_sino, const float *angles,
+                                 size_t n_ang, float *d_volume, void *stream)
+{
+    MR_GUARD_BEGIN
+    return joseph_common(geom, d_sino, angles, n_ang, d_volume, stream, false);
+    MR_GUARD_END
+}

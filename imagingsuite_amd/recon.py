@@ -108,10 +108,12 @@ class BackProjector:
     ``width``/``slices`` are the block's SizeU/SizeV, ``center`` is fCenter
     (ROI-relative), ``direction`` 0 = CW, 1 = CCW, ``buffer_size`` is
     ProjectionBufferSize (16 reproduces the reference exactly, including its
-    dropped projections; 0 back-projects every projection)."""
+    dropped projections; 0 back-projects every projection).  ``fused`` folds
+    the interpolation into fused multiply-adds (float rounding differs from
+    the reference; otherwise results are bit-identical)."""
 
     def __init__(self, name: str, width: int, slices: int, center: float, direction: int = 0,
-                 buffer_size: int = 16, roi: Optional[Sequence[int]] = None):
+                 buffer_size: int = 16, roi: Optional[Sequence[int]] = None, fused: bool = False):
         if name not in BP_MODULES:
             raise MrError(_lib.MR_ERR_INVALID, f"unknown back-projector {name}")
         L = _lib.load()
@@ -120,6 +122,7 @@ class BackProjector:
         p.interp = BP_MODULES[name]["interp"]
         p.multiprojbp = BP_MODULES[name]["multiprojbp"]
         p.buffer_size = int(buffer_size)
+        p.fused = int(bool(fused))
         p.use_roi = 0 if roi is None else 1
         if roi is not None:
             for i in range(4):

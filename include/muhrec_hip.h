@@ -101,6 +101,9 @@ typedef struct {
     int multiprojbp;     /* 1: MultiProjBP column-B offset (differs only when slices%4 != 0) */
     int use_roi;         /* MatrixInfo.bUseROI */
     size_t roi[4];       /* MatrixInfo.roi x0,y0,x1,y1 (inclusive) */
+    int fused;           /* 0: the reference's float operation order per voxel (bit-identical
+                            results); 1: interpolation folded into fused multiply-adds
+                            (differs by float rounding only) */
 } mr_bp_params;
 
 typedef struct mr_bp_plan mr_bp_plan;
@@ -143,6 +146,10 @@ typedef struct {
 
 typedef struct mr_fdk_plan mr_fdk_plan;
 
+/* Host-only: the detector-row window (CBCT_roi x0,y0,x1,y1) of the slab in
+ * params->roi, as ReconEngine::Run3DFull computes it
+ * (frameworks/tomography/Framework/ReconFramework/src/ReconEngine.cpp:954-1008). */
+int mr_fdk_window(const mr_fdk_params *params, size_t *cbct_roi);
 int mr_fdk_plan_create(mr_fdk_plan **plan, const mr_fdk_params *params);
 int mr_fdk_plan_destroy(mr_fdk_plan *plan);
 /* Detector window (CBCT_roi x0,y0,x1,y1) whose rows the caller hands to
@@ -154,6 +161,29 @@ int mr_fdk_plan_window(const mr_fdk_plan *plan, size_t *cbct_roi);
  * overwritten with the mirrored/flipped result (fdkreconbase.cpp:259-270). */
 int mr_fdk_bp(mr_fdk_plan *plan, float *d_proj, size_t n_proj, size_t win_rows,
               const float *angles, const float *weights, float *d_volume, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Matched Joseph projector pair for gradient-based reconstruction.
+ * No reference counterpart: the reference's only forward projector,
+ * LinearForwardProjector::project
+ * (frameworks/tomography/Framework/ReconAlgorithms/ReconAlgorithms/linearforwardprojector.cpp:15-143),
+ * is 2-D and not the adjoint of any back-projector.  The pair uses the
+ * parallel-beam geometry of StdBackProjectorBase (.../StdBackProjectorBase.cpp:73-80)
+ * and mr_joseph_adjoint is the exact transpose of mr_joseph_forward.
+ * volume [nx(y)][nx(x)][nz] (MatrixZXY), sinogram [n_ang][n_det][nz]; nz % 4 == 0.
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    size_t nx;      /* volume is nx x nx x nz */
+    size_t nz;      /* slices (multiple of 4) */
+    size_t n_det;   /* detector columns */
+    float center;   /* detector column of the rotation axis (fCenter) */
+    int direction;  /* 0 CW, 1 CCW */
+} mr_joseph_geom;
+
+int mr_joseph_forward(const mr_joseph_geom *geom, const float *d_volume, const float *angles,
+                      size_t n_ang, float *d_sino, void *stream);
+int mr_joseph_adjoint(const mr_joseph_geom *geom, const float *d_sino, const float *angles,
+                      size_t n_ang, float *d_volume, void *stream);
 
 #ifdef __cplusplus
 }

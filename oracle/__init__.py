@@ -91,6 +91,8 @@ def _setup(L):
     L.orc_fdk_filter_f64.argtypes = [_dp, _sz, _sz]
     L.orc_fdk_bp_f32.argtypes = [_fp, _sz, _sz, _sz, _fp, _fp, ctypes.POINTER(FdkGeom), _sp, _sp, _fp]
     L.orc_fdk_bp_f64.argtypes = [_fp, _sz, _sz, _sz, _dp, _dp, ctypes.POINTER(FdkGeom), _sp, _sp, _dp]
+    L.orc_joseph_forward_f64.argtypes = [_dp, _sz, _sz, _sz, ctypes.c_double, ctypes.c_int, _dp, _sz, _dp]
+    L.orc_joseph_adjoint_f64.argtypes = [_dp, _sz, _sz, _sz, ctypes.c_double, ctypes.c_int, _dp, _sz, _dp]
 
 
 def _f(a):
@@ -293,6 +295,31 @@ def fdk_bp(proj: np.ndarray, angles, weights, g: FdkGeom, f64: bool = False) -> 
                              _f(np.ascontiguousarray(weights, np.float32)), ctypes.byref(g),
                              _s(f), _s(s), _f(vol))
     return vol
+
+
+# --------------------------------------------------------------------------
+# Matched Joseph pair (FP64)
+# --------------------------------------------------------------------------
+def joseph_forward_f64(vol: np.ndarray, angles, n_det: int, center: float,
+                       direction: int = 0) -> np.ndarray:
+    """vol [nx, nx, nz] (ZXY) -> sino [n_ang, n_det, nz]."""
+    v = np.ascontiguousarray(vol, np.float64)
+    nx, _, nz = v.shape
+    a = np.ascontiguousarray(angles, np.float64)
+    out = np.zeros((a.size, n_det, nz), np.float64)
+    lib().orc_joseph_forward_f64(_d(v), nx, nz, n_det, center, direction, _d(a), a.size, _d(out))
+    return out
+
+
+def joseph_adjoint_f64(sino: np.ndarray, angles, nx: int, center: float,
+                       direction: int = 0) -> np.ndarray:
+    """sino [n_ang, n_det, nz] -> vol [nx, nx, nz] (ZXY)."""
+    s = np.ascontiguousarray(sino, np.float64)
+    na, nd, nz = s.shape
+    a = np.ascontiguousarray(angles, np.float64)
+    out = np.zeros((nx, nx, nz), np.float64)
+    lib().orc_joseph_adjoint_f64(_d(s), nx, nz, nd, center, direction, _d(a), na, _d(out))
+    return out
 
 
 # --------------------------------------------------------------------------

@@ -1,0 +1,124 @@
+"""GPU: matched Joseph pair A / A^T behind torch.autograd.Function.
+
+Checks (tolerances in the asserts):
+* A and A^T against the FP64 oracle: relRMSE <= 1e-6;
+* dot-product test <A x, y> = <x, A^T y> to 1e-5 relative (GPU float32);
+* FP64 finite differences of the objective ||A x - y||^2 + lam TV(x) on a
+  64 x 64 x 4 sub-problem against the autograd directional derivative: 1e-4;
+* reference gradient (FP64 oracle A^T(2(Ax - y)) + lam grad TV): relRMSE <= 1e-5;
+* 50 gradient-descent iterations: loss decreases; result vs the same
+  iterations in FP64 on the CPU: relRMSE <= 1e-4.
+"""
+import numpy as np
+import pytest
+import torch
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(nx=64, nz=4, n_ang=45, seed=0):
+    from recon_autograd import Geometry
+    rng = np.random.default_rng(seed)
+    ang = np.sort(rng.uniform(0, 180, n_ang)).astype(np.float32)
+    nd = nx + 6
+    geom = Geometry(nx, nz, nd, ang, center=nd / 2 - 0.25, direction=0)
+    return rng, geom
+
+
+@pytest.mark.parametrize("nx,nz", [(64, 4), (96, 16), (50, 8)])
+def test_pair_matches_fp64_oracle(orc, dev, nx, nz):
+    rng, geom = _setup(nx, nz, 37, seed=nx)
+    x = rng.random((nx, nx, nz)).astype(np.float32)
+    y = rng.random((geom.angles.size, geom.n_det, nz)).astype(np.float32)
+    Ax = geom.forward(torch.from_numpy(x).to(dev)).cpu().numpy()
+    Aty = geom.adjoint(torch.from_numpy(y).to(dev)).cpu().numpy()
+    ref_a = orc.joseph_forward_f64(x, geom.angles.astype(np.float64), geom.n_det, float(geom.g.center))
+    ref_t = orc.joseph_adjoint_f64(y, geom.angles.astype(np.float64), nx, float(geom.g.center))
+    ea, et = rel_err(Ax, ref_a), rel_err(Aty, ref_t)
+    print(f"A vs FP64 {ea[0]:.2e}/{ea[1]:.2e}, A^T vs FP64 {et[0]:.2e}/{et[1]:.2e}")
+    assert ea[0] <= 1e-6 and et[0] <= 1e-6
+
+
+def test_dot_product_gpu(dev):
+    rng, geom = _setup(128, 8, 90, seed=3)
+    x = torch.rand((128, 128, 8), device=dev)
+    y = torch.rand((90, geom.n_det, 8), device=dev)
+    lhs = float((geom.forward(x).double() * y.double()).sum())
+    rhs = float((x.double() * geom.adjoint(y).double()).sum())
+    print(f"<Ax,y>={lhs:.8e} <x,ATy>={rhs:.8e} rel {abs(lhs - rhs) / abs(lhs):.2e}")
+    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+
+
+def _objective_f64(orc, x, y, geom, lam, eps):
+    r = orc.joseph_forward_f64(x, geom.angles.astype(np.float64), geom.n_det, float(geom.g.center)) - y
+    xt = torch.from_numpy(x)
+    from recon_autograd import tv
+    return float((r * r).sum()) + lam * float(tv(xt, eps))
+
+
+def test_finite_differences_and_reference_gradient(orc, dev):
+    from recon_autograd import objective, tv
+    lam, eps = 1e-3, 1e-2
+    rng, geom = _setup(64, 4, 40, seed=5)
+    xt = rng.random((64, 64, 4))
+    y = orc.joseph_forward_f64(xt, geom.angles.astype(np.float64), geom.n_det, float(geom.g.center))
+    y += rng.normal(scale=0.5, size=y.shape)
+    x0 = rng.random((64, 64, 4))
+    # GPU autograd gradient
+    xg = torch.from_numpy(x0.astype(np.float32)).to(dev).requires_grad_(True)
+    yg = torch.from_numpy(y.astype(np.float32)).to(dev)
+    from recon_autograd import JosephProject
+    r = JosephProject.apply(xg, geom) - yg
+    loss = (r * r).sum() + lam * tv(xg, eps)
+    (g,) = torch.autograd.grad(loss, xg)
+    g = g.double().cpu().numpy()
+    # reference gradient in FP64: 2 A^T (A x - y) + lam grad TV
+    res = orc.joseph_forward_f64(x0, geom.angles.astype(np.float64), geom.n_det, float(geom.g.center)) - y
+    gref = 2.0 * orc.joseph_adjoint_f64(res, geom.angles.astype(np.float64), 64, float(geom.g.center))
+    xd = torch.from_numpy(x0).requires_grad_(True)
+    (gtv,) = torch.autograd.grad(tv(xd, eps), xd)
+    gref = gref + lam * gtv.numpy()
+    e = rel_err(g, gref)
+    print(f"gradient vs FP64 reference: relRMSE {e[0]:.2e} max {e[1]:.2e}")
+    assert e[0] <= 1e-5
+    # FP64 central differences along random directions
+    for k in range(3):
+        d = rng.normal(size=x0.shape)
+        h = 1e-4
+        fd = (_objective_f64(orc, x0 + h * d, y, geom, lam, eps)
+              - _objective_f64(orc, x0 - h * d, y, geom, lam, eps)) / (2 * h)
+        ad = float((g * d).sum())
+        print(f"direction {k}: autograd {ad:.8e} finite diff {fd:.8e}")
+        assert abs(ad - fd) <= 1e-4 * abs(fd)
+
+
+def test_gradient_descent_50_iterations(orc, dev):
+    from recon_autograd import gradient_descent
+    lam = 1e-3
+    rng, geom = _setup(64, 4, 60, seed=9)
+    from imagingsuite_amd import phantom
+    ells = phantom.random_ellipsoids(5)
+    xt = phantom.voxelize(ells, 64, 4).numpy().astype(np.float64)
+    y = orc.joseph_forward_f64(xt, geom.angles.astype(np.float64), geom.n_det, float(geom.g.center))
+    yg = torch.from_numpy(y.astype(np.float32)).to(dev)
+    x, losses, step = gradient_descent(yg, geom, iters=50, lam=lam)
+    assert all(b <= a * (1 + 1e-6) for a, b in zip(losses, losses[1:]))
+    assert losses[-1] < 0.2 * losses[0]
+    # the same 50 iterations in FP64 on the CPU (oracle A / A^T, torch double TV)
+    from recon_autograd import tv
+    xd = np.zeros_like(xt)
+    a64 = geom.angles.astype(np.float64)
+    for _ in range(50):
+        res = orc.joseph_forward_f64(xd, a64, geom.n_det, float(geom.g.center)) - y
+        g = 2.0 * orc.joseph_adjoint_f64(res, a64, 64, float(geom.g.center))
+        t = torch.from_numpy(xd).requires_grad_(True)
+        (gt,) = torch.autograd.grad(tv(t, 1e-2), t)
+        xd = xd - step * (g + lam * gt.numpy())
+    e64 = rel_err(x.cpu().numpy(), xd)
+    err_truth = rel_err(x.cpu().numpy(), xt)
+    print(f"GD 50 iters: loss {losses[0]:.3e} -> {losses[-1]:.3e}; vs FP64 GD relRMSE {e64[0]:.2e}; "
+          f"vs phantom relRMSE {err_truth[0]:.3e}")
+    assert e64[0] <= 1e-4
+    assert err_truth[0] < 0.5

@@ -67,6 +67,23 @@ def test_bp_bitexact(orc, dev, name, nv, bufsize):
     assert nbad == 0
 
 
+@pytest.mark.parametrize("nv", [32, 8])
+def test_bp_fused_mode_tolerance(orc, dev, nv):
+    """fused=True only changes float rounding: relRMSE <= 1e-6 of the range."""
+    from imagingsuite_amd.recon import BackProjector
+    rng = np.random.default_rng(11 + nv)
+    nu, n_p = 128, 90
+    proj = _random_stack(rng, n_p, nv, nu)
+    ang = np.arange(n_p, dtype=np.float32) * np.float32(180.0 / n_p)
+    w = np.full(n_p, 1.0 / n_p, np.float32)
+    bp = BackProjector("MultiProjBPparallel", nu, nv, 64.25, buffer_size=16, fused=True)
+    vol = bp.process(_gpu(proj, dev), ang, w).cpu().numpy()
+    ref = orc.bp_parallel_f32(proj, ang, w, 64.25, 0, "linear", 16)
+    e = rel_err(vol, ref)
+    print(f"fused nv={nv}: relRMSE {e[0]:.2e} max {e[1]:.2e}")
+    assert e[0] <= 1e-6 and e[1] <= 1e-5
+
+
 def test_bp_ccw_and_accumulate(orc, dev):
     from imagingsuite_amd.recon import BackProjector
     rng = np.random.default_rng(5)
