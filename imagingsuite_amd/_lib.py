@@ -27,6 +27,7 @@ EXPORTED_SYMBOLS = (
     "mr_bp_plan_create", "mr_bp_plan_destroy", "mr_bp_plan_mask", "mr_bp_parallel",
     "mr_fbp_parallel",
     "mr_fdk_window", "mr_fdk_plan_create", "mr_fdk_plan_destroy", "mr_fdk_plan_window", "mr_fdk_bp",
+    "mr_joseph_forward", "mr_joseph_adjoint",
 )
 
 
