@@ -1,23 +1,32 @@
 #!/usr/bin/env python3
-"""Benchmark of the MuhRec parallel-beam FBP hot path on one MI355X.
+"""Benchmark of the MuhRec reconstruction hot path on one MI355X.
 
-One step = ProjectionFilterSingle (Hanning, cut-off 0.5, bias 0.1, padding
-doubler 2) + MultiProjBPparallel over the whole synthetic stack, inputs
-resident in HBM: 2048 x 2048 x 256 voxels from 1500 projections
-(2048 detector columns x 256 rows, [0, 180) deg, centre offset +3.5 px),
-seeded ellipsoid phantom, I0 = 5e4 Poisson noise, -log.
+Default (the headline): config 2 -- one step = ProjectionFilterSingle
+(Hanning, cut-off 0.5, bias 0.1, padding doubler 2) + MultiProjBPparallel
+over the whole stack, inputs resident in HBM: 2048 x 2048 x 256 voxels from
+1500 projections (2048 detector columns x 256 rows over [0, 180) deg, centre
+offset +3.5 px), seeded 10-ellipsoid phantom, I0 = 5e4 Poisson noise, -log.
 
-Prints one JSON line: GUPS = voxel*projection updates per second over the
-bounding box with every configured projection, the roofline of the dominant
-kernel (the back-projector) and a CPU baseline (the oracle restatement of
-the same two stages, OpenMP on this host's cores) on a bounded sample.
+Prints one JSON line.  GUPS = voxel*projection updates per second over the
+bounding box with every configured projection.  The roofline entry is the
+dominant kernel (the back-projector), timed with HIP events around that
+kernel on its stream; the CPU baseline is the oracle restatement of the same
+two stages (OpenMP on this host's cores) on a bounded sample.
 
-Usage: python bench.py [--steps K] [--warmup W] [--config 1|2] [--mode all|compat]
+Other configurations (reported by their own JSON line):
+  --config 1  parallel FBP 256^2 x 64, 180 projections, Ram-Lak
+  --config 3  FDK 1024^3 from 720 cone-beam projections (1024^2 detector)
+  --config 4  32 uneven parallel-beam stacks: persistent batched launch
+              (hipGraph replay) vs 32 sequential plain launches
+  --config 5  matched Joseph pair A / A^T on 1024^2 x 64, 900 angles, and
+              50 gradient-descent iterations
+  --calibrate-copy  a plain 1 GiB device copy (for rocprofv3 counter scaling)
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -35,53 +44,62 @@ CONFIGS = {
             n_proj=1500, arc=(0.0, 179.88), center_off=3.5, pixel_mm=0.0125, i0=5e4, seed=2,
             filtertype="Hanning"),
 }
-PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 (vector/matrix) peak
+PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 peak
 PEAK_HBM_GBS = 8000.0
-BP_FLOP_PER_UPDATE = 4     # 2 mul + 2 add per voxel*projection update (linear interpolation)
+BP_FLOP_PER_UPDATE = 4     # linear interpolation: 2 products + 2 sums per update
+FDK_FLOP_PER_UPDATE = 11
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def emit(d):
+    print(json.dumps(d), flush=True)
+
+
+def base_result(metric, value, unit, steps, warmup, ms, dtype, config, higher=True):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": round(ms, 3), "higher_is_better": higher,
+            "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+            "data": "synthetic (seeded 10-ellipsoid phantom, analytic line integrals, Poisson I0, -log)",
+            "config": config}
+
+
 def cpu_baseline(host_sino, angles, weights, cfg, n_sample_proj, n_sample_slices):
-    """Oracle (faithful float restatement, OpenMP) on a bounded sample."""
+    """Oracle (float restatement, OpenMP) on a bounded sample."""
     import oracle
     nu = cfg["width"]
     sample = np.ascontiguousarray(host_sino[:n_sample_proj, :n_sample_slices, :])
-    a = angles[:n_sample_proj]
-    w = weights[:n_sample_proj]
     t0 = time.perf_counter()
     filt = oracle.filter_rows_f32(sample.reshape(-1, nu), cfg["filtertype"]).reshape(sample.shape)
     t1 = time.perf_counter()
-    oracle.bp_parallel_f32(filt, a, w, nu / 2 + cfg["center_off"], 0, "linear", 0)
+    oracle.bp_parallel_f32(filt, angles[:n_sample_proj], weights[:n_sample_proj],
+                           nu / 2 + cfg["center_off"], 0, "linear", 0)
     t2 = time.perf_counter()
     upd = nu * nu * n_sample_slices * n_sample_proj
-    return dict(value=upd / (t2 - t0) / 1e9, unit="GUPS", cores=len(os.sched_getaffinity(0)),
+    return dict(value=round(upd / (t2 - t0) / 1e9, 3), unit="GUPS", cores=len(os.sched_getaffinity(0)),
                 kind="port",
                 sample=f"{nu}^2 x {n_sample_slices} slices x {n_sample_proj} proj "
-                       f"(filter {t1 - t0:.2f} s + BP {t2 - t1:.2f} s)",
-                filter_s=t1 - t0, bp_s=t2 - t1)
+                       f"(filter {t1 - t0:.2f} s + BP {t2 - t1:.2f} s)")
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=2, choices=sorted(CONFIGS))
-    ap.add_argument("--mode", default="all", choices=["all", "compat"],
-                    help="all: every projection back-projected; compat: MuhRec's "
-                         "ProjectionBufferSize=16 drop of every 16th projection")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-proj", type=int, default=720)
-    ap.add_argument("--cpu-sample-slices", type=int, default=32)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
-              "(the path shards by z-slab with no exchange; see DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
+def timed(fn, steps, warmup, log_name):
+    import torch
+    for i in range(warmup):
+        fn()
+        log(f"[bench] {log_name} warmup {i + 1}/{warmup} done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        fn()
+        log(f"[bench] {log_name} step {i + 1}/{steps} done")
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps * 1e3
 
+
+# --------------------------------------------------------------------------
+def bench_parallel(args):
     import torch
     from imagingsuite_amd import _lib, phantom, scan
     from imagingsuite_amd.recon import BackProjector, ProjectionFilterSingle, fbp_parallel
@@ -102,83 +120,246 @@ def main():
 
     filt = ProjectionFilterSingle(nu, filtertype=cfg["filtertype"], cutoff=0.5, usebias=True,
                                   biasweight=0.1, paddingdoubler=2)
-    bp = BackProjector("MultiProjBPparallel", nu, nv, center, direction=0,
-                       buffer_size=0 if args.mode == "all" else 16)
+    buf = 0 if args.mode == "all" else 16
     n_used = n_p if args.mode == "all" else n_p - n_p // 16
     work = torch.empty_like(pristine)
     vol = torch.empty((nu, nu, nv), dtype=torch.float32, device=dev)
+    res = {}
+    for fused in ([True, False] if args.precision == "both" else [args.precision == "fused"]):
+        bp = BackProjector("MultiProjBPparallel", nu, nv, center, direction=0, buffer_size=buf,
+                           fused=fused)
 
-    def step():
-        work.copy_(pristine)
-        fbp_parallel(filt, bp, work, angles, weights, vol)
+        def step():
+            work.copy_(pristine)
+            fbp_parallel(filt, bp, work, angles, weights, vol)
 
-    for i in range(args.warmup):
-        step()
-        log(f"[bench] warmup {i + 1}/{args.warmup} done")
-    torch.cuda.synchronize()
-    _lib.profile(True)
-    _lib.profile_reset()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step()
-        log(f"[bench] step {i + 1}/{args.steps} done")
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    bp_ms, bp_n = _lib.profile_query("bp_parallel")
-    fl_ms, fl_n = _lib.profile_query("row_filter")
-    pr_ms, pr_n = _lib.profile_query("bp_prep")
-    _lib.profile(False)
-    assert torch.isfinite(vol).all().item()
-
-    ms_step = elapsed / args.steps * 1e3
+        _lib.profile(False)
+        for i in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        _lib.profile(True)
+        _lib.profile_reset()
+        ms = timed(step, args.steps, 0, "fused" if fused else "exact")
+        bp_ms, bp_n = _lib.profile_query("bp_parallel")
+        fl_ms, fl_n = _lib.profile_query("row_filter")
+        pr_ms, pr_n = _lib.profile_query("bp_prep")
+        _lib.profile(False)
+        assert torch.isfinite(vol).all().item()
+        res[fused] = dict(ms=ms, bp=bp_ms / max(bp_n, 1), filt=fl_ms / max(fl_n, 1), prep=pr_ms / max(pr_n, 1))
+        del bp
+    head = res[True] if True in res else res[False]
     updates = nu * nu * nv * n_p
-    gups = updates / (ms_step / 1e3) / 1e9
-    bp_avg = bp_ms / max(bp_n, 1)
+    gups = updates / (head["ms"] / 1e3) / 1e9
     flops = BP_FLOP_PER_UPDATE * nu * nu * nv * n_used
-    achieved = flops / (bp_avg / 1e3) / 1e12
+    achieved = flops / (head["bp"] / 1e3) / 1e12
     bp_bytes = 4.0 * (nu * nv * n_used + nu * nu * nv)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"bp_parallel_traffic_c{args.config}.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
-    result = {
-        "metric": "GUPS (voxel*proj updates/s) FBP 2048^2x1500 proj",
-        "value": round(gups, 2),
-        "unit": "GUPS",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded 10-ellipsoid phantom, analytic line integrals, Poisson I0, -log)",
-        "config": {"workload": cfg["name"], "width": nu, "slices": nv, "n_proj": n_p,
-                   "n_proj_backprojected": n_used, "mode": args.mode, "center": center,
-                   "filter": cfg["filtertype"], "slice_block": nv},
-        "roofline": {"bound": "mfma", "pipe": "fp32 VALU (no MFMA: not a contraction)",
+    r = base_result("GUPS (voxel*proj updates/s) FBP 2048^2x1500 proj" if args.config == 2 else
+                    f"GUPS (voxel*proj updates/s) FBP config {args.config}",
+                    round(gups, 2), "GUPS", args.steps, args.warmup, head["ms"], "f32",
+                    {"workload": cfg["name"], "width": nu, "slices": nv, "n_proj": n_p,
+                     "n_proj_backprojected": n_used, "mode": args.mode, "center": center,
+                     "filter": cfg["filtertype"], "slice_block": nv,
+                     "precision": "fused" if True in res else "exact"})
+    r["roofline"] = {"bound": "mfma", "pipe": "fp32 VALU (no MFMA: not a contraction)",
                      "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
-                     "kernel": "bp_kernel<16,0>", "kernel_ms": round(bp_avg, 3),
-                     "flop_per_update": BP_FLOP_PER_UPDATE,
+                     "kernel": "bp_lds_kernel<16,0,%s,16>" % ("false" if True in res else "true"),
+                     "kernel_ms": round(head["bp"], 3), "flop_per_update": BP_FLOP_PER_UPDATE,
                      "hbm_algorithmic_bytes": bp_bytes,
-                     "hbm_frac": round(bp_bytes / (bp_avg / 1e3) / 1e9 / PEAK_HBM_GBS, 5)},
-        "kernels_ms": {"row_filter": round(fl_ms / max(fl_n, 1), 3),
-                       "bp_prep": round(pr_ms / max(pr_n, 1), 3),
-                       "bp_parallel": round(bp_avg, 3)},
-    }
+                     "hbm_frac": round(bp_bytes / (head["bp"] / 1e3) / 1e9 / PEAK_HBM_GBS, 5)}
+    r["kernels_ms"] = {"row_filter": round(head["filt"], 3), "bp_prep": round(head["prep"], 3),
+                       "bp_parallel": round(head["bp"], 3)}
+    if False in res and True in res:
+        ex = res[False]
+        r["exact_mode"] = {"value": round(updates / (ex["ms"] / 1e3) / 1e9, 2), "ms_per_step": round(ex["ms"], 3),
+                           "bp_kernel_ms": round(ex["bp"], 3),
+                           "note": "reference float order per voxel, bit-identical to MultiProjBPparallel"}
     if not args.no_cpu_baseline:
         host = pristine[:args.cpu_sample_proj, :args.cpu_sample_slices].cpu().numpy()
         log("[bench] timing CPU baseline (oracle) ...")
-        cb = cpu_baseline(host, angles, weights, cfg, min(args.cpu_sample_proj, n_p),
-                          min(args.cpu_sample_slices, nv))
-        cb.pop("filter_s"); cb.pop("bp_s")
-        cb["value"] = round(cb["value"], 3)
-        result["cpu_baseline"] = cb
-    print(json.dumps(result), flush=True)
+        r["cpu_baseline"] = cpu_baseline(host, angles, weights, cfg, min(args.cpu_sample_proj, n_p),
+                                         min(args.cpu_sample_slices, nv))
+    emit(r)
+
+
+# --------------------------------------------------------------------------
+def fdk_matrix(sod, sdd, pix, angle, off_center=0.0, direction=0):
+    """FDKbp_single projection matrix in closed form (double), for input synthesis."""
+    a = (-angle if direction == 0 else angle) * math.pi / 180.0
+    nrm = np.array([math.cos(a), math.sin(a), 0.0])
+    plt = np.array([nrm[1], -nrm[0], 0.0])
+    pup = np.cross(plt, nrm)
+    return np.array([[*(-plt / pix[0]), -off_center / pix[0]],
+                     [*(-pup / pix[1]), 0.0],
+                     [*(-nrm / sdd), sod / sdd]])
+
+
+def bench_fdk(args):
+    import torch
+    from imagingsuite_amd import _lib, phantom
+    from imagingsuite_amd.recon import FDKbp_single
+    dev = torch.device("cuda:0")
+    n, n_p = 1024, 720
+    sod, sdd, pix, vox = 500.0, 1000.0, (0.2, 0.2), (0.1, 0.1, 0.1)
+    ppoint = (n / 2, n / 2)
+    roi = (0, 0, n, n)
+    fdk = FDKbp_single(sod, sdd, pix, vox, ppoint, n / 2, roi, n - 1)
+    cb = fdk.window
+    log(f"[bench] FDK 1024^3, window {cb}")
+    angles = np.arange(n_p, dtype=np.float32) * np.float32(360.0 / n_p)
+    weights = np.full(n_p, np.float32(1.0 / n_p), np.float32)
+    ells = phantom.random_ellipsoids(3)
+    ic = (ppoint[0] - cb[0], ppoint[1] - cb[1])
+    t = time.perf_counter()
+    proj = phantom.cone_projections(ells, n, cb[3] - cb[1], angles,
+                                    lambda a: fdk_matrix(sod, sdd, pix, a), ic, n * vox[0] * 0.8, device=dev)
+    proj = phantom.add_poisson_log(proj, 2e4, seed=3)
+    torch.cuda.synchronize()
+    log(f"[bench] cone-beam projections {tuple(proj.shape)} in {time.perf_counter() - t:.1f} s")
+    work = torch.empty_like(proj)
+    vol = torch.empty((n, n, n), dtype=torch.float32, device=dev)
+
+    def step():
+        work.copy_(proj)
+        fdk.process(work, angles, weights, vol)
+
+    step()
+    torch.cuda.synchronize()
+    _lib.profile(True)
+    _lib.profile_reset()
+    ms = timed(step, args.steps, 0, "fdk")
+    bp_ms, bp_n = _lib.profile_query("fdk_bp")
+    fl_ms, fl_n = _lib.profile_query("row_filter")
+    _lib.profile(False)
+    updates = float(n) ** 3 * n_p
+    kms = bp_ms / max(bp_n, 1)
+    achieved = FDK_FLOP_PER_UPDATE * updates / (kms / 1e3) / 1e12
+    r = base_result("GUPS (voxel*proj updates/s) FDK 1024^3 x 720 proj", round(updates / (ms / 1e3) / 1e9, 2),
+                    "GUPS", args.steps, 1, ms, "f32",
+                    {"workload": "FDK 1024^3 from 720 projections, 1024^2 detector, SOD 500, SDD 1000",
+                     "window_rows": cb[3] - cb[1]})
+    r["roofline"] = {"bound": "mfma", "pipe": "fp32 VALU", "achieved": round(achieved, 3),
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "traffic": None, "kernel": "fdk_kernel<16>", "kernel_ms": round(kms, 3),
+                     "flop_per_update": FDK_FLOP_PER_UPDATE}
+    r["kernels_ms"] = {"row_filter_fdk": round(fl_ms / max(fl_n, 1), 3), "fdk_bp": round(kms, 3)}
+    emit(r)
+
+
+# --------------------------------------------------------------------------
+def bench_batch(args):
+    import torch
+    from imagingsuite_amd import _lib, phantom, scan
+    from imagingsuite_amd.recon import BatchReconstructor
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(4)
+    stacks = []
+    for i in range(32):
+        nu = int(rng.choice([512, 768, 1024, 1536, 2048]))
+        n_p = int(rng.integers(360, 1801))
+        nv = int(rng.integers(8, 129))
+        off = float(rng.uniform(-5, 5))
+        a, w = scan.engine_angles_weights(n_p, 0.0, 180.0 - 180.0 / n_p, 25.6 / nu)
+        ells = phantom.random_ellipsoids(100 + i)
+        s = phantom.parallel_sinogram(ells, nu, nv, a, nu / 2 + off, 25.6 / nu, device=dev)
+        s = phantom.add_poisson_log(s, 1e4, seed=100 + i)
+        stacks.append(dict(proj=s, angles=a, weights=w, center=nu / 2 + off, filtertype="Hamming",
+                           buffer_size=0))
+    torch.cuda.synchronize()
+    log("[bench] 32 stacks generated; building batch plan (autotuner) ...")
+    br = BatchReconstructor(stacks, tune=True)
+    stream = torch.cuda.Stream()
+    res = {}
+    for name, fn in [("graph", br.run_graph), ("persistent", br.run), ("sequential", br.run_sequential)]:
+        def step(fn=fn):
+            with torch.cuda.stream(stream):
+                fn(stream=stream)
+            stream.synchronize()
+        res[name] = timed(step, args.steps, args.warmup, name)
+    upd = br.updates
+    r = base_result("GUPS (voxel*proj updates/s) batched 32 uneven stacks", round(upd / (res["graph"] / 1e3) / 1e9, 2),
+                    "GUPS", args.steps, args.warmup, res["graph"], "f32",
+                    {"workload": "32 uneven parallel-beam FBP stacks (width 512-2048, 360-1800 proj, 8-128 slices)",
+                     "slice_tile": br.slice_tile, "proj_chunk": br.proj_chunk, "work_items": br.n_items})
+    r["batched_vs_plain"] = {k: {"ms": round(v, 3), "GUPS": round(upd / (v / 1e3) / 1e9, 2)} for k, v in res.items()}
+    emit(r)
+
+
+# --------------------------------------------------------------------------
+def bench_joseph(args):
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from imagingsuite_amd import _lib, phantom
+    from recon_autograd import Geometry, gradient_descent
+    dev = torch.device("cuda:0")
+    nx, nz, n_a = 1024, 64, 900
+    ang = np.arange(n_a, dtype=np.float32) * np.float32(180.0 / n_a)
+    geom = Geometry(nx, nz, nx, ang, center=nx / 2, direction=0)
+    x = phantom.voxelize(phantom.random_ellipsoids(5), nx, nz, device=dev)
+    y = geom.forward(x)
+    torch.cuda.synchronize()
+    ms_a = timed(lambda: geom.forward(x), args.steps, 1, "A")
+    ms_t = timed(lambda: geom.adjoint(y), args.steps, 1, "AT")
+    upd = float(nx) * nx * nz * n_a
+    t0 = time.perf_counter()
+    xr, losses, _ = gradient_descent(y, geom, iters=50, lam=1e-3)
+    torch.cuda.synchronize()
+    gd_s = time.perf_counter() - t0
+    r = base_result("GUPS (voxel*angle updates/s) Joseph A and A^T, 1024^2x64x900",
+                    round(upd / (ms_a / 1e3) / 1e9, 2), "GUPS", args.steps, 1, ms_a, "f32",
+                    {"workload": "matched Joseph pair 1024^2 x 64 slices x 900 angles"})
+    r["adjoint"] = {"ms": round(ms_t, 3), "GUPS": round(upd / (ms_t / 1e3) / 1e9, 2)}
+    r["gradient_descent_50"] = {"seconds": round(gd_s, 2), "loss_first": losses[0], "loss_last": losses[-1]}
+    emit(r)
+
+
+def calibrate_copy():
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    emit({"calibration": "device copy", "bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 3})
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5])
+    ap.add_argument("--mode", default="all", choices=["all", "compat"],
+                    help="all: every projection back-projected; compat: MuhRec's "
+                         "ProjectionBufferSize=16 drop of every 16th projection")
+    ap.add_argument("--precision", default="both", choices=["both", "fused", "exact"],
+                    help="fused (headline): interpolation with FMAs; exact: the reference's "
+                         "float order (bit-identical); both: report both")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-proj", type=int, default=1500)
+    ap.add_argument("--cpu-sample-slices", type=int, default=96)
+    ap.add_argument("--calibrate-copy", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
+              "(the path shards by z-slab with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    if args.calibrate_copy:
+        return calibrate_copy()
+    if args.config in (1, 2):
+        return bench_parallel(args)
+    if args.config == 3:
+        return bench_fdk(args)
+    if args.config == 4:
+        return bench_batch(args)
+    return bench_joseph(args)
 
 
 if __name__ == "__main__":

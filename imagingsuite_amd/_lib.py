@@ -28,7 +28,24 @@ EXPORTED_SYMBOLS = (
     "mr_fbp_parallel",
     "mr_fdk_window", "mr_fdk_plan_create", "mr_fdk_plan_destroy", "mr_fdk_plan_window", "mr_fdk_bp",
     "mr_joseph_forward", "mr_joseph_adjoint",
+    "mr_batch_plan_create", "mr_batch_plan_info", "mr_batch_run", "mr_batch_run_graph",
+    "mr_batch_run_sequential", "mr_batch_plan_destroy",
 )
+
+
+class JosephGeom(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_size_t), ("nz", ctypes.c_size_t), ("n_det", ctypes.c_size_t),
+                ("center", ctypes.c_float), ("direction", ctypes.c_int)]
+
+
+class BatchStack(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_size_t), ("slices", ctypes.c_size_t), ("n_proj", ctypes.c_size_t),
+                ("center", ctypes.c_float), ("direction", ctypes.c_int),
+                ("filter_type", ctypes.c_int), ("cutoff", ctypes.c_float), ("order", ctypes.c_float),
+                ("use_bias", ctypes.c_int), ("bias_weight", ctypes.c_float),
+                ("padding_doubler", ctypes.c_size_t), ("buffer_size", ctypes.c_int),
+                ("d_proj", ctypes.c_void_p), ("angles", ctypes.POINTER(ctypes.c_float)),
+                ("weights", ctypes.POINTER(ctypes.c_float)), ("d_volume", ctypes.c_void_p)]
 
 
 class MrError(RuntimeError):
@@ -93,6 +110,14 @@ def load():
     L.mr_fdk_plan_destroy.argtypes = [vp]
     L.mr_fdk_plan_window.argtypes = [vp, szp]
     L.mr_fdk_bp.argtypes = [vp, vp, sz, sz, fp, fp, vp, vp]
+    for f in (L.mr_joseph_forward, L.mr_joseph_adjoint):
+        f.argtypes = [ctypes.POINTER(JosephGeom), vp, fp, sz, vp, vp]
+    L.mr_batch_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(BatchStack), sz, i32]
+    L.mr_batch_plan_info.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), szp,
+                                     ctypes.POINTER(ctypes.c_double)]
+    for n in ("mr_batch_run", "mr_batch_run_graph", "mr_batch_run_sequential"):
+        getattr(L, n).argtypes = [vp, vp]
+    L.mr_batch_plan_destroy.argtypes = [vp]
     for name in EXPORTED_SYMBOLS:
         if name not in ("mr_last_error", "mr_version", "mr_device_count"):
             getattr(L, name).restype = i32

@@ -133,19 +133,23 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a)
 // which spreads 16 distinct columns over distinct LDS banks.
 // EXACT = true keeps the reference's float operation order per voxel
 // (bit-identical); EXACT = false folds the interpolation into two FMAs.
+constexpr int kBpSpan = 26;
+
+// One 16 x 16 x ZT tile (bx, by, bz) of one volume; shared by the plain grid
+// kernel and the persistent batched kernel.  All 256 threads of the block
+// must call it together (it synchronises internally).
 template <int ZT, int INTERP, bool EXACT, int KC>
-__global__ __launch_bounds__(256) void bp_lds_kernel(BpArgs a)
+__device__ __forceinline__ void bp_lds_tile(const BpArgs &a, int bx, int by, int bz,
+                                            float (&sm)[KC][kBpSpan][ZT + 4], int (&sb0)[KC])
 {
     constexpr int T = 16;
-    constexpr int SPAN = 26;
+    constexpr int SPAN = kBpSpan;
     constexpr int RS = ZT + 4;
     constexpr int G = ZT / 4;
-    __shared__ float sm[KC][SPAN][RS];
-    __shared__ int sb0[KC];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    const int x0 = blockIdx.x * T, y0 = blockIdx.y * T;
+    const int x0 = bx * T, y0 = by * T;
     const int x = x0 + tx, y = y0 + ty;
-    const int z0 = blockIdx.z * ZT;
+    const int z0 = bz * ZT;
     const bool in_vol = x < a.nu && y < a.nu;
     const int zn = min(ZT, a.nv - z0);
     const int zl = max(0, min(ZT, a.nz4 - z0));
@@ -247,6 +251,40 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(BpArgs a)
             if (z < zn) out[z] = acc[z];
     }
 }
+
+template <int ZT, int INTERP, bool EXACT, int KC>
+__global__ __launch_bounds__(256) void bp_lds_kernel(BpArgs a)
+{
+    __shared__ float sm[KC][kBpSpan][ZT + 4];
+    __shared__ int sb0[KC];
+    bp_lds_tile<ZT, INTERP, EXACT, KC>(a, blockIdx.x, blockIdx.y, blockIdx.z, sm, sb0);
+}
+
+// Persistent batched back-projection: a grid sized to the device pulls work
+// items (stack, tile x, tile y, slice chunk) from a global counter until the
+// list is exhausted; every block reaches the exit test each iteration.
+template <int ZT, int INTERP, bool EXACT, int KC>
+__global__ __launch_bounds__(256) void bp_batch_kernel(const BpArgs *__restrict__ stacks,
+                                                       const int4 *__restrict__ items, int n_items,
+                                                       int *counter)
+{
+    __shared__ float sm[KC][kBpSpan][ZT + 4];
+    __shared__ int sb0[KC];
+    __shared__ int item;
+    for (;;) {
+        if (threadIdx.x == 0) item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = item;
+        __syncthreads();
+        if (it >= n_items) break;
+        const int4 w = items[it];
+        const BpArgs a = stacks[w.x];
+        bp_lds_tile<ZT, INTERP, EXACT, KC>(a, w.y, w.z, w.w, sm, sb0);
+        __syncthreads();
+    }
+}
+
+__global__ void reset_counter_kernel(int *counter) { *counter = 0; }
 
 template <int ZT, int INTERP>
 static int launch_bp(const BpArgs &a, hipStream_t s)
@@ -460,5 +498,345 @@ extern "C" int mr_fbp_parallel(mr_filter_plan *fplan, mr_bp_plan *bplan, float *
     int rc = mr_filter_rows(fplan, d_proj, n_proj * bplan->p.slices, stream);
     if (rc != MR_OK) return rc;
     return mr_bp_parallel(bplan, d_proj, n_proj, angles, weights, d_volume, 0, stream);
+    MR_GUARD_END
+}
+
+// --------------------------------------------------------------------------
+// Batched back-projection of many uneven stacks (C ABI mr_batch_*).
+// Per stack: ProjectionFilterSingle (out of place, the raw stack is kept),
+// weighting/transposition into the stack's sinogram, then ONE persistent
+// launch over the work items of all stacks.  mr_batch_run only enqueues
+// kernels (no host synchronisation, no host copies), so it can be captured
+// in a hipGraph; mr_batch_run_graph does that on its first call.
+// --------------------------------------------------------------------------
+#include "mr_filter.h"
+
+namespace {
+
+struct BatchStack {
+    mr_batch_stack desc{};
+    mr_filter_plan *fplan = nullptr;
+    float *d_filt = nullptr;   // filtered copy of the raw stack
+    float *d_sino = nullptr;   // [n_used][width][vpad]
+    float4 *d_tab = nullptr;
+    int2 *d_mask = nullptr;
+    int *d_idx = nullptr;
+    float *d_w = nullptr;
+    size_t vpad = 0, n_used = 0;
+    mr::BpArgs args{};
+};
+
+struct Variant {
+    int zt, kc;
+};
+const Variant kVariants[] = {{16, 16}, {16, 8}, {16, 32}, {8, 16}, {8, 32}};
+
+}  // namespace
+
+struct mr_batch_plan {
+    std::vector<BatchStack> stacks;
+    mr::BpArgs *d_args = nullptr;
+    int4 *d_items[2] = {nullptr, nullptr};   // item lists for ZT = 16 and ZT = 8
+    int n_items[2] = {0, 0};
+    int *d_counter = nullptr;
+    int variant = 0;
+    int grid[sizeof(kVariants) / sizeof(kVariants[0])] = {0};
+    double updates = 0.0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t graph_stream = nullptr;
+};
+
+namespace {
+
+int batch_launch_bp(mr_batch_plan *p, int variant, hipStream_t s)
+{
+    const Variant v = kVariants[variant];
+    const int li = v.zt == 16 ? 0 : 1;
+    hipLaunchKernelGGL(mr::reset_counter_kernel, dim3(1), dim3(1), 0, s, p->d_counter);
+    MR_HIP(hipGetLastError());
+    mr::ProfScope prof("bp_batch", s);
+    dim3 grid((unsigned)p->grid[variant]), block(256);
+#define MR_BATCH_CASE(ZT, KC)                                                                         \
+    if (v.zt == ZT && v.kc == KC)                                                                     \
+        hipLaunchKernelGGL((mr::bp_batch_kernel<ZT, 0, true, KC>), grid, block, 0, s, p->d_args,       \
+                           p->d_items[li], p->n_items[li], p->d_counter);
+    MR_BATCH_CASE(16, 16)
+    MR_BATCH_CASE(16, 8)
+    MR_BATCH_CASE(16, 32)
+    MR_BATCH_CASE(8, 16)
+    MR_BATCH_CASE(8, 32)
+#undef MR_BATCH_CASE
+    MR_HIP(hipGetLastError());
+    return MR_OK;
+}
+
+int batch_prepare(mr_batch_plan *p, hipStream_t s)
+{
+    for (auto &b : p->stacks) {
+        const mr_batch_stack &d = b.desc;
+        int rc = mr::filter_rows_to(b.fplan, d.d_proj, b.d_filt, d.n_proj * d.slices, s);
+        if (rc != MR_OK) return rc;
+        if (b.n_used == 0) continue;
+        dim3 g((unsigned)((d.width + 31) / 32), (unsigned)((b.vpad + 31) / 32), (unsigned)b.n_used);
+        mr::ProfScope prof("bp_prep", s);
+        hipLaunchKernelGGL(mr::bp_prep_kernel, g, dim3(256), 0, s, b.d_filt, b.d_sino, b.d_idx, b.d_w,
+                           (int)d.width, (int)d.slices, (int)b.vpad);
+        MR_HIP(hipGetLastError());
+    }
+    return MR_OK;
+}
+
+void batch_free(mr_batch_plan *p)
+{
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    for (auto &b : p->stacks) {
+        if (b.fplan) mr_filter_plan_destroy(b.fplan);
+        for (void *ptr : {(void *)b.d_filt, (void *)b.d_sino, (void *)b.d_tab, (void *)b.d_mask,
+                          (void *)b.d_idx, (void *)b.d_w})
+            if (ptr) (void)hipFree(ptr);
+    }
+    for (void *ptr : {(void *)p->d_args, (void *)p->d_items[0], (void *)p->d_items[1], (void *)p->d_counter})
+        if (ptr) (void)hipFree(ptr);
+    delete p;
+}
+
+template <int ZT, int KC>
+int occupancy_grid(int &grid)
+{
+    int dev = 0, ncu = 0, nb = 0;
+    MR_HIP(hipGetDevice(&dev));
+    MR_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    MR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mr::bp_batch_kernel<ZT, 0, true, KC>, 256, 0));
+    grid = std::max(1, nb) * std::max(1, ncu);
+    return MR_OK;
+}
+
+}  // namespace
+
+extern "C" int mr_batch_plan_create(mr_batch_plan **plan, const mr_batch_stack *stacks, size_t n_stacks,
+                                    int tune)
+{
+    MR_GUARD_BEGIN
+    mr::clear_error();
+    MR_REQUIRE(plan != nullptr && stacks != nullptr && n_stacks > 0, "NULL argument or empty batch");
+    *plan = nullptr;
+    auto *p = new mr_batch_plan();
+    p->stacks.resize(n_stacks);
+    std::vector<std::pair<double, int4>> items16, items8;
+    for (size_t i = 0; i < n_stacks; ++i) {
+        BatchStack &b = p->stacks[i];
+        b.desc = stacks[i];
+        const mr_batch_stack &d = b.desc;
+        if (!(d.width >= 2 && d.slices >= 1 && d.n_proj >= 1 && d.d_proj && d.d_volume && d.angles &&
+              d.weights) || d.direction < 0 || d.direction > 1) {
+            batch_free(p);
+            mr::set_error("invalid stack " + std::to_string(i));
+            return MR_ERR_INVALID;
+        }
+        int rc = mr_filter_plan_create(&b.fplan, d.width, d.filter_type, d.cutoff, d.order, d.use_bias,
+                                       d.bias_weight, d.padding_doubler);
+        if (rc != MR_OK) {
+            batch_free(p);
+            return rc;
+        }
+        b.vpad = (d.slices + 3) & ~(size_t)3;
+        std::vector<size_t> used = mr::used_projections(d.n_proj, (size_t)std::max(0, d.buffer_size));
+        b.n_used = used.size();
+        std::vector<float> tab;
+        mr::bp_tables(d.angles, used.data(), b.n_used, d.direction, d.center, d.width / 2, tab);
+        std::vector<int> idx(b.n_used);
+        for (size_t k = 0; k < b.n_used; ++k) idx[k] = (int)used[k];
+        std::vector<size_t> first, second;
+        mr::circle_mask(d.width, d.width, false, nullptr, false, first, second);
+        std::vector<int2> m(d.width);
+        for (size_t y = 0; y < d.width; ++y) {
+            m[y].x = (int)first[y];
+            m[y].y = (int)std::max(first[y], second[y]);
+        }
+        hipError_t e = hipMalloc(&b.d_filt, sizeof(float) * d.n_proj * d.slices * d.width);
+        if (e == hipSuccess) e = hipMalloc(&b.d_sino, sizeof(float) * std::max<size_t>(b.n_used, 1) * d.width * b.vpad);
+        if (e == hipSuccess) e = hipMalloc(&b.d_tab, sizeof(float4) * std::max<size_t>(b.n_used, 1));
+        if (e == hipSuccess) e = hipMalloc(&b.d_mask, sizeof(int2) * d.width);
+        if (e == hipSuccess) e = hipMalloc(&b.d_idx, sizeof(int) * std::max<size_t>(b.n_used, 1));
+        if (e == hipSuccess) e = hipMalloc(&b.d_w, sizeof(float) * d.n_proj);
+        if (e == hipSuccess && b.n_used)
+            e = hipMemcpy(b.d_tab, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(b.d_mask, m.data(), sizeof(int2) * d.width, hipMemcpyHostToDevice);
+        if (e == hipSuccess && b.n_used)
+            e = hipMemcpy(b.d_idx, idx.data(), sizeof(int) * b.n_used, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(b.d_w, d.weights, sizeof(float) * d.n_proj, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            batch_free(p);
+            return mr::hip_fail(e, "mr_batch_plan_create", __FILE__, __LINE__);
+        }
+        mr::BpArgs &a = b.args;
+        a.sino = b.d_sino; a.vol = d.d_volume; a.tab = b.d_tab; a.mask = b.d_mask;
+        a.nu = (int)d.width; a.nv = (int)d.slices; a.vpad = (int)b.vpad; a.nk = (int)b.n_used;
+        a.nz4 = (int)((d.slices / 4) * 4); a.bstride = (int)b.vpad; a.accumulate = 0;
+        p->updates += (double)d.width * d.width * d.slices * d.n_proj;
+        // work items, cost ~ projections x voxels inside the circle x slices
+        const int tiles = (int)((d.width + 15) / 16);
+        for (int zt = 0; zt < 2; ++zt) {
+            const int ZT = zt == 0 ? 16 : 8;
+            const int zc = (int)((d.slices + ZT - 1) / ZT);
+            for (int bz = 0; bz < zc; ++bz)
+                for (int by = 0; by < tiles; ++by)
+                    for (int bx = 0; bx < tiles; ++bx) {
+                        double inside = 0;
+                        for (int yy = by * 16; yy < std::min<int>((int)d.width, by * 16 + 16); ++yy) {
+                            const int lo = std::max(m[yy].x, bx * 16), hi = std::min(m[yy].y, bx * 16 + 16);
+                            inside += std::max(0, hi - lo);
+                        }
+                        const double cost = 1.0 + inside * (double)b.n_used;
+                        (zt == 0 ? items16 : items8).push_back({cost, make_int4((int)i, bx, by, bz)});
+                    }
+        }
+    }
+    for (int li = 0; li < 2; ++li) {
+        auto &v = li == 0 ? items16 : items8;
+        std::stable_sort(v.begin(), v.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+        std::vector<int4> flat(v.size());
+        for (size_t k = 0; k < v.size(); ++k) flat[k] = v[k].second;
+        p->n_items[li] = (int)flat.size();
+        hipError_t e = hipMalloc(&p->d_items[li], sizeof(int4) * std::max<size_t>(flat.size(), 1));
+        if (e == hipSuccess)
+            e = hipMemcpy(p->d_items[li], flat.data(), sizeof(int4) * flat.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            batch_free(p);
+            return mr::hip_fail(e, "mr_batch_plan_create items", __FILE__, __LINE__);
+        }
+    }
+    std::vector<mr::BpArgs> args(n_stacks);
+    for (size_t i = 0; i < n_stacks; ++i) args[i] = p->stacks[i].args;
+    hipError_t e = hipMalloc(&p->d_args, sizeof(mr::BpArgs) * n_stacks);
+    if (e == hipSuccess) e = hipMemcpy(p->d_args, args.data(), sizeof(mr::BpArgs) * n_stacks, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&p->d_counter, sizeof(int));
+    if (e != hipSuccess) {
+        batch_free(p);
+        return mr::hip_fail(e, "mr_batch_plan_create args", __FILE__, __LINE__);
+    }
+    int rc = MR_OK;
+    if (rc == MR_OK) rc = occupancy_grid<16, 16>(p->grid[0]);
+    if (rc == MR_OK) rc = occupancy_grid<16, 8>(p->grid[1]);
+    if (rc == MR_OK) rc = occupancy_grid<16, 32>(p->grid[2]);
+    if (rc == MR_OK) rc = occupancy_grid<8, 16>(p->grid[3]);
+    if (rc == MR_OK) rc = occupancy_grid<8, 32>(p->grid[4]);
+    if (rc != MR_OK) {
+        batch_free(p);
+        return rc;
+    }
+    if (tune) {
+        // Autotuner: time each (slice tile, projection chunk) variant on the
+        // real batch and keep the fastest.
+        hipStream_t s = nullptr;
+        hipEvent_t a0, a1;
+        MR_HIP(hipEventCreate(&a0));
+        MR_HIP(hipEventCreate(&a1));
+        rc = batch_prepare(p, s);
+        float best = 1e30f;
+        const int nv = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
+        for (int v = 0; v < nv && rc == MR_OK; ++v) {
+            rc = batch_launch_bp(p, v, s);  // warm-up
+            if (rc != MR_OK) break;
+            MR_HIP(hipEventRecord(a0, s));
+            rc = batch_launch_bp(p, v, s);
+            MR_HIP(hipEventRecord(a1, s));
+            MR_HIP(hipEventSynchronize(a1));
+            float ms = 0.0f;
+            MR_HIP(hipEventElapsedTime(&ms, a0, a1));
+            if (ms < best) {
+                best = ms;
+                p->variant = v;
+            }
+        }
+        (void)hipEventDestroy(a0);
+        (void)hipEventDestroy(a1);
+        if (rc != MR_OK) {
+            batch_free(p);
+            return rc;
+        }
+    }
+    *plan = p;
+    return MR_OK;
+    MR_GUARD_END
+}
+
+extern "C" int mr_batch_plan_info(const mr_batch_plan *plan, int *slice_tile, int *proj_chunk,
+                                  size_t *n_items, double *updates)
+{
+    MR_GUARD_BEGIN
+    mr::clear_error();
+    MR_REQUIRE(plan != nullptr, "plan is NULL");
+    const Variant v = kVariants[plan->variant];
+    if (slice_tile) *slice_tile = v.zt;
+    if (proj_chunk) *proj_chunk = v.kc;
+    if (n_items) *n_items = (size_t)plan->n_items[v.zt == 16 ? 0 : 1];
+    if (updates) *updates = plan->updates;
+    return MR_OK;
+    MR_GUARD_END
+}
+
+extern "C" int mr_batch_run(mr_batch_plan *plan, void *stream)
+{
+    MR_GUARD_BEGIN
+    mr::clear_error();
+    MR_REQUIRE(plan != nullptr, "plan is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = batch_prepare(plan, s);
+    if (rc != MR_OK) return rc;
+    return batch_launch_bp(plan, plan->variant, s);
+    MR_GUARD_END
+}
+
+extern "C" int mr_batch_run_graph(mr_batch_plan *plan, void *stream)
+{
+    MR_GUARD_BEGIN
+    mr::clear_error();
+    MR_REQUIRE(plan != nullptr, "plan is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    MR_REQUIRE(s != nullptr, "graph launches need a non-default stream");
+    if (!plan->exec) {
+        MR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int rc = batch_prepare(plan, s);
+        if (rc == MR_OK) rc = batch_launch_bp(plan, plan->variant, s);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(s, &g);
+        if (rc != MR_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return mr::hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+        plan->graph = g;
+        MR_HIP(hipGraphInstantiate(&plan->exec, g, nullptr, nullptr, 0));
+    }
+    mr::ProfScope prof("bp_batch_graph", s);
+    MR_HIP(hipGraphLaunch(plan->exec, s));
+    return MR_OK;
+    MR_GUARD_END
+}
+
+extern "C" int mr_batch_run_sequential(mr_batch_plan *plan, void *stream)
+{
+    MR_GUARD_BEGIN
+    mr::clear_error();
+    MR_REQUIRE(plan != nullptr, "plan is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = batch_prepare(plan, s);
+    if (rc != MR_OK) return rc;
+    for (auto &b : plan->stacks) {
+        rc = mr::run_bp(b.args, 0, 1, 1, s);
+        if (rc != MR_OK) return rc;
+    }
+    return MR_OK;
+    MR_GUARD_END
+}
+
+extern "C" int mr_batch_plan_destroy(mr_batch_plan *plan)
+{
+    MR_GUARD_BEGIN
+    if (plan) batch_free(plan);
+    return MR_OK;
     MR_GUARD_END
 }

@@ -20,9 +20,18 @@ struct RowFilterArgs {
     float post0, post1, post2;  // sequential output multipliers
     const float *pre_w;         // mode 1: per-projection input weight (nullable)
     int rows_per_proj;          // rows of one projection (mode 1)
+    float *out;                 // destination rows (nullptr: in place)
 };
 
 int run_row_filter(const RowFilterArgs &a, int P, hipStream_t s);
+}  // namespace mr
+
+struct mr_filter_plan;
+
+namespace mr {
+// ProjectionFilterSingle from `in` into `out` (out may equal in).
+int filter_rows_to(mr_filter_plan *plan, const float *in, float *out, size_t n_rows, hipStream_t s);
+size_t filter_width(const mr_filter_plan *plan);
 int run_fdk_margin_rows(float *img, int n, int h, long n_proj, hipStream_t s);
 std::vector<float> make_twiddles(size_t P);
 // Zero-fills n floats with a kernel (no hipMemset, so callers may capture it in a graph).

@@ -77,17 +77,19 @@ __global__ __launch_bounds__(NT) void row_filter_kernel(RowFilterArgs a)
     }
     __syncthreads();
     fft_lds<P, NT, true>(line, a.tw);
+    float *o0 = a.out ? a.out + r0 * a.n : row0;
+    float *o1 = o0 + a.n;
     if (a.mode == 0) {
         for (int i = t; i < a.n; i += NT) {
             const float2 v = line[i + a.insert];
-            row0[i] = v.x * a.post0;
-            if (has1) row1[i] = v.y * a.post0;
+            o0[i] = v.x * a.post0;
+            if (has1) o1[i] = v.y * a.post0;
         }
     } else {
         for (int i = t; i < a.n; i += NT) {
             const float2 v = line[i + a.insert];
-            row0[i] = ((v.x * a.post0) * a.post1) * a.post2;
-            if (has1) row1[i] = ((v.y * a.post0) * a.post1) * a.post2;
+            o0[i] = ((v.x * a.post0) * a.post1) * a.post2;
+            if (has1) o1[i] = ((v.y * a.post0) * a.post1) * a.post2;
         }
     }
 }
@@ -252,17 +254,25 @@ extern "C" int mr_filter_plan_info(const mr_filter_plan *plan, size_t *fft_size,
     MR_GUARD_END
 }
 
+int mr::filter_rows_to(mr_filter_plan *plan, const float *in, float *out, size_t n_rows, hipStream_t s)
+{
+    if (n_rows == 0) return MR_OK;
+    mr::RowFilterArgs a{};
+    a.rows = const_cast<float *>(in); a.n_rows = (long)n_rows; a.n = (int)plan->width; a.mode = 0;
+    a.insert = (int)plan->n_insert; a.taper = plan->d_taper; a.hs = plan->d_hs; a.tw = plan->d_tw;
+    a.post0 = plan->scale; a.post1 = 1.0f; a.post2 = 1.0f; a.pre_w = nullptr; a.rows_per_proj = 1;
+    a.out = out;
+    return mr::run_row_filter(a, (int)plan->P, s);
+}
+
+size_t mr::filter_width(const mr_filter_plan *plan) { return plan->width; }
+
 extern "C" int mr_filter_rows(mr_filter_plan *plan, float *d_rows, size_t n_rows, void *stream)
 {
     MR_GUARD_BEGIN
     mr::clear_error();
     MR_REQUIRE(plan != nullptr, "plan is NULL");
     MR_REQUIRE(d_rows != nullptr || n_rows == 0, "rows pointer is NULL");
-    if (n_rows == 0) return MR_OK;
-    mr::RowFilterArgs a{};
-    a.rows = d_rows; a.n_rows = (long)n_rows; a.n = (int)plan->width; a.mode = 0;
-    a.insert = (int)plan->n_insert; a.taper = plan->d_taper; a.hs = plan->d_hs; a.tw = plan->d_tw;
-    a.post0 = plan->scale; a.post1 = 1.0f; a.post2 = 1.0f; a.pre_w = nullptr; a.rows_per_proj = 1;
-    return mr::run_row_filter(a, (int)plan->P, (hipStream_t)stream);
+    return mr::filter_rows_to(plan, d_rows, d_rows, n_rows, (hipStream_t)stream);
     MR_GUARD_END
 }

@@ -36,6 +36,8 @@ std::atomic<bool> g_prof_on{false};
 ProfScope::ProfScope(const char *name, hipStream_t s) : slot_(-1), s_(s)
 {
     if (!g_prof_on.load()) return;
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &st) != hipSuccess || st != hipStreamCaptureStatusNone) return;
     ProfRec r{name, nullptr, nullptr};
     if (hipEventCreate(&r.a) != hipSuccess) return;
     if (hipEventCreate(&r.b) != hipSuccess) { (void)hipEventDestroy(r.a); return; }

@@ -137,3 +137,31 @@ def voxelize(ells: Sequence[Ellipsoid], width: int, slices: int, device="cpu"):
         inside = (q * (q @ A.T)).sum(-1) <= 1.0
         vol = vol + e.mu * inside.double()
     return vol.to(torch.float32)
+
+
+def cone_projections(ells: Sequence[Ellipsoid], n_det_u: int, n_det_v: int, angles_deg, pm_fn,
+                     ic, fov_mm: float, device="cpu", dtype=None):
+    """Noise-free cone-beam line integrals [n_proj, n_det_v, n_det_u] (mu in
+    1/cm, lengths in mm -> 0.1 * mm).  ``pm_fn(angle)`` returns the 3x4
+    projection matrix (FDKbp_single::getProjMatrix); detector column c and
+    row r of the window satisfy (c - ic0, r - ic1) = (p0/p2, p1/p2).  The
+    phantom is scaled to a field of view ``fov_mm`` centred on the rotation
+    axis."""
+    import torch
+    dtype = dtype or torch.float64
+    out = torch.empty((len(angles_deg), n_det_v, n_det_u), dtype=torch.float32, device=device)
+    cc = torch.arange(n_det_u, dtype=dtype, device=device) - float(ic[0])
+    rr = torch.arange(n_det_v, dtype=dtype, device=device) - float(ic[1])
+    R, C = torch.meshgrid(rr, cc, indexing="ij")
+    q = torch.stack([C, R, torch.ones_like(C)], dim=-1)
+    scale = (fov_mm, fov_mm, fov_mm)
+    for i, ang in enumerate(angles_deg):
+        P = np.asarray(pm_fn(float(ang)), np.float64).reshape(3, 4)
+        Minv = np.linalg.inv(P[:, :3])
+        src = -Minv @ P[:, 3]
+        Mt = torch.tensor(Minv, dtype=dtype, device=device)
+        d = q @ Mt.T
+        d = d / torch.linalg.norm(d, dim=-1, keepdim=True)
+        p0 = torch.tensor(src, dtype=dtype, device=device).expand_as(d)
+        out[i] = (_chords(p0, d, ells, scale, torch, dtype) * 0.1).to(torch.float32)
+    return out

@@ -238,3 +238,61 @@ class FDKbp_single:
             self._plan = ctypes.c_void_p()
 
     __del__ = close
+
+
+class BatchReconstructor:
+    """Many parallel-beam stacks of uneven geometry in one persistent launch
+    (mr_batch_*).  ``stacks`` is a list of dicts with keys proj (CUDA
+    [n_proj, slices, width]), angles, weights, center and optionally
+    filtertype, cutoff, direction, buffer_size; volumes are allocated here."""
+
+    def __init__(self, stacks, tune: bool = True):
+        import torch
+        L = _lib.load()
+        self._keep = []
+        arr = (_lib.BatchStack * len(stacks))()
+        self.volumes = []
+        self.updates = 0
+        for i, s in enumerate(stacks):
+            proj = s["proj"]
+            _require_cuda_f32(proj, 3, "proj")
+            n, nv, nu = proj.shape
+            a = _f32arr(s["angles"], n)
+            w = _f32arr(s["weights"], n)
+            vol = torch.empty((nu, nu, nv), dtype=torch.float32, device=proj.device)
+            self._keep += [a, w, proj]
+            self.volumes.append(vol)
+            d = arr[i]
+            d.width, d.slices, d.n_proj = nu, nv, n
+            d.center, d.direction = float(s["center"]), int(s.get("direction", 0))
+            d.filter_type = _lib.FILTER_TYPES[s.get("filtertype", "Hamming").lower()]
+            d.cutoff, d.order = float(s.get("cutoff", 0.5)), float(s.get("order", 1.0))
+            d.use_bias, d.bias_weight = int(s.get("usebias", True)), float(s.get("biasweight", 0.1))
+            d.padding_doubler = int(s.get("paddingdoubler", 2))
+            d.buffer_size = int(s.get("buffer_size", 16))
+            d.d_proj = proj.data_ptr()
+            d.angles = a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+            d.weights = w.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+            d.d_volume = vol.data_ptr()
+        self._plan = ctypes.c_void_p()
+        check(L.mr_batch_plan_create(ctypes.byref(self._plan), arr, len(stacks), int(bool(tune))))
+        zt, kc, ni, upd = ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t(), ctypes.c_double()
+        check(L.mr_batch_plan_info(self._plan, ctypes.byref(zt), ctypes.byref(kc), ctypes.byref(ni),
+                                   ctypes.byref(upd)))
+        self.slice_tile, self.proj_chunk, self.n_items, self.updates = zt.value, kc.value, ni.value, upd.value
+
+    def run(self, stream=None):
+        check(_lib.load().mr_batch_run(self._plan, _stream(stream)))
+
+    def run_graph(self, stream=None):
+        check(_lib.load().mr_batch_run_graph(self._plan, _stream(stream)))
+
+    def run_sequential(self, stream=None):
+        check(_lib.load().mr_batch_run_sequential(self._plan, _stream(stream)))
+
+    def close(self):
+        if getattr(self, "_plan", None) and self._plan.value:
+            _lib.load().mr_batch_plan_destroy(self._plan)
+            self._plan = ctypes.c_void_p()
+
+    __del__ = close

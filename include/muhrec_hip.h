@@ -163,6 +163,46 @@ int mr_fdk_bp(mr_fdk_plan *plan, float *d_proj, size_t n_proj, size_t win_rows,
               const float *angles, const float *weights, float *d_volume, void *stream);
 
 /* ------------------------------------------------------------------------
+ * Batched reconstruction of many parallel-beam stacks of uneven geometry
+ * (no reference counterpart: the reference reconstructs one stack per engine
+ * run, ReconEngine::Run3DFull, frameworks/tomography/Framework/ReconFramework/src/ReconEngine.cpp:838-1185).
+ * Each stack is filtered (ProjectionFilterSingle, out of place: d_proj is not
+ * modified), weighted and back-projected (MultiProjBPparallel order of
+ * operations, bit-identical per stack).  mr_batch_run enqueues kernels only
+ * and ends with ONE persistent back-projection launch over a work queue of
+ * all stacks' tiles; mr_batch_run_graph captures it into a hipGraph on the
+ * first call and replays it after; mr_batch_run_sequential launches one
+ * plain back-projection per stack (the comparison baseline).
+ * Device buffers are bound at plan creation.
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    size_t width, slices, n_proj;   /* detector width, rows, projections */
+    float center;                   /* fCenter (ROI-relative) */
+    int direction;                  /* 0 CW, 1 CCW */
+    int filter_type;                /* mr_filter_type */
+    float cutoff, order;
+    int use_bias;
+    float bias_weight;
+    size_t padding_doubler;
+    int buffer_size;                /* ProjectionBufferSize drop (0 = all) */
+    const float *d_proj;            /* [n_proj][slices][width] raw projections */
+    const float *angles, *weights;  /* host, n_proj each */
+    float *d_volume;                /* [width][width][slices] */
+} mr_batch_stack;
+
+typedef struct mr_batch_plan mr_batch_plan;
+
+/* tune != 0: the autotuner times every (slice tile, projection chunk) variant
+ * of the persistent kernel on this batch and keeps the fastest. */
+int mr_batch_plan_create(mr_batch_plan **plan, const mr_batch_stack *stacks, size_t n_stacks, int tune);
+int mr_batch_plan_info(const mr_batch_plan *plan, int *slice_tile, int *proj_chunk, size_t *n_items,
+                       double *updates);
+int mr_batch_run(mr_batch_plan *plan, void *stream);
+int mr_batch_run_graph(mr_batch_plan *plan, void *stream);
+int mr_batch_run_sequential(mr_batch_plan *plan, void *stream);
+int mr_batch_plan_destroy(mr_batch_plan *plan);
+
+/* ------------------------------------------------------------------------
  * Matched Joseph projector pair for gradient-based reconstruction.
  * No reference counterpart: the reference's only forward projector,
  * LinearForwardProjector::project

@@ -16,21 +16,11 @@ import torch
 from imagingsuite_amd import _lib
 
 
-class JosephGeom(ctypes.Structure):
-    _fields_ = [("nx", ctypes.c_size_t), ("nz", ctypes.c_size_t), ("n_det", ctypes.c_size_t),
-                ("center", ctypes.c_float), ("direction", ctypes.c_int)]
+JosephGeom = _lib.JosephGeom
 
 
 def _lib_j():
-    L = _lib.load()
-    if not getattr(L, "_joseph_ready", False):
-        fp = ctypes.POINTER(ctypes.c_float)
-        for f in (L.mr_joseph_forward, L.mr_joseph_adjoint):
-            f.argtypes = [ctypes.POINTER(JosephGeom), ctypes.c_void_p, fp, ctypes.c_size_t,
-                          ctypes.c_void_p, ctypes.c_void_p]
-            f.restype = ctypes.c_int
-        L._joseph_ready = True
-    return L
+    return _lib.load()
 
 
 class Geometry:
