@@ -297,10 +297,9 @@ static int launch_bp(const BpArgs &a, hipStream_t s)
     return MR_OK;
 }
 
-template <int ZT, int INTERP, bool EXACT>
+template <int ZT, int INTERP, bool EXACT, int KC>
 static int launch_bp_lds(const BpArgs &a, hipStream_t s)
 {
-    constexpr int KC = 16;
     dim3 grid((unsigned)((a.nu + 15) / 16), (unsigned)((a.nu + 15) / 16),
               (unsigned)((a.nv + ZT - 1) / ZT));
     ProfScope prof("bp_parallel", s);
@@ -309,17 +308,46 @@ static int launch_bp_lds(const BpArgs &a, hipStream_t s)
     return MR_OK;
 }
 
+template <int ZT, int KC>
+static int launch_bp_lds_mode(const BpArgs &a, int interp, int exact, hipStream_t s)
+{
+    if (interp == 1) return launch_bp_lds<ZT, 1, true, KC>(a, s);
+    return exact ? launch_bp_lds<ZT, 0, true, KC>(a, s) : launch_bp_lds<ZT, 0, false, KC>(a, s);
+}
+
+// Slice tile / projection chunk of the LDS kernel (MR_BP_ZT / MR_BP_KC
+// override the defaults for tuning experiments).
+static void lds_shape(int nv, int &zt, int &kc)
+{
+    zt = nv >= 16 ? 16 : 8;
+    kc = 16;
+    if (const char *e = std::getenv("MR_BP_ZT")) {
+        const int v = std::atoi(e);
+        if ((v == 32 && nv >= 32) || v == 16 || v == 8) zt = v;
+    }
+    if (const char *e = std::getenv("MR_BP_KC")) {
+        const int v = std::atoi(e);
+        if (v == 8 || v == 16 || v == 32) kc = v;
+    }
+    if (zt == 32) kc = 8;
+}
+
 // impl: 0 = direct global gathers, 1 = LDS-staged tiles.  exact: bit-faithful
 // float order (otherwise fused multiply-adds).
 int run_bp(const BpArgs &a, int interp, int impl, int exact, hipStream_t s)
 {
     if (impl == 1) {
-        if (a.nv >= 16) {
-            if (interp == 1) return launch_bp_lds<16, 1, true>(a, s);
-            return exact ? launch_bp_lds<16, 0, true>(a, s) : launch_bp_lds<16, 0, false>(a, s);
+        int zt, kc;
+        lds_shape(a.nv, zt, kc);
+        if (zt == 32) return launch_bp_lds_mode<32, 8>(a, interp, exact, s);
+        if (zt == 16) {
+            if (kc == 8) return launch_bp_lds_mode<16, 8>(a, interp, exact, s);
+            if (kc == 32) return launch_bp_lds_mode<16, 32>(a, interp, exact, s);
+            return launch_bp_lds_mode<16, 16>(a, interp, exact, s);
         }
-        if (interp == 1) return launch_bp_lds<8, 1, true>(a, s);
-        return exact ? launch_bp_lds<8, 0, true>(a, s) : launch_bp_lds<8, 0, false>(a, s);
+        if (kc == 8) return launch_bp_lds_mode<8, 8>(a, interp, exact, s);
+        if (kc == 32) return launch_bp_lds_mode<8, 32>(a, interp, exact, s);
+        return launch_bp_lds_mode<8, 16>(a, interp, exact, s);
     }
     if (a.nv >= 16) return interp == 0 ? launch_bp<16, 0>(a, s) : launch_bp<16, 1>(a, s);
     if (a.nv >= 8) return interp == 0 ? launch_bp<8, 0>(a, s) : launch_bp<8, 1>(a, s);
