@@ -145,7 +145,8 @@ def bench_parallel(args):
         pr_ms, pr_n = _lib.profile_query("bp_prep")
         _lib.profile(False)
         assert torch.isfinite(vol).all().item()
-        res[fused] = dict(ms=ms, bp=bp_ms / max(bp_n, 1), filt=fl_ms / max(fl_n, 1), prep=pr_ms / max(pr_n, 1))
+        res[fused] = dict(ms=ms, bp=bp_ms / max(bp_n, 1), filt=fl_ms / max(fl_n, 1), prep=pr_ms / max(pr_n, 1),
+                          kernel=bp.kernel)
         del bp
     head = res[True] if True in res else res[False]
     updates = nu * nu * nv * n_p
@@ -157,7 +158,10 @@ def bench_parallel(args):
     tpath = os.path.join(ROOT, "profiles", f"bp_parallel_traffic_c{args.config}.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tj = json.load(f)
+        # counters are only valid for the kernel variant they were taken on
+        if tj.get("kernel") == head["kernel"]:
+            traffic = tj.get("hbm_bytes_per_launch")
     r = base_result("GUPS (voxel*proj updates/s) FBP 2048^2x1500 proj" if args.config == 2 else
                     f"GUPS (voxel*proj updates/s) FBP config {args.config}",
                     round(gups, 2), "GUPS", args.steps, args.warmup, head["ms"], "f32",
@@ -168,7 +172,7 @@ def bench_parallel(args):
     r["roofline"] = {"bound": "mfma", "pipe": "fp32 VALU (no MFMA: not a contraction)",
                      "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
-                     "kernel": "bp_lds_kernel<16,0,%s,16>" % ("false" if True in res else "true"),
+                     "kernel": head["kernel"],
                      "kernel_ms": round(head["bp"], 3), "flop_per_update": BP_FLOP_PER_UPDATE,
                      "hbm_algorithmic_bytes": bp_bytes,
                      "hbm_frac": round(bp_bytes / (head["bp"] / 1e3) / 1e9 / PEAK_HBM_GBS, 5)}
@@ -177,7 +181,7 @@ def bench_parallel(args):
     if False in res and True in res:
         ex = res[False]
         r["exact_mode"] = {"value": round(updates / (ex["ms"] / 1e3) / 1e9, 2), "ms_per_step": round(ex["ms"], 3),
-                           "bp_kernel_ms": round(ex["bp"], 3),
+                           "bp_kernel_ms": round(ex["bp"], 3), "kernel": ex["kernel"],
                            "note": "reference float order per voxel, bit-identical to MultiProjBPparallel"}
     if not args.no_cpu_baseline:
         host = pristine[:args.cpu_sample_proj, :args.cpu_sample_slices].cpu().numpy()

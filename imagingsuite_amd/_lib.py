@@ -24,7 +24,7 @@ EXPORTED_SYMBOLS = (
     "mr_last_error", "mr_version", "mr_device_count",
     "mr_profile_enable", "mr_profile_reset", "mr_profile_query",
     "mr_filter_plan_create", "mr_filter_plan_destroy", "mr_filter_plan_info", "mr_filter_rows",
-    "mr_bp_plan_create", "mr_bp_plan_destroy", "mr_bp_plan_mask", "mr_bp_parallel",
+    "mr_bp_plan_create", "mr_bp_plan_destroy", "mr_bp_plan_mask", "mr_bp_plan_kernel", "mr_bp_parallel",
     "mr_fbp_parallel",
     "mr_fdk_window", "mr_fdk_plan_create", "mr_fdk_plan_destroy", "mr_fdk_plan_window", "mr_fdk_bp",
     "mr_joseph_forward", "mr_joseph_adjoint",
@@ -70,7 +70,9 @@ class FdkParams(ctypes.Structure):
                 ("ppoint", ctypes.c_float * 2), ("center", ctypes.c_float),
                 ("direction", ctypes.c_int), ("roi", ctypes.c_size_t * 4),
                 ("projection_roi3", ctypes.c_size_t), ("use_roi", ctypes.c_int),
-                ("matrix_roi", ctypes.c_size_t * 4)]
+                ("matrix_roi", ctypes.c_size_t * 4), ("module", ctypes.c_int)]
+
+FDK_MODULES = {"FDKbp_single": 0, "FDKbp": 1}
 
 
 _lib = None
@@ -103,6 +105,7 @@ def load():
     L.mr_bp_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(BpParams)]
     L.mr_bp_plan_destroy.argtypes = [vp]
     L.mr_bp_plan_mask.argtypes = [vp, szp, szp]
+    L.mr_bp_plan_kernel.argtypes = [vp, ctypes.c_char_p, sz]
     L.mr_bp_parallel.argtypes = [vp, vp, sz, fp, fp, vp, i32, vp]
     L.mr_fbp_parallel.argtypes = [vp, vp, vp, sz, fp, fp, vp, vp]
     L.mr_fdk_window.argtypes = [ctypes.POINTER(FdkParams), szp]

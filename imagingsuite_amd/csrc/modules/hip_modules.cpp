@@ -6,6 +6,7 @@
 //   MultiProjBPparallel, MultiProjBP, NearestNeighborBP
 //                           frameworks/tomography/Backprojectors/StdBackProjectors/src/StdBackProjectorBase.cpp:21-342
 //   FDKbp_single            frameworks/tomography/Backprojectors/FDKBackProjectors/src/fdkreconbase.cpp:20-330
+//   FDKbp                   .../FDKBackProjectors/src/fdkbp.cpp, fdkbackproj.cpp:29-30
 //   GetModule/Destroy/LibVersion/GetModuleList
 //                           .../StdBackProjectors/src/StdBackProjectors.cpp:25-88,
 //                           .../StdPreprocModules/src/StdPreprocModules.cpp:40-170
@@ -390,16 +391,16 @@ private:
 };
 
 // -------------------------------------------------------------------------
-// FDKbp_single
+// FDKbp_single (float geometry) and FDKbp (double geometry)
 // -------------------------------------------------------------------------
-class HipFdkSingle : public BackProjectorModuleBase {
+class HipFdk : public BackProjectorModuleBase {
 public:
-    explicit HipFdkSingle(kipl::interactors::InteractionBase *interactor)
-        : BackProjectorModuleBase("muhrec", "FDKbp_single", MatrixXYZ, interactor)
+    HipFdk(const std::string &name, int module, kipl::interactors::InteractionBase *interactor)
+        : BackProjectorModuleBase("muhrec", name, MatrixXYZ, interactor), module_(module)
     {
         nSubVolume[0] = nSubVolume[1] = 1;
     }
-    ~HipFdkSingle() override
+    ~HipFdk() override
     {
         if (plan_) mr_fdk_plan_destroy(plan_);
     }
@@ -457,6 +458,7 @@ public:
         q.projection_roi3 = P.projection_roi[3];
         q.use_roi = mConfig.MatrixInfo.bUseROI ? 1 : 0;
         for (int i = 0; i < 4; ++i) q.matrix_roi[i] = mConfig.MatrixInfo.roi[i];
+        q.module = module_;
         MRM_CHECK(mr_fdk_plan_create(&plan_, &q));
     }
 
@@ -511,11 +513,12 @@ public:
 private:
     size_t SizeU = 0, SizeV = 0;
     size_t nProjectionBufferSize = 2000, nSliceBlock = 32, nSubVolume[2];
+    int module_ = MR_FDK_SINGLE;
     mr_fdk_plan *plan_ = nullptr;
     DevBuf dproj_, dvol_;
 };
 
-const char *kBpNames[] = {"MultiProjBP", "MultiProjBPparallel", "NearestNeighborBP", "FDKbp_single"};
+const char *kBpNames[] = {"MultiProjBP", "MultiProjBPparallel", "NearestNeighborBP", "FDKbp_single", "FDKbp"};
 
 }  // namespace
 
@@ -536,7 +539,8 @@ extern "C" void *GetModule(const char *application, const char *name, void *vint
             if (n == "MultiProjBP") return static_cast<BackProjectorModuleBase *>(new HipStdBackProjector(n, 0, 1, interactor));
             if (n == "MultiProjBPparallel") return static_cast<BackProjectorModuleBase *>(new HipStdBackProjector(n, 0, 0, interactor));
             if (n == "NearestNeighborBP") return static_cast<BackProjectorModuleBase *>(new HipStdBackProjector(n, 1, 0, interactor));
-            if (n == "FDKbp_single") return static_cast<BackProjectorModuleBase *>(new HipFdkSingle(interactor));
+            if (n == "FDKbp_single") return static_cast<BackProjectorModuleBase *>(new HipFdk(n, MR_FDK_SINGLE, interactor));
+            if (n == "FDKbp") return static_cast<BackProjectorModuleBase *>(new HipFdk(n, MR_FDK_DOUBLE, interactor));
         }
     } catch (...) {
         return nullptr;

@@ -19,7 +19,9 @@
 #include "mr_host.h"
 
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 namespace mr {
@@ -56,7 +58,42 @@ struct BpArgs {
     const float4 *tab;   // (fStartU, fCos, fSin, 0) per used projection
     const int2 *mask;    // [first, second) per y, 0-based x
     int nu, nv, vpad, nk, nz4, bstride, accumulate;
+    int order;           // block -> tile mapping (tile_of_block)
+    int ntx, nty, ntz;   // tiles along x, y and slice chunks
+    int sw;              // stripe width (tiles) of the XCD-striped order
 };
+
+// Block -> tile mapping of the tiled kernels.  order 0: the 3-D grid itself.
+// order 1: a 1-D grid.  Consecutive blocks are dispatched to consecutive XCDs
+// (round robin over the 8 XCDs), so XCD c runs blocks c, c+8, c+16, ...;
+// those take the c-th contiguous eighth of a tile list ordered by slice
+// chunk, then stripes of sw tile columns, then rows inside a stripe.  The
+// blocks resident on one XCD at a time then cover a compact patch of the
+// plane, and the projection spans they stage overlap in that XCD's L2.
+// Blocks past the end of the list return before touching anything.
+__device__ __forceinline__ bool tile_of_block(const BpArgs &a, int &bx, int &by, int &bz)
+{
+    if (a.order == 0) {
+        bx = blockIdx.x;
+        by = blockIdx.y;
+        bz = blockIdx.z;
+        return true;
+    }
+    const int n = a.ntx * a.nty * a.ntz;
+    const int per = (n + 7) >> 3;
+    const int b = (int)blockIdx.x;
+    const int t = (b & 7) * per + (b >> 3);
+    if (t >= n) return false;
+    const int plane = a.ntx * a.nty;
+    bz = t / plane;
+    const int r = t - bz * plane;
+    const int s = r / (a.sw * a.nty);
+    const int q = r - s * a.sw * a.nty;
+    const int w = min(a.sw, a.ntx - a.sw * s);
+    by = q / w;
+    bx = a.sw * s + (q - by * w);
+    return true;
+}
 
 template <int ZT, int INTERP>
 __global__ __launch_bounds__(256) void bp_kernel(BpArgs a)
@@ -257,7 +294,170 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(BpArgs a)
 {
     __shared__ float sm[KC][kBpSpan][ZT + 4];
     __shared__ int sb0[KC];
-    bp_lds_tile<ZT, INTERP, EXACT, KC>(a, blockIdx.x, blockIdx.y, blockIdx.z, sm, sb0);
+    int bx, by, bz;
+    if (!tile_of_block(a, bx, by, bz)) return;
+    bp_lds_tile<ZT, INTERP, EXACT, KC>(a, bx, by, bz, sm, sb0);
+}
+
+// Quad variant (linear interpolation only).  A block owns a 32 x 32 voxel
+// tile and ZT slices; each thread owns a 2 x 2 group of voxel columns.  The
+// four detector positions of a group differ by at most |sin| + |cos| <= sqrt 2,
+// so their truncated bins n lie in {nb, nb+1, nb+2} and the group needs only
+// the four staged columns nb..nb+3: four 16-byte LDS reads serve 16 updates
+// where the one-column kernel needs eight.  Each voxel gets weight 1-w on its
+// column n and w on n+1, zero on the other two:
+//   fused:  acc = fma(c3,w3, fma(c2,w2, fma(c1,w1, fma(c0,w0, acc))))
+//           which, the zero terms being exact, equals fma(B,w,fma(A,1-w,acc));
+//   exact:  acc = acc + ((c0*w0 + c1*w1) + (c2*w2 + c3*w3))
+//           which equals the reference's acc + (A*(1-w) + B*w) bit for bit
+//           (x + 0 and 0 + x are exact), MultiProjBPparallel.cpp:149-161.
+// Both assume finite projection values (Inf * 0 would give NaN).
+// The tile's columns span at most 31*sqrt 2 + 1 < 45 bins, and a group reads
+// up to 3 past its lowest bin: 48 staged columns plus 2 of margin.
+constexpr int kQuadSpan = 50;
+
+template <int ZT, bool EXACT, int KC>
+__device__ __forceinline__ void bp_quad_tile(const BpArgs &a, int bx, int by, int bz,
+                                             float (&sm)[KC][kQuadSpan][ZT + 4], int (&sb0)[KC])
+{
+    constexpr int T = 32;
+    constexpr int SPAN = kQuadSpan;
+    constexpr int RS = ZT + 4;
+    constexpr int G = ZT / 4;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int x0 = bx * T, y0 = by * T;
+    const int xq = x0 + 2 * tx, yq = y0 + 2 * ty;   // group voxels (xq + (q&1), yq + (q>>1))
+    const int z0 = bz * ZT;
+    const int zn = min(ZT, a.nv - z0);
+    const int zl = max(0, min(ZT, a.nz4 - z0));
+    float acc[4][ZT];
+    bool in_vol[4], inside[4];
+    bool any = false;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int x = xq + (q & 1), y = yq + (q >> 1);
+        in_vol[q] = x < a.nu && y < a.nu;
+        const float *out = a.vol + ((size_t)min(y, a.nu - 1) * a.nu + min(x, a.nu - 1)) * a.nv + z0;
+#pragma unroll
+        for (int z = 0; z < ZT; ++z) acc[q][z] = (in_vol[q] && a.accumulate && z < zn) ? out[z] : 0.0f;
+        inside[q] = false;
+        if (in_vol[q]) {
+            const int2 m = a.mask[y];
+            inside[q] = x >= m.x && x < m.y;
+        }
+        any = any || inside[q];
+    }
+    const bool any_inside = __syncthreads_or(any ? 1 : 0) != 0;
+    if (any_inside && zl > 0) {
+        const float xf0 = (float)(xq + 1), xf1 = (float)(xq + 2);
+        const float yf0 = (float)(yq + 1), yf1 = (float)(yq + 2);
+        const int num2 = a.nu - 2;
+        for (int kb = 0; kb < a.nk; kb += KC) {
+            const int kc = min(KC, a.nk - kb);
+            if (threadIdx.x < kc) {
+                const float4 t = a.tab[kb + threadIdx.x];
+                const float ya = (float)(y0 + 1), yb = (float)(y0 + T);
+                const float xa = (float)(x0 + 1), xb = (float)(x0 + T);
+                const float la = t.x + t.y * ya, lb = t.x + t.y * yb;
+                int n0 = min(min((int)(la - t.z * xa), (int)(la - t.z * xb)),
+                             min((int)(lb - t.z * xa), (int)(lb - t.z * xb)));
+                sb0[threadIdx.x] = max(n0, 0);
+            }
+            __syncthreads();
+            for (int e = threadIdx.x; e < kc * SPAN * G; e += 256) {
+                const int kk = e / (SPAN * G);
+                const int r = e - kk * SPAN * G;
+                const int b = r / G, g = r - b * G;
+                const int col = sb0[kk] + b;
+                float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (col < a.nu && z0 + 4 * g < a.vpad)
+                    v = *reinterpret_cast<const float4 *>(
+                        a.sino + ((size_t)(kb + kk) * a.nu + col) * a.vpad + z0 + 4 * g);
+                *reinterpret_cast<float4 *>(&sm[kk][b][4 * g]) = v;
+            }
+            __syncthreads();
+            if (any) {
+                for (int kk = 0; kk < kc; ++kk) {
+                    const float4 t = a.tab[kb + kk];
+                    const float l0 = t.x + t.y * yf0, l1 = t.x + t.y * yf1;
+                    const float u[4] = {l0 - t.z * xf0, l0 - t.z * xf1, l1 - t.z * xf0, l1 - t.z * xf1};
+                    int n[4];
+                    bool ok[4];
+                    int nb = 0x7fffffff;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        n[q] = (int)u[q];
+                        ok[q] = inside[q] && n[q] >= 0 && n[q] <= num2;
+                        if (ok[q]) nb = min(nb, n[q]);
+                    }
+                    if (nb == 0x7fffffff) continue;
+                    const int loc = min(max(nb - sb0[kk], 0), SPAN - 4);
+                    const int base = sb0[kk] + loc;
+                    float w0[4], w1[4], w2[4], w3[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int d = ok[q] ? n[q] - base : -1;
+                        const float wb = fabsf(u[q] - (float)n[q]);
+                        const float wa = 1.0f - wb;
+                        w0[q] = d == 0 ? wa : 0.0f;
+                        w1[q] = d == 0 ? wb : (d == 1 ? wa : 0.0f);
+                        w2[q] = d == 1 ? wb : (d == 2 ? wa : 0.0f);
+                        w3[q] = d == 2 ? wb : 0.0f;
+                    }
+                    const float *A = &sm[kk][loc][0];
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        if (4 * g < zl) {
+                            const float4 c0 = *reinterpret_cast<const float4 *>(A + 4 * g);
+                            const float4 c1 = *reinterpret_cast<const float4 *>(A + RS + 4 * g);
+                            const float4 c2 = *reinterpret_cast<const float4 *>(A + 2 * RS + 4 * g);
+                            const float4 c3 = *reinterpret_cast<const float4 *>(A + 3 * RS + 4 * g);
+                            const float r0[4] = {c0.x, c0.y, c0.z, c0.w}, r1[4] = {c1.x, c1.y, c1.z, c1.w};
+                            const float r2[4] = {c2.x, c2.y, c2.z, c2.w}, r3[4] = {c3.x, c3.y, c3.z, c3.w};
+#pragma unroll
+                            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                                for (int c = 0; c < 4; ++c) {
+                                    float &s = acc[q][4 * g + c];
+                                    if (EXACT)
+                                        s = s + ((r0[c] * w0[q] + r1[c] * w1[q]) + (r2[c] * w2[q] + r3[c] * w3[q]));
+                                    else
+                                        s = __builtin_fmaf(r3[c], w3[q], __builtin_fmaf(r2[c], w2[q],
+                                            __builtin_fmaf(r1[c], w1[q], __builtin_fmaf(r0[c], w0[q], s))));
+                                }
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (!in_vol[q]) continue;
+        const int x = xq + (q & 1), y = yq + (q >> 1);
+        float *out = a.vol + ((size_t)y * a.nu + x) * a.nv + z0;
+        if ((a.nv & 3) == 0 && zn == ZT) {
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+                *reinterpret_cast<float4 *>(out + 4 * g) =
+                    make_float4(acc[q][4 * g], acc[q][4 * g + 1], acc[q][4 * g + 2], acc[q][4 * g + 3]);
+        } else {
+#pragma unroll
+            for (int z = 0; z < ZT; ++z)
+                if (z < zn) out[z] = acc[q][z];
+        }
+    }
+}
+
+template <int ZT, bool EXACT, int KC>
+__global__ __launch_bounds__(256) void bp_quad_kernel(BpArgs a)
+{
+    __shared__ float sm[KC][kQuadSpan][ZT + 4];
+    __shared__ int sb0[KC];
+    int bx, by, bz;
+    if (!tile_of_block(a, bx, by, bz)) return;
+    bp_quad_tile<ZT, EXACT, KC>(a, bx, by, bz, sm, sb0);
 }
 
 // Persistent batched back-projection: a grid sized to the device pulls work
@@ -297,11 +497,30 @@ static int launch_bp(const BpArgs &a, hipStream_t s)
     return MR_OK;
 }
 
-template <int ZT, int INTERP, bool EXACT, int KC>
-static int launch_bp_lds(const BpArgs &a, hipStream_t s)
+static int env_int(const char *name, int dflt)
 {
-    dim3 grid((unsigned)((a.nu + 15) / 16), (unsigned)((a.nu + 15) / 16),
-              (unsigned)((a.nv + ZT - 1) / ZT));
+    const char *e = std::getenv(name);
+    return e ? std::atoi(e) : dflt;
+}
+
+// Grid of a tiled kernel with tile side T: the 3-D tile grid (order 0) or
+// the XCD-striped 1-D grid (order 1, see tile_of_block).
+static dim3 tiled_grid(BpArgs &a, int T, int ZT, int order)
+{
+    a.ntx = a.nty = (a.nu + T - 1) / T;
+    a.ntz = (a.nv + ZT - 1) / ZT;
+    const long n = (long)a.ntx * a.nty * a.ntz;
+    a.order = n < (1L << 30) ? order : 0;   // tile numbers stay in int range
+    const int sw = env_int("MR_BP_SW", 4);
+    a.sw = (sw == 2 || sw == 8 || sw == 16) ? sw : 4;
+    if (a.order == 0) return dim3((unsigned)a.ntx, (unsigned)a.nty, (unsigned)a.ntz);
+    return dim3((unsigned)(8 * ((n + 7) / 8)));
+}
+
+template <int ZT, int INTERP, bool EXACT, int KC>
+static int launch_bp_lds(BpArgs a, int order, hipStream_t s)
+{
+    const dim3 grid = tiled_grid(a, 16, ZT, order);
     ProfScope prof("bp_parallel", s);
     hipLaunchKernelGGL((bp_lds_kernel<ZT, INTERP, EXACT, KC>), grid, dim3(256), 0, s, a);
     MR_HIP(hipGetLastError());
@@ -309,49 +528,101 @@ static int launch_bp_lds(const BpArgs &a, hipStream_t s)
 }
 
 template <int ZT, int KC>
-static int launch_bp_lds_mode(const BpArgs &a, int interp, int exact, hipStream_t s)
+static int launch_bp_lds_mode(const BpArgs &a, int interp, int exact, int order, hipStream_t s)
 {
-    if (interp == 1) return launch_bp_lds<ZT, 1, true, KC>(a, s);
-    return exact ? launch_bp_lds<ZT, 0, true, KC>(a, s) : launch_bp_lds<ZT, 0, false, KC>(a, s);
+    if (interp == 1) return launch_bp_lds<ZT, 1, true, KC>(a, order, s);
+    return exact ? launch_bp_lds<ZT, 0, true, KC>(a, order, s) : launch_bp_lds<ZT, 0, false, KC>(a, order, s);
 }
 
-// Slice tile / projection chunk of the LDS kernel (MR_BP_ZT / MR_BP_KC
+template <int ZT, int KC>
+static int launch_bp_quad(BpArgs a, int exact, int order, hipStream_t s)
+{
+    const dim3 grid = tiled_grid(a, 32, ZT, order);
+    ProfScope prof("bp_parallel", s);
+    if (exact)
+        hipLaunchKernelGGL((bp_quad_kernel<ZT, true, KC>), grid, dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((bp_quad_kernel<ZT, false, KC>), grid, dim3(256), 0, s, a);
+    MR_HIP(hipGetLastError());
+    return MR_OK;
+}
+
+// Slice tile / projection chunk of the LDS kernels (MR_BP_ZT / MR_BP_KC
 // override the defaults for tuning experiments).
 static void lds_shape(int nv, int &zt, int &kc)
 {
-    zt = nv >= 16 ? 16 : 8;
-    kc = 16;
-    if (const char *e = std::getenv("MR_BP_ZT")) {
-        const int v = std::atoi(e);
-        if ((v == 32 && nv >= 32) || v == 16 || v == 8) zt = v;
-    }
-    if (const char *e = std::getenv("MR_BP_KC")) {
-        const int v = std::atoi(e);
-        if (v == 8 || v == 16 || v == 32) kc = v;
-    }
-    if (zt == 32) kc = 8;
+    zt = nv >= 32 ? 32 : (nv >= 16 ? 16 : 8);
+    kc = zt == 32 ? 4 : 8;
+    const int ez = env_int("MR_BP_ZT", 0), ek = env_int("MR_BP_KC", 0);
+    if ((ez == 64 && nv >= 64) || (ez == 32 && nv >= 32) || ez == 16 || ez == 8) zt = ez;
+    if (ek == 2 || ek == 4 || ek == 8 || ek == 16 || ek == 32) kc = ek;
+    if (zt == 64) kc = kc >= 4 ? 4 : 2;
+    if (zt == 32) kc = kc >= 8 ? 8 : 4;
+    if (zt == 16) kc = kc <= 4 ? 4 : (kc >= 16 ? (kc >= 32 ? 32 : 16) : 8);
+    if (zt == 8) kc = kc <= 8 ? 8 : (kc >= 32 ? 32 : 16);
 }
 
-// impl: 0 = direct global gathers, 1 = LDS-staged tiles.  exact: bit-faithful
-// float order (otherwise fused multiply-adds).
+// impl: 0 = direct global gathers, 1 = LDS-staged 16 x 16 tiles, 2 = LDS-staged
+// 32 x 32 tiles with 2 x 2 voxel groups (linear interpolation; nearest
+// neighbour falls back to impl 1).  exact: bit-faithful float order
+// (otherwise fused multiply-adds).  MR_BP_ORDER selects the block order of
+// the tiled kernels (0 grid order, 1 XCD stripes).
 int run_bp(const BpArgs &a, int interp, int impl, int exact, hipStream_t s)
 {
-    if (impl == 1) {
+    const int order = env_int("MR_BP_ORDER", 1) ? 1 : 0;
+    if (impl == 2 && interp == 0) {
+        const int ez = env_int("MR_BP_ZT", 0), ek = env_int("MR_BP_KC", 0);
+        const int zt = (ez == 8 || a.nv < 16) ? 8 : 16;
+        const int kc = ek == 16 ? 16 : 8;
+        if (zt == 16) return kc == 16 ? launch_bp_quad<16, 16>(a, exact, order, s)
+                                      : launch_bp_quad<16, 8>(a, exact, order, s);
+        return kc == 16 ? launch_bp_quad<8, 16>(a, exact, order, s) : launch_bp_quad<8, 8>(a, exact, order, s);
+    }
+    if (impl >= 1) {
         int zt, kc;
         lds_shape(a.nv, zt, kc);
-        if (zt == 32) return launch_bp_lds_mode<32, 8>(a, interp, exact, s);
+        if (zt == 64) return kc == 2 ? launch_bp_lds_mode<64, 2>(a, interp, exact, order, s)
+                                     : launch_bp_lds_mode<64, 4>(a, interp, exact, order, s);
+        if (zt == 32) return kc == 4 ? launch_bp_lds_mode<32, 4>(a, interp, exact, order, s)
+                                     : launch_bp_lds_mode<32, 8>(a, interp, exact, order, s);
         if (zt == 16) {
-            if (kc == 8) return launch_bp_lds_mode<16, 8>(a, interp, exact, s);
-            if (kc == 32) return launch_bp_lds_mode<16, 32>(a, interp, exact, s);
-            return launch_bp_lds_mode<16, 16>(a, interp, exact, s);
+            if (kc == 4) return launch_bp_lds_mode<16, 4>(a, interp, exact, order, s);
+            if (kc == 8) return launch_bp_lds_mode<16, 8>(a, interp, exact, order, s);
+            if (kc == 32) return launch_bp_lds_mode<16, 32>(a, interp, exact, order, s);
+            return launch_bp_lds_mode<16, 16>(a, interp, exact, order, s);
         }
-        if (kc == 8) return launch_bp_lds_mode<8, 8>(a, interp, exact, s);
-        if (kc == 32) return launch_bp_lds_mode<8, 32>(a, interp, exact, s);
-        return launch_bp_lds_mode<8, 16>(a, interp, exact, s);
+        if (kc == 8) return launch_bp_lds_mode<8, 8>(a, interp, exact, order, s);
+        if (kc == 32) return launch_bp_lds_mode<8, 32>(a, interp, exact, order, s);
+        return launch_bp_lds_mode<8, 16>(a, interp, exact, order, s);
     }
     if (a.nv >= 16) return interp == 0 ? launch_bp<16, 0>(a, s) : launch_bp<16, 1>(a, s);
     if (a.nv >= 8) return interp == 0 ? launch_bp<8, 0>(a, s) : launch_bp<8, 1>(a, s);
     return interp == 0 ? launch_bp<4, 0>(a, s) : launch_bp<4, 1>(a, s);
+}
+
+// The kernel run_bp launches for these settings, e.g.
+// "bp_lds_kernel<32,0,false,4> order=1 sw=4".
+std::string describe_bp(int nv, int interp, int impl, int exact)
+{
+    const int order = env_int("MR_BP_ORDER", 1) ? 1 : 0;
+    const int swe = env_int("MR_BP_SW", 4);
+    const int sw = (swe == 2 || swe == 8 || swe == 16) ? swe : 4;
+    const std::string ord = order ? " order=1 sw=" + std::to_string(sw) : " order=0";
+    const char *ex = exact ? "true" : "false";
+    if (impl == 2 && interp == 0) {
+        const int ez = env_int("MR_BP_ZT", 0), ek = env_int("MR_BP_KC", 0);
+        const int zt = (ez == 8 || nv < 16) ? 8 : 16;
+        const int kc = ek == 16 ? 16 : 8;
+        return "bp_quad_kernel<" + std::to_string(zt) + "," + ex + "," + std::to_string(kc) + ">" + ord;
+    }
+    if (impl >= 1) {
+        int zt, kc;
+        lds_shape(nv, zt, kc);
+        return "bp_lds_kernel<" + std::to_string(zt) + "," + std::to_string(interp) + "," +
+               (interp == 1 ? "true" : ex) + "," + std::to_string(kc) + ">" + ord;
+    }
+    const int zt = nv >= 16 ? 16 : (nv >= 8 ? 8 : 4);
+    return "bp_kernel<" + std::to_string(zt) + "," + std::to_string(interp) + ">";
 }
 
 }  // namespace mr
@@ -428,7 +699,10 @@ extern "C" int mr_bp_plan_create(mr_bp_plan **plan, const mr_bp_params *params)
     // MultiProjBP's B-column offset only differs from the padded column when
     // slices % 4 != 0; that case stays on the direct-gather kernel.
     p->impl = (q.multiprojbp && (q.slices & 3)) ? 0 : 1;
-    if (const char *e = std::getenv("MR_BP_IMPL")) p->impl = std::atoi(e) == 0 ? 0 : p->impl;
+    if (const char *e = std::getenv("MR_BP_IMPL")) {
+        const int v = std::atoi(e);
+        p->impl = v == 0 ? 0 : (p->impl == 0 ? 0 : (v == 2 ? 2 : 1));
+    }
     mr::circle_mask(q.width, q.width, q.use_roi != 0, q.roi, false, p->first, p->second);
     std::vector<int2> m(q.width);
     for (size_t y = 0; y < q.width; ++y) {
@@ -457,6 +731,17 @@ extern "C" int mr_bp_plan_destroy(mr_bp_plan *plan)
     if (plan->d_idx) (void)hipFree(plan->d_idx);
     if (plan->d_w) (void)hipFree(plan->d_w);
     delete plan;
+    return MR_OK;
+    MR_GUARD_END
+}
+
+extern "C" int mr_bp_plan_kernel(const mr_bp_plan *plan, char *name, size_t len)
+{
+    MR_GUARD_BEGIN
+    mr::clear_error();
+    MR_REQUIRE(plan != nullptr && name != nullptr && len > 0, "NULL argument");
+    const std::string d = mr::describe_bp((int)plan->p.slices, plan->p.interp, plan->impl, plan->p.fused ? 0 : 1);
+    std::snprintf(name, len, "%s", d.c_str());
     return MR_OK;
     MR_GUARD_END
 }

@@ -84,6 +84,76 @@ __global__ __launch_bounds__(256) void fdk_kernel(FdkArgs a)
             a.vol[((size_t)(a.nz - 1 - (k0 + kk)) * a.ny + j) * a.nx + (a.nx - 1 - i)] = acc[kk];
 }
 
+// FDKbp (fdkbp.cpp:324-510, 612-631): the same loop with the geometry in
+// double.  Voxel coordinates are float expressions widened to double
+// (origin + i*spacing in float), the per-projection coefficients and the
+// per-slice row term are the reference's double xip/yip/zip products
+// (evaluated on the host), the depth reciprocal, the sampling positions and
+// the bounds tests are double, and the float voxel takes vol + dw*dw*val
+// evaluated in double, once per projection.  With -ffp-contract=off and the
+// correctly rounded double division every step matches the reference.
+struct FdkdArgs {
+    const float *img;
+    float *vol;
+    const double *ptab;  // 8 doubles per projection: cx0 cx1 cx2 cy0 cy1 cy2 zc0 zc2
+    const double *zt1;   // [np][nz]
+    const int2 *mask;
+    int nx, ny, nz, w, h, np;
+    float ox, oy, sx, sy;
+};
+
+template <int KT>
+__global__ __launch_bounds__(256) void fdkd_kernel(FdkdArgs a)
+{
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int k0 = blockIdx.z * KT;
+    if (i >= a.nx || j >= a.ny) return;
+    const int kn = min(KT, a.nz - k0);
+    float acc[KT];
+#pragma unroll
+    for (int kk = 0; kk < KT; ++kk) acc[kk] = 0.0f;
+    const int2 m = a.mask[j];
+    if (i > m.x && i <= m.y) {
+        const double x = (double)(a.ox + (float)i * a.sx);
+        const double y = (double)(a.oy + (float)j * a.sy);
+        const double fw = (double)a.w, fh = (double)a.h;
+        for (int p = 0; p < a.np; ++p) {
+            const double *t = a.ptab + 8 * p;
+            const double xip0 = x * t[0], xip1 = x * t[1], xip2 = x * t[2];
+            const double yip0 = y * t[3], yip1 = y * t[4], yip2 = y * t[5];
+            const double acc20 = t[6] + yip0;
+            const double acc22 = t[7] + yip2;
+            const double acc30 = acc20 + xip0;
+            const double acc32 = acc22 + xip2;
+            const double dw = 1.0 / acc32;
+            double c = acc30 * dw;
+            c = c + 0.5;
+            if (c < 0.0 || c >= fw) continue;
+            const int cc = (int)c;
+            const double dw2 = dw * dw;
+            const float *col = a.img + (size_t)p * a.h * a.w + cc;
+            const double *zt = a.zt1 + (size_t)p * a.nz + k0;
+#pragma unroll
+            for (int kk = 0; kk < KT; ++kk) {
+                if (kk < kn) {
+                    const double acc21 = zt[kk] + yip1;
+                    const double acc31 = acc21 + xip1;
+                    double r = acc31 * dw;
+                    r = r + 0.5;
+                    float val = 0.0f;
+                    if (!(r < 0.0) && r < fh) val = col[(size_t)(int)r * a.w];
+                    acc[kk] = (float)((double)acc[kk] + dw2 * (double)val);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int kk = 0; kk < KT; ++kk)
+        if (kk < kn)
+            a.vol[((size_t)(a.nz - 1 - (k0 + kk)) * a.ny + j) * a.nx + (a.nx - 1 - i)] = acc[kk];
+}
+
 }  // namespace mr
 
 struct mr_fdk_plan {
@@ -94,6 +164,9 @@ struct mr_fdk_plan {
     size_t cbct[4] = {0, 0, 0, 0};
     float origin[3] = {0, 0, 0};
     float ic[2] = {0, 0};
+    double icd[2] = {0, 0};   // FDKbp: ic from its own (wrapping) window
+    double *d_ptabd = nullptr;
+    double *d_zt1d = nullptr;
     int2 *d_mask = nullptr;
     float *d_hs = nullptr;
     float2 *d_tw = nullptr;
@@ -132,6 +205,7 @@ extern "C" int mr_fdk_plan_create(mr_fdk_plan **plan, const mr_fdk_params *param
     MR_REQUIRE(q.roi[2] > q.roi[0] && q.roi[3] > q.roi[1], "ROI x1<=x0 or y1<=y0");
     MR_REQUIRE(q.sdd > 0 && q.sod > 0 && q.pixel[0] > 0 && q.pixel[1] > 0, "invalid distances/pixel size");
     MR_REQUIRE(q.direction == 0 || q.direction == 1, "direction must be 0 (CW) or 1 (CCW)");
+    MR_REQUIRE(q.module == MR_FDK_SINGLE || q.module == MR_FDK_DOUBLE, "module must be MR_FDK_SINGLE or MR_FDK_DOUBLE");
     auto *p = new mr_fdk_plan();
     p->p = q;
     mr::FdkGeometry &g = p->g;
@@ -153,6 +227,12 @@ extern "C" int mr_fdk_plan_create(mr_fdk_plan **plan, const mr_fdk_params *param
     p->origin[2] = -(V - (q.ppoint[1] - (float)q.roi[1])) * q.voxel[2] - q.voxel[2] / 2;
     p->ic[0] = q.ppoint[0] - (float)p->cbct[0];
     p->ic[1] = q.ppoint[1] - (float)p->cbct[1];
+    {
+        size_t cbd[4];
+        mr::fdk_cbct_roi(g, p->nx, cbd, true);
+        p->icd[0] = (double)(q.ppoint[0] - (float)cbd[0]);
+        p->icd[1] = (double)(q.ppoint[1] - (float)cbd[1]);
+    }
     std::vector<size_t> first, second;
     mr::circle_mask(p->nx, p->nx, q.use_roi != 0, q.matrix_roi, true, first, second);
     std::vector<int2> m(p->nx);
@@ -161,10 +241,22 @@ extern "C" int mr_fdk_plan_create(mr_fdk_plan **plan, const mr_fdk_params *param
     const size_t P = p->P;
     std::vector<float> ramp(P);
     const float DEGTORADF = 3.1415926535897932384626f / 180.0f;
-    for (size_t i = 0; i < P; ++i) {
-        ramp[i] = i < P / 2 ? (float)(unsigned)i : (float)(unsigned)(P - i);
-        ramp[i] *= (0.54 + 0.46 * (std::cos((float)(unsigned)i * DEGTORADF * 360 / (float)(unsigned)P) + 1));
-        ramp[i] /= (double)(unsigned)P * 0.5;
+    if (q.module == MR_FDK_SINGLE) {
+        for (size_t i = 0; i < P; ++i) {
+            ramp[i] = i < P / 2 ? (float)(unsigned)i : (float)(unsigned)(P - i);
+            ramp[i] *= (0.54 + 0.46 * (std::cos((float)(unsigned)i * DEGTORADF * 360 / (float)(unsigned)P) + 1));
+            ramp[i] /= (double)(unsigned)P * 0.5;
+        }
+    } else {
+        // FDKbp's ramp in double (fdkbp.cpp:895-901), rounded to float for the
+        // float spectrum
+        const double DEGTORAD = 3.14159265358979323846264338327950288 / 180.0;
+        for (size_t i = 0; i < P; ++i) {
+            double r = i < P / 2 ? (double)i : (double)(P - i);
+            r *= (0.54 + 0.46 * (std::cos((double)(unsigned)i * DEGTORAD * 360 / (double)(unsigned)P) + 1));
+            r /= (double)(unsigned)P * 0.5;
+            ramp[i] = (float)r;
+        }
     }
     std::vector<float> tw = mr::make_twiddles(P);
     hipError_t e = hipMalloc(&p->d_mask, sizeof(int2) * p->nx);
@@ -187,7 +279,8 @@ extern "C" int mr_fdk_plan_destroy(mr_fdk_plan *plan)
     MR_GUARD_BEGIN
     if (!plan) return MR_OK;
     for (void *ptr : {(void *)plan->d_mask, (void *)plan->d_hs, (void *)plan->d_tw,
-                      (void *)plan->d_ptab, (void *)plan->d_zt1, (void *)plan->d_w})
+                      (void *)plan->d_ptab, (void *)plan->d_zt1, (void *)plan->d_w,
+                      (void *)plan->d_ptabd, (void *)plan->d_zt1d})
         if (ptr) (void)hipFree(ptr);
     delete plan;
     return MR_OK;
@@ -216,39 +309,72 @@ extern "C" int mr_fdk_bp(mr_fdk_plan *plan, float *d_proj, size_t n_proj, size_t
     MR_REQUIRE(n_proj <= 65535, "too many projections for one call");
     hipStream_t s = (hipStream_t)stream;
     const size_t nx = plan->nx, nz = plan->nz, w = nx, h = win_rows;
+    const bool dbl = plan->p.module == MR_FDK_DOUBLE;
     if (n_proj > plan->cap_np) {
         for (float **ptr : {&plan->d_ptab, &plan->d_zt1, &plan->d_w})
+            if (*ptr) { (void)hipFree(*ptr); *ptr = nullptr; }
+        for (double **ptr : {&plan->d_ptabd, &plan->d_zt1d})
             if (*ptr) { (void)hipFree(*ptr); *ptr = nullptr; }
         plan->cap_np = 0;
         MR_HIP(hipMalloc(&plan->d_ptab, sizeof(float) * 8 * n_proj));
         MR_HIP(hipMalloc(&plan->d_zt1, sizeof(float) * nz * n_proj));
         MR_HIP(hipMalloc(&plan->d_w, sizeof(float) * n_proj));
+        MR_HIP(hipMalloc(&plan->d_ptabd, sizeof(double) * 8 * n_proj));
+        MR_HIP(hipMalloc(&plan->d_zt1d, sizeof(double) * nz * n_proj));
         plan->cap_np = n_proj;
     }
-    // per-projection coefficients (fdkbp_single.cpp:420-456)
-    std::vector<float> ptab(8 * std::max<size_t>(n_proj, 1)), zt1(nz * std::max<size_t>(n_proj, 1));
-    const float *ic = plan->ic;
+    // per-projection coefficients (fdkbp_single.cpp:420-456; fdkbp.cpp:430-465)
+    const size_t npx = std::max<size_t>(n_proj, 1);
+    std::vector<float> ptab(dbl ? 0 : 8 * npx), zt1(dbl ? 0 : nz * npx);
+    std::vector<double> ptabd(dbl ? 8 * npx : 0), zt1d(dbl ? nz * npx : 0);
     for (size_t ip = 0; ip < n_proj; ++ip) {
-        float pm[12];
-        mr::fdk_proj_matrix(plan->g, angles[ip], pm);
-        if (pm[2] != 0.0f || pm[10] != 0.0f) {
-            mr::set_error("projection matrix depends on z in x/depth (tilted geometry) - unsupported");
-            return MR_ERR_UNSUPPORTED;
-        }
-        float *t = &ptab[8 * ip];
-        t[0] = pm[0] + ic[0] * pm[8];
-        t[1] = pm[4] + ic[1] * pm[8];
-        t[2] = pm[8];
-        t[3] = pm[1] + ic[0] * pm[9];
-        t[4] = pm[5] + ic[1] * pm[9];
-        t[5] = pm[9];
-        t[6] = ic[0] * pm[11] + pm[3];
-        t[7] = pm[11];
-        const float a1 = pm[6] + ic[1] * pm[10];
-        const float b1 = ic[1] * pm[11];
-        for (size_t k = 0; k < nz; ++k) {
-            const float z = plan->origin[2] + (float)(long)k * plan->p.voxel[2];
-            zt1[ip * nz + k] = z * a1 + b1 + pm[7];
+        if (!dbl) {
+            const float *ic = plan->ic;
+            float pm[12];
+            mr::fdk_proj_matrix(plan->g, angles[ip], pm);
+            if (pm[2] != 0.0f || pm[10] != 0.0f) {
+                mr::set_error("projection matrix depends on z in x/depth (tilted geometry) - unsupported");
+                return MR_ERR_UNSUPPORTED;
+            }
+            float *t = &ptab[8 * ip];
+            t[0] = pm[0] + ic[0] * pm[8];
+            t[1] = pm[4] + ic[1] * pm[8];
+            t[2] = pm[8];
+            t[3] = pm[1] + ic[0] * pm[9];
+            t[4] = pm[5] + ic[1] * pm[9];
+            t[5] = pm[9];
+            t[6] = ic[0] * pm[11] + pm[3];
+            t[7] = pm[11];
+            const float a1 = pm[6] + ic[1] * pm[10];
+            const float b1 = ic[1] * pm[11];
+            for (size_t k = 0; k < nz; ++k) {
+                const float z = plan->origin[2] + (float)(long)k * plan->p.voxel[2];
+                zt1[ip * nz + k] = z * a1 + b1 + pm[7];
+            }
+        } else {
+            const double *ic = plan->icd;
+            double pm[12];
+            mr::fdkd_proj_matrix(plan->g, angles[ip], pm);
+            if (pm[2] != 0.0 || pm[10] != 0.0) {
+                mr::set_error("projection matrix depends on z in x/depth (tilted geometry) - unsupported");
+                return MR_ERR_UNSUPPORTED;
+            }
+            double *t = &ptabd[8 * ip];
+            t[0] = pm[0] + ic[0] * pm[8];
+            t[1] = pm[4] + ic[1] * pm[8];
+            t[2] = pm[8];
+            t[3] = pm[1] + ic[0] * pm[9];
+            t[4] = pm[5] + ic[1] * pm[9];
+            t[5] = pm[9];
+            // zip x and depth terms do not depend on the slice when pm[2] and
+            // pm[10] are zero: z*(+-0) + ic0*pm11 + pm3 and z*(+-0) + pm11
+            const double z0 = (double)(plan->origin[2] + (float)(long)0 * plan->p.voxel[2]);
+            t[6] = z0 * (pm[2] + ic[0] * pm[10]) + ic[0] * pm[11] + pm[3];
+            t[7] = z0 * pm[10] + pm[11];
+            for (size_t k = 0; k < nz; ++k) {
+                const double z = (double)(plan->origin[2] + (float)(long)k * plan->p.voxel[2]);
+                zt1d[ip * nz + k] = z * (pm[6] + ic[1] * pm[10]) + ic[1] * pm[11] + pm[7];
+            }
         }
     }
     if (n_proj == 0) {
@@ -258,29 +384,44 @@ extern "C" int mr_fdk_bp(mr_fdk_plan *plan, float *d_proj, size_t n_proj, size_t
         MR_HIP(hipStreamSynchronize(s));
         return MR_OK;
     }
-    MR_HIP(hipMemcpyAsync(plan->d_ptab, ptab.data(), sizeof(float) * 8 * n_proj, hipMemcpyHostToDevice, s));
-    MR_HIP(hipMemcpyAsync(plan->d_zt1, zt1.data(), sizeof(float) * nz * n_proj, hipMemcpyHostToDevice, s));
+    if (!dbl) {
+        MR_HIP(hipMemcpyAsync(plan->d_ptab, ptab.data(), sizeof(float) * 8 * n_proj, hipMemcpyHostToDevice, s));
+        MR_HIP(hipMemcpyAsync(plan->d_zt1, zt1.data(), sizeof(float) * nz * n_proj, hipMemcpyHostToDevice, s));
+    } else {
+        MR_HIP(hipMemcpyAsync(plan->d_ptabd, ptabd.data(), sizeof(double) * 8 * n_proj, hipMemcpyHostToDevice, s));
+        MR_HIP(hipMemcpyAsync(plan->d_zt1d, zt1d.data(), sizeof(double) * nz * n_proj, hipMemcpyHostToDevice, s));
+    }
     MR_HIP(hipMemcpyAsync(plan->d_w, weights, sizeof(float) * n_proj, hipMemcpyHostToDevice, s));
-    // weight, ramp filter, 1/P, (SOD/SDD)^2, SOD/SDD  (fdkreconbase.cpp:247, fdkbp_single.cpp:376-389)
+    // weight, ramp filter, 1/P, (SOD/SDD)^2, then SOD/SDD (FDKbp_single) or
+    // SOD/SDD/nProj (FDKbp)  (fdkreconbase.cpp:247, fdkbp_single.cpp:376-389,
+    // fdkbp.cpp:335,413-418)
     const float sod = plan->p.sod, sdd = plan->p.sdd;
     mr::RowFilterArgs fa{};
     fa.rows = d_proj; fa.n_rows = (long)(n_proj * h); fa.n = (int)w; fa.mode = 1;
     fa.insert = (int)((plan->P - w) / 2); fa.taper = nullptr; fa.hs = plan->d_hs; fa.tw = plan->d_tw;
-    fa.post0 = 1.0f / (float)plan->P; fa.post1 = (sod * sod) / (sdd * sdd); fa.post2 = sod / sdd;
+    fa.post0 = 1.0f / (float)plan->P; fa.post1 = (sod * sod) / (sdd * sdd);
+    fa.post2 = dbl ? sod / sdd / (float)n_proj : sod / sdd;
     fa.pre_w = plan->d_w; fa.rows_per_proj = (int)h;
     int rc = mr::run_row_filter(fa, (int)plan->P, s);
     if (rc != MR_OK) return rc;
     rc = mr::run_fdk_margin_rows(d_proj, (int)w, (int)h, (long)n_proj, s);
     if (rc != MR_OK) return rc;
-    mr::FdkArgs a{};
-    a.img = d_proj; a.vol = d_volume; a.ptab = plan->d_ptab; a.zt1 = plan->d_zt1; a.mask = plan->d_mask;
-    a.nx = (int)nx; a.ny = (int)nx; a.nz = (int)nz; a.w = (int)w; a.h = (int)h; a.np = (int)n_proj;
-    a.ox = plan->origin[0]; a.oy = plan->origin[1]; a.sx = plan->p.voxel[0]; a.sy = plan->p.voxel[1];
     constexpr int KT = 16;
     dim3 grid((unsigned)((nx + 63) / 64), (unsigned)((nx + 3) / 4), (unsigned)((nz + KT - 1) / KT));
-    {
+    if (!dbl) {
+        mr::FdkArgs a{};
+        a.img = d_proj; a.vol = d_volume; a.ptab = plan->d_ptab; a.zt1 = plan->d_zt1; a.mask = plan->d_mask;
+        a.nx = (int)nx; a.ny = (int)nx; a.nz = (int)nz; a.w = (int)w; a.h = (int)h; a.np = (int)n_proj;
+        a.ox = plan->origin[0]; a.oy = plan->origin[1]; a.sx = plan->p.voxel[0]; a.sy = plan->p.voxel[1];
         mr::ProfScope prof("fdk_bp", s);
         hipLaunchKernelGGL(mr::fdk_kernel<KT>, grid, dim3(256), 0, s, a);
+    } else {
+        mr::FdkdArgs a{};
+        a.img = d_proj; a.vol = d_volume; a.ptab = plan->d_ptabd; a.zt1 = plan->d_zt1d; a.mask = plan->d_mask;
+        a.nx = (int)nx; a.ny = (int)nx; a.nz = (int)nz; a.w = (int)w; a.h = (int)h; a.np = (int)n_proj;
+        a.ox = plan->origin[0]; a.oy = plan->origin[1]; a.sx = plan->p.voxel[0]; a.sy = plan->p.voxel[1];
+        mr::ProfScope prof("fdkd_bp", s);
+        hipLaunchKernelGGL(mr::fdkd_kernel<KT>, grid, dim3(256), 0, s, a);
     }
     MR_HIP(hipGetLastError());
     MR_HIP(hipStreamSynchronize(s));

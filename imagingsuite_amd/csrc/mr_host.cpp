@@ -174,7 +174,57 @@ void fdk_proj_matrix(const FdkGeometry &g, float angle, float pm[12])
         }
 }
 
-void fdk_cbct_roi(const FdkGeometry &g, size_t nvol, size_t cb[4])
+void fdkd_proj_matrix(const FdkGeometry &g, float angle, double pm[12])
+{
+    double ext[16] = {0}, intr[12] = {0};
+    const double sad = g.sod, sid = g.sdd;
+    // a float expression in the reference (float operands), then widened
+    const double off_center = (g.ppoint[0] - static_cast<float>(g.roi[0]) - g.center) * g.voxel[0];
+    constexpr double kDPi = 3.14159265358979323846264338327950288;
+    double cam[3] = {0.0, 0.0, 0.0};
+    if (g.direction == 0) {
+        cam[0] = sad * std::cos(-angle * kDPi / 180);
+        cam[1] = sad * std::sin(-angle * kDPi / 180);
+    } else {
+        cam[0] = sad * std::cos(angle * kDPi / 180);
+        cam[1] = sad * std::sin(angle * kDPi / 180);
+    }
+    const double vup[3] = {0.0, 0.0, 1.0}, tgt[3] = {0.0, 0.0, 0.0};
+    double nrm[3], plt[3], pup[3];
+    for (int i = 0; i < 3; ++i) nrm[i] = cam[i] - tgt[i];
+    const double nn = std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+    for (int i = 0; i < 3; ++i) nrm[i] /= nn;
+    plt[0] = nrm[1] * vup[2] - nrm[2] * vup[1];
+    plt[1] = nrm[2] * vup[0] - nrm[0] * vup[2];
+    plt[2] = nrm[0] * vup[1] - nrm[1] * vup[0];
+    const double np_ = std::sqrt(plt[0] * plt[0] + plt[1] * plt[1] + plt[2] * plt[2]);
+    for (int i = 0; i < 3; ++i) plt[i] /= np_;
+    pup[0] = plt[1] * nrm[2] - plt[2] * nrm[1];
+    pup[1] = plt[2] * nrm[0] - plt[0] * nrm[2];
+    pup[2] = plt[0] * nrm[1] - plt[1] * nrm[0];
+    const double nu_ = std::sqrt(pup[0] * pup[0] + pup[1] * pup[1] + pup[2] * pup[2]);
+    for (int i = 0; i < 3; ++i) pup[i] /= nu_;
+    for (int i = 0; i < 3; ++i) {
+        ext[i] = -1.0 * plt[i];
+        ext[4 + i] = -1.0 * pup[i];
+        ext[8 + i] = -1.0 * nrm[i];
+    }
+    ext[15] = 1.0;
+    ext[3] = plt[0] * tgt[0] + plt[1] * tgt[1] + plt[2] * tgt[2] - off_center;
+    ext[7] = pup[0] * tgt[0] + pup[1] * tgt[1] + pup[2] * tgt[2];
+    ext[11] = nrm[0] * tgt[0] + nrm[1] * tgt[1] + nrm[2] * tgt[2] + sad;
+    intr[0] = 1 / g.pixel[0];   // int / float: a float quotient
+    intr[5] = 1 / g.pixel[1];
+    intr[10] = 1 / sid;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 4; ++j) {
+            float acc = 0.0f;
+            for (int k = 0; k < 4; ++k) acc = static_cast<float>(acc + intr[i * 4 + k] * ext[k * 4 + j]);
+            pm[i * 4 + j] = acc;
+        }
+}
+
+void fdk_cbct_roi(const FdkGeometry &g, size_t nvol, size_t cb[4], bool fdkbp_variant)
 {
     const float radius = static_cast<float>(nvol) * g.voxel[0] / 2;
     const float pp1 = g.ppoint[1];
@@ -203,7 +253,7 @@ void fdk_cbct_roi(const FdkGeometry &g, size_t nvol, size_t cb[4])
         cb[3] = value2 >= static_cast<float>(g.projection_roi3) ? g.projection_roi3
                                                                  : static_cast<size_t>(value2);
     }
-    if (cb[1] != 0) cb[1] -= 8;
+    if (fdkbp_variant || cb[1] != 0) cb[1] -= 8;
     if (cb[3] + 8 <= g.projection_roi3) cb[3] += 8;
 }
 

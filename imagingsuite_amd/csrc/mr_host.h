@@ -42,8 +42,13 @@ struct FdkGeometry {
 // FDKbp_single::getProjMatrix (frameworks/tomography/Backprojectors/
 // FDKBackProjectors/src/fdkbp_single.cpp:83-255).
 void fdk_proj_matrix(const FdkGeometry &g, float angle, float pm[12]);
-// CBCT_roi window (fdkbp_single.cpp:322-367).
-void fdk_cbct_roi(const FdkGeometry &g, size_t nvol, size_t cb[4]);
+// FDKbp::getProjMatrix (fdkbp.cpp:83-257): double vectors and matrices, the
+// product accumulated in a float, so every entry is a float value.
+void fdkd_proj_matrix(const FdkGeometry &g, float angle, double pm[12]);
+// CBCT_roi window (fdkbp_single.cpp:322-367).  fdkbp_variant: FDKbp's copy
+// (fdkbp.cpp:361-406), whose "CBCT_roi[1]-8>=0" test on a size_t always
+// passes, so a window starting at row 0 wraps to 2^64-8.
+void fdk_cbct_roi(const FdkGeometry &g, size_t nvol, size_t cb[4], bool fdkbp_variant = false);
 // prepareFFT pad width (fdkbp_single.cpp:960-963).
 size_t fdk_padwidth(size_t width);
 

@@ -139,6 +139,13 @@ class BackProjector:
         check(_lib.load().mr_bp_plan_mask(self._plan, f.ctypes.data_as(szp), s.ctypes.data_as(szp)))
         return f.astype(np.int64), s.astype(np.int64)
 
+    @property
+    def kernel(self) -> str:
+        """The kernel variant this plan launches (mr_bp_plan_kernel)."""
+        buf = ctypes.create_string_buffer(128)
+        check(_lib.load().mr_bp_plan_kernel(self._plan, buf, 128))
+        return buf.value.decode()
+
     def process(self, proj, angles, weights, volume=None, accumulate: bool = False, stream=None):
         """Back-projects ``proj`` [n_proj, slices, width] (CUDA) into a
         [width, width, slices] ZXY volume (allocated when not given)."""
@@ -184,13 +191,17 @@ def fbp_parallel(filt: ProjectionFilterSingle, bp: BackProjector, proj, angles, 
 
 
 class FDKbp_single:
-    """FDK cone-beam back-projector for one slab (SetROI(roi) + Process(3D))."""
+    """FDK cone-beam back-projector for one slab (SetROI(roi) + Process(3D)),
+    float geometry (fdkbp_single.cpp)."""
+
+    module = "FDKbp_single"
 
     def __init__(self, sod: float, sdd: float, pixel, voxel, ppoint, center: float,
                  roi: Sequence[int], projection_roi3: int, direction: int = 0,
                  matrix_roi: Optional[Sequence[int]] = None):
         L = _lib.load()
         p = FdkParams()
+        p.module = _lib.FDK_MODULES[self.module]
         p.sod, p.sdd = sod, sdd
         p.pixel[0], p.pixel[1] = pixel
         p.voxel[0], p.voxel[1], p.voxel[2] = voxel
@@ -238,6 +249,13 @@ class FDKbp_single:
             self._plan = ctypes.c_void_p()
 
     __del__ = close
+
+
+class FDKbp(FDKbp_single):
+    """The double-geometry FDK module (fdkbp.cpp): projection matrix, sampling
+    positions and accumulation in double, normalised by SOD/SDD/nProj."""
+
+    module = "FDKbp"
 
 
 class BatchReconstructor:

@@ -110,6 +110,11 @@ typedef struct mr_bp_plan mr_bp_plan;
 
 int mr_bp_plan_create(mr_bp_plan **plan, const mr_bp_params *params);
 int mr_bp_plan_destroy(mr_bp_plan *plan);
+/* Name of the kernel variant mr_bp_parallel launches for this plan (slice
+ * tile, projection chunk, block order), e.g.
+ * "bp_lds_kernel<32,0,false,4> order=1 sw=4".  Diagnostics only; no
+ * counterpart in the reference. */
+int mr_bp_plan_kernel(const mr_bp_plan *plan, char *name, size_t len);
 /* Host copy of the circle mask: for row y the voxels first[y] <= x < second[y]
  * are reconstructed (BuildCircleMask's pair, with `second` exclusive). */
 int mr_bp_plan_mask(const mr_bp_plan *plan, size_t *first, size_t *second);
@@ -125,12 +130,16 @@ int mr_fbp_parallel(mr_filter_plan *fplan, mr_bp_plan *bplan, float *d_proj, siz
                     const float *angles, const float *weights, float *d_volume, void *stream);
 
 /* ------------------------------------------------------------------------
- * FDK cone-beam back-projection (FDKbp_single).
+ * FDK cone-beam back-projection (FDKbp_single and FDKbp).
  * Replaces FdkReconBase::SetROI/Process(3D)
  * (frameworks/tomography/Backprojectors/FDKBackProjectors/src/fdkreconbase.cpp:91-277),
  * FDKbp_single::getProjMatrix / ramp_filter_tuned / project_volume_onto_image_c
- * (.../src/fdkbp_single.cpp:83-255, 832-958, 293-531, 641-683).
+ * (.../src/fdkbp_single.cpp:83-255, 832-958, 293-531, 641-683) and the
+ * double-geometry FDKbp versions of the same three (.../src/fdkbp.cpp:83-257,
+ * 822-948, 324-510, 612-631).
  * ------------------------------------------------------------------------ */
+#define MR_FDK_SINGLE 0  /* FDKbp_single: float geometry, scale SOD/SDD */
+#define MR_FDK_DOUBLE 1  /* FDKbp: double geometry and sampling, scale SOD/SDD/nProj */
 typedef struct {
     float sod, sdd;         /* ProjectionInfo.fSOD, fSDD (mm) */
     float pixel[2];         /* ProjectionInfo.fResolution (mm) */
@@ -142,6 +151,7 @@ typedef struct {
     size_t projection_roi3; /* ProjectionInfo.projection_roi[3] */
     int use_roi;            /* MatrixInfo.bUseROI */
     size_t matrix_roi[4];   /* MatrixInfo.roi */
+    int module;             /* MR_FDK_SINGLE or MR_FDK_DOUBLE */
 } mr_fdk_params;
 
 typedef struct mr_fdk_plan mr_fdk_plan;

@@ -91,6 +91,10 @@ def _setup(L):
     L.orc_fdk_filter_f64.argtypes = [_dp, _sz, _sz]
     L.orc_fdk_bp_f32.argtypes = [_fp, _sz, _sz, _sz, _fp, _fp, ctypes.POINTER(FdkGeom), _sp, _sp, _fp]
     L.orc_fdk_bp_f64.argtypes = [_fp, _sz, _sz, _sz, _dp, _dp, ctypes.POINTER(FdkGeom), _sp, _sp, _dp]
+    L.orc_fdkd_projmatrix.argtypes = [ctypes.POINTER(FdkGeom), ctypes.c_float, _dp]
+    L.orc_fdkd_cbct_roi.argtypes = [ctypes.POINTER(FdkGeom), _sz, ctypes.POINTER(ctypes.c_uint64)]
+    L.orc_fdkd_filter.argtypes = [_fp, _sz, _sz]
+    L.orc_fdkd_bp.argtypes = [_fp, _sz, _sz, _sz, _fp, _fp, ctypes.POINTER(FdkGeom), _sp, _sp, _fp]
     L.orc_joseph_forward_f64.argtypes = [_dp, _sz, _sz, _sz, ctypes.c_double, ctypes.c_int, _dp, _sz, _dp]
     L.orc_joseph_adjoint_f64.argtypes = [_dp, _sz, _sz, _sz, ctypes.c_double, ctypes.c_int, _dp, _sz, _dp]
 
@@ -276,14 +280,45 @@ def fdk_filter_f64(img: np.ndarray) -> np.ndarray:
     return a
 
 
-def fdk_bp(proj: np.ndarray, angles, weights, g: FdkGeom, f64: bool = False) -> np.ndarray:
-    """FDKbp_single::Process(3D) for one slab; proj is [np, h, w] (window rows).
-    Returns the XYZ volume [nz, ny, nx] after the final mirror/flip."""
+def fdkd_projmatrix(g: FdkGeom, angle: float) -> np.ndarray:
+    """FDKbp::getProjMatrix (double vectors, float accumulator)."""
+    pm = np.zeros(12, np.float64)
+    lib().orc_fdkd_projmatrix(ctypes.byref(g), ctypes.c_float(angle), _d(pm))
+    return pm
+
+
+def fdkd_cbct_roi(g: FdkGeom, nvol: int) -> np.ndarray:
+    """FDKbp's own window (row 0 wraps to 2^64-8); returned as uint64."""
+    out = (ctypes.c_uint64 * 4)()
+    lib().orc_fdkd_cbct_roi(ctypes.byref(g), nvol, out)
+    return np.array(list(out), np.uint64)
+
+
+def fdkd_filter(img: np.ndarray) -> np.ndarray:
+    """FDKbp::ramp_filter_tuned (double transform, float result)."""
+    a = np.ascontiguousarray(img, np.float32).copy()
+    h, w = a.shape
+    lib().orc_fdkd_filter(_f(a), w, h)
+    return a
+
+
+def fdk_bp(proj: np.ndarray, angles, weights, g: FdkGeom, f64: bool = False,
+           module: str = "FDKbp_single") -> np.ndarray:
+    """FDKbp_single::Process(3D) (or FDKbp's with module="FDKbp") for one
+    slab; proj is [np, h, w] (window rows).  Returns the XYZ volume
+    [nz, ny, nx] after the final mirror/flip.  f64: the FDKbp_single algorithm
+    in double precision (FDKbp divides that by the projection count)."""
     p = np.ascontiguousarray(proj, np.float32)
     n_p, h, w = p.shape
     nx = int(g.roi[2] - g.roi[0]); nz = int(g.roi[3] - g.roi[1])
     first, second = circle_mask(nx)
     f = first.astype(np.uintp); s = second.astype(np.uintp)
+    if module == "FDKbp" and not f64:
+        vol = np.zeros((nz, nx, nx), np.float32)
+        lib().orc_fdkd_bp(_f(p), w, h, n_p, _f(np.ascontiguousarray(angles, np.float32)),
+                          _f(np.ascontiguousarray(weights, np.float32)), ctypes.byref(g),
+                          _s(f), _s(s), _f(vol))
+        return vol
     if f64:
         vol = np.zeros((nz, nx, nx), np.float64)
         lib().orc_fdk_bp_f64(_f(p), w, h, n_p, _d(np.ascontiguousarray(angles, np.float64)),
@@ -294,6 +329,8 @@ def fdk_bp(proj: np.ndarray, angles, weights, g: FdkGeom, f64: bool = False) -> 
         lib().orc_fdk_bp_f32(_f(p), w, h, n_p, _f(np.ascontiguousarray(angles, np.float32)),
                              _f(np.ascontiguousarray(weights, np.float32)), ctypes.byref(g),
                              _s(f), _s(s), _f(vol))
+    if module == "FDKbp":
+        vol /= n_p
     return vol
 
 

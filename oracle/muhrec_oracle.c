@@ -25,6 +25,7 @@
  *   frameworks/tomography/Backprojectors/StdBackProjectors/src/NNMultiProjBP.cpp
  *   frameworks/tomography/Backprojectors/FDKBackProjectors/src/fdkreconbase.cpp
  *   frameworks/tomography/Backprojectors/FDKBackProjectors/src/fdkbp_single.cpp
+ *   frameworks/tomography/Backprojectors/FDKBackProjectors/src/fdkbp.cpp
  *   frameworks/tomography/Framework/ReconAlgorithms/ReconAlgorithms/linearforwardprojector.cpp
  *
  * Parity pinning of each function is documented beside its tests
@@ -538,7 +539,22 @@ void orc_fdk_projmatrix_f32(const orc_fdk_geom *g, float angle, float *pm /*12*/
 
 /* Detector-row window, fdkbp_single.cpp:322-367 (identical to the engine's
  * CBCT_roi, RF/src/ReconEngine.cpp:954-1008).  nvol = volume.Size(1). */
+static void orc_fdk_cbct_roi_impl(const orc_fdk_geom *g, size_t nvol, uint64_t *cb, int always_sub);
 void orc_fdk_cbct_roi(const orc_fdk_geom *g, size_t nvol, uint64_t *cb)
+{
+    orc_fdk_cbct_roi_impl(g, nvol, cb, 0);
+}
+
+/* FDKbp's own copy of the window (fdkbp.cpp:361-406) differs in one line:
+ * "if (CBCT_roi[1]-8>=0) CBCT_roi[1]-=8" is always true for a size_t, so a
+ * window starting at row 0 wraps to 2^64-8 (the single-precision module
+ * tests CBCT_roi[1]!=0 first). */
+void orc_fdkd_cbct_roi(const orc_fdk_geom *g, size_t nvol, uint64_t *cb)
+{
+    orc_fdk_cbct_roi_impl(g, nvol, cb, 1);
+}
+
+static void orc_fdk_cbct_roi_impl(const orc_fdk_geom *g, size_t nvol, uint64_t *cb, int always_sub)
 {
     const float radius = (float)nvol * g->vox[0] / 2;
     const float pp1 = g->ppoint[1];
@@ -566,7 +582,7 @@ void orc_fdk_cbct_roi(const orc_fdk_geom *g, size_t nvol, uint64_t *cb)
         if (value2 >= (float)g->proj_roi3) cb[3] = g->proj_roi3;
         else cb[3] = (uint64_t)value2;
     }
-    if (cb[1] != 0) cb[1] -= 8;
+    if (always_sub || cb[1] != 0) cb[1] -= 8;
     if (cb[3] + 8 <= g->proj_roi3) cb[3] += 8;
 }
 
@@ -786,6 +802,195 @@ void orc_fdk_bp_f64(const float *proj, size_t w, size_t h, size_t np,
             for (size_t i = 0; i < nx; ++i)
                 vol[((nz - 1 - k) * ny + j) * nx + i] = cv[(k * ny + j) * nx + (nx - 1 - i)];
     free(cv); free(img);
+}
+
+/* ------------------------------------------------------------------------ */
+/* FDKbp (double-precision geometry), FDK/fdkbp.cpp                         */
+/* ------------------------------------------------------------------------ */
+
+/* getProjMatrix, fdkbp.cpp:83-257: camera, panel vectors and both matrices in
+ * double; the product accumulates in a float ("float acc", :259-266), so
+ * every matrix entry is a float value held in a double.  offCenter and
+ * 1/fResolution are float expressions (float operands). */
+void orc_fdkd_projmatrix(const orc_fdk_geom *g, float angle, double *pm /*12*/)
+{
+    double ext[16] = {0}, intr[12] = {0};
+    const double sad = g->sod, sid = g->sdd;
+    const double offCenter = (double)(((g->ppoint[0] - (float)g->roi[0]) - g->center) * g->vox[0]);
+    double cam[3] = {0.0, 0.0, 0.0};
+    if (g->direction == 0) {
+        cam[0] = sad * cos(-angle * ORC_DPI / 180);
+        cam[1] = sad * sin(-angle * ORC_DPI / 180);
+    } else {
+        cam[0] = sad * cos(angle * ORC_DPI / 180);
+        cam[1] = sad * sin(angle * ORC_DPI / 180);
+    }
+    const double vup[3] = {0.0, 0.0, 1.0}, tgt[3] = {0.0, 0.0, 0.0};
+    double nrm[3], plt[3], pup[3];
+    nrm[0] = cam[0] - tgt[0]; nrm[1] = cam[1] - tgt[1]; nrm[2] = cam[2] - tgt[2];
+    double nn = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+    nrm[0] /= nn; nrm[1] /= nn; nrm[2] /= nn;
+    plt[0] = nrm[1] * vup[2] - nrm[2] * vup[1];
+    plt[1] = nrm[2] * vup[0] - nrm[0] * vup[2];
+    plt[2] = nrm[0] * vup[1] - nrm[1] * vup[0];
+    double np_ = sqrt(plt[0] * plt[0] + plt[1] * plt[1] + plt[2] * plt[2]);
+    plt[0] /= np_; plt[1] /= np_; plt[2] /= np_;
+    pup[0] = plt[1] * nrm[2] - plt[2] * nrm[1];
+    pup[1] = plt[2] * nrm[0] - plt[0] * nrm[2];
+    pup[2] = plt[0] * nrm[1] - plt[1] * nrm[0];
+    double nu_ = sqrt(pup[0] * pup[0] + pup[1] * pup[1] + pup[2] * pup[2]);
+    pup[0] /= nu_; pup[1] /= nu_; pup[2] /= nu_;
+    ext[0] = -1.0 * plt[0]; ext[1] = -1.0 * plt[1]; ext[2] = -1.0 * plt[2];
+    ext[4] = -1.0 * pup[0]; ext[5] = -1.0 * pup[1]; ext[6] = -1.0 * pup[2];
+    ext[8] = -1.0 * nrm[0]; ext[9] = -1.0 * nrm[1]; ext[10] = -1.0 * nrm[2];
+    ext[15] = 1.0;
+    ext[3] = plt[0] * tgt[0] + plt[1] * tgt[1] + plt[2] * tgt[2] - offCenter;
+    ext[7] = pup[0] * tgt[0] + pup[1] * tgt[1] + pup[2] * tgt[2];
+    ext[11] = nrm[0] * tgt[0] + nrm[1] * tgt[1] + nrm[2] * tgt[2] + sad;
+    intr[0] = (double)(1 / g->pix[0]);
+    intr[5] = (double)(1 / g->pix[1]);
+    intr[10] = 1 / sid;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 4; ++j) {
+            float acc = 0.0f;
+            for (int k = 0; k < 4; ++k) acc = (float)((double)acc + intr[i * 4 + k] * ext[k * 4 + j]);
+            pm[i * 4 + j] = acc;
+        }
+}
+
+/* FDKbp::ramp_filter_tuned, fdkbp.cpp:822-948: the float image padded as in
+ * FDKbp_single, transformed in double (FFTW double c2c there, the exact radix-2
+ * DFT here), ramp |k| x (0.54+0.46(cos(i*DEGTORAD*360/P)+1)) / (P*0.5) in
+ * double, inverse, /P in double, rounded to float. */
+void orc_fdkd_filter(float *img, size_t w, size_t h)
+{
+    const unsigned int MARGIN = 5;
+    const size_t P = orc_fdk_padwidth(w);
+    const size_t padf = (P - w) / 2;
+    float *pad = (float *)calloc(P * h, sizeof(float));
+    for (size_t r = 0; r < h; ++r) memcpy(pad + r * P + padf, img + r * w, w * sizeof(float));
+    for (size_t r = 0; r < MARGIN; ++r) memcpy(pad + r * P, pad + MARGIN * P, P * sizeof(float));
+    for (size_t r = h - MARGIN; r < h; ++r) memcpy(pad + r * P, pad + (h - MARGIN - 1) * P, P * sizeof(float));
+    for (size_t r = 0; r < h; ++r) {
+        for (size_t c = 0; c < MARGIN; ++c) pad[r * P + c] = pad[r * P + MARGIN];
+        for (size_t c = P - MARGIN; c < P; ++c) pad[r * P + c] = pad[r * P + P - MARGIN - 1];
+    }
+    double *ramp = (double *)malloc(sizeof(double) * P);
+    const double DEGTORAD = ORC_DPI / 180.0;
+    for (size_t i = 0; i < P / 2; ++i) ramp[i] = (double)i;
+    for (size_t i = P / 2; i < P; ++i) ramp[i] = (double)(P - i);
+    for (size_t i = 0; i < P; ++i) {
+        ramp[i] *= (0.54 + 0.46 * (cos((double)(unsigned)i * DEGTORAD * 360 / (double)(unsigned)P) + 1));
+        ramp[i] /= (double)(unsigned)P * 0.5;
+    }
+    orc_cplx *tw = orc_twiddles(P);
+#pragma omp parallel
+    {
+        orc_cplx *a = (orc_cplx *)malloc(sizeof(orc_cplx) * P);
+#pragma omp for schedule(static)
+        for (long r = 0; r < (long)h; ++r) {
+            for (size_t i = 0; i < P; ++i) { a[i].re = (double)pad[r * P + i]; a[i].im = 0.0; }
+            orc_fft(a, P, -1, tw);
+            for (size_t c = 0; c < P; ++c) { a[c].re *= ramp[c]; a[c].im *= ramp[c]; }
+            orc_fft(a, P, +1, tw);
+            for (size_t c = 0; c < w; ++c) img[r * w + c] = (float)(a[padf + c].re / (double)(unsigned)P);
+        }
+        free(a);
+    }
+    free(tw); free(ramp); free(pad);
+}
+
+/* Full FDKbp::Process(3D) on one slab (fdkreconbase.cpp:217-277 with
+ * fdkbp.cpp:272-510, 612-631).  Differences from FDKbp_single:
+ *   - scale = SOD/SDD/nProj (float), nProj = projections in the call;
+ *   - img *= sad_sid_2 with sad_sid_2 a double holding the float
+ *     (SOD*SOD)/(SDD*SDD); img *= scale in float;
+ *   - ic, origin, spacing as float expressions; xip/yip/zip, the depth, its
+ *     reciprocal and the sampling positions in double;
+ *   - get_pixel_value_c takes double r, c (r += 0.5, bounds, truncation);
+ *   - volume (float) += dw*dw*val evaluated in double;
+ *   - the window origin of ic[1] uses orc_fdkd_cbct_roi (see there). */
+void orc_fdkd_bp(const float *proj, size_t w, size_t h, size_t np,
+                 const float *angles, const float *weights, const orc_fdk_geom *g,
+                 const size_t *mfirst, const size_t *msecond, float *vol)
+{
+    const size_t nx = (size_t)(g->roi[2] - g->roi[0]), ny = nx;
+    const size_t nz = (size_t)(g->roi[3] - g->roi[1]);
+    float *cv = (float *)calloc(nx * ny * nz, sizeof(float));
+    float *img = (float *)malloc(sizeof(float) * w * h);
+    const float scale = g->sod / g->sdd / (float)np;
+    const double sad_sid_2 = (double)((g->sod * g->sod) / (g->sdd * g->sdd));
+    const float sp[3] = {g->vox[0], g->vox[1], g->vox[2]};
+    const float U = (float)(g->roi[2] - g->roi[0]);
+    const float V = (float)(g->roi[3] - g->roi[1]);
+    float origin[3];
+    origin[0] = -(U - g->center) * sp[0] - sp[0] / 2;
+    origin[1] = -(U - g->center) * sp[1] - sp[1] / 2;
+    origin[2] = -(V - (g->ppoint[1] - (float)g->roi[1])) * sp[2] - sp[2] / 2;
+    uint64_t cb[4];
+    orc_fdkd_cbct_roi(g, ny, cb);
+    const double ic[2] = {(double)(g->ppoint[0] - (float)cb[0]), (double)(g->ppoint[1] - (float)cb[1])};
+    double *xip = (double *)malloc(sizeof(double) * 3 * nx);
+    double *yip = (double *)malloc(sizeof(double) * 3 * ny);
+    double *zip = (double *)malloc(sizeof(double) * 3 * nz);
+    for (size_t ip = 0; ip < np; ++ip) {
+        const float *src = proj + ip * w * h;
+        for (size_t i = 0; i < w * h; ++i) img[i] = src[i] * weights[ip];
+        double pm[12];
+        orc_fdkd_projmatrix(g, angles[ip], pm);
+        orc_fdkd_filter(img, w, h);
+        for (size_t i = 0; i < w * h; ++i) {
+            img[i] = (float)((double)img[i] * sad_sid_2);
+            img[i] *= scale;
+        }
+        for (size_t i = 0; i < nx; ++i) {
+            double x = (double)(origin[0] + (float)(long)i * sp[0]);
+            xip[i * 3 + 0] = x * (pm[0] + ic[0] * pm[8]);
+            xip[i * 3 + 1] = x * (pm[4] + ic[1] * pm[8]);
+            xip[i * 3 + 2] = x * pm[8];
+        }
+        for (size_t j = 0; j < ny; ++j) {
+            double y = (double)(origin[1] + (float)(long)j * sp[1]);
+            yip[j * 3 + 0] = y * (pm[1] + ic[0] * pm[9]);
+            yip[j * 3 + 1] = y * (pm[5] + ic[1] * pm[9]);
+            yip[j * 3 + 2] = y * pm[9];
+        }
+        for (size_t k = 0; k < nz; ++k) {
+            double z = (double)(origin[2] + (float)(long)k * sp[2]);
+            zip[k * 3 + 0] = z * (pm[2] + ic[0] * pm[10]) + ic[0] * pm[11] + pm[3];
+            zip[k * 3 + 1] = z * (pm[6] + ic[1] * pm[10]) + ic[1] * pm[11] + pm[7];
+            zip[k * 3 + 2] = z * pm[10] + pm[11];
+        }
+#pragma omp parallel for schedule(static)
+        for (long k = 0; k < (long)nz; ++k) {
+            for (size_t j = 0; j < ny; ++j) {
+                double acc2[3];
+                acc2[0] = zip[3 * k] + yip[3 * j];
+                acc2[1] = zip[3 * k + 1] + yip[3 * j + 1];
+                acc2[2] = zip[3 * k + 2] + yip[3 * j + 2];
+                float *row = cv + ((size_t)k * ny + j) * nx;
+                for (size_t i = mfirst[j] + 1; i <= msecond[j]; ++i) {
+                    double acc3[3];
+                    acc3[0] = acc2[0] + xip[3 * i];
+                    acc3[1] = acc2[1] + xip[3 * i + 1];
+                    acc3[2] = acc2[2] + xip[3 * i + 2];
+                    const double dw = 1.0 / acc3[2];
+                    double r = acc3[1] * dw, c = acc3[0] * dw;
+                    float val = 0.0f;
+                    r += 0.5;
+                    c += 0.5;
+                    if (!(r < 0) && !(r >= (double)h) && !(c < 0) && !(c >= (double)w))
+                        val = img[(size_t)(int)r * w + (size_t)(int)c];
+                    row[i] = (float)((double)row[i] + dw * dw * (double)val);
+                }
+            }
+        }
+    }
+    for (size_t k = 0; k < nz; ++k)
+        for (size_t j = 0; j < ny; ++j)
+            for (size_t i = 0; i < nx; ++i)
+                vol[((nz - 1 - k) * ny + j) * nx + i] = cv[(k * ny + j) * nx + (nx - 1 - i)];
+    free(cv); free(img); free(xip); free(yip); free(zip);
 }
 
 /* ------------------------------------------------------------------------ */

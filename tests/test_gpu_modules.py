@@ -52,15 +52,16 @@ def test_engine_parallel_matches_direct_and_oracle(orc, dev, bp_name, slices, sb
     assert e[0] <= 1e-4
 
 
-def test_engine_fdk_slabs(orc, dev):
-    """FDKbp_single through the driver: per slab the engine computes the
-    detector window, hands those rows to Process and transfers the slab."""
+@pytest.mark.parametrize("module", ["FDKbp_single", "FDKbp"])
+def test_engine_fdk_slabs(orc, dev, module):
+    """FDKbp_single / FDKbp through the driver: per slab the engine computes
+    the detector window, hands those rows to Process and transfers the slab."""
     from imagingsuite_amd import engine
     n, n_p, slices = 96, 48, 64
     cfg = {"geometry": "cone", "center": 48.0, "roi": f"0 16 {n} {16 + slices}", "resolution": "0.2 0.2",
            "voxelsize": "0.1 0.1 0.1", "sod": 500, "sdd": 1000, "ppoint": "48 48",
            "projection_roi3": n - 1}
-    eng = engine.Engine("FDKbp_single", {"ProjectionBufferSize": 2000, "SliceBlock": 32, "SubVolume": "1 1"},
+    eng = engine.Engine(module, {"ProjectionBufferSize": 2000, "SliceBlock": 32, "SubVolume": "1 1"},
                         cfg)
     rng = np.random.default_rng(3)
     proj = rng.random((n_p, n, n)).astype(np.float32)
@@ -71,7 +72,7 @@ def test_engine_fdk_slabs(orc, dev):
         slab = (0, 16 + 32 * b, n, 48 + 32 * b)
         g = orc.fdk_geom(500.0, 1000.0, (0.2, 0.2), (0.1, 0.1, 0.1), (48.0, 48.0), 48.0, 0, slab, n - 1)
         cb = orc.fdk_cbct_roi(g, n)
-        ref = orc.fdk_bp(np.ascontiguousarray(proj[:, cb[1]:cb[3], :]), angles, weights, g)
+        ref = orc.fdk_bp(np.ascontiguousarray(proj[:, cb[1]:cb[3], :]), angles, weights, g, module=module)
         e = rel_err(vol[32 * b:32 * b + 32], ref)
-        print(f"FDK slab {b}: relRMSE {e[0]:.2e} max {e[1]:.2e}")
+        print(f"{module} slab {b}: relRMSE {e[0]:.2e} max {e[1]:.2e}")
         assert e[0] <= 1e-5

@@ -31,7 +31,7 @@ def test_plugin_quartet():
 
 def test_module_lists():
     bp = engine.module_list("muhrecbp")
-    for name in ["MultiProjBPparallel", "MultiProjBP", "NearestNeighborBP", "FDKbp_single"]:
+    for name in ["MultiProjBPparallel", "MultiProjBP", "NearestNeighborBP", "FDKbp_single", "FDKbp"]:
         assert name in bp
         assert bp[name]["ProjectionBufferSize"] and bp[name]["SliceBlock"]
     assert bp["MultiProjBPparallel"]["ProjectionBufferSize"] == "16"
