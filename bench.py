@@ -249,9 +249,27 @@ def bench_fdk(args):
                      "window_rows": cb[3] - cb[1]})
     r["roofline"] = {"bound": "mfma", "pipe": "fp32 VALU", "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                     "traffic": None, "kernel": "fdk_kernel<16>", "kernel_ms": round(kms, 3),
+                     "traffic": None, "kernel": "fdk_kernel<%s>" % os.environ.get("MR_FDK_KT", "16"),
+                     "kernel_ms": round(kms, 3),
                      "flop_per_update": FDK_FLOP_PER_UPDATE}
     r["kernels_ms"] = {"row_filter_fdk": round(fl_ms / max(fl_n, 1), 3), "fdk_bp": round(kms, 3)}
+    # the double-geometry module on the same slab (FDKbp, fdkbp.cpp)
+    from imagingsuite_amd.recon import FDKbp
+    fdkd = FDKbp(sod, sdd, pix, vox, ppoint, n / 2, roi, n - 1)
+
+    def step_d():
+        work.copy_(proj)
+        fdkd.process(work, angles, weights, vol)
+
+    step_d()
+    torch.cuda.synchronize()
+    _lib.profile(True)
+    _lib.profile_reset()
+    msd = timed(step_d, max(1, args.steps // 2), 0, "fdkbp")
+    bpd_ms, bpd_n = _lib.profile_query("fdkd_bp")
+    _lib.profile(False)
+    r["fdkbp_double"] = {"ms_per_step": round(msd, 3), "GUPS": round(updates / (msd / 1e3) / 1e9, 2),
+                         "kernel": "fdkd_kernel<16>", "kernel_ms": round(bpd_ms / max(bpd_n, 1), 3)}
     emit(r)
 
 

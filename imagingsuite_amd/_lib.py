@@ -28,6 +28,7 @@ EXPORTED_SYMBOLS = (
     "mr_fbp_parallel",
     "mr_fdk_window", "mr_fdk_plan_create", "mr_fdk_plan_destroy", "mr_fdk_plan_window", "mr_fdk_bp",
     "mr_joseph_forward", "mr_joseph_adjoint",
+    "mr_lognorm_reference", "mr_lognorm", "mr_lognorm_logdose",
     "mr_batch_plan_create", "mr_batch_plan_info", "mr_batch_run", "mr_batch_run_graph",
     "mr_batch_run_sequential", "mr_batch_plan_destroy",
 )
@@ -115,6 +116,9 @@ def load():
     L.mr_fdk_bp.argtypes = [vp, vp, sz, sz, fp, fp, vp, vp]
     for f in (L.mr_joseph_forward, L.mr_joseph_adjoint):
         f.argtypes = [ctypes.POINTER(JosephGeom), vp, fp, sz, vp, vp]
+    L.mr_lognorm_reference.argtypes = [vp, vp, sz, f32, vp]
+    L.mr_lognorm.argtypes = [vp, sz, sz, vp, vp, fp, f32, vp]
+    L.mr_lognorm_logdose.argtypes = [vp, sz, sz, vp, vp, fp, vp]
     L.mr_batch_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(BatchStack), sz, i32]
     L.mr_batch_plan_info.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), szp,
                                      ctypes.POINTER(ctypes.c_double)]

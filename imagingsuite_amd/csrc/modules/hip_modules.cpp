@@ -3,6 +3,7 @@
 // Module names, parameter keys and the plugin quartet follow the reference
 // libraries they replace:
 //   ProjectionFilterSingle  frameworks/tomography/Preprocessing/StdPreprocModules/src/ProjectionFilter.cpp:78-375
+//   FullLogNorm             .../StdPreprocModules/src/NormPlugins.cpp:20-558 (non-LUT path)
 //   MultiProjBPparallel, MultiProjBP, NearestNeighborBP
 //                           frameworks/tomography/Backprojectors/StdBackProjectors/src/StdBackProjectorBase.cpp:21-342
 //   FDKbp_single            frameworks/tomography/Backprojectors/FDKBackProjectors/src/fdkreconbase.cpp:20-330
@@ -17,9 +18,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <limits>
 #include <sstream>
+#include <vector>
 
 #include "mr/module_api.h"
 #include "muhrec_hip.h"
@@ -181,6 +184,131 @@ private:
     size_t m_nPaddingDoubler = 2, nImageSize = 0;
     mr_filter_plan *plan_ = nullptr;
     DevBuf buf_;
+};
+
+// -------------------------------------------------------------------------
+// FullLogNorm (NormPlugins.cpp:20-90, 232-558), non-LUT path.  Loading and
+// averaging the open-beam / dark images from files (NormBase::ReferenceLoader)
+// is I/O: the caller hands the averaged images and their doses to
+// SetReferenceImages, the public entry the reference's loader itself ends in.
+// -------------------------------------------------------------------------
+class HipFullLogNorm : public PreprocModuleBase {
+public:
+    explicit HipFullLogNorm(kipl::interactors::InteractionBase *interactor)
+        : PreprocModuleBase("FullLogNorm", interactor)
+    {
+    }
+
+    int Configure(ReconConfig config, std::map<std::string, std::string> parameters) override
+    {
+        m_Config = config;
+        nOBCount = config.ProjectionInfo.nOBCount;
+        nDCCount = config.ProjectionInfo.nDCCount;
+        bUseNormROI = string2bool(GetStringParameter(parameters, "usenormregion"));
+        bUseLUT = optional_bool(parameters, "uselut", false);
+        if (bUseLUT)
+            throw ReconException("FullLogNorm: the look-up-table path (uselut=true) is not provided by the HIP module",
+                                 __FILE__, __LINE__);
+        auto it = parameters.find("referenceaverage");
+        sReferenceAverage = it == parameters.end() ? "ImageWeightedAverage" : it->second;
+        return 0;
+    }
+
+    std::map<std::string, std::string> GetParameters() override
+    {
+        std::map<std::string, std::string> p;
+        p["usenormregion"] = bUseNormROI ? "true" : "false";
+        p["uselut"] = bUseLUT ? "true" : "false";
+        p["referenceaverage"] = sReferenceAverage;
+        return p;
+    }
+
+    bool SetROI(size_t *roi) override
+    {
+        if ((nOBCount != 0 || nDCCount != 0) && !refs_set_)
+            throw ReconException("FullLogNorm: reference images must be given with SetReferenceImages "
+                                 "(file loading is not part of the HIP module)",
+                                 __FILE__, __LINE__);
+        (void)roi;
+        return true;
+    }
+
+    // The averaged references and their doses (NormBase::ReferenceLoader's
+    // results; flat_dose already has dark_dose removed, as there).
+    void SetReferenceImages(kipl::base::TImage<float, 2> dark, kipl::base::TImage<float, 2> flat, float dark_dose,
+                            float flat_dose)
+    {
+        fDarkDose = dark_dose;
+        fFlatDose = flat_dose;
+        npix_ = 0;
+        if (nOBCount != 0) {
+            npix_ = flat.Size();
+            dflat_.reserve(npix_);
+            MRM_HIP(hipMemcpy(dflat_.p, flat.GetDataPtr(), npix_ * sizeof(float), hipMemcpyHostToDevice));
+        }
+        if (nDCCount != 0) {
+            if (npix_ && dark.Size() != npix_)
+                throw ReconException("Dark and flat images differ in size", __FILE__, __LINE__);
+            npix_ = dark.Size();
+            ddark_.reserve(npix_);
+            MRM_HIP(hipMemcpy(ddark_.p, dark.GetDataPtr(), npix_ * sizeof(float), hipMemcpyHostToDevice));
+        }
+        if (nOBCount != 0)
+            MRM_CHECK(mr_lognorm_reference(dflat_.p, nDCCount != 0 ? ddark_.p : nullptr, npix_, fFlatDose, nullptr));
+        refs_set_ = true;
+    }
+
+protected:
+    int ProcessCore(kipl::base::TImage<float, 2> &img, std::map<std::string, std::string> &coeff) override
+    {
+        // the 2-D path replaces a dose <= 0 by 1 (:343-357), the 3-D one a dose < 1
+        float dose = 1.0f;
+        if (bUseNormROI) {
+            dose = GetFloatParameter(coeff, "dose") - fDarkDose;
+            dose = dose <= 0.0f ? 1.0f : dose;
+        }
+        if (dose != dose) throw ReconException("The projection dose is a NaN", __FILE__, __LINE__);
+        const float cLogDose = std::log(dose);
+        const size_t N = img.Size();
+        if ((nOBCount != 0 || nDCCount != 0) && npix_ != N)
+            throw ReconException("Flat field does not match projection", __FILE__, __LINE__);
+        buf_.reserve(N);
+        MRM_HIP(hipMemcpy(buf_.p, img.GetDataPtr(), N * sizeof(float), hipMemcpyHostToDevice));
+        MRM_CHECK(mr_lognorm_logdose(buf_.p, 1, N, nOBCount != 0 ? dflat_.p : nullptr,
+                                     nDCCount != 0 ? ddark_.p : nullptr, &cLogDose, nullptr));
+        MRM_HIP(hipMemcpy(img.GetDataPtr(), buf_.p, N * sizeof(float), hipMemcpyDeviceToHost));
+        return 0;
+    }
+
+    int ProcessCore(kipl::base::TImage<float, 3> &img, std::map<std::string, std::string> &coeff) override
+    {
+        const size_t N = img.Size(0) * img.Size(1), nProj = img.Size(2);
+        if (nOBCount != 0 && npix_ != N) {
+            std::ostringstream msg;
+            msg << "Flat field (" << npix_ << " pixels) does not match projection (" << img.Size(0) << ", "
+                << img.Size(1) << ")";
+            throw ReconException(msg.str(), __FILE__, __LINE__);
+        }
+        if (nDCCount != 0 && npix_ != N) throw ReconException("Dark field does not match projection", __FILE__, __LINE__);
+        std::vector<float> dose(nProj);
+        if (bUseNormROI) GetFloatParameterVector(coeff, "dose", dose.data(), (int)nProj);
+        if (UpdateStatus(0.0f, m_sModuleName)) return 0;
+        buf_.reserve(img.Size());
+        MRM_HIP(hipMemcpy(buf_.p, img.GetDataPtr(), img.Size() * sizeof(float), hipMemcpyHostToDevice));
+        MRM_CHECK(mr_lognorm(buf_.p, nProj, N, nOBCount != 0 ? dflat_.p : nullptr, nDCCount != 0 ? ddark_.p : nullptr,
+                             bUseNormROI ? dose.data() : nullptr, fDarkDose, nullptr));
+        MRM_HIP(hipMemcpy(img.GetDataPtr(), buf_.p, img.Size() * sizeof(float), hipMemcpyDeviceToHost));
+        UpdateStatus(1.0f, m_sModuleName);
+        return 0;
+    }
+
+private:
+    ReconConfig m_Config;
+    size_t nOBCount = 0, nDCCount = 0, npix_ = 0;
+    bool bUseNormROI = true, bUseLUT = false, refs_set_ = false;
+    std::string sReferenceAverage = "ImageWeightedAverage";
+    float fDarkDose = 0.0f, fFlatDose = 1.0f;
+    DevBuf dflat_, ddark_, buf_;
 };
 
 // -------------------------------------------------------------------------
@@ -533,6 +661,7 @@ extern "C" void *GetModule(const char *application, const char *name, void *vint
     try {
         if (app == "muhrec") {
             if (n == "ProjectionFilterSingle") return static_cast<PreprocModuleBase *>(new HipProjectionFilterSingle(interactor));
+            if (n == "FullLogNorm") return static_cast<PreprocModuleBase *>(new HipFullLogNorm(interactor));
             return nullptr;
         }
         if (app == "muhrecbp") {
@@ -573,6 +702,8 @@ extern "C" int GetModuleList(const char *application, void *listptr)
     if (app == "muhrec") {
         HipProjectionFilterSingle f(nullptr);
         (*list)["ProjectionFilterSingle"] = f.GetParameters();
+        HipFullLogNorm g(nullptr);
+        (*list)["FullLogNorm"] = g.GetParameters();
         return 0;
     }
     if (app == "muhrecbp") {

@@ -20,6 +20,7 @@
 #include "mr_filter.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <vector>
 
@@ -33,14 +34,71 @@ struct FdkArgs {
     const int2 *mask;   // (first, second]: reference loop i = first+1 .. second
     int nx, ny, nz, w, h, np;
     float ox, oy, sx, sy;
+    int order, ntx, nty, ntz;
 };
+
+// Block -> tile mapping: order 0 is the 3-D grid; order 1 a 1-D grid whose
+// consecutive blocks go to consecutive XCDs (round robin over 8), each XCD
+// taking a contiguous eighth of the tiles in slice-chunk, row, column order,
+// so the blocks resident on one XCD share detector rows in its L2.
+__device__ __forceinline__ bool fdk_tile(int order, int ntx, int nty, int ntz, int &bx, int &by, int &bz)
+{
+    if (order == 0) {
+        bx = blockIdx.x;
+        by = blockIdx.y;
+        bz = blockIdx.z;
+        return true;
+    }
+    const int n = ntx * nty * ntz;
+    const int per = (n + 7) >> 3;
+    const int b = (int)blockIdx.x;
+    const int t = (b & 7) * per + (b >> 3);
+    if (t >= n) return false;
+    bz = t / (ntx * nty);
+    const int r = t - bz * ntx * nty;
+    by = r / ntx;
+    bx = r - by * ntx;
+    return true;
+}
+
+// Detector samples are read with buffer loads whose range is one projection
+// (h * w floats): the row index is clamped to [-1, h] and floored, so rows
+// outside the window (r < 0 or r >= h, the reference's two tests) give byte
+// offsets outside the range, which the hardware returns as 0.0 - the value
+// get_pixel_value_c returns there.  Inside the window floor(r) = (int)r.
+// The KT row terms are fetched first, then all KT samples are requested
+// before any of them is used, so the loads of one projection are in flight
+// together.
+template <int KT, bool FULL>
+__device__ __forceinline__ void fdk_column(float (&acc)[KT], const float *__restrict__ zt, int kn,
+                                           __amdgpu_buffer_rsrc_t img, float yip1, float xip1, float dw,
+                                           float dw2, float fh, int w4, int cc4)
+{
+    float ztv[KT];
+#pragma unroll
+    for (int kk = 0; kk < KT; ++kk) ztv[kk] = (FULL || kk < kn) ? zt[kk] : 0.0f;
+    float val[KT];
+#pragma unroll
+    for (int kk = 0; kk < KT; ++kk) {
+        const float acc21 = ztv[kk] + yip1;
+        float r = dw * (acc21 + xip1);
+        r = r + 0.5f;
+        const int rr = (int)floorf(__builtin_amdgcn_fmed3f(r, -1.0f, fh));
+        val[kk] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(img, __mul24(rr, w4) + cc4, 0, 0));
+    }
+#pragma unroll
+    for (int kk = 0; kk < KT; ++kk)
+        if (FULL || kk < kn) acc[kk] = acc[kk] + dw2 * val[kk];
+}
 
 template <int KT>
 __global__ __launch_bounds__(256) void fdk_kernel(FdkArgs a)
 {
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int k0 = blockIdx.z * KT;
+    int bx, by, bz;
+    if (!fdk_tile(a.order, a.ntx, a.nty, a.ntz, bx, by, bz)) return;
+    const int i = bx * 64 + (threadIdx.x & 63);
+    const int j = by * 4 + (threadIdx.x >> 6);
+    const int k0 = bz * KT;
     if (i >= a.nx || j >= a.ny) return;
     const int kn = min(KT, a.nz - k0);
     float acc[KT];
@@ -51,6 +109,8 @@ __global__ __launch_bounds__(256) void fdk_kernel(FdkArgs a)
         const float x = a.ox + (float)i * a.sx;
         const float y = a.oy + (float)j * a.sy;
         const float fw = (float)a.w, fh = (float)a.h;
+        const int w4 = 4 * a.w;
+        const int pbytes = 4 * a.w * a.h;
         for (int p = 0; p < a.np; ++p) {
             const float *t = a.ptab + 8 * p;
             const float xip0 = x * t[0], xip1 = x * t[1], xip2 = x * t[2];
@@ -61,21 +121,13 @@ __global__ __launch_bounds__(256) void fdk_kernel(FdkArgs a)
             float c = dw * (acc20 + xip0);
             c = c + 0.5f;
             if (c < 0.0f || c >= fw) continue;
-            const int cc = (int)c;
+            const int cc4 = 4 * (int)c;
             const float dw2 = dw * dw;
-            const float *col = a.img + (size_t)p * a.h * a.w + cc;
+            const __amdgpu_buffer_rsrc_t img = __builtin_amdgcn_make_buffer_rsrc(
+                (void *)(a.img + (size_t)p * a.h * a.w), (short)0, pbytes, 0x00020000);
             const float *zt = a.zt1 + (size_t)p * a.nz + k0;
-#pragma unroll
-            for (int kk = 0; kk < KT; ++kk) {
-                if (kk < kn) {
-                    const float acc21 = zt[kk] + yip1;
-                    float r = dw * (acc21 + xip1);
-                    r = r + 0.5f;
-                    float val = 0.0f;
-                    if (!(r < 0.0f) && r < fh) val = col[(size_t)(int)r * a.w];
-                    acc[kk] = acc[kk] + dw2 * val;
-                }
-            }
+            if (kn == KT) fdk_column<KT, true>(acc, zt, kn, img, yip1, xip1, dw, dw2, fh, w4, cc4);
+            else fdk_column<KT, false>(acc, zt, kn, img, yip1, xip1, dw, dw2, fh, w4, cc4);
         }
     }
 #pragma unroll
@@ -100,14 +152,37 @@ struct FdkdArgs {
     const int2 *mask;
     int nx, ny, nz, w, h, np;
     float ox, oy, sx, sy;
+    int order, ntx, nty, ntz;
 };
+
+template <int KT, bool FULL>
+__device__ __forceinline__ void fdkd_column(float (&acc)[KT], const double *__restrict__ zt, int kn,
+                                            __amdgpu_buffer_rsrc_t img, double yip1, double xip1, double dw,
+                                            double dw2, double fh, int w4, int cc4)
+{
+    float val[KT];
+#pragma unroll
+    for (int kk = 0; kk < KT; ++kk) {
+        const double acc21 = ((FULL || kk < kn) ? zt[kk] : 0.0) + yip1;
+        const double acc31 = acc21 + xip1;
+        double r = acc31 * dw;
+        r = r + 0.5;
+        const int rr = (int)floor(fmin(fmax(r, -1.0), fh));
+        val[kk] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(img, __mul24(rr, w4) + cc4, 0, 0));
+    }
+#pragma unroll
+    for (int kk = 0; kk < KT; ++kk)
+        if (FULL || kk < kn) acc[kk] = (float)((double)acc[kk] + dw2 * (double)val[kk]);
+}
 
 template <int KT>
 __global__ __launch_bounds__(256) void fdkd_kernel(FdkdArgs a)
 {
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int k0 = blockIdx.z * KT;
+    int bx, by, bz;
+    if (!fdk_tile(a.order, a.ntx, a.nty, a.ntz, bx, by, bz)) return;
+    const int i = bx * 64 + (threadIdx.x & 63);
+    const int j = by * 4 + (threadIdx.x >> 6);
+    const int k0 = bz * KT;
     if (i >= a.nx || j >= a.ny) return;
     const int kn = min(KT, a.nz - k0);
     float acc[KT];
@@ -118,6 +193,8 @@ __global__ __launch_bounds__(256) void fdkd_kernel(FdkdArgs a)
         const double x = (double)(a.ox + (float)i * a.sx);
         const double y = (double)(a.oy + (float)j * a.sy);
         const double fw = (double)a.w, fh = (double)a.h;
+        const int w4 = 4 * a.w;
+        const int pbytes = 4 * a.w * a.h;
         for (int p = 0; p < a.np; ++p) {
             const double *t = a.ptab + 8 * p;
             const double xip0 = x * t[0], xip1 = x * t[1], xip2 = x * t[2];
@@ -130,22 +207,13 @@ __global__ __launch_bounds__(256) void fdkd_kernel(FdkdArgs a)
             double c = acc30 * dw;
             c = c + 0.5;
             if (c < 0.0 || c >= fw) continue;
-            const int cc = (int)c;
+            const int cc4 = 4 * (int)c;
             const double dw2 = dw * dw;
-            const float *col = a.img + (size_t)p * a.h * a.w + cc;
+            const __amdgpu_buffer_rsrc_t img = __builtin_amdgcn_make_buffer_rsrc(
+                (void *)(a.img + (size_t)p * a.h * a.w), (short)0, pbytes, 0x00020000);
             const double *zt = a.zt1 + (size_t)p * a.nz + k0;
-#pragma unroll
-            for (int kk = 0; kk < KT; ++kk) {
-                if (kk < kn) {
-                    const double acc21 = zt[kk] + yip1;
-                    const double acc31 = acc21 + xip1;
-                    double r = acc31 * dw;
-                    r = r + 0.5;
-                    float val = 0.0f;
-                    if (!(r < 0.0) && r < fh) val = col[(size_t)(int)r * a.w];
-                    acc[kk] = (float)((double)acc[kk] + dw2 * (double)val);
-                }
-            }
+            if (kn == KT) fdkd_column<KT, true>(acc, zt, kn, img, yip1, xip1, dw, dw2, fh, w4, cc4);
+            else fdkd_column<KT, false>(acc, zt, kn, img, yip1, xip1, dw, dw2, fh, w4, cc4);
         }
     }
 #pragma unroll
@@ -409,19 +477,35 @@ extern "C" int mr_fdk_bp(mr_fdk_plan *plan, float *d_proj, size_t n_proj, size_t
     constexpr int KT = 16;
     dim3 grid((unsigned)((nx + 63) / 64), (unsigned)((nx + 3) / 4), (unsigned)((nz + KT - 1) / KT));
     if (!dbl) {
+        MR_REQUIRE(w * h * 4 < (size_t)1 << 31, "FDK window larger than 2^29 samples");
         mr::FdkArgs a{};
         a.img = d_proj; a.vol = d_volume; a.ptab = plan->d_ptab; a.zt1 = plan->d_zt1; a.mask = plan->d_mask;
         a.nx = (int)nx; a.ny = (int)nx; a.nz = (int)nz; a.w = (int)w; a.h = (int)h; a.np = (int)n_proj;
         a.ox = plan->origin[0]; a.oy = plan->origin[1]; a.sx = plan->p.voxel[0]; a.sy = plan->p.voxel[1];
+        const char *eo = std::getenv("MR_FDK_ORDER");
+        const char *ek = std::getenv("MR_FDK_KT");
+        const int kt = (ek && std::atoi(ek) == 32) ? 32 : 16;
+        a.ntx = (int)((nx + 63) / 64); a.nty = (int)((nx + 3) / 4); a.ntz = (int)((nz + kt - 1) / kt);
+        const long ntiles = (long)a.ntx * a.nty * a.ntz;
+        a.order = (eo && std::atoi(eo) == 0) || ntiles >= (1L << 30) ? 0 : 1;
+        const dim3 g = a.order ? dim3((unsigned)(8 * ((ntiles + 7) / 8)))
+                               : dim3((unsigned)a.ntx, (unsigned)a.nty, (unsigned)a.ntz);
         mr::ProfScope prof("fdk_bp", s);
-        hipLaunchKernelGGL(mr::fdk_kernel<KT>, grid, dim3(256), 0, s, a);
+        if (kt == 32) hipLaunchKernelGGL(mr::fdk_kernel<32>, g, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL(mr::fdk_kernel<16>, g, dim3(256), 0, s, a);
     } else {
+        MR_REQUIRE(w * h * 4 < (size_t)1 << 31, "FDK window larger than 2^29 samples");
         mr::FdkdArgs a{};
         a.img = d_proj; a.vol = d_volume; a.ptab = plan->d_ptabd; a.zt1 = plan->d_zt1d; a.mask = plan->d_mask;
         a.nx = (int)nx; a.ny = (int)nx; a.nz = (int)nz; a.w = (int)w; a.h = (int)h; a.np = (int)n_proj;
         a.ox = plan->origin[0]; a.oy = plan->origin[1]; a.sx = plan->p.voxel[0]; a.sy = plan->p.voxel[1];
+        a.ntx = (int)((nx + 63) / 64); a.nty = (int)((nx + 3) / 4); a.ntz = (int)((nz + KT - 1) / KT);
+        const long ntiles = (long)a.ntx * a.nty * a.ntz;
+        const char *eo = std::getenv("MR_FDK_ORDER");
+        a.order = (eo && std::atoi(eo) == 0) || ntiles >= (1L << 30) ? 0 : 1;
+        const dim3 g = a.order ? dim3((unsigned)(8 * ((ntiles + 7) / 8))) : grid;
         mr::ProfScope prof("fdkd_bp", s);
-        hipLaunchKernelGGL(mr::fdkd_kernel<KT>, grid, dim3(256), 0, s, a);
+        hipLaunchKernelGGL(mr::fdkd_kernel<KT>, g, dim3(256), 0, s, a);
     }
     MR_HIP(hipGetLastError());
     MR_HIP(hipStreamSynchronize(s));

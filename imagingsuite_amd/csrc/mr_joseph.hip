@@ -33,6 +33,46 @@ struct JosephArgs {
     int nx, nz, nd, na;
 };
 
+// Loads below are never predicated: out-of-volume neighbours read a clamped,
+// valid address with weight zero, so the compiler can keep the loads of
+// several unrolled steps in flight together.
+template <int ZT>
+__device__ __forceinline__ void lerp_accumulate(float (&acc)[ZT], const float *p0, const float *p1, float w0,
+                                                float w1)
+{
+#pragma unroll
+    for (int g = 0; g < ZT / 4; ++g) {
+        const float4 v0 = *reinterpret_cast<const float4 *>(p0 + 4 * g);
+        const float4 v1 = *reinterpret_cast<const float4 *>(p1 + 4 * g);
+        acc[4 * g + 0] += w0 * v0.x + w1 * v1.x;
+        acc[4 * g + 1] += w0 * v0.y + w1 * v1.y;
+        acc[4 * g + 2] += w0 * v0.z + w1 * v1.z;
+        acc[4 * g + 3] += w0 * v0.w + w1 * v1.w;
+    }
+}
+
+// Range [lo, hi) of the step index t = 0..N-1 for which the crossing
+// position q(t) = A + B t can fall in [-1, N) (where a voxel pair gets a
+// nonzero weight), widened by two steps against rounding; the loop body
+// still tests every step exactly.
+__device__ __forceinline__ void step_range(float A, float B, int N, int &lo, int &hi)
+{
+    lo = 0;
+    hi = N;
+    if (fabsf(B) > 1e-20f) {
+        float t1 = (-1.0f - A) / B, t2 = ((float)N - A) / B;
+        if (t1 > t2) {
+            const float tmp = t1;
+            t1 = t2;
+            t2 = tmp;
+        }
+        lo = max(0, (int)fmaxf(floorf(t1) - 2.0f, -1.0f));
+        hi = min(N, (int)fminf(ceilf(t2) + 2.0f, (float)N));
+    } else if (A < -1.0f || A >= (float)N) {
+        hi = 0;
+    }
+}
+
 // Forward: one thread per (angle, bin) and ZT slices.
 template <int ZT>
 __global__ __launch_bounds__(256) void joseph_forward_kernel(JosephArgs a)
@@ -51,44 +91,36 @@ __global__ __launch_bounds__(256) void joseph_forward_kernel(JosephArgs a)
     if (fabsf(s) >= fabsf(c)) {
         // step over rows y: x(y) solves u(x, y) = n
         const float inv_s = 1.0f / s;
-        for (int y = 0; y < N; ++y) {
+        int lo, hi;
+        step_range((su + c - fn) * inv_s - 1.0f, c * inv_s, N, lo, hi);
+#pragma unroll 2
+        for (int y = lo; y < hi; ++y) {
             const float xf = (su + c * (float)(y + 1) - fn) * inv_s - 1.0f;
             const float fl = floorf(xf);
             const int x0 = (int)fl;
             const float f = xf - fl;
-            if (x0 < -1 || x0 >= N) continue;
+            const float w0 = (x0 >= 0 && x0 < N) ? 1.0f - f : 0.0f;
+            const float w1 = (x0 + 1 >= 0 && x0 + 1 < N) ? f : 0.0f;
+            const int xa = min(max(x0, 0), N - 1), xb = min(max(x0 + 1, 0), N - 1);
             const float *row = a.src + ((size_t)y * N) * a.nz + z0;
-#pragma unroll
-            for (int g = 0; g < ZT / 4; ++g) {
-                float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;
-                if (x0 >= 0) v0 = *reinterpret_cast<const float4 *>(row + (size_t)x0 * a.nz + 4 * g);
-                if (x0 + 1 < N) v1 = *reinterpret_cast<const float4 *>(row + (size_t)(x0 + 1) * a.nz + 4 * g);
-                acc[4 * g + 0] += (1.0f - f) * v0.x + f * v1.x;
-                acc[4 * g + 1] += (1.0f - f) * v0.y + f * v1.y;
-                acc[4 * g + 2] += (1.0f - f) * v0.z + f * v1.z;
-                acc[4 * g + 3] += (1.0f - f) * v0.w + f * v1.w;
-            }
+            lerp_accumulate<ZT>(acc, row + (size_t)xa * a.nz, row + (size_t)xb * a.nz, w0, w1);
         }
     } else {
         // step over columns x: y(x) solves u(x, y) = n
         const float inv_c = 1.0f / c;
-        for (int x = 0; x < N; ++x) {
+        int lo, hi;
+        step_range((fn - su + s) * inv_c - 1.0f, s * inv_c, N, lo, hi);
+#pragma unroll 2
+        for (int x = lo; x < hi; ++x) {
             const float yf = (fn - su + s * (float)(x + 1)) * inv_c - 1.0f;
             const float fl = floorf(yf);
             const int y0 = (int)fl;
             const float f = yf - fl;
-            if (y0 < -1 || y0 >= N) continue;
+            const float w0 = (y0 >= 0 && y0 < N) ? 1.0f - f : 0.0f;
+            const float w1 = (y0 + 1 >= 0 && y0 + 1 < N) ? f : 0.0f;
+            const int ya = min(max(y0, 0), N - 1), yb = min(max(y0 + 1, 0), N - 1);
             const float *col = a.src + (size_t)x * a.nz + z0;
-#pragma unroll
-            for (int g = 0; g < ZT / 4; ++g) {
-                float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;
-                if (y0 >= 0) v0 = *reinterpret_cast<const float4 *>(col + (size_t)y0 * N * a.nz + 4 * g);
-                if (y0 + 1 < N) v1 = *reinterpret_cast<const float4 *>(col + (size_t)(y0 + 1) * N * a.nz + 4 * g);
-                acc[4 * g + 0] += (1.0f - f) * v0.x + f * v1.x;
-                acc[4 * g + 1] += (1.0f - f) * v0.y + f * v1.y;
-                acc[4 * g + 2] += (1.0f - f) * v0.z + f * v1.z;
-                acc[4 * g + 3] += (1.0f - f) * v0.w + f * v1.w;
-            }
+            lerp_accumulate<ZT>(acc, col + (size_t)ya * N * a.nz, col + (size_t)yb * N * a.nz, w0, w1);
         }
     }
     float *out = a.dst + ((size_t)ang * a.nd + n) * a.nz + z0;
@@ -111,6 +143,7 @@ __global__ __launch_bounds__(256) void joseph_adjoint_kernel(JosephArgs a)
 #pragma unroll
     for (int z = 0; z < ZT; ++z) acc[z] = 0.0f;
     const float xf = (float)(x + 1), yf = (float)(y + 1);
+#pragma unroll 2
     for (int ang = 0; ang < a.na; ++ang) {
         const float4 t = a.tab[ang];
         const float u = (t.x + t.y * yf) - t.z * xf;
@@ -118,20 +151,12 @@ __global__ __launch_bounds__(256) void joseph_adjoint_kernel(JosephArgs a)
         const int n1 = (int)fl;
         const float f = u - fl;
         const float inv_m = t.w;
-        const float w1 = fmaxf(0.0f, 1.0f - f * inv_m) * inv_m;
-        const float w2 = fmaxf(0.0f, 1.0f - (1.0f - f) * inv_m) * inv_m;
-        const float *base = a.src + (size_t)ang * a.nd * a.nz + z0;
         const bool ok1 = n1 >= 0 && n1 < a.nd, ok2 = n1 + 1 >= 0 && n1 + 1 < a.nd;
-#pragma unroll
-        for (int g = 0; g < ZT / 4; ++g) {
-            float4 v1 = make_float4(0.f, 0.f, 0.f, 0.f), v2 = v1;
-            if (ok1) v1 = *reinterpret_cast<const float4 *>(base + (size_t)n1 * a.nz + 4 * g);
-            if (ok2) v2 = *reinterpret_cast<const float4 *>(base + (size_t)(n1 + 1) * a.nz + 4 * g);
-            acc[4 * g + 0] += w1 * v1.x + w2 * v2.x;
-            acc[4 * g + 1] += w1 * v1.y + w2 * v2.y;
-            acc[4 * g + 2] += w1 * v1.z + w2 * v2.z;
-            acc[4 * g + 3] += w1 * v1.w + w2 * v2.w;
-        }
+        const float w1 = ok1 ? fmaxf(0.0f, 1.0f - f * inv_m) * inv_m : 0.0f;
+        const float w2 = ok2 ? fmaxf(0.0f, 1.0f - (1.0f - f) * inv_m) * inv_m : 0.0f;
+        const int na = min(max(n1, 0), a.nd - 1), nb = min(max(n1 + 1, 0), a.nd - 1);
+        const float *base = a.src + (size_t)ang * a.nd * a.nz + z0;
+        lerp_accumulate<ZT>(acc, base + (size_t)na * a.nz, base + (size_t)nb * a.nz, w1, w2);
     }
     float *out = a.dst + ((size_t)y * N + x) * a.nz + z0;
 #pragma unroll

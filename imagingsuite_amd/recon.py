@@ -190,6 +190,53 @@ def fbp_parallel(filt: ProjectionFilterSingle, bp: BackProjector, proj, angles, 
     return volume
 
 
+class FullLogNorm:
+    """FullLogNorm (StdPreprocModules/src/NormPlugins.cpp:300-341, 411-558),
+    non-LUT path, in place on device stacks.  ``flat`` / ``dark`` are the
+    averaged reference images (CUDA [rows, width] float32, or None when no
+    open-beam / dark-current images are configured); ``flat_dose`` and
+    ``dark_dose`` are the doses NormBase::ReferenceLoader computes."""
+
+    def __init__(self, flat=None, dark=None, flat_dose: float = 1.0, dark_dose: float = 0.0,
+                 usenormregion: bool = True, stream=None):
+        _lib.load()
+        self.dark = dark
+        self.dark_dose = float(dark_dose)
+        self.usenormregion = bool(usenormregion)
+        self.flat_log = None
+        if flat is not None:
+            _require_cuda_f32(flat, 2, "flat")
+            if dark is not None:
+                _require_cuda_f32(dark, 2, "dark")
+            self.flat_log = flat.clone()
+            check(_lib.load().mr_lognorm_reference(_ptr(self.flat_log), _ptr(dark) if dark is not None else None,
+                                                   self.flat_log.numel(), float(flat_dose), _stream(stream)))
+        elif dark is not None:
+            _require_cuda_f32(dark, 2, "dark")
+
+    def process(self, proj, dose=None, stream=None):
+        """``proj`` CUDA [n_proj, rows, width], normalised in place; ``dose``
+        the per-projection doses (the "dose" parameter list)."""
+        _require_cuda_f32(proj, 3, "proj")
+        n_proj = proj.shape[0]
+        n_pix = proj.shape[1] * proj.shape[2]
+        for ref in (self.flat_log, self.dark):
+            if ref is not None and ref.numel() != n_pix:
+                raise MrError(_lib.MR_ERR_INVALID, "reference image does not match the projection size")
+        d = None
+        if self.usenormregion:
+            if dose is None:
+                raise MrError(_lib.MR_ERR_INVALID, "usenormregion needs the per-projection dose list")
+            d = _f32arr(dose, n_proj)
+        fp = ctypes.POINTER(ctypes.c_float)
+        check(_lib.load().mr_lognorm(_ptr(proj), n_proj, n_pix,
+                                     _ptr(self.flat_log) if self.flat_log is not None else None,
+                                     _ptr(self.dark) if self.dark is not None else None,
+                                     d.ctypes.data_as(fp) if d is not None else None,
+                                     self.dark_dose, _stream(stream)))
+        return proj
+
+
 class FDKbp_single:
     """FDK cone-beam back-projector for one slab (SetROI(roi) + Process(3D)),
     float geometry (fdkbp_single.cpp)."""

@@ -138,6 +138,10 @@ public:
         float fpPoint[2] = {500.0f, 500.0f};
         bool bCorrectTilt = false;
         float fTiltAngle = 0.0f, fTiltPivotPosition = 0.0f;
+        // reference images (ReconConfig.h:103-113)
+        std::string sReferencePath, sOBFileMask, sDCFileMask;
+        size_t nOBFirstIndex = 1, nOBCount = 0, nDCFirstIndex = 1, nDCCount = 0;
+        size_t dose_roi[4] = {0, 0, 0, 0};
     } ProjectionInfo;
     struct cMatrix {
         float fRotation = 0.0f;

@@ -130,6 +130,28 @@ int mr_fbp_parallel(mr_filter_plan *fplan, mr_bp_plan *bplan, float *d_proj, siz
                     const float *angles, const float *weights, float *d_volume, void *stream);
 
 /* ------------------------------------------------------------------------
+ * FullLogNorm (the normalisation in front of the filter), non-LUT path.
+ * Replaces FullLogNorm::SetReferenceImages and ProcessCore(3D)
+ * (frameworks/tomography/Preprocessing/StdPreprocModules/src/NormPlugins.cpp:300-341, 411-558).
+ * Reading and averaging the reference images (NormBase::ReferenceLoader) is
+ * file I/O and stays with the caller.
+ * ------------------------------------------------------------------------ */
+/* d_flat (device, n_pix) becomes log((flat - dark) / flat_dose), 0 where
+ * flat - dark <= 0.  d_dark may be NULL (no dark current images). */
+int mr_lognorm_reference(float *d_flat, const float *d_dark, size_t n_pix, float flat_dose, void *stream);
+/* In place on d_proj [n_proj][n_pix]: p = img - dark (or img),
+ * img = p <= 0 ? 0 : flat_log - log(p) + log(max(dose[j] - dark_dose, 1)).
+ * d_flat_log / d_dark may be NULL (nOBCount / nDCCount == 0); dose (host,
+ * n_proj values) may be NULL (usenormregion = false: dose term 0). */
+int mr_lognorm(float *d_proj, size_t n_proj, size_t n_pix, const float *d_flat_log, const float *d_dark,
+               const float *dose, float dark_dose, void *stream);
+/* The same with the per-projection dose term given directly (host, n_proj
+ * values, or NULL for 0): the 2-D path's log(dose <= 0 ? 1 : dose)
+ * (ProcessCore(2D), NormPlugins.cpp:343-357). */
+int mr_lognorm_logdose(float *d_proj, size_t n_proj, size_t n_pix, const float *d_flat_log, const float *d_dark,
+                       const float *dose_log, void *stream);
+
+/* ------------------------------------------------------------------------
  * FDK cone-beam back-projection (FDKbp_single and FDKbp).
  * Replaces FdkReconBase::SetROI/Process(3D)
  * (frameworks/tomography/Backprojectors/FDKBackProjectors/src/fdkreconbase.cpp:91-277),

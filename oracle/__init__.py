@@ -95,6 +95,8 @@ def _setup(L):
     L.orc_fdkd_cbct_roi.argtypes = [ctypes.POINTER(FdkGeom), _sz, ctypes.POINTER(ctypes.c_uint64)]
     L.orc_fdkd_filter.argtypes = [_fp, _sz, _sz]
     L.orc_fdkd_bp.argtypes = [_fp, _sz, _sz, _sz, _fp, _fp, ctypes.POINTER(FdkGeom), _sp, _sp, _fp]
+    L.orc_lognorm_flat.argtypes = [_fp, _fp, _sz, ctypes.c_float]
+    L.orc_lognorm_f32.argtypes = [_fp, _sz, _sz, _fp, _fp, _fp, ctypes.c_float]
     L.orc_joseph_forward_f64.argtypes = [_dp, _sz, _sz, _sz, ctypes.c_double, ctypes.c_int, _dp, _sz, _dp]
     L.orc_joseph_adjoint_f64.argtypes = [_dp, _sz, _sz, _sz, ctypes.c_double, ctypes.c_int, _dp, _sz, _dp]
 
@@ -332,6 +334,34 @@ def fdk_bp(proj: np.ndarray, angles, weights, g: FdkGeom, f64: bool = False,
     if module == "FDKbp":
         vol /= n_p
     return vol
+
+
+# --------------------------------------------------------------------------
+# FullLogNorm (non-LUT path)
+# --------------------------------------------------------------------------
+def _fopt(a):
+    return None if a is None else _f(a)
+
+
+def lognorm_flat(flat: np.ndarray, dark: Optional[np.ndarray], flat_dose: float) -> np.ndarray:
+    """FullLogNorm::SetReferenceImages: log((flat - dark) / flat_dose), 0 where <= 0."""
+    f = np.ascontiguousarray(flat, np.float32).copy()
+    d = None if dark is None else np.ascontiguousarray(dark, np.float32)
+    lib().orc_lognorm_flat(_f(f), _fopt(d), f.size, flat_dose)
+    return f
+
+
+def lognorm(proj: np.ndarray, flat_log: Optional[np.ndarray], dark: Optional[np.ndarray],
+            dose: Optional[np.ndarray], dark_dose: float = 0.0) -> np.ndarray:
+    """FullLogNorm::ProcessCore(3D) on [n_proj, rows, width] (copy returned)."""
+    p = np.ascontiguousarray(proj, np.float32).copy()
+    n_p = p.shape[0]
+    n_pix = p.size // n_p
+    fl = None if flat_log is None else np.ascontiguousarray(flat_log, np.float32)
+    dk = None if dark is None else np.ascontiguousarray(dark, np.float32)
+    ds = None if dose is None else np.ascontiguousarray(dose, np.float32)
+    lib().orc_lognorm_f32(_f(p), n_p, n_pix, _fopt(fl), _fopt(dk), _fopt(ds), dark_dose)
+    return p
 
 
 # --------------------------------------------------------------------------

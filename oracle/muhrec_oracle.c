@@ -26,6 +26,7 @@
  *   frameworks/tomography/Backprojectors/FDKBackProjectors/src/fdkreconbase.cpp
  *   frameworks/tomography/Backprojectors/FDKBackProjectors/src/fdkbp_single.cpp
  *   frameworks/tomography/Backprojectors/FDKBackProjectors/src/fdkbp.cpp
+ *   frameworks/tomography/Preprocessing/StdPreprocModules/src/NormPlugins.cpp
  *   frameworks/tomography/Framework/ReconAlgorithms/ReconAlgorithms/linearforwardprojector.cpp
  *
  * Parity pinning of each function is documented beside its tests
@@ -991,6 +992,47 @@ void orc_fdkd_bp(const float *proj, size_t w, size_t h, size_t np,
             for (size_t i = 0; i < nx; ++i)
                 vol[((nz - 1 - k) * ny + j) * nx + i] = cv[(k * ny + j) * nx + (nx - 1 - i)];
     free(cv); free(img); free(xip); free(yip); free(zip);
+}
+
+/* ------------------------------------------------------------------------ */
+/* FullLogNorm (StdPreprocModules/src/NormPlugins.cpp), non-LUT path.  The   */
+/* reference's log(float) resolves to the float overload (sharedbuffer.h     */
+/* pulls <cmath> and "using namespace std" into every kipl image user).      */
+/* ------------------------------------------------------------------------ */
+
+/* FullLogNorm::SetReferenceImages (:300-341): the flat field becomes
+ * log((flat - dark) / flat_dose), 0 where flat - dark <= 0; dark == NULL when
+ * no dark current images are configured (nDCCount == 0). */
+void orc_lognorm_flat(float *flat, const float *dark, size_t n, float flat_dose)
+{
+    const float dose = 1.0f / flat_dose;
+    for (size_t i = 0; i < n; ++i) {
+        const float p = dark ? flat[i] - dark[i] : flat[i];
+        flat[i] = p <= 0 ? 0.0f : logf(p * dose);
+    }
+}
+
+/* FullLogNorm::ProcessCore(3D) (:411-558, bUseLUT == false) on n_proj
+ * projections of n_pix pixels.  flat_log: the prepared flat (NULL when
+ * nOBCount == 0); dark: NULL when nDCCount == 0; dose: per-projection doses
+ * (NULL when usenormregion is false, i.e. one dose log of 0). */
+void orc_lognorm_f32(float *img, size_t n_proj, size_t n_pix, const float *flat_log, const float *dark,
+                     const float *dose, float dark_dose)
+{
+#pragma omp parallel for schedule(static)
+    for (long j = 0; j < (long)n_proj; ++j) {
+        float dl = 0.0f;
+        if (dose) {
+            const float d = dose[j] - dark_dose;
+            dl = logf(d < 1 ? 1.0f : d);
+        }
+        float *p = img + (size_t)j * n_pix;
+        for (size_t i = 0; i < n_pix; ++i) {
+            const float v = dark ? p[i] - dark[i] : p[i];
+            if (v <= 0.0f) p[i] = 0.0f;
+            else p[i] = flat_log ? flat_log[i] - logf(v) + dl : -logf(v) + dl;
+        }
+    }
 }
 
 /* ------------------------------------------------------------------------ */

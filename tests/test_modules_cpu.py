@@ -40,6 +40,8 @@ def test_module_lists():
     assert set(pre["ProjectionFilterSingle"]) == {"filtertype", "cutoff", "order", "usebias",
                                                   "biasweight", "paddingdoubler"}
     assert pre["ProjectionFilterSingle"]["filtertype"] == "Hamming"
+    assert set(pre["FullLogNorm"]) == {"usenormregion", "uselut", "referenceaverage"}
+    assert pre["FullLogNorm"]["uselut"] == "false"
     assert engine.module_list("other")["__rc__"] == -1
 
 
