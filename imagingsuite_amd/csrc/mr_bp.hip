@@ -165,7 +165,7 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a)
 // LDS-staged variant.  A block owns a 16 x 16 voxel tile and ZT slices.  For
 // each chunk of KC projections it stages, per projection, the SPAN detector
 // columns the tile can reach (the tile diagonal spans at most 15*sqrt(2)+2
-// columns) x ZT slices into LDS, then every thread gathers its two columns
+// columns, plus one column of rounding margin below) x ZT slices into LDS, then every thread gathers its two columns
 // from LDS with 16-byte reads.  Rows of the staged tile are ZT+4 floats apart,
 // which spreads 16 distinct columns over distinct LDS banks.
 // EXACT = true keeps the reference's float operation order per voxel
@@ -213,7 +213,9 @@ __device__ __forceinline__ void bp_lds_tile(const BpArgs &a, int bx, int by, int
                 const float la = t.x + t.y * ya, lb = t.x + t.y * yb;
                 int n0 = min(min((int)(la - t.z * xa), (int)(la - t.z * xb)),
                              min((int)(lb - t.z * xa), (int)(lb - t.z * xb)));
-                sb0[threadIdx.x] = max(n0, 0);
+                // one bin of margin: an interior voxel may round one ulp below a
+                // corner and truncate to the bin below
+                sb0[threadIdx.x] = max(n0 - 1, 0);
             }
             __syncthreads();
             for (int e = threadIdx.x; e < kc * SPAN * G; e += 256) {
@@ -312,8 +314,9 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(BpArgs a)
 //           which equals the reference's acc + (A*(1-w) + B*w) bit for bit
 //           (x + 0 and 0 + x are exact), MultiProjBPparallel.cpp:149-161.
 // Both assume finite projection values (Inf * 0 would give NaN).
-// The tile's columns span at most 31*sqrt 2 + 1 < 45 bins, and a group reads
-// up to 3 past its lowest bin: 48 staged columns plus 2 of margin.
+// The tile's columns span at most 31*sqrt 2 + 1 < 45 bins, a group reads up
+// to 3 past its lowest bin and one rounding-margin column is staged below the
+// lowest: 49 staged columns plus 1 spare.
 constexpr int kQuadSpan = 50;
 
 template <int ZT, bool EXACT, int KC>
@@ -361,7 +364,9 @@ __device__ __forceinline__ void bp_quad_tile(const BpArgs &a, int bx, int by, in
                 const float la = t.x + t.y * ya, lb = t.x + t.y * yb;
                 int n0 = min(min((int)(la - t.z * xa), (int)(la - t.z * xb)),
                              min((int)(lb - t.z * xa), (int)(lb - t.z * xb)));
-                sb0[threadIdx.x] = max(n0, 0);
+                // one bin of margin: an interior voxel may round one ulp below a
+                // corner and truncate to the bin below
+                sb0[threadIdx.x] = max(n0 - 1, 0);
             }
             __syncthreads();
             for (int e = threadIdx.x; e < kc * SPAN * G; e += 256) {
@@ -460,23 +465,41 @@ __global__ __launch_bounds__(256) void bp_quad_kernel(BpArgs a)
     bp_quad_tile<ZT, EXACT, KC>(a, bx, by, bz, sm, sb0);
 }
 
-// Persistent batched back-projection: a grid sized to the device pulls work
-// items (stack, tile x, tile y, slice chunk) from a global counter until the
-// list is exhausted; every block reaches the exit test each iteration.
+// Persistent batched back-projection: a grid sized to the device (every
+// block resident at once, so block b runs on XCD b % 8 by the round-robin
+// dispatch) pulls work items (stack, tile x, tile y, slice chunk) from eight
+// partitions, one per XCD.  Each partition is a contiguous, spatially ordered
+// run of the item list with about an eighth of the work, so the blocks of
+// one XCD stage overlapping projection spans in its L2.  A block whose
+// partition is exhausted moves on to the next one; after at most eight
+// empty partitions it leaves, so every block reaches the exit.
 template <int ZT, int INTERP, bool EXACT, int KC>
 __global__ __launch_bounds__(256) void bp_batch_kernel(const BpArgs *__restrict__ stacks,
-                                                       const int4 *__restrict__ items, int n_items,
-                                                       int *counter)
+                                                       const int4 *__restrict__ items,
+                                                       const int *__restrict__ part_begin, int *counters)
 {
     __shared__ float sm[KC][kBpSpan][ZT + 4];
     __shared__ int sb0[KC];
     __shared__ int item;
+    int part = (int)(blockIdx.x & 7);
     for (;;) {
-        if (threadIdx.x == 0) item = atomicAdd(counter, 1);
+        if (threadIdx.x == 0) {
+            int it = -1;
+            for (int tries = 0; tries < 8; ++tries) {
+                const int k = atomicAdd(&counters[part], 1);
+                const int idx = part_begin[part] + k;
+                if (idx < part_begin[part + 1]) {
+                    it = idx;
+                    break;
+                }
+                part = (part + 1) & 7;
+            }
+            item = it;
+        }
         __syncthreads();
         const int it = item;
         __syncthreads();
-        if (it >= n_items) break;
+        if (it < 0) break;
         const int4 w = items[it];
         const BpArgs a = stacks[w.x];
         bp_lds_tile<ZT, INTERP, EXACT, KC>(a, w.y, w.z, w.w, sm, sb0);
@@ -484,7 +507,12 @@ __global__ __launch_bounds__(256) void bp_batch_kernel(const BpArgs *__restrict_
     }
 }
 
-__global__ void reset_counter_kernel(int *counter) { *counter = 0; }
+// Clears the eight partition counters (a kernel, not a memset, so that the
+// launch sequence can be captured in a graph).
+__global__ void reset_counter_kernel(int *counter)
+{
+    if (threadIdx.x < 8) counter[threadIdx.x] = 0;
+}
 
 template <int ZT, int INTERP>
 static int launch_bp(const BpArgs &a, hipStream_t s)
@@ -842,18 +870,22 @@ struct BatchStack {
 struct Variant {
     int zt, kc;
 };
-const Variant kVariants[] = {{16, 16}, {16, 8}, {16, 32}, {8, 16}, {8, 32}};
+const Variant kVariants[] = {{32, 4}, {32, 8}, {16, 8}, {16, 4}, {16, 16}};
+constexpr int kNumVariants = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
+// item list index of a slice tile (lists for ZT = 32 and 16)
+inline int zt_list(int zt) { return zt == 32 ? 0 : 1; }
 
 }  // namespace
 
 struct mr_batch_plan {
     std::vector<BatchStack> stacks;
     mr::BpArgs *d_args = nullptr;
-    int4 *d_items[2] = {nullptr, nullptr};   // item lists for ZT = 16 and ZT = 8
+    int4 *d_items[2] = {nullptr, nullptr};   // item lists for ZT = 32 and ZT = 16
+    int *d_parts[2] = {nullptr, nullptr};    // 9 partition boundaries per list
     int n_items[2] = {0, 0};
-    int *d_counter = nullptr;
+    int *d_counter = nullptr;                // 8 partition counters
     int variant = 0;
-    int grid[sizeof(kVariants) / sizeof(kVariants[0])] = {0};
+    int grid[kNumVariants] = {0};
     double updates = 0.0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -865,20 +897,20 @@ namespace {
 int batch_launch_bp(mr_batch_plan *p, int variant, hipStream_t s)
 {
     const Variant v = kVariants[variant];
-    const int li = v.zt == 16 ? 0 : 1;
-    hipLaunchKernelGGL(mr::reset_counter_kernel, dim3(1), dim3(1), 0, s, p->d_counter);
+    const int li = zt_list(v.zt);
+    hipLaunchKernelGGL(mr::reset_counter_kernel, dim3(1), dim3(64), 0, s, p->d_counter);
     MR_HIP(hipGetLastError());
     mr::ProfScope prof("bp_batch", s);
     dim3 grid((unsigned)p->grid[variant]), block(256);
 #define MR_BATCH_CASE(ZT, KC)                                                                         \
     if (v.zt == ZT && v.kc == KC)                                                                     \
         hipLaunchKernelGGL((mr::bp_batch_kernel<ZT, 0, true, KC>), grid, block, 0, s, p->d_args,       \
-                           p->d_items[li], p->n_items[li], p->d_counter);
-    MR_BATCH_CASE(16, 16)
+                           p->d_items[li], p->d_parts[li], p->d_counter);
+    MR_BATCH_CASE(32, 4)
+    MR_BATCH_CASE(32, 8)
     MR_BATCH_CASE(16, 8)
-    MR_BATCH_CASE(16, 32)
-    MR_BATCH_CASE(8, 16)
-    MR_BATCH_CASE(8, 32)
+    MR_BATCH_CASE(16, 4)
+    MR_BATCH_CASE(16, 16)
 #undef MR_BATCH_CASE
     MR_HIP(hipGetLastError());
     return MR_OK;
@@ -910,7 +942,8 @@ void batch_free(mr_batch_plan *p)
                           (void *)b.d_idx, (void *)b.d_w})
             if (ptr) (void)hipFree(ptr);
     }
-    for (void *ptr : {(void *)p->d_args, (void *)p->d_items[0], (void *)p->d_items[1], (void *)p->d_counter})
+    for (void *ptr : {(void *)p->d_args, (void *)p->d_items[0], (void *)p->d_items[1], (void *)p->d_parts[0],
+                      (void *)p->d_parts[1], (void *)p->d_counter})
         if (ptr) (void)hipFree(ptr);
     delete p;
 }
@@ -937,7 +970,7 @@ extern "C" int mr_batch_plan_create(mr_batch_plan **plan, const mr_batch_stack *
     *plan = nullptr;
     auto *p = new mr_batch_plan();
     p->stacks.resize(n_stacks);
-    std::vector<std::pair<double, int4>> items16, items8;
+    std::vector<std::pair<double, int4>> items[2];   // ZT = 32, ZT = 16
     for (size_t i = 0; i < n_stacks; ++i) {
         BatchStack &b = p->stacks[i];
         b.desc = stacks[i];
@@ -989,33 +1022,47 @@ extern "C" int mr_batch_plan_create(mr_batch_plan **plan, const mr_batch_stack *
         a.nu = (int)d.width; a.nv = (int)d.slices; a.vpad = (int)b.vpad; a.nk = (int)b.n_used;
         a.nz4 = (int)((d.slices / 4) * 4); a.bstride = (int)b.vpad; a.accumulate = 0;
         p->updates += (double)d.width * d.width * d.slices * d.n_proj;
-        // work items, cost ~ projections x voxels inside the circle x slices
+        // work items in spatial order (slice chunk, stripes of 4 tile
+        // columns, rows, columns), cost ~ projections x voxels inside the circle
         const int tiles = (int)((d.width + 15) / 16);
-        for (int zt = 0; zt < 2; ++zt) {
-            const int ZT = zt == 0 ? 16 : 8;
+        for (int li = 0; li < 2; ++li) {
+            const int ZT = li == 0 ? 32 : 16;
             const int zc = (int)((d.slices + ZT - 1) / ZT);
             for (int bz = 0; bz < zc; ++bz)
-                for (int by = 0; by < tiles; ++by)
-                    for (int bx = 0; bx < tiles; ++bx) {
-                        double inside = 0;
-                        for (int yy = by * 16; yy < std::min<int>((int)d.width, by * 16 + 16); ++yy) {
-                            const int lo = std::max(m[yy].x, bx * 16), hi = std::min(m[yy].y, bx * 16 + 16);
-                            inside += std::max(0, hi - lo);
+                for (int sx = 0; sx < tiles; sx += 4)
+                    for (int by = 0; by < tiles; ++by)
+                        for (int bx = sx; bx < std::min(tiles, sx + 4); ++bx) {
+                            double inside = 0;
+                            for (int yy = by * 16; yy < std::min<int>((int)d.width, by * 16 + 16); ++yy) {
+                                const int lo = std::max(m[yy].x, bx * 16), hi = std::min(m[yy].y, bx * 16 + 16);
+                                inside += std::max(0, hi - lo);
+                            }
+                            const double cost = 1.0 + inside * (double)b.n_used * std::min<int>(ZT, (int)d.slices - bz * ZT);
+                            items[li].push_back({cost, make_int4((int)i, bx, by, bz)});
                         }
-                        const double cost = 1.0 + inside * (double)b.n_used;
-                        (zt == 0 ? items16 : items8).push_back({cost, make_int4((int)i, bx, by, bz)});
-                    }
         }
     }
     for (int li = 0; li < 2; ++li) {
-        auto &v = li == 0 ? items16 : items8;
-        std::stable_sort(v.begin(), v.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+        // eight contiguous partitions of equal cost, one per XCD
+        const auto &v = items[li];
+        double total = 0;
+        for (const auto &it : v) total += it.first;
+        std::vector<int> parts(9, (int)v.size());
+        parts[0] = 0;
+        double acc = 0;
+        int next = 1;
+        for (size_t k = 0; k < v.size() && next < 8; ++k) {
+            acc += v[k].first;
+            while (next < 8 && acc >= total * next / 8.0) parts[next++] = (int)k + 1;
+        }
         std::vector<int4> flat(v.size());
         for (size_t k = 0; k < v.size(); ++k) flat[k] = v[k].second;
         p->n_items[li] = (int)flat.size();
         hipError_t e = hipMalloc(&p->d_items[li], sizeof(int4) * std::max<size_t>(flat.size(), 1));
-        if (e == hipSuccess)
+        if (e == hipSuccess && !flat.empty())
             e = hipMemcpy(p->d_items[li], flat.data(), sizeof(int4) * flat.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMalloc(&p->d_parts[li], sizeof(int) * 9);
+        if (e == hipSuccess) e = hipMemcpy(p->d_parts[li], parts.data(), sizeof(int) * 9, hipMemcpyHostToDevice);
         if (e != hipSuccess) {
             batch_free(p);
             return mr::hip_fail(e, "mr_batch_plan_create items", __FILE__, __LINE__);
@@ -1025,17 +1072,17 @@ extern "C" int mr_batch_plan_create(mr_batch_plan **plan, const mr_batch_stack *
     for (size_t i = 0; i < n_stacks; ++i) args[i] = p->stacks[i].args;
     hipError_t e = hipMalloc(&p->d_args, sizeof(mr::BpArgs) * n_stacks);
     if (e == hipSuccess) e = hipMemcpy(p->d_args, args.data(), sizeof(mr::BpArgs) * n_stacks, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMalloc(&p->d_counter, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&p->d_counter, sizeof(int) * 8);
     if (e != hipSuccess) {
         batch_free(p);
         return mr::hip_fail(e, "mr_batch_plan_create args", __FILE__, __LINE__);
     }
     int rc = MR_OK;
-    if (rc == MR_OK) rc = occupancy_grid<16, 16>(p->grid[0]);
-    if (rc == MR_OK) rc = occupancy_grid<16, 8>(p->grid[1]);
-    if (rc == MR_OK) rc = occupancy_grid<16, 32>(p->grid[2]);
-    if (rc == MR_OK) rc = occupancy_grid<8, 16>(p->grid[3]);
-    if (rc == MR_OK) rc = occupancy_grid<8, 32>(p->grid[4]);
+    if (rc == MR_OK) rc = occupancy_grid<32, 4>(p->grid[0]);
+    if (rc == MR_OK) rc = occupancy_grid<32, 8>(p->grid[1]);
+    if (rc == MR_OK) rc = occupancy_grid<16, 8>(p->grid[2]);
+    if (rc == MR_OK) rc = occupancy_grid<16, 4>(p->grid[3]);
+    if (rc == MR_OK) rc = occupancy_grid<16, 16>(p->grid[4]);
     if (rc != MR_OK) {
         batch_free(p);
         return rc;
@@ -1049,8 +1096,7 @@ extern "C" int mr_batch_plan_create(mr_batch_plan **plan, const mr_batch_stack *
         MR_HIP(hipEventCreate(&a1));
         rc = batch_prepare(p, s);
         float best = 1e30f;
-        const int nv = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
-        for (int v = 0; v < nv && rc == MR_OK; ++v) {
+        for (int v = 0; v < kNumVariants && rc == MR_OK; ++v) {
             rc = batch_launch_bp(p, v, s);  // warm-up
             if (rc != MR_OK) break;
             MR_HIP(hipEventRecord(a0, s));
@@ -1085,7 +1131,7 @@ extern "C" int mr_batch_plan_info(const mr_batch_plan *plan, int *slice_tile, in
     const Variant v = kVariants[plan->variant];
     if (slice_tile) *slice_tile = v.zt;
     if (proj_chunk) *proj_chunk = v.kc;
-    if (n_items) *n_items = (size_t)plan->n_items[v.zt == 16 ? 0 : 1];
+    if (n_items) *n_items = (size_t)plan->n_items[zt_list(v.zt)];
     if (updates) *updates = plan->updates;
     return MR_OK;
     MR_GUARD_END

@@ -487,7 +487,8 @@ extern "C" int mr_fdk_bp(mr_fdk_plan *plan, float *d_proj, size_t n_proj, size_t
         const int kt = (ek && std::atoi(ek) == 32) ? 32 : 16;
         a.ntx = (int)((nx + 63) / 64); a.nty = (int)((nx + 3) / 4); a.ntz = (int)((nz + kt - 1) / kt);
         const long ntiles = (long)a.ntx * a.nty * a.ntz;
-        a.order = (eo && std::atoi(eo) == 0) || ntiles >= (1L << 30) ? 0 : 1;
+        // grid order measured faster for the float kernel (1024^3: 218 vs 236 ms)
+        a.order = (eo && std::atoi(eo) == 1) && ntiles < (1L << 30) ? 1 : 0;
         const dim3 g = a.order ? dim3((unsigned)(8 * ((ntiles + 7) / 8)))
                                : dim3((unsigned)a.ntx, (unsigned)a.nty, (unsigned)a.ntz);
         mr::ProfScope prof("fdk_bp", s);
@@ -502,6 +503,7 @@ extern "C" int mr_fdk_bp(mr_fdk_plan *plan, float *d_proj, size_t n_proj, size_t
         a.ntx = (int)((nx + 63) / 64); a.nty = (int)((nx + 3) / 4); a.ntz = (int)((nz + KT - 1) / KT);
         const long ntiles = (long)a.ntx * a.nty * a.ntz;
         const char *eo = std::getenv("MR_FDK_ORDER");
+        // XCD order measured faster for the double kernel (1024^3: 258 vs 302 ms)
         a.order = (eo && std::atoi(eo) == 0) || ntiles >= (1L << 30) ? 0 : 1;
         const dim3 g = a.order ? dim3((unsigned)(8 * ((ntiles + 7) / 8))) : grid;
         mr::ProfScope prof("fdkd_bp", s);

@@ -95,4 +95,121 @@ __device__ __forceinline__ void fft_lds(float2 *a, const float2 *__restrict__ tw
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Second LDS FFT: radix-8 Stockham passes (one radix-2 or radix-4 pass first
+// when log2 P is not a multiple of 3), so P = 8192 takes 5 passes instead of
+// 7.  The line is stored with one pad entry every 16 (index lpad(i)), which
+// spreads the strided writes of the early passes over all LDS banks.  The
+// twiddles come from an LDS copy of the first P/8 entries of the table
+// (w = exp(-+2 pi i base / P), base < P/8 in every radix-8 pass); the other
+// six powers of a butterfly are products of w (at most three multiplications
+// deep).  Length of the padded line: lds_len(P).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
+constexpr int lds_len(int P) { return P + P / 16; }
+
+template <bool INV>
+__device__ __forceinline__ void dft4(float2 &v0, float2 &v1, float2 &v2, float2 &v3)
+{
+    const float2 a0 = cadd(v0, v2), a1 = csub(v0, v2), a2 = cadd(v1, v3), d = csub(v1, v3);
+    const float2 a3 = INV ? make_float2(-d.y, d.x) : make_float2(d.y, -d.x);
+    v0 = cadd(a0, a2);
+    v2 = csub(a0, a2);
+    v1 = cadd(a1, a3);
+    v3 = csub(a1, a3);
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft_r(float2 (&v)[R])
+{
+    if constexpr (R == 2) {
+        const float2 t = v[0];
+        v[0] = cadd(t, v[1]);
+        v[1] = csub(t, v[1]);
+    } else if constexpr (R == 4) {
+        dft4<INV>(v[0], v[1], v[2], v[3]);
+    } else {
+        constexpr float c = 0.70710678118654752440f;
+        float2 a[4], b[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            a[r] = cadd(v[r], v[r + 4]);
+            b[r] = csub(v[r], v[r + 4]);
+        }
+        // b_r *= exp(-+2 pi i r / 8)
+        b[1] = INV ? make_float2(c * (b[1].x - b[1].y), c * (b[1].x + b[1].y))
+                   : make_float2(c * (b[1].x + b[1].y), c * (b[1].y - b[1].x));
+        b[2] = INV ? make_float2(-b[2].y, b[2].x) : make_float2(b[2].y, -b[2].x);
+        b[3] = INV ? make_float2(-c * (b[3].x + b[3].y), c * (b[3].x - b[3].y))
+                   : make_float2(c * (b[3].y - b[3].x), -c * (b[3].x + b[3].y));
+        dft4<INV>(a[0], a[1], a[2], a[3]);
+        dft4<INV>(b[0], b[1], b[2], b[3]);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            v[2 * m] = a[m];
+            v[2 * m + 1] = b[m];
+        }
+    }
+}
+
+template <int P, int NT, int R, bool INV>
+__device__ __forceinline__ void fft_pass(float2 *a, const float2 *__restrict__ twl, int ns, int lns)
+{
+    constexpr int B = P / R;
+    constexpr int PER = (B + NT - 1) / NT;
+    constexpr int LR = ilog2c(R);
+    constexpr int L = ilog2c(P);
+    const int t = threadIdx.x;
+    float2 v[PER][R];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int j = t + q * NT;
+        if (j < B) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[q][r] = a[lpad(j + r * B)];
+            if (R == 8 && ns > 1) {
+                const int k = j & (ns - 1);
+                float2 w1 = twl[k << (L - 3 - lns)];
+                if (INV) w1.y = -w1.y;
+                const float2 w2 = cmul(w1, w1), w3 = cmul(w1, w2), w4 = cmul(w2, w2);
+                const float2 w5 = cmul(w1, w4), w6 = cmul(w3, w3), w7 = cmul(w3, w4);
+                const float2 w[8] = {make_float2(1.f, 0.f), w1, w2, w3, w4, w5, w6, w7};
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], w[r]);
+            }
+            dft_r<R, INV>(v[q]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int j = t + q * NT;
+        if (j < B) {
+            const int k = j & (ns - 1);
+            const int d0 = ((j >> lns) << (lns + LR)) + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) a[lpad(d0 + r * ns)] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <int P, int NT, bool INV>
+__device__ __forceinline__ void fft_lds2(float2 *a, const float2 *__restrict__ twl)
+{
+    constexpr int L = ilog2c(P);
+    int ns = 1, lns = 0;
+    if constexpr (L % 3 == 1) {
+        fft_pass<P, NT, 2, INV>(a, twl, 1, 0);
+        ns = 2;
+        lns = 1;
+    } else if constexpr (L % 3 == 2) {
+        fft_pass<P, NT, 4, INV>(a, twl, 1, 0);
+        ns = 4;
+        lns = 2;
+    }
+    for (; ns < P; ns <<= 3, lns += 3) fft_pass<P, NT, 8, INV>(a, twl, ns, lns);
+}
+
 }  // namespace mr

@@ -13,6 +13,7 @@
 #include "mr_filter.h"
 
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 namespace mr {
@@ -94,6 +95,62 @@ __global__ __launch_bounds__(NT) void row_filter_kernel(RowFilterArgs a)
     }
 }
 
+// Same filter on the radix-8 padded FFT (fft_lds2): dynamic LDS holds the
+// padded line (lds_len(P) float2) followed by the P/8 twiddles it uses.
+template <int P, int NT>
+__global__ __launch_bounds__(NT) void row_filter_kernel2(RowFilterArgs a)
+{
+    extern __shared__ float2 line[];
+    float2 *twl = line + lds_len(P);
+    const long r0 = 2 * (long)blockIdx.x;
+    const bool has1 = r0 + 1 < a.n_rows;
+    float *row0 = a.rows + r0 * a.n;
+    float *row1 = has1 ? row0 + a.n : row0;
+    const int t = threadIdx.x;
+    for (int i = t; i < P / 8; i += NT) twl[i] = a.tw[i];
+    if (a.mode == 0) {
+        const float f0 = row0[0], l0 = row0[a.n - 1];
+        const float f1 = row1[0], l1 = row1[a.n - 1];
+        for (int i = t; i < P; i += NT) {
+            const float x0 = pb_pad(row0, a.n, i, a.insert, P, a.taper, f0, l0);
+            const float x1 = has1 ? pb_pad(row1, a.n, i, a.insert, P, a.taper, f1, l1) : 0.0f;
+            line[lpad(i)] = make_float2(x0, x1);
+        }
+    } else {
+        const float w0 = a.pre_w ? a.pre_w[r0 / a.rows_per_proj] : 1.0f;
+        const float w1 = (a.pre_w && has1) ? a.pre_w[(r0 + 1) / a.rows_per_proj] : 1.0f;
+        for (int i = t; i < P; i += NT) {
+            const float x0 = fdk_pad(row0, a.n, i, a.insert, P, w0);
+            const float x1 = has1 ? fdk_pad(row1, a.n, i, a.insert, P, w1) : 0.0f;
+            line[lpad(i)] = make_float2(x0, x1);
+        }
+    }
+    __syncthreads();
+    fft_lds2<P, NT, false>(line, twl);
+    for (int i = t; i < P; i += NT) {
+        const float h = a.hs[i];
+        const float2 v = line[lpad(i)];
+        line[lpad(i)] = make_float2(v.x * h, v.y * h);
+    }
+    __syncthreads();
+    fft_lds2<P, NT, true>(line, twl);
+    float *o0 = a.out ? a.out + r0 * a.n : row0;
+    float *o1 = o0 + a.n;
+    if (a.mode == 0) {
+        for (int i = t; i < a.n; i += NT) {
+            const float2 v = line[lpad(i + a.insert)];
+            o0[i] = v.x * a.post0;
+            if (has1) o1[i] = v.y * a.post0;
+        }
+    } else {
+        for (int i = t; i < a.n; i += NT) {
+            const float2 v = line[lpad(i + a.insert)];
+            o0[i] = ((v.x * a.post0) * a.post1) * a.post2;
+            if (has1) o1[i] = ((v.y * a.post0) * a.post1) * a.post2;
+        }
+    }
+}
+
 // FDK margin rows: rows 0..4 take the filtered row 5, rows h-5..h-1 the
 // filtered row h-6 (fdkbp_single.cpp:880-886).  Filtering is row-local, so
 // copying the filtered rows equals filtering the copied raw rows.
@@ -112,18 +169,29 @@ template <int P>
 static int launch_row_filter(const RowFilterArgs &a, hipStream_t s)
 {
     constexpr int NT = (P / 16) < 64 ? 64 : ((P / 16) > 512 ? 512 : (P / 16));
-    const size_t lds = sizeof(float2) * P;
-    if (lds > 65536)
-        MR_HIP(hipFuncSetAttribute((const void *)row_filter_kernel<P, NT>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const long pairs = (a.n_rows + 1) / 2;
     if (pairs == 0) return MR_OK;
     if (pairs > 0x7fffffffL) {
         set_error("too many rows for one launch");
         return MR_ERR_INVALID;
     }
+    // MR_FILTER_IMPL=1 selects the radix-4 FFT (comparison runs)
+    const char *e = std::getenv("MR_FILTER_IMPL");
+    const bool radix4 = e && std::atoi(e) == 1;
     ProfScope prof(a.mode == 0 ? "row_filter_pb" : "row_filter_fdk", s);
-    hipLaunchKernelGGL((row_filter_kernel<P, NT>), dim3((unsigned)pairs), dim3(NT), lds, s, a);
+    if (radix4) {
+        const size_t lds = sizeof(float2) * P;
+        if (lds > 65536)
+            MR_HIP(hipFuncSetAttribute((const void *)row_filter_kernel<P, NT>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL((row_filter_kernel<P, NT>), dim3((unsigned)pairs), dim3(NT), lds, s, a);
+    } else {
+        const size_t lds = sizeof(float2) * (lds_len(P) + P / 8);
+        if (lds > 65536)
+            MR_HIP(hipFuncSetAttribute((const void *)row_filter_kernel2<P, NT>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL((row_filter_kernel2<P, NT>), dim3((unsigned)pairs), dim3(NT), lds, s, a);
+    }
     MR_HIP(hipGetLastError());
     return MR_OK;
 }

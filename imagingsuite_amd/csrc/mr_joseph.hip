@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 namespace mr {
@@ -165,6 +166,102 @@ __global__ __launch_bounds__(256) void joseph_adjoint_kernel(JosephArgs a)
             make_float4(acc[4 * g], acc[4 * g + 1], acc[4 * g + 2], acc[4 * g + 3]);
 }
 
+// LDS-staged adjoint: the structure of the parallel back-projector's tiled
+// kernel (16 x 16 voxel tile, ZT slices, KC angles staged per chunk, the
+// XCD-striped tile order) with the Joseph kernel K as weights.  For each
+// angle the tile stages the SPAN detector bins from floor(u) of its lowest
+// corner; bins outside [0, n_det) are staged as zeros, which is where the
+// direct kernel's clamped loads have weight zero.
+constexpr int kJSpan = 26;
+
+template <int ZT, int KC>
+__global__ __launch_bounds__(256) void joseph_adjoint_lds_kernel(JosephArgs a, int ntx, int nty, int ntz)
+{
+    constexpr int T = 16, SPAN = kJSpan, G = ZT / 4;
+    __shared__ float sm[KC][SPAN][ZT + 4];
+    __shared__ int sb0[KC];
+    // block -> tile: XCD c (= block % 8) takes the c-th eighth of the tiles in
+    // slice-chunk, 4-column stripe, row order
+    const int n = ntx * nty * ntz;
+    const int per = (n + 7) >> 3;
+    const int b = (int)blockIdx.x;
+    const int tidx = (b & 7) * per + (b >> 3);
+    if (tidx >= n) return;
+    const int plane = ntx * nty;
+    const int bz = tidx / plane;
+    const int r = tidx - bz * plane;
+    const int sw = 4;
+    const int s = r / (sw * nty);
+    const int q = r - s * sw * nty;
+    const int w = min(sw, ntx - sw * s);
+    const int by = q / w, bx = sw * s + (q - by * w);
+
+    const int N = a.nx;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int x0 = bx * T, y0 = by * T;
+    const int x = x0 + tx, y = y0 + ty;
+    const int z0 = bz * ZT;
+    const bool in_vol = x < N && y < N;
+    float acc[ZT];
+#pragma unroll
+    for (int z = 0; z < ZT; ++z) acc[z] = 0.0f;
+    const float xf = (float)(x + 1), yf = (float)(y + 1);
+    for (int kb = 0; kb < a.na; kb += KC) {
+        const int kc = min(KC, a.na - kb);
+        if (threadIdx.x < kc) {
+            const float4 t = a.tab[kb + threadIdx.x];
+            const float ya = (float)(y0 + 1), yb = (float)(y0 + T);
+            const float xa = (float)(x0 + 1), xb = (float)(x0 + T);
+            const float la = t.x + t.y * ya, lb = t.x + t.y * yb;
+            const float umin = fminf(fminf(la - t.z * xa, la - t.z * xb), fminf(lb - t.z * xa, lb - t.z * xb));
+            sb0[threadIdx.x] = (int)floorf(umin) - 1;   // one bin of rounding margin
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < kc * SPAN * G; e += 256) {
+            const int kk = e / (SPAN * G);
+            const int rr = e - kk * SPAN * G;
+            const int bb = rr / G, g = rr - bb * G;
+            const int col = sb0[kk] + bb;
+            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (col >= 0 && col < a.nd)
+                v = *reinterpret_cast<const float4 *>(a.src + ((size_t)(kb + kk) * a.nd + col) * a.nz + z0 + 4 * g);
+            *reinterpret_cast<float4 *>(&sm[kk][bb][4 * g]) = v;
+        }
+        __syncthreads();
+        if (in_vol) {
+            for (int kk = 0; kk < kc; ++kk) {
+                const float4 t = a.tab[kb + kk];
+                const float u = (t.x + t.y * yf) - t.z * xf;
+                const float fl = floorf(u);
+                const int n1 = (int)fl;
+                const float f = u - fl;
+                const float inv_m = t.w;
+                const float w1 = fmaxf(0.0f, 1.0f - f * inv_m) * inv_m;
+                const float w2 = fmaxf(0.0f, 1.0f - (1.0f - f) * inv_m) * inv_m;
+                const int loc = min(max(n1 - sb0[kk], 0), SPAN - 2);
+                const float *A = &sm[kk][loc][0];
+                const float *B = A + (ZT + 4);
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    const float4 va = *reinterpret_cast<const float4 *>(A + 4 * g);
+                    const float4 vb = *reinterpret_cast<const float4 *>(B + 4 * g);
+                    acc[4 * g + 0] += w1 * va.x + w2 * vb.x;
+                    acc[4 * g + 1] += w1 * va.y + w2 * vb.y;
+                    acc[4 * g + 2] += w1 * va.z + w2 * vb.z;
+                    acc[4 * g + 3] += w1 * va.w + w2 * vb.w;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!in_vol) return;
+    float *out = a.dst + ((size_t)y * N + x) * a.nz + z0;
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+        *reinterpret_cast<float4 *>(out + 4 * g) =
+            make_float4(acc[4 * g], acc[4 * g + 1], acc[4 * g + 2], acc[4 * g + 3]);
+}
+
 std::vector<float> joseph_tables(const mr_joseph_geom &g, const float *angles, size_t n_ang)
 {
     std::vector<float> tab(4 * n_ang);
@@ -218,7 +315,17 @@ int joseph_common(const mr_joseph_geom *g, const float *src, const float *angles
         }
     } else {
         mr::ProfScope prof("joseph_adjoint", s);
-        if (g->nz % 16 == 0) {
+        const char *ei = std::getenv("MR_JOSEPH_IMPL");
+        const int ntx = (int)((g->nx + 15) / 16);
+        if (g->nz % 32 == 0 && !(ei && std::atoi(ei) == 0)) {
+            const long nt = (long)ntx * ntx * (long)(g->nz / 32);
+            hipLaunchKernelGGL((mr::joseph_adjoint_lds_kernel<32, 4>), dim3((unsigned)(8 * ((nt + 7) / 8))), dim3(256),
+                               0, s, a, ntx, ntx, (int)(g->nz / 32));
+        } else if (g->nz % 16 == 0 && !(ei && std::atoi(ei) == 0)) {
+            const long nt = (long)ntx * ntx * (long)(g->nz / 16);
+            hipLaunchKernelGGL((mr::joseph_adjoint_lds_kernel<16, 8>), dim3((unsigned)(8 * ((nt + 7) / 8))), dim3(256),
+                               0, s, a, ntx, ntx, (int)(g->nz / 16));
+        } else if (g->nz % 16 == 0) {
             dim3 grid((unsigned)((g->nx + 15) / 16), (unsigned)((g->nx + 15) / 16), (unsigned)(g->nz / 16));
             hipLaunchKernelGGL(mr::joseph_adjoint_kernel<16>, grid, dim3(256), 0, s, a);
         } else {
@@ -230,6 +337,8 @@ int joseph_common(const mr_joseph_geom *g, const float *src, const float *angles
     hipError_t e2 = hipFreeAsync(d_tab, s);
     if (e != hipSuccess) return mr::hip_fail(e, "joseph kernel launch", __FILE__, __LINE__);
     if (e2 != hipSuccess) return mr::hip_fail(e2, "hipFreeAsync", __FILE__, __LINE__);
+    // the host table must outlive the asynchronous copy
+    MR_HIP(hipStreamSynchronize(s));
     return MR_OK;
 }
 

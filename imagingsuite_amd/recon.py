@@ -96,7 +96,10 @@ class ProjectionFilterSingle:
 
     def close(self):
         if getattr(self, "_plan", None) and self._plan.value:
-            _lib.load().mr_filter_plan_destroy(self._plan)
+            try:
+                _lib.load().mr_filter_plan_destroy(self._plan)
+            except (TypeError, AttributeError):   # interpreter shutdown
+                pass
             self._plan = ctypes.c_void_p()
 
     __del__ = close
@@ -169,7 +172,10 @@ class BackProjector:
 
     def close(self):
         if getattr(self, "_plan", None) and self._plan.value:
-            _lib.load().mr_bp_plan_destroy(self._plan)
+            try:
+                _lib.load().mr_bp_plan_destroy(self._plan)
+            except (TypeError, AttributeError):   # interpreter shutdown
+                pass
             self._plan = ctypes.c_void_p()
 
     __del__ = close
@@ -292,7 +298,10 @@ class FDKbp_single:
 
     def close(self):
         if getattr(self, "_plan", None) and self._plan.value:
-            _lib.load().mr_fdk_plan_destroy(self._plan)
+            try:
+                _lib.load().mr_fdk_plan_destroy(self._plan)
+            except (TypeError, AttributeError):   # interpreter shutdown
+                pass
             self._plan = ctypes.c_void_p()
 
     __del__ = close
@@ -357,7 +366,10 @@ class BatchReconstructor:
 
     def close(self):
         if getattr(self, "_plan", None) and self._plan.value:
-            _lib.load().mr_batch_plan_destroy(self._plan)
+            try:
+                _lib.load().mr_batch_plan_destroy(self._plan)
+            except (TypeError, AttributeError):   # interpreter shutdown
+                pass
             self._plan = ctypes.c_void_p()
 
     __del__ = close
