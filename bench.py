@@ -212,7 +212,8 @@ def bench_fdk(args):
     sod, sdd, pix, vox = 500.0, 1000.0, (0.2, 0.2), (0.1, 0.1, 0.1)
     ppoint = (n / 2, n / 2)
     roi = (0, 0, n, n)
-    fdk = FDKbp_single(sod, sdd, pix, vox, ppoint, n / 2, roi, n - 1)
+    fdk = FDKbp_single(sod, sdd, pix, vox, ppoint, n / 2, roi, n - 1, fused=True)
+    fdk_exact = FDKbp_single(sod, sdd, pix, vox, ppoint, n / 2, roi, n - 1)
     cb = fdk.window
     log(f"[bench] FDK 1024^3, window {cb}")
     angles = np.arange(n_p, dtype=np.float32) * np.float32(360.0 / n_p)
@@ -246,13 +247,28 @@ def bench_fdk(args):
     r = base_result("GUPS (voxel*proj updates/s) FDK 1024^3 x 720 proj", round(updates / (ms / 1e3) / 1e9, 2),
                     "GUPS", args.steps, 1, ms, "f32",
                     {"workload": "FDK 1024^3 from 720 projections, 1024^2 detector, SOD 500, SDD 1000",
-                     "window_rows": cb[3] - cb[1]})
+                     "window_rows": cb[3] - cb[1], "precision": "fused"})
     r["roofline"] = {"bound": "mfma", "pipe": "fp32 VALU", "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                     "traffic": None, "kernel": "fdk_kernel<%s>" % os.environ.get("MR_FDK_KT", "16"),
+                     "traffic": None, "kernel": "fdk_kernel<%s,true>" % os.environ.get("MR_FDK_KT", "16"),
                      "kernel_ms": round(kms, 3),
                      "flop_per_update": FDK_FLOP_PER_UPDATE}
     r["kernels_ms"] = {"row_filter_fdk": round(fl_ms / max(fl_n, 1), 3), "fdk_bp": round(kms, 3)}
+    # the reference float operation order (exact positions and sums)
+    def step_e():
+        work.copy_(proj)
+        fdk_exact.process(work, angles, weights, vol)
+
+    step_e()
+    torch.cuda.synchronize()
+    _lib.profile(True)
+    _lib.profile_reset()
+    mse = timed(step_e, max(1, args.steps // 2), 0, "fdk exact")
+    bpe_ms, bpe_n = _lib.profile_query("fdk_bp")
+    _lib.profile(False)
+    r["exact_mode"] = {"ms_per_step": round(mse, 3), "GUPS": round(updates / (mse / 1e3) / 1e9, 2),
+                       "kernel_ms": round(bpe_ms / max(bpe_n, 1), 3),
+                       "note": "reference float operation order: positions and sums bit-identical to FDKbp_single"}
     # the double-geometry module on the same slab (FDKbp, fdkbp.cpp)
     from imagingsuite_amd.recon import FDKbp
     fdkd = FDKbp(sod, sdd, pix, vox, ppoint, n / 2, roi, n - 1)

@@ -539,6 +539,8 @@ public:
         nProjectionBufferSize = (size_t)GetIntParameter(parameters, "ProjectionBufferSize");
         nSliceBlock = (size_t)GetIntParameter(parameters, "SliceBlock");
         GetUIntParameterVector(parameters, "SubVolume", nSubVolume, 2);
+        // optional, HIP-only: FMA form of the sampling (FDKbp_single)
+        fused_ = module_ == MR_FDK_SINGLE && optional_bool(parameters, "fused", false);
         return 0;
     }
     int Initialize() override { return 0; }
@@ -587,6 +589,7 @@ public:
         q.use_roi = mConfig.MatrixInfo.bUseROI ? 1 : 0;
         for (int i = 0; i < 4; ++i) q.matrix_roi[i] = mConfig.MatrixInfo.roi[i];
         q.module = module_;
+        q.fused = fused_ ? 1 : 0;
         MRM_CHECK(mr_fdk_plan_create(&plan_, &q));
     }
 
@@ -642,6 +645,7 @@ private:
     size_t SizeU = 0, SizeV = 0;
     size_t nProjectionBufferSize = 2000, nSliceBlock = 32, nSubVolume[2];
     int module_ = MR_FDK_SINGLE;
+    bool fused_ = false;
     mr_fdk_plan *plan_ = nullptr;
     DevBuf dproj_, dvol_;
 };

@@ -69,7 +69,7 @@ __device__ __forceinline__ bool fdk_tile(int order, int ntx, int nty, int ntz, i
 // The KT row terms are fetched first, then all KT samples are requested
 // before any of them is used, so the loads of one projection are in flight
 // together.
-template <int KT, bool FULL>
+template <int KT, bool FULL, bool FUSED>
 __device__ __forceinline__ void fdk_column(float (&acc)[KT], const float *__restrict__ zt, int kn,
                                            __amdgpu_buffer_rsrc_t img, float yip1, float xip1, float dw,
                                            float dw2, float fh, int w4, int cc4)
@@ -77,21 +77,32 @@ __device__ __forceinline__ void fdk_column(float (&acc)[KT], const float *__rest
     float ztv[KT];
 #pragma unroll
     for (int kk = 0; kk < KT; ++kk) ztv[kk] = (FULL || kk < kn) ? zt[kk] : 0.0f;
+    // fused: r = dw * zt + (dw * (yip1 + xip1) + 0.5), one FMA per slice
+    const float rb = FUSED ? __builtin_fmaf(dw, yip1 + xip1, 0.5f) : 0.0f;
     float val[KT];
 #pragma unroll
     for (int kk = 0; kk < KT; ++kk) {
-        const float acc21 = ztv[kk] + yip1;
-        float r = dw * (acc21 + xip1);
-        r = r + 0.5f;
+        float r;
+        if (FUSED) {
+            r = __builtin_fmaf(dw, ztv[kk], rb);
+        } else {
+            const float acc21 = ztv[kk] + yip1;
+            r = dw * (acc21 + xip1);
+            r = r + 0.5f;
+        }
         const int rr = (int)floorf(__builtin_amdgcn_fmed3f(r, -1.0f, fh));
         val[kk] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(img, __mul24(rr, w4) + cc4, 0, 0));
     }
 #pragma unroll
     for (int kk = 0; kk < KT; ++kk)
-        if (FULL || kk < kn) acc[kk] = acc[kk] + dw2 * val[kk];
+        if (FULL || kk < kn) acc[kk] = FUSED ? __builtin_fmaf(dw2, val[kk], acc[kk]) : acc[kk] + dw2 * val[kk];
 }
 
-template <int KT>
+// FUSED = false: the reference's float operations (bit-identical positions
+// and sums).  FUSED = true: the row position folded to one FMA per slice and
+// the accumulation to an FMA (positions can round differently at pixel
+// edges, where nearest-neighbour sampling picks the neighbouring pixel).
+template <int KT, bool FUSED>
 __global__ __launch_bounds__(256) void fdk_kernel(FdkArgs a)
 {
     int bx, by, bz;
@@ -126,8 +137,8 @@ __global__ __launch_bounds__(256) void fdk_kernel(FdkArgs a)
             const __amdgpu_buffer_rsrc_t img = __builtin_amdgcn_make_buffer_rsrc(
                 (void *)(a.img + (size_t)p * a.h * a.w), (short)0, pbytes, 0x00020000);
             const float *zt = a.zt1 + (size_t)p * a.nz + k0;
-            if (kn == KT) fdk_column<KT, true>(acc, zt, kn, img, yip1, xip1, dw, dw2, fh, w4, cc4);
-            else fdk_column<KT, false>(acc, zt, kn, img, yip1, xip1, dw, dw2, fh, w4, cc4);
+            if (kn == KT) fdk_column<KT, true, FUSED>(acc, zt, kn, img, yip1, xip1, dw, dw2, fh, w4, cc4);
+            else fdk_column<KT, false, FUSED>(acc, zt, kn, img, yip1, xip1, dw, dw2, fh, w4, cc4);
         }
     }
 #pragma unroll
@@ -274,6 +285,7 @@ extern "C" int mr_fdk_plan_create(mr_fdk_plan **plan, const mr_fdk_params *param
     MR_REQUIRE(q.sdd > 0 && q.sod > 0 && q.pixel[0] > 0 && q.pixel[1] > 0, "invalid distances/pixel size");
     MR_REQUIRE(q.direction == 0 || q.direction == 1, "direction must be 0 (CW) or 1 (CCW)");
     MR_REQUIRE(q.module == MR_FDK_SINGLE || q.module == MR_FDK_DOUBLE, "module must be MR_FDK_SINGLE or MR_FDK_DOUBLE");
+    MR_REQUIRE(!(q.fused && q.module == MR_FDK_DOUBLE), "fused mode is defined for FDKbp_single only");
     auto *p = new mr_fdk_plan();
     p->p = q;
     mr::FdkGeometry &g = p->g;
@@ -492,8 +504,14 @@ extern "C" int mr_fdk_bp(mr_fdk_plan *plan, float *d_proj, size_t n_proj, size_t
         const dim3 g = a.order ? dim3((unsigned)(8 * ((ntiles + 7) / 8)))
                                : dim3((unsigned)a.ntx, (unsigned)a.nty, (unsigned)a.ntz);
         mr::ProfScope prof("fdk_bp", s);
-        if (kt == 32) hipLaunchKernelGGL(mr::fdk_kernel<32>, g, dim3(256), 0, s, a);
-        else hipLaunchKernelGGL(mr::fdk_kernel<16>, g, dim3(256), 0, s, a);
+        const bool fused = plan->p.fused != 0;
+        if (kt == 32) {
+            if (fused) hipLaunchKernelGGL((mr::fdk_kernel<32, true>), g, dim3(256), 0, s, a);
+            else hipLaunchKernelGGL((mr::fdk_kernel<32, false>), g, dim3(256), 0, s, a);
+        } else {
+            if (fused) hipLaunchKernelGGL((mr::fdk_kernel<16, true>), g, dim3(256), 0, s, a);
+            else hipLaunchKernelGGL((mr::fdk_kernel<16, false>), g, dim3(256), 0, s, a);
+        }
     } else {
         MR_REQUIRE(w * h * 4 < (size_t)1 << 31, "FDK window larger than 2^29 samples");
         mr::FdkdArgs a{};

@@ -251,10 +251,11 @@ class FDKbp_single:
 
     def __init__(self, sod: float, sdd: float, pixel, voxel, ppoint, center: float,
                  roi: Sequence[int], projection_roi3: int, direction: int = 0,
-                 matrix_roi: Optional[Sequence[int]] = None):
+                 matrix_roi: Optional[Sequence[int]] = None, fused: bool = False):
         L = _lib.load()
         p = FdkParams()
         p.module = _lib.FDK_MODULES[self.module]
+        p.fused = int(bool(fused))
         p.sod, p.sdd = sod, sdd
         p.pixel[0], p.pixel[1] = pixel
         p.voxel[0], p.voxel[1], p.voxel[2] = voxel

@@ -174,6 +174,9 @@ typedef struct {
     int use_roi;            /* MatrixInfo.bUseROI */
     size_t matrix_roi[4];   /* MatrixInfo.roi */
     int module;             /* MR_FDK_SINGLE or MR_FDK_DOUBLE */
+    int fused;              /* MR_FDK_SINGLE only.  0: the reference's float operations per sample
+                               (bit-identical positions); 1: row position and accumulation folded
+                               into FMAs (a position on a pixel edge may pick the neighbour) */
 } mr_fdk_params;
 
 typedef struct mr_fdk_plan mr_fdk_plan;

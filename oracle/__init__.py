@@ -95,6 +95,9 @@ def _setup(L):
     L.orc_fdkd_cbct_roi.argtypes = [ctypes.POINTER(FdkGeom), _sz, ctypes.POINTER(ctypes.c_uint64)]
     L.orc_fdkd_filter.argtypes = [_fp, _sz, _sz]
     L.orc_fdkd_bp.argtypes = [_fp, _sz, _sz, _sz, _fp, _fp, ctypes.POINTER(FdkGeom), _sp, _sp, _fp]
+    L.orc_filter_rows_adjoint_f64.argtypes = [_dp, _dp, _sz, _sz, _sz, _dp, _dp, _sz]
+    L.orc_bp_parallel_adjoint_f64.argtypes = [_dp, _sz, _sz, _sz, _fp, _fp, ctypes.c_float, ctypes.c_int,
+                                              ctypes.c_int, _sz, _sp, _sp, _dp]
     L.orc_lognorm_flat.argtypes = [_fp, _fp, _sz, ctypes.c_float]
     L.orc_lognorm_f32.argtypes = [_fp, _sz, _sz, _fp, _fp, _fp, ctypes.c_float]
     L.orc_joseph_forward_f64.argtypes = [_dp, _sz, _sz, _sz, ctypes.c_double, ctypes.c_int, _dp, _sz, _dp]
@@ -234,6 +237,35 @@ def bp_parallel_f64(proj: np.ndarray, angles, weights, center: float, direction:
     lib().orc_bp_parallel_f64(_d(p), nu, nv, n_p, _d(a), _d(w), center, direction,
                               1 if interp == "nearest" else 0, bufsize, _s(f), _s(s), _d(vol))
     return vol
+
+
+def filter_rows_adjoint_f64(grad: np.ndarray, ftype="ram-lak", cutoff=0.5, order=1.0, usebias=True,
+                            biasweight=0.1, paddingdoubler=2) -> np.ndarray:
+    """Adjoint of ProjectionFilterSingle (double) on the last axis of ``grad``."""
+    g = np.ascontiguousarray(grad, np.float64)
+    n = g.shape[-1]
+    P = filter_size(n, paddingdoubler)
+    H = filter_kernel(ftype, P, cutoff, order, usebias, biasweight, f64=True)
+    tap = filter_taper(n, P, f64=True)
+    tap_buf = np.zeros(P // 2 + 1, np.float64); tap_buf[:tap.size] = tap
+    out = np.zeros(g.shape, np.float64)
+    lib().orc_filter_rows_adjoint_f64(_d(g), _d(out), g.size // n, n, P, _d(H), _d(tap_buf), tap.size)
+    return out
+
+
+def bp_parallel_adjoint_f64(grad_vol: np.ndarray, n_proj: int, angles, weights, center: float,
+                            direction: int = 0, interp: str = "linear", bufsize: int = 16) -> np.ndarray:
+    """Adjoint of bp_parallel_f32 (multiprojbp = 0): volume-shaped gradient
+    [nu, nu, nv] -> projection-shaped [n_proj, nv, nu] (double)."""
+    g = np.ascontiguousarray(grad_vol, np.float64)
+    nu, _, nv = g.shape
+    a = np.ascontiguousarray(angles, np.float32); w = np.ascontiguousarray(weights, np.float32)
+    first, second = circle_mask(nu)
+    f = first.astype(np.uintp); s = second.astype(np.uintp)
+    out = np.zeros((n_proj, nv, nu), np.float64)
+    lib().orc_bp_parallel_adjoint_f64(_d(g), nu, nv, n_proj, _f(a), _f(w), center, direction,
+                                      1 if interp == "nearest" else 0, bufsize, _s(f), _s(s), _d(out))
+    return out
 
 
 # --------------------------------------------------------------------------

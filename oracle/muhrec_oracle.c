@@ -279,6 +279,44 @@ void orc_filter_rows_f64(const float *in, double *out, size_t nrows, size_t n, s
     free(tw);
 }
 
+/* Adjoint of the filter (the project's own operator for gradients; the
+ * reference has none).  The forward is out = scale * M pad(x) restricted to
+ * [nInsert, nInsert+n), with M = c2r(Hs * r2c(.)) symmetric (Hs real and
+ * even), so the adjoint embeds g at nInsert, applies scale * M, and folds
+ * the two tapers back onto the first and last sample. */
+void orc_filter_rows_adjoint_f64(const double *g, double *out, size_t nrows, size_t n, size_t P,
+                                 const double *H, const double *taper, size_t nInsert)
+{
+    const size_t N2 = P / 2, nDest = P + 16;
+    orc_cplx *tw = orc_twiddles(P);
+    const double scale = ORC_DPI / (2.0 * (double)P);
+#pragma omp parallel
+    {
+        orc_cplx *a = (orc_cplx *)malloc(sizeof(orc_cplx) * P);
+#pragma omp for schedule(static)
+        for (long r = 0; r < (long)nrows; ++r) {
+            for (size_t i = 0; i < P; ++i) { a[i].re = 0.0; a[i].im = 0.0; }
+            for (size_t i = 0; i < n; ++i) a[nInsert + i].re = g[(size_t)r * n + i] * scale;
+            orc_fft(a, P, -1, tw);
+            for (size_t k = 0; k <= N2; ++k) {
+                double h = (k < N2) ? H[k] : 1.0;
+                a[k].re *= h; a[k].im = (k == 0 || k == N2) ? 0.0 : a[k].im * h;
+            }
+            for (size_t k = 1; k < N2; ++k) { a[P - k].re = a[k].re; a[P - k].im = -a[k].im; }
+            orc_fft(a, P, +1, tw);
+            double *o = out + (size_t)r * n;
+            for (size_t i = 0; i < n; ++i) o[i] = a[nInsert + i].re;
+            for (size_t i = 0; i < nInsert; ++i) {
+                o[0] += taper[i] * a[i].re;
+                const size_t j = nDest - 1 - i;
+                if (j < P) o[n - 1] += taper[i] * a[j].re;
+            }
+        }
+        free(a);
+    }
+    free(tw);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Circle mask, RF/src/BackProjectorModuleBase.cpp:153-218 (no tilt).        */
 /* roi == NULL: no MatrixInfo ROI.  roi = {x0,y0,x1,y1} inclusive,           */
@@ -426,6 +464,52 @@ void orc_bp_parallel_f32(const float *proj, size_t nu, size_t nv, size_t np,
         free(lsu);
     }
     free(buf); free(used); free(fsin); free(fcos); free(fsu);
+}
+
+/* Adjoint of orc_bp_parallel_f32 with multiprojbp = 0 (MultiProjBPparallel,
+ * NearestNeighborBP): the float positions n, w of the forward, sums in
+ * double.  g: volume-shaped [nu(y)][nu(x)][nv]; out: [np][nv][nu], zero for
+ * dropped projections and for slices z >= 4*(nv/4). */
+void orc_bp_parallel_adjoint_f64(const double *g, size_t nu, size_t nv, size_t np,
+                                 const float *angles, const float *weights, float center, int direction,
+                                 int interp, size_t bufsize, const size_t *mfirst, const size_t *msecond,
+                                 double *out)
+{
+    const size_t sizeZ4 = (nv / 4) * 4;
+    const int sizeUm2 = (int)nu - 2;
+    size_t *used = (size_t *)malloc(sizeof(size_t) * (np + 1));
+    size_t nused = orc_used_projections(np, bufsize, used);
+    float *fsin = (float *)malloc(sizeof(float) * np);
+    float *fcos = (float *)malloc(sizeof(float) * np);
+    float *fsu = (float *)malloc(sizeof(float) * np);
+    orc_bp_tables_f32(angles, np, direction, center, nu / 2, fsin, fcos, fsu);
+    memset(out, 0, sizeof(double) * np * nv * nu);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long k = 0; k < (long)nused; ++k) {
+        const size_t p = used[k];
+        const double wk = (double)weights[p];
+        double *o = out + p * nv * nu;
+        for (long y = 1; y <= (long)nu; ++y) {
+            const float lsu = fsu[p] + fcos[p] * (float)y;
+            for (size_t x = mfirst[y - 1] + 1; x <= msecond[y - 1]; ++x) {
+                const float u = lsu - fsin[p] * (float)x;
+                const int nn = (int)u;
+                if ((nn < 0) || (sizeUm2 < nn)) continue;
+                const double *col = g + ((size_t)(y - 1) * nu + (x - 1)) * nv;
+                if (interp == 1) {
+                    for (size_t z = 0; z < sizeZ4; ++z) o[z * nu + nn] += wk * col[z];
+                } else {
+                    const float wb = fabsf(u - (float)nn);
+                    const float wa = 1.0f - wb;
+                    for (size_t z = 0; z < sizeZ4; ++z) {
+                        o[z * nu + nn] += wk * (double)wa * col[z];
+                        o[z * nu + nn + 1] += wk * (double)wb * col[z];
+                    }
+                }
+            }
+        }
+    }
+    free(used); free(fsin); free(fcos); free(fsu);
 }
 
 /* The same back-projection in double precision (tables, positions,

@@ -116,3 +116,29 @@ def test_fdkbp_top_slab_window_wrap(orc, dev):
     vol = fdk.process(torch.from_numpy(win).to(dev), angles, weights).cpu().numpy()
     assert np.all(orc.fdk_bp(win, angles, weights, g, module="FDKbp") == 0.0)
     assert np.all(vol == 0.0)
+
+
+@pytest.mark.parametrize("slab", [(0, 48, 128, 80), (0, 96, 128, 128)])
+def test_fdk_fused_mode(orc, dev, slab):
+    """fused=True folds the row position into one FMA per slice: positions on
+    a pixel edge may round to the neighbouring pixel, so the bound is the
+    float-vs-double one: relRMSE <= 1e-4 against the float oracle on random
+    data and against the FP64 oracle on smooth data."""
+    from imagingsuite_amd.recon import FDKbp_single
+    from test_oracle_fdk import smooth_window
+    n, n_p = 128, 60
+    g = _geom(orc, slab, n, 0)
+    cb = orc.fdk_cbct_roi(g, n)
+    rng = np.random.default_rng(slab[1] + 11)
+    win = np.ascontiguousarray(rng.random((n_p, n, n)).astype(np.float32)[:, cb[1]:cb[3], :])
+    angles = np.arange(n_p, dtype=np.float32) * np.float32(360.0 / n_p)
+    weights = np.full(n_p, 1.0 / n_p, np.float32)
+    fdk = FDKbp_single(500.0, 1000.0, (0.2, 0.2), (0.1, 0.1, 0.1), (n / 2 + 0.3, n / 2 - 0.7),
+                       n / 2 + 0.3, slab, n - 1, fused=True)
+    vol = fdk.process(torch.from_numpy(win).to(dev), angles, weights).cpu().numpy()
+    e32 = rel_err(vol, orc.fdk_bp(win, angles, weights, g))
+    sm = smooth_window(n_p, win.shape[1], n)
+    vol_s = fdk.process(torch.from_numpy(sm).to(dev), angles, weights).cpu().numpy()
+    e64 = rel_err(vol_s, orc.fdk_bp(sm, angles, weights, g, f64=True))
+    print(f"FDK fused slab {slab}: vs float oracle {e32[0]:.2e}, vs FP64 (smooth) {e64[0]:.2e}")
+    assert e32[0] <= 1e-4 and e64[0] <= 1e-4
