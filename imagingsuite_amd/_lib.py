@@ -24,6 +24,7 @@ EXPORTED_SYMBOLS = (
     "mr_last_error", "mr_version", "mr_device_count",
     "mr_profile_enable", "mr_profile_reset", "mr_profile_query",
     "mr_filter_plan_create", "mr_filter_plan_destroy", "mr_filter_plan_info", "mr_filter_rows",
+    "mr_filter_rows_adjoint", "mr_bp_parallel_adjoint",
     "mr_bp_plan_create", "mr_bp_plan_destroy", "mr_bp_plan_mask", "mr_bp_plan_kernel", "mr_bp_parallel",
     "mr_fbp_parallel",
     "mr_fdk_window", "mr_fdk_plan_create", "mr_fdk_plan_destroy", "mr_fdk_plan_window", "mr_fdk_bp",
@@ -103,6 +104,8 @@ def load():
     L.mr_filter_plan_destroy.argtypes = [vp]
     L.mr_filter_plan_info.argtypes = [vp, szp, szp, fp]
     L.mr_filter_rows.argtypes = [vp, vp, sz, vp]
+    L.mr_filter_rows_adjoint.argtypes = [vp, vp, sz, vp]
+    L.mr_bp_parallel_adjoint.argtypes = [vp, vp, sz, fp, fp, vp, vp]
     L.mr_bp_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(BpParams)]
     L.mr_bp_plan_destroy.argtypes = [vp]
     L.mr_bp_plan_mask.argtypes = [vp, szp, szp]

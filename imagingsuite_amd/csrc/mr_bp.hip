@@ -16,6 +16,7 @@
 // Geometry tables are read with scalar loads; detector columns are read as
 // float4 vectors along the slice axis.
 #include "mr_common.h"
+#include "mr_filter.h"
 #include "mr_host.h"
 
 #include <algorithm>
@@ -531,6 +532,82 @@ static int env_int(const char *name, int dflt)
     return e ? std::atoi(e) : dflt;
 }
 
+// Adjoint of the parallel back-projection (the project's own operator for
+// gradients through FBP; no reference counterpart).  The forward adds, for
+// every used projection k and voxel (x, y) inside the mask, w_k ((1-f) S[n] +
+// f S[n+1]) with n = (int)u, f = |u - n| and u computed exactly as the
+// forward does (MultiProjBPparallel.cpp:130-149).  One thread per (used
+// projection, detector bin u, 4 slices) gathers the voxels whose n is u or
+// u-1: it walks the rows (|sin| >= |cos|) or the columns and tests the few
+// candidates whose u lies in [u-2, u+2], recomputing u with the forward's
+// float operations, so n and f are the forward's exactly.  Slices at or
+// beyond 4*(nv/4) and dropped projections get no gradient (the caller's
+// buffer is zero-filled first).
+template <int INTERP>
+__global__ __launch_bounds__(256) void bp_adjoint_kernel(const float *__restrict__ g, float *__restrict__ out,
+                                                         const float4 *__restrict__ tab, const int2 *__restrict__ mask,
+                                                         const int *__restrict__ idx, const float *__restrict__ wts,
+                                                         int nu, int nv)
+{
+    const int u = blockIdx.x * 256 + threadIdx.x;
+    const int k = blockIdx.y;
+    const int z0 = blockIdx.z * 4;
+    if (u >= nu) return;
+    const float4 t = tab[k];
+    const float su = t.x, c = t.y, s = t.z;
+    const int num2 = nu - 2;
+    const float fu = (float)u;
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    auto visit = [&](int x, int y) {  // 1-based, as in the forward
+        const int2 m = mask[y - 1];
+        if (x - 1 < m.x || x - 1 >= m.y) return;
+        const float lsu = su + c * (float)y;
+        const float uu = lsu - s * (float)x;
+        const int n = (int)uu;
+        if (n < 0 || n > num2) return;
+        float wgt;
+        if (INTERP == 1) {
+            if (n != u) return;
+            wgt = 1.0f;
+        } else {
+            const float wb = fabsf(uu - (float)n);
+            if (n == u) wgt = 1.0f - wb;
+            else if (n == u - 1) wgt = wb;
+            else return;
+        }
+        const float4 v = *reinterpret_cast<const float4 *>(g + ((size_t)(y - 1) * nu + (x - 1)) * nv + z0);
+        acc.x += wgt * v.x;
+        acc.y += wgt * v.y;
+        acc.z += wgt * v.z;
+        acc.w += wgt * v.w;
+    };
+    if (fabsf(s) >= fabsf(c)) {
+        const float inv = 1.0f / s;
+        for (int y = 1; y <= nu; ++y) {
+            const float lsu = su + c * (float)y;
+            const float xa = (lsu - (fu + 2.0f)) * inv, xb = (lsu - (fu - 2.0f)) * inv;
+            const int lo = max(1, (int)floorf(fminf(xa, xb)) - 1);
+            const int hi = min(nu, (int)ceilf(fmaxf(xa, xb)) + 1);
+            for (int x = lo; x <= hi; ++x) visit(x, y);
+        }
+    } else {
+        const float inv = 1.0f / c;
+        for (int x = 1; x <= nu; ++x) {
+            const float b = s * (float)x - su;
+            const float ya = (fu - 2.0f + b) * inv, yb = (fu + 2.0f + b) * inv;
+            const int lo = max(1, (int)floorf(fminf(ya, yb)) - 1);
+            const int hi = min(nu, (int)ceilf(fmaxf(ya, yb)) + 1);
+            for (int y = lo; y <= hi; ++y) visit(x, y);
+        }
+    }
+    const float w = wts[idx[k]];
+    float *o = out + (size_t)idx[k] * nv * nu + u;
+    o[(size_t)(z0 + 0) * nu] = w * acc.x;
+    o[(size_t)(z0 + 1) * nu] = w * acc.y;
+    o[(size_t)(z0 + 2) * nu] = w * acc.z;
+    o[(size_t)(z0 + 3) * nu] = w * acc.w;
+}
+
 // Grid of a tiled kernel with tile side T: the 3-D tile grid (order 0) or
 // the XCD-striped 1-D grid (order 1, see tile_of_block).
 static dim3 tiled_grid(BpArgs &a, int T, int ZT, int order)
@@ -824,6 +901,52 @@ extern "C" int mr_bp_parallel(mr_bp_plan *plan, const float *d_proj, size_t n_pr
     rc = mr::run_bp(a, q.interp, plan->impl, q.fused ? 0 : 1, s);
     if (rc != MR_OK) return rc;
     // Host buffers (tables) must stay alive until the copies have been consumed.
+    MR_HIP(hipStreamSynchronize(s));
+    return MR_OK;
+    MR_GUARD_END
+}
+
+extern "C" int mr_bp_parallel_adjoint(mr_bp_plan *plan, const float *d_grad_volume, size_t n_proj,
+                                      const float *angles, const float *weights, float *d_grad_proj, void *stream)
+{
+    MR_GUARD_BEGIN
+    mr::clear_error();
+    MR_REQUIRE(plan != nullptr, "plan is NULL");
+    MR_REQUIRE(d_grad_volume != nullptr && d_grad_proj != nullptr, "NULL gradient pointer");
+    MR_REQUIRE(n_proj == 0 || (angles && weights), "NULL angle/weight pointer");
+    const mr_bp_params &q = plan->p;
+    MR_REQUIRE(!(q.multiprojbp && (q.slices & 3)), "no adjoint for MultiProjBP's column offset with slices % 4 != 0");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = mr::fill_zero(d_grad_proj, n_proj * q.slices * q.width, s);
+    if (rc != MR_OK) return rc;
+    std::vector<size_t> used = mr::used_projections(n_proj, (size_t)q.buffer_size);
+    const size_t nk = used.size();
+    MR_REQUIRE(nk <= 65535, "at most 65535 projections per call");
+    const size_t nz4 = (q.slices / 4) * 4;
+    if (nk == 0 || nz4 == 0) {
+        MR_HIP(hipStreamSynchronize(s));
+        return MR_OK;
+    }
+    rc = ensure_capacity(plan, n_proj, nk);
+    if (rc != MR_OK) return rc;
+    std::vector<float> tab;
+    mr::bp_tables(angles, used.data(), nk, q.direction, q.center, q.width / 2, tab);
+    std::vector<int> idx(nk);
+    for (size_t k = 0; k < nk; ++k) idx[k] = (int)used[k];
+    MR_HIP(hipMemcpyAsync(plan->d_tab, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice, s));
+    MR_HIP(hipMemcpyAsync(plan->d_idx, idx.data(), sizeof(int) * nk, hipMemcpyHostToDevice, s));
+    MR_HIP(hipMemcpyAsync(plan->d_w, weights, sizeof(float) * n_proj, hipMemcpyHostToDevice, s));
+    dim3 grid((unsigned)((q.width + 255) / 256), (unsigned)nk, (unsigned)(nz4 / 4));
+    {
+        mr::ProfScope prof("bp_adjoint", s);
+        if (q.interp == MR_INTERP_NEAREST)
+            hipLaunchKernelGGL(mr::bp_adjoint_kernel<1>, grid, dim3(256), 0, s, d_grad_volume, d_grad_proj,
+                               plan->d_tab, plan->d_mask, plan->d_idx, plan->d_w, (int)q.width, (int)q.slices);
+        else
+            hipLaunchKernelGGL(mr::bp_adjoint_kernel<0>, grid, dim3(256), 0, s, d_grad_volume, d_grad_proj,
+                               plan->d_tab, plan->d_mask, plan->d_idx, plan->d_w, (int)q.width, (int)q.slices);
+    }
+    MR_HIP(hipGetLastError());
     MR_HIP(hipStreamSynchronize(s));
     return MR_OK;
     MR_GUARD_END

@@ -12,7 +12,8 @@ struct RowFilterArgs {
     float *rows;                // n_rows x n, filtered in place
     long n_rows;
     int n;                      // row width
-    int mode;                   // 0 ProjectionFilterSingle, 1 FDK ramp_filter_tuned
+    int mode;                   // 0 ProjectionFilterSingle, 1 FDK ramp_filter_tuned,
+                                // 2 adjoint of mode 0
     int insert;                 // mode 0: nInsert; mode 1: (P - n) / 2
     const float *taper;         // mode 0: nInsert sigmoid weights
     const float *hs;            // P real spectral multipliers

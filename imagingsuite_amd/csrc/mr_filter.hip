@@ -116,12 +116,21 @@ __global__ __launch_bounds__(NT) void row_filter_kernel2(RowFilterArgs a)
             const float x1 = has1 ? pb_pad(row1, a.n, i, a.insert, P, a.taper, f1, l1) : 0.0f;
             line[lpad(i)] = make_float2(x0, x1);
         }
-    } else {
+    } else if (a.mode == 1) {
         const float w0 = a.pre_w ? a.pre_w[r0 / a.rows_per_proj] : 1.0f;
         const float w1 = (a.pre_w && has1) ? a.pre_w[(r0 + 1) / a.rows_per_proj] : 1.0f;
         for (int i = t; i < P; i += NT) {
             const float x0 = fdk_pad(row0, a.n, i, a.insert, P, w0);
             const float x1 = has1 ? fdk_pad(row1, a.n, i, a.insert, P, w1) : 0.0f;
+            line[lpad(i)] = make_float2(x0, x1);
+        }
+    } else {
+        // mode 2, adjoint of mode 0: the scaled rows embedded at nInsert, zeros elsewhere
+        for (int i = t; i < P; i += NT) {
+            const int d = i - a.insert;
+            const bool in = d >= 0 && d < a.n;
+            const float x0 = in ? row0[d] * a.post0 : 0.0f;
+            const float x1 = (in && has1) ? row1[d] * a.post0 : 0.0f;
             line[lpad(i)] = make_float2(x0, x1);
         }
     }
@@ -141,6 +150,52 @@ __global__ __launch_bounds__(NT) void row_filter_kernel2(RowFilterArgs a)
             const float2 v = line[lpad(i + a.insert)];
             o0[i] = v.x * a.post0;
             if (has1) o1[i] = v.y * a.post0;
+        }
+    } else if (a.mode == 2) {
+        // adjoint of Pad: the tapers fold back onto the first and last sample
+        // (left taper at 0..nInsert-1, right taper at P+15-j, inside the
+        // FFT window only for j >= 16)
+        __shared__ float red[NT / 64 + 1][4];
+        float sl0 = 0.0f, sl1 = 0.0f, sr0 = 0.0f, sr1 = 0.0f;
+        for (int j = t; j < a.insert; j += NT) {
+            const float w = a.taper[j];
+            const float2 l = line[lpad(j)];
+            sl0 += w * l.x;
+            sl1 += w * l.y;
+            const int jj = P + 15 - j;
+            if (jj < P) {
+                const float2 r = line[lpad(jj)];
+                sr0 += w * r.x;
+                sr1 += w * r.y;
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            sl0 += __shfl_xor(sl0, off);
+            sl1 += __shfl_xor(sl1, off);
+            sr0 += __shfl_xor(sr0, off);
+            sr1 += __shfl_xor(sr1, off);
+        }
+        if ((t & 63) == 0) {
+            red[t >> 6][0] = sl0;
+            red[t >> 6][1] = sl1;
+            red[t >> 6][2] = sr0;
+            red[t >> 6][3] = sr1;
+        }
+        __syncthreads();
+        float L0 = 0.0f, L1 = 0.0f, R0 = 0.0f, R1 = 0.0f;
+        for (int wv = 0; wv < (NT + 63) / 64; ++wv) {
+            L0 += red[wv][0];
+            L1 += red[wv][1];
+            R0 += red[wv][2];
+            R1 += red[wv][3];
+        }
+        for (int i = t; i < a.n; i += NT) {
+            const float2 v = line[lpad(i + a.insert)];
+            float y0 = v.x, y1 = v.y;
+            if (i == 0) { y0 += L0; y1 += L1; }
+            if (i == a.n - 1) { y0 += R0; y1 += R1; }
+            o0[i] = y0;
+            if (has1) o1[i] = y1;
         }
     } else {
         for (int i = t; i < a.n; i += NT) {
@@ -178,7 +233,7 @@ static int launch_row_filter(const RowFilterArgs &a, hipStream_t s)
     // MR_FILTER_IMPL=1 selects the radix-4 FFT (comparison runs)
     const char *e = std::getenv("MR_FILTER_IMPL");
     const bool radix4 = e && std::atoi(e) == 1;
-    ProfScope prof(a.mode == 0 ? "row_filter_pb" : "row_filter_fdk", s);
+    ProfScope prof(a.mode == 0 ? "row_filter_pb" : (a.mode == 1 ? "row_filter_fdk" : "row_filter_adjoint"), s);
     if (radix4) {
         const size_t lds = sizeof(float2) * P;
         if (lds > 65536)
@@ -334,6 +389,24 @@ int mr::filter_rows_to(mr_filter_plan *plan, const float *in, float *out, size_t
 }
 
 size_t mr::filter_width(const mr_filter_plan *plan) { return plan->width; }
+
+extern "C" int mr_filter_rows_adjoint(mr_filter_plan *plan, float *d_rows, size_t n_rows, void *stream)
+{
+    MR_GUARD_BEGIN
+    mr::clear_error();
+    MR_REQUIRE(plan != nullptr, "plan is NULL");
+    MR_REQUIRE(d_rows != nullptr || n_rows == 0, "rows pointer is NULL");
+    if (n_rows == 0) return MR_OK;
+    MR_REQUIRE(!(std::getenv("MR_FILTER_IMPL") && std::atoi(std::getenv("MR_FILTER_IMPL")) == 1),
+               "the adjoint filter runs on the radix-8 FFT only");
+    mr::RowFilterArgs a{};
+    a.rows = d_rows; a.n_rows = (long)n_rows; a.n = (int)plan->width; a.mode = 2;
+    a.insert = (int)plan->n_insert; a.taper = plan->d_taper; a.hs = plan->d_hs; a.tw = plan->d_tw;
+    a.post0 = plan->scale; a.post1 = 1.0f; a.post2 = 1.0f; a.pre_w = nullptr; a.rows_per_proj = 1;
+    a.out = nullptr;
+    return mr::run_row_filter(a, (int)plan->P, (hipStream_t)stream);
+    MR_GUARD_END
+}
 
 extern "C" int mr_filter_rows(mr_filter_plan *plan, float *d_rows, size_t n_rows, void *stream)
 {

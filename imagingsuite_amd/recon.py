@@ -94,6 +94,16 @@ class ProjectionFilterSingle:
                                          _stream(stream)))
         return proj
 
+    def adjoint(self, grad, stream=None):
+        """Applies the adjoint of ``process`` to ``grad`` ([..., width] CUDA
+        float32) in place (gradients through the filter)."""
+        _require_cuda_f32(grad, grad.dim(), "grad")
+        if grad.shape[-1] != self.width:
+            raise MrError(_lib.MR_ERR_INVALID, "row width does not match the configured ROI")
+        check(_lib.load().mr_filter_rows_adjoint(self._plan, _ptr(grad), grad.numel() // self.width,
+                                                 _stream(stream)))
+        return grad
+
     def close(self):
         if getattr(self, "_plan", None) and self._plan.value:
             try:
@@ -169,6 +179,24 @@ class BackProjector:
                                          w.ctypes.data_as(fp), _ptr(volume), int(bool(accumulate)),
                                          _stream(stream)))
         return volume
+
+    def adjoint(self, grad_volume, angles, weights, out=None, stream=None):
+        """Adjoint of ``process``: a [width, width, slices] volume gradient ->
+        [n_proj, slices, width] projection gradient (mr_bp_parallel_adjoint)."""
+        import torch
+        _require_cuda_f32(grad_volume, 3, "grad_volume")
+        if tuple(grad_volume.shape) != (self.width, self.width, self.slices):
+            raise MrError(_lib.MR_ERR_INVALID, "volume size does not match the ROI")
+        a = np.ascontiguousarray(np.asarray(angles, dtype=np.float32))
+        n_proj = a.size
+        w = _f32arr(weights, n_proj)
+        if out is None:
+            out = torch.empty((n_proj, self.slices, self.width), dtype=torch.float32, device=grad_volume.device)
+        _require_cuda_f32(out, 3, "out")
+        fp = ctypes.POINTER(ctypes.c_float)
+        check(_lib.load().mr_bp_parallel_adjoint(self._plan, _ptr(grad_volume), n_proj, a.ctypes.data_as(fp),
+                                                 w.ctypes.data_as(fp), _ptr(out), _stream(stream)))
+        return out
 
     def close(self):
         if getattr(self, "_plan", None) and self._plan.value:

@@ -79,6 +79,11 @@ int mr_filter_plan_info(const mr_filter_plan *plan, size_t *fft_size, size_t *n_
 /* Filters n_rows rows of `width` samples in place (device memory).  A projection
  * stack [n_proj][rows][width] is n_proj*rows rows. */
 int mr_filter_rows(mr_filter_plan *plan, float *d_rows, size_t n_rows, void *stream);
+/* Adjoint of mr_filter_rows, in place (no reference counterpart; for
+ * gradients through FBP): the rows are embedded at nInsert, convolved with the
+ * same symmetric ramp, cropped, and the two tapers are folded back onto the
+ * first and last sample. */
+int mr_filter_rows_adjoint(mr_filter_plan *plan, float *d_rows, size_t n_rows, void *stream);
 
 /* ------------------------------------------------------------------------
  * Parallel-beam back-projection.
@@ -124,6 +129,13 @@ int mr_bp_plan_mask(const mr_bp_plan *plan, size_t *first, size_t *second);
  * accumulate = 0 overwrites the volume (SetROI zeroes it); 1 adds to it. */
 int mr_bp_parallel(mr_bp_plan *plan, const float *d_proj, size_t n_proj, const float *angles,
                    const float *weights, float *d_volume, int accumulate, void *stream);
+/* Adjoint of mr_bp_parallel (the project's own operator for gradients; no
+ * reference counterpart): d_grad_volume [width(y)][width(x)][slices] ->
+ * d_grad_proj [n_proj][slices][width] (overwritten; zero for dropped
+ * projections and slices >= 4*(slices/4)).  Uses the forward's float
+ * positions exactly.  Not available for MultiProjBP with slices % 4 != 0. */
+int mr_bp_parallel_adjoint(mr_bp_plan *plan, const float *d_grad_volume, size_t n_proj, const float *angles,
+                           const float *weights, float *d_grad_proj, void *stream);
 /* Filter (in place on d_proj) followed by back-projection: one engine block of
  * ReconEngine::Process3D (frameworks/tomography/Framework/ReconFramework/src/ReconEngine.cpp:1499-1511,1663-1714). */
 int mr_fbp_parallel(mr_filter_plan *fplan, mr_bp_plan *bplan, float *d_proj, size_t n_proj,

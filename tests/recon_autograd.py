@@ -2,6 +2,9 @@
 
 * JosephProject:      y = A x   (mr_joseph_forward), backward g -> A^T g
 * JosephBackproject:  x = A^T y (mr_joseph_adjoint), backward g -> A g
+* FilteredBackprojection: vol = BP(F p) with MuhRec's ProjectionFilterSingle
+  and MultiProjBPparallel, backward g -> F^T BP^T g (mr_filter_rows_adjoint,
+  mr_bp_parallel_adjoint)
 * tv():               smoothed isotropic total variation (torch autograd)
 * gradient_descent(): min ||A x - y||^2 + lam * TV(x), recomputing A x every
   iteration instead of storing intermediates.
@@ -107,3 +110,41 @@ def gradient_descent(y, geom: Geometry, iters: int = 50, lam: float = 1e-3, step
         with torch.no_grad():
             x = x - step * g
     return x.detach(), losses, step
+
+
+class FBPOperator:
+    """ProjectionFilterSingle + MultiProjBPparallel on one block, with its
+    adjoint; the projections are [n_proj, slices, width]."""
+
+    def __init__(self, width: int, slices: int, center: float, angles, weights, filtertype="hamming",
+                 buffer_size: int = 16, fused: bool = False):
+        from imagingsuite_amd.recon import BackProjector, ProjectionFilterSingle
+        self.filt = ProjectionFilterSingle(width, filtertype=filtertype, cutoff=0.5, usebias=True,
+                                           biasweight=0.1, paddingdoubler=2)
+        self.bp = BackProjector("MultiProjBPparallel", width, slices, center, direction=0,
+                                buffer_size=buffer_size, fused=fused)
+        self.angles = np.ascontiguousarray(angles, np.float32)
+        self.weights = np.ascontiguousarray(weights, np.float32)
+
+    def forward(self, proj: torch.Tensor) -> torch.Tensor:
+        work = proj.contiguous().clone()
+        self.filt.process(work)
+        return self.bp.process(work, self.angles, self.weights)
+
+    def adjoint(self, grad_vol: torch.Tensor) -> torch.Tensor:
+        g = self.bp.adjoint(grad_vol.contiguous(), self.angles, self.weights)
+        return self.filt.adjoint(g)
+
+
+class FilteredBackprojection(torch.autograd.Function):
+    """vol = FBP(proj); the backward recomputes nothing and stores nothing:
+    the adjoint needs only the geometry."""
+
+    @staticmethod
+    def forward(ctx, proj, op: FBPOperator):
+        ctx.op = op
+        return op.forward(proj)
+
+    @staticmethod
+    def backward(ctx, g):
+        return ctx.op.adjoint(g), None

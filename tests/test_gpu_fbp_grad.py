@@ -1,0 +1,107 @@
+"""GPU: gradients through the MuhRec filtered back-projection.
+
+* BP^T (mr_bp_parallel_adjoint) against the FP64 adjoint oracle (relRMSE
+  <= 1e-6) and the dot-product test <BP s, G> = <s, BP^T G> to 1e-5;
+* F^T (mr_filter_rows_adjoint) against the FP64 oracle (max <= 2e-5 of the
+  range, the filter's float FFT tolerance) and its dot-product test to 1e-5;
+* the autograd function vol = FBP(p): FP64 finite differences of
+  L(p) = ||FBP(p) - t||^2 along a random direction against the autograd
+  directional derivative (1e-4), and the reference gradient
+  F^T BP^T 2 (FBP(p) - t) from the FP64 oracles (relRMSE <= 1e-5).
+"""
+import numpy as np
+import pytest
+import torch
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(seed, nu=64, nv=8, n_p=45):
+    rng = np.random.default_rng(seed)
+    ang = (np.arange(n_p) * (180.0 / n_p)).astype(np.float32)
+    w = rng.uniform(0.5, 1.5, n_p).astype(np.float32)
+    return rng, ang, w
+
+
+@pytest.mark.parametrize("nu,nv,bufsize,interp", [(64, 8, 16, "linear"), (97, 12, 0, "linear"),
+                                                  (64, 4, 16, "nearest"), (150, 6, 16, "linear")])
+def test_bp_adjoint_matches_oracle(orc, dev, nu, nv, bufsize, interp):
+    from imagingsuite_amd.recon import BackProjector
+    rng, ang, w = _case(nu + nv, nu, nv)
+    n_p = ang.size
+    center = nu / 2 - 0.75
+    name = "NearestNeighborBP" if interp == "nearest" else "MultiProjBPparallel"
+    bp = BackProjector(name, nu, nv, center, buffer_size=bufsize)
+    G = rng.normal(size=(nu, nu, nv)).astype(np.float32)
+    got = bp.adjoint(torch.from_numpy(G).to(dev), ang, w).cpu().numpy()
+    ref = orc.bp_parallel_adjoint_f64(G, n_p, ang, w, center, 0, interp, bufsize)
+    e = rel_err(got, ref)
+    s = rng.normal(size=(n_p, nv, nu)).astype(np.float32)
+    bps = bp.process(torch.from_numpy(s).to(dev), ang, w).cpu().numpy().astype(np.float64)
+    lhs = float((bps * G).sum())
+    rhs = float((s.astype(np.float64) * got.astype(np.float64)).sum())
+    print(f"BP^T {name} nu={nu} nv={nv}: vs FP64 {e[0]:.2e}/{e[1]:.2e}, dot {abs(lhs - rhs) / abs(lhs):.2e}")
+    assert e[0] <= 1e-6
+    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+
+
+@pytest.mark.parametrize("nu,ftype", [(64, "hamming"), (577, "hanning"), (2048, "ram-lak"), (30, "shepp-logan")])
+def test_filter_adjoint_matches_oracle(orc, dev, nu, ftype):
+    from imagingsuite_amd.recon import ProjectionFilterSingle
+    rng = np.random.default_rng(nu)
+    f = ProjectionFilterSingle(nu, filtertype=ftype, cutoff=0.5, usebias=True, biasweight=0.1, paddingdoubler=2)
+    g = rng.normal(size=(7, nu)).astype(np.float32)
+    got = f.adjoint(torch.from_numpy(g.copy()).to(dev)).cpu().numpy()
+    ref = orc.filter_rows_adjoint_f64(g, ftype, 0.5, 1.0, True, 0.1, 2)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    x = rng.normal(size=(7, nu)).astype(np.float32)
+    fx = f.process(torch.from_numpy(x.copy()).to(dev)).cpu().numpy().astype(np.float64)
+    lhs = float((fx * g).sum())
+    rhs = float((x.astype(np.float64) * got.astype(np.float64)).sum())
+    print(f"F^T {ftype} nu={nu}: max rel err {err:.2e}, dot {abs(lhs - rhs) / abs(lhs):.2e}")
+    assert err <= 2e-5
+    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+
+
+def test_fbp_autograd_finite_differences_and_reference_gradient(orc, dev):
+    from recon_autograd import FBPOperator, FilteredBackprojection
+    nu, nv, n_p = 48, 4, 30
+    rng, ang, w = _case(5, nu, nv, n_p)
+    center = nu / 2 - 0.5
+    op = FBPOperator(nu, nv, center, ang, w, "hamming", buffer_size=16)
+    p0 = rng.normal(size=(n_p, nv, nu)).astype(np.float32)
+    target = rng.normal(size=(nu, nu, nv)).astype(np.float32)
+    p = torch.from_numpy(p0).to(dev).requires_grad_(True)
+    t = torch.from_numpy(target).to(dev)
+    vol = FilteredBackprojection.apply(p, op)
+    loss = ((vol - t) ** 2).sum()
+    loss.backward()
+    grad = p.grad.detach().cpu().numpy().astype(np.float64)
+
+    # FP64 loss through the oracles (the FP64 BP uses double positions; the
+    # direction is smooth enough that no position flips matter at 1e-4)
+    def loss64(pp):
+        f = orc.filter_rows_f64(pp.reshape(-1, nu).astype(np.float32), "hamming", 0.5, 1.0, True, 0.1, 2)
+        v = orc.bp_parallel_f64(f.reshape(pp.shape), ang.astype(np.float64), w.astype(np.float64), center, 0,
+                                "linear", 16)
+        return float(((v - target.astype(np.float64)) ** 2).sum())
+
+    d = rng.normal(size=p0.shape)
+    d /= np.linalg.norm(d)
+    h = 1e-2
+    fd = (loss64(p0 + h * d) - loss64(p0 - h * d)) / (2 * h)
+    ad = float((grad * d).sum())
+    print(f"FBP directional derivative: autograd {ad:.6e}, FP64 central difference {fd:.6e}")
+    assert abs(ad - fd) <= 1e-4 * max(abs(fd), 1e-12)
+
+    # reference gradient F^T BP^T 2 (FBP(p) - t) in FP64
+    f = orc.filter_rows_f32(p0.reshape(-1, nu), "hamming", 0.5, 1.0, True, 0.1, 2).reshape(p0.shape)
+    v = orc.bp_parallel_f32(f, ang, w, center, 0, "linear", 16, False).astype(np.float64)
+    r = 2.0 * (v - target.astype(np.float64))
+    gref = orc.filter_rows_adjoint_f64(orc.bp_parallel_adjoint_f64(r, n_p, ang, w, center, 0, "linear", 16)
+                                       .reshape(-1, nu), "hamming", 0.5, 1.0, True, 0.1, 2).reshape(p0.shape)
+    e = rel_err(grad, gref)
+    print(f"FBP gradient vs FP64 reference gradient: relRMSE {e[0]:.2e}")
+    assert e[0] <= 1e-5
