@@ -72,7 +72,8 @@ class FdkParams(ctypes.Structure):
                 ("ppoint", ctypes.c_float * 2), ("center", ctypes.c_float),
                 ("direction", ctypes.c_int), ("roi", ctypes.c_size_t * 4),
                 ("projection_roi3", ctypes.c_size_t), ("use_roi", ctypes.c_int),
-                ("matrix_roi", ctypes.c_size_t * 4), ("module", ctypes.c_int), ("fused", ctypes.c_int)]
+                ("matrix_roi", ctypes.c_size_t * 4), ("module", ctypes.c_int), ("fused", ctypes.c_int),
+                ("filter", ctypes.c_int), ("cosine_weight", ctypes.c_int)]
 
 FDK_MODULES = {"FDKbp_single": 0, "FDKbp": 1}
 

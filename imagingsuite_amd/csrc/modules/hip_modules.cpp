@@ -541,6 +541,16 @@ public:
         GetUIntParameterVector(parameters, "SubVolume", nSubVolume, 2);
         // optional, HIP-only: FMA form of the sampling (FDKbp_single)
         fused_ = module_ == MR_FDK_SINGLE && optional_bool(parameters, "fused", false);
+        // optional, HIP-only filter extension: "filter" = "muhrec" (default) or
+        // "shepp-logan"; "cosineweight" = Feldkamp cosine pre-weighting
+        const auto f = parameters.find("filter");
+        if (f == parameters.end() || f->second == "muhrec")
+            filter_ = 0;
+        else if (f->second == "shepp-logan")
+            filter_ = 1;
+        else
+            throw ReconException("FDK filter must be muhrec or shepp-logan", __FILE__, __LINE__);
+        cosine_ = optional_bool(parameters, "cosineweight", false);
         return 0;
     }
     int Initialize() override { return 0; }
@@ -590,6 +600,8 @@ public:
         for (int i = 0; i < 4; ++i) q.matrix_roi[i] = mConfig.MatrixInfo.roi[i];
         q.module = module_;
         q.fused = fused_ ? 1 : 0;
+        q.filter = filter_;
+        q.cosine_weight = cosine_ ? 1 : 0;
         MRM_CHECK(mr_fdk_plan_create(&plan_, &q));
     }
 
@@ -644,8 +656,8 @@ public:
 private:
     size_t SizeU = 0, SizeV = 0;
     size_t nProjectionBufferSize = 2000, nSliceBlock = 32, nSubVolume[2];
-    int module_ = MR_FDK_SINGLE;
-    bool fused_ = false;
+    int module_ = MR_FDK_SINGLE, filter_ = 0;
+    bool fused_ = false, cosine_ = false;
     mr_fdk_plan *plan_ = nullptr;
     DevBuf dproj_, dvol_;
 };

@@ -286,6 +286,8 @@ extern "C" int mr_fdk_plan_create(mr_fdk_plan **plan, const mr_fdk_params *param
     MR_REQUIRE(q.direction == 0 || q.direction == 1, "direction must be 0 (CW) or 1 (CCW)");
     MR_REQUIRE(q.module == MR_FDK_SINGLE || q.module == MR_FDK_DOUBLE, "module must be MR_FDK_SINGLE or MR_FDK_DOUBLE");
     MR_REQUIRE(!(q.fused && q.module == MR_FDK_DOUBLE), "fused mode is defined for FDKbp_single only");
+    MR_REQUIRE(q.filter == 0 || q.filter == 1, "filter must be 0 (reference ramp) or 1 (Shepp-Logan)");
+    MR_REQUIRE(q.cosine_weight == 0 || q.cosine_weight == 1, "cosine_weight must be 0 or 1");
     auto *p = new mr_fdk_plan();
     p->p = q;
     mr::FdkGeometry &g = p->g;
@@ -321,7 +323,15 @@ extern "C" int mr_fdk_plan_create(mr_fdk_plan **plan, const mr_fdk_params *param
     const size_t P = p->P;
     std::vector<float> ramp(P);
     const float DEGTORADF = 3.1415926535897932384626f / 180.0f;
-    if (q.module == MR_FDK_SINGLE) {
+    if (q.filter == 1) {
+        // Shepp-Logan: |k| / (P/2) * sin(pi k / P) / (pi k / P), evaluated in double
+        const double pi = 3.14159265358979323846264338327950288;
+        for (size_t i = 0; i < P; ++i) {
+            const double k = i < P / 2 ? (double)i : (double)(P - i);
+            const double x = pi * k / (double)P;
+            ramp[i] = i == 0 ? 0.0f : (float)(k / ((double)P * 0.5) * std::sin(x) / x);
+        }
+    } else if (q.module == MR_FDK_SINGLE) {
         for (size_t i = 0; i < P; ++i) {
             ramp[i] = i < P / 2 ? (float)(unsigned)i : (float)(unsigned)(P - i);
             ramp[i] *= (0.54 + 0.46 * (std::cos((float)(unsigned)i * DEGTORADF * 360 / (float)(unsigned)P) + 1));
@@ -482,6 +492,12 @@ extern "C" int mr_fdk_bp(mr_fdk_plan *plan, float *d_proj, size_t n_proj, size_t
     fa.post0 = 1.0f / (float)plan->P; fa.post1 = (sod * sod) / (sdd * sdd);
     fa.post2 = dbl ? sod / sdd / (float)n_proj : sod / sdd;
     fa.pre_w = plan->d_w; fa.rows_per_proj = (int)h;
+    if (plan->p.cosine_weight) {
+        fa.cosw = 1;
+        fa.cw_u = plan->p.pixel[0] / sdd; fa.cw_v = plan->p.pixel[1] / sdd;
+        fa.cw_c0 = dbl ? (float)plan->icd[0] : plan->ic[0];
+        fa.cw_r0 = dbl ? (float)plan->icd[1] : plan->ic[1];
+    }
     int rc = mr::run_row_filter(fa, (int)plan->P, s);
     if (rc != MR_OK) return rc;
     rc = mr::run_fdk_margin_rows(d_proj, (int)w, (int)h, (long)n_proj, s);

@@ -21,6 +21,9 @@ struct RowFilterArgs {
     float post0, post1, post2;  // sequential output multipliers
     const float *pre_w;         // mode 1: per-projection input weight (nullable)
     int rows_per_proj;          // rows of one projection (mode 1)
+    int cosw;                   // mode 1: multiply by the Feldkamp cosine weight
+    float cw_u, cw_v;           // pix_u / SDD, pix_v / SDD
+    float cw_c0, cw_r0;         // piercing point (column, row) in the window
     float *out;                 // destination rows (nullptr: in place)
 };
 

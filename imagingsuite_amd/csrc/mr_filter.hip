@@ -35,11 +35,27 @@ __device__ __forceinline__ float pb_pad(const float *row, int n, int i, int ins,
 // ramp_filter_tuned zero padding with the 5-column edge copies
 // (fdkbp_single.cpp:855-897); the row itself is pre-multiplied by the
 // projection weight (FdkReconBase::Process, fdkreconbase.cpp:243-249).
-__device__ __forceinline__ float fdk_pad(const float *row, int n, int c, int ins, int P, float w)
+// With a.cosw set (an extension without reference counterpart) each sample
+// is also multiplied by the Feldkamp cosine weight SDD / sqrt(SDD^2 + u^2 + v^2),
+// u = (d - cw_c0) pix_u and v = (row - cw_r0) pix_v in mm from the piercing
+// point; vv2 = (v / SDD)^2 of the row.  The 5-row edge copies run after the
+// filter, so the copied rows carry the weight of the row they copy.
+__device__ __forceinline__ float fdk_pad(const float *row, int n, int c, int ins, int P, float w,
+                                         const RowFilterArgs &a, float vv2)
 {
     const int cc = c < 5 ? 5 : (c >= P - 5 ? P - 6 : c);
     const int d = cc - ins;
-    return (d >= 0 && d < n) ? row[d] * w : 0.0f;
+    if (d < 0 || d >= n) return 0.0f;
+    const float x = row[d] * w;
+    if (!a.cosw) return x;
+    const float uu = ((float)d - a.cw_c0) * a.cw_u;
+    return x * (1.0f / sqrtf(fmaf(uu, uu, 1.0f + vv2)));
+}
+
+__device__ __forceinline__ float fdk_row_vv2(const RowFilterArgs &a, long r)
+{
+    const float vv = ((float)(r % a.rows_per_proj) - a.cw_r0) * a.cw_v;
+    return vv * vv;
 }
 
 template <int P, int NT>
@@ -63,9 +79,10 @@ __global__ __launch_bounds__(NT) void row_filter_kernel(RowFilterArgs a)
     } else {
         const float w0 = a.pre_w ? a.pre_w[r0 / a.rows_per_proj] : 1.0f;
         const float w1 = (a.pre_w && has1) ? a.pre_w[(r0 + 1) / a.rows_per_proj] : 1.0f;
+        const float v0 = fdk_row_vv2(a, r0), v1 = fdk_row_vv2(a, r0 + 1);
         for (int i = t; i < P; i += NT) {
-            const float x0 = fdk_pad(row0, a.n, i, a.insert, P, w0);
-            const float x1 = has1 ? fdk_pad(row1, a.n, i, a.insert, P, w1) : 0.0f;
+            const float x0 = fdk_pad(row0, a.n, i, a.insert, P, w0, a, v0);
+            const float x1 = has1 ? fdk_pad(row1, a.n, i, a.insert, P, w1, a, v1) : 0.0f;
             line[i] = make_float2(x0, x1);
         }
     }
@@ -119,9 +136,10 @@ __global__ __launch_bounds__(NT) void row_filter_kernel2(RowFilterArgs a)
     } else if (a.mode == 1) {
         const float w0 = a.pre_w ? a.pre_w[r0 / a.rows_per_proj] : 1.0f;
         const float w1 = (a.pre_w && has1) ? a.pre_w[(r0 + 1) / a.rows_per_proj] : 1.0f;
+        const float v0 = fdk_row_vv2(a, r0), v1 = fdk_row_vv2(a, r0 + 1);
         for (int i = t; i < P; i += NT) {
-            const float x0 = fdk_pad(row0, a.n, i, a.insert, P, w0);
-            const float x1 = has1 ? fdk_pad(row1, a.n, i, a.insert, P, w1) : 0.0f;
+            const float x0 = fdk_pad(row0, a.n, i, a.insert, P, w0, a, v0);
+            const float x1 = has1 ? fdk_pad(row1, a.n, i, a.insert, P, w1, a, v1) : 0.0f;
             line[lpad(i)] = make_float2(x0, x1);
         }
     } else {

@@ -273,17 +273,27 @@ class FullLogNorm:
 
 class FDKbp_single:
     """FDK cone-beam back-projector for one slab (SetROI(roi) + Process(3D)),
-    float geometry (fdkbp_single.cpp)."""
+    float geometry (fdkbp_single.cpp).
+
+    ``filter="shepp-logan"`` and ``cosine_weight=True`` select the filter
+    extension, which the reference does not have: the Shepp-Logan ramp
+    |k|/(P/2) sinc(k/P) instead of ramp_filter_tuned's roll-off, and the
+    Feldkamp pre-weight SDD/sqrt(SDD^2+u^2+v^2) per detector pixel."""
 
     module = "FDKbp_single"
 
     def __init__(self, sod: float, sdd: float, pixel, voxel, ppoint, center: float,
                  roi: Sequence[int], projection_roi3: int, direction: int = 0,
-                 matrix_roi: Optional[Sequence[int]] = None, fused: bool = False):
+                 matrix_roi: Optional[Sequence[int]] = None, fused: bool = False,
+                 filter: str = "muhrec", cosine_weight: bool = False):
         L = _lib.load()
         p = FdkParams()
         p.module = _lib.FDK_MODULES[self.module]
         p.fused = int(bool(fused))
+        if filter not in ("muhrec", "shepp-logan"):
+            raise MrError(_lib.MR_ERR_INVALID, "FDK filter must be 'muhrec' or 'shepp-logan'")
+        p.filter = 0 if filter == "muhrec" else 1
+        p.cosine_weight = int(bool(cosine_weight))
         p.sod, p.sdd = sod, sdd
         p.pixel[0], p.pixel[1] = pixel
         p.voxel[0], p.voxel[1], p.voxel[2] = voxel

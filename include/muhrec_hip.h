@@ -189,6 +189,12 @@ typedef struct {
     int fused;              /* MR_FDK_SINGLE only.  0: the reference's float operations per sample
                                (bit-identical positions); 1: row position and accumulation folded
                                into FMAs (a position on a pixel edge may pick the neighbour) */
+    /* Filter extension without reference counterpart (both modules; 0/0 is the reference):
+     * filter 0 = ramp_filter_tuned's ramp and roll-off, 1 = Shepp-Logan |k|/(P/2) sinc(k/P);
+     * cosine_weight 1 multiplies each detector pixel by SDD / sqrt(SDD^2 + u^2 + v^2)
+     * before the filter (u, v in mm from the piercing point). */
+    int filter;
+    int cosine_weight;
 } mr_fdk_params;
 
 typedef struct mr_fdk_plan mr_fdk_plan;

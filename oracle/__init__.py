@@ -89,8 +89,11 @@ def _setup(L):
     L.orc_fdk_padwidth.argtypes = [_sz]; L.orc_fdk_padwidth.restype = _sz
     L.orc_fdk_filter_f32.argtypes = [_fp, _sz, _sz]
     L.orc_fdk_filter_f64.argtypes = [_dp, _sz, _sz]
+    L.orc_fdk_filter_ext_f64.argtypes = [_dp, _sz, _sz, ctypes.c_int]
     L.orc_fdk_bp_f32.argtypes = [_fp, _sz, _sz, _sz, _fp, _fp, ctypes.POINTER(FdkGeom), _sp, _sp, _fp]
     L.orc_fdk_bp_f64.argtypes = [_fp, _sz, _sz, _sz, _dp, _dp, ctypes.POINTER(FdkGeom), _sp, _sp, _dp]
+    L.orc_fdk_bp_ext_f64.argtypes = [_fp, _sz, _sz, _sz, _dp, _dp, ctypes.POINTER(FdkGeom), _sp, _sp, _dp,
+                                     ctypes.c_int, ctypes.c_int]
     L.orc_fdkd_projmatrix.argtypes = [ctypes.POINTER(FdkGeom), ctypes.c_float, _dp]
     L.orc_fdkd_cbct_roi.argtypes = [ctypes.POINTER(FdkGeom), _sz, ctypes.POINTER(ctypes.c_uint64)]
     L.orc_fdkd_filter.argtypes = [_fp, _sz, _sz]
@@ -312,6 +315,29 @@ def fdk_filter_f64(img: np.ndarray) -> np.ndarray:
     h, w = a.shape
     lib().orc_fdk_filter_f64(_d(a), w, h)
     return a
+
+
+def fdk_filter_ext_f64(img: np.ndarray, shepp_logan: bool = True) -> np.ndarray:
+    a = np.ascontiguousarray(img, np.float64).copy()
+    h, w = a.shape
+    lib().orc_fdk_filter_ext_f64(_d(a), w, h, int(shepp_logan))
+    return a
+
+
+def fdk_bp_ext_f64(proj: np.ndarray, angles, weights, g: FdkGeom, shepp_logan: bool = True,
+                   cosine: bool = True) -> np.ndarray:
+    """FP64 FDK with the project's filter extension (Shepp-Logan ramp and/or
+    Feldkamp cosine pre-weighting; no reference counterpart)."""
+    p = np.ascontiguousarray(proj, np.float32)
+    n_p, h, w = p.shape
+    nx = int(g.roi[2] - g.roi[0]); nz = int(g.roi[3] - g.roi[1])
+    first, second = circle_mask(nx)
+    f = first.astype(np.uintp); s = second.astype(np.uintp)
+    vol = np.zeros((nz, nx, nx), np.float64)
+    lib().orc_fdk_bp_ext_f64(_f(p), w, h, n_p, _d(np.ascontiguousarray(angles, np.float64)),
+                             _d(np.ascontiguousarray(weights, np.float64)), ctypes.byref(g),
+                             _s(f), _s(s), _d(vol), int(shepp_logan), int(cosine))
+    return vol
 
 
 def fdkd_projmatrix(g: FdkGeom, angle: float) -> np.ndarray:

@@ -680,7 +680,7 @@ size_t orc_fdk_padwidth(size_t width)
 /* ramp_filter_tuned, fdkbp_single.cpp:832-958, on one w x h image in place.
  * f64 != 0: evaluate in double (window and scale in double); the image stays
  * float only for the f32 flavour. */
-static void orc_fdk_filter_impl(float *imgf, double *imgd, size_t w, size_t h)
+static void orc_fdk_filter_impl(float *imgf, double *imgd, size_t w, size_t h, int kind)
 {
     const unsigned int MARGINF = 5;
     const size_t P = orc_fdk_padwidth(w);
@@ -705,7 +705,13 @@ static void orc_fdk_filter_impl(float *imgf, double *imgd, size_t w, size_t h)
         rampd[i] = rampf[i];
         rampf[i] = (float)(rampf[i] * (0.54 + 0.46 * (cosf((float)(unsigned)i * DEGTORADF * 360 / (float)(unsigned)P) + 1)));
         rampf[i] = (float)(rampf[i] / ((double)P * 0.5));
-        rampd[i] = rampd[i] * (0.54 + 0.46 * (cos(2.0 * ORC_DPI * (double)i / (double)P) + 1)) / ((double)P * 0.5);
+        if (kind == 0) {
+            rampd[i] = rampd[i] * (0.54 + 0.46 * (cos(2.0 * ORC_DPI * (double)i / (double)P) + 1)) / ((double)P * 0.5);
+        } else {
+            /* extension: Shepp-Logan, |k| sinc(k / P) */
+            const double k = rampd[i], x = ORC_DPI * k / (double)P;
+            rampd[i] = k == 0.0 ? 0.0 : (k / ((double)P * 0.5)) * sin(x) / x;
+        }
     }
     const float rpad = 1.0f / (float)P;
     orc_cplx *tw = orc_twiddles(P);
@@ -738,8 +744,10 @@ static void orc_fdk_filter_impl(float *imgf, double *imgd, size_t w, size_t h)
     free(tw); free(rampf); free(rampd); free(pad);
 }
 
-void orc_fdk_filter_f32(float *img, size_t w, size_t h) { orc_fdk_filter_impl(img, NULL, w, h); }
-void orc_fdk_filter_f64(double *img, size_t w, size_t h) { orc_fdk_filter_impl(NULL, img, w, h); }
+void orc_fdk_filter_f32(float *img, size_t w, size_t h) { orc_fdk_filter_impl(img, NULL, w, h, 0); }
+void orc_fdk_filter_f64(double *img, size_t w, size_t h) { orc_fdk_filter_impl(NULL, img, w, h, 0); }
+/* kind 0: the reference's ramp and roll-off; kind 1: the Shepp-Logan extension */
+void orc_fdk_filter_ext_f64(double *img, size_t w, size_t h, int kind) { orc_fdk_filter_impl(NULL, img, w, h, kind); }
 
 /* Full FDKbp_single::Process(3D) on one slab.
  * proj : np x h x w window projections (the rows the engine hands over),
@@ -824,11 +832,34 @@ void orc_fdk_bp_f32(const float *proj, size_t w, size_t h, size_t np,
     free(cv); free(img); free(xip); free(yip); free(zip);
 }
 
+static void orc_fdk_bp_f64_impl(const float *proj, size_t w, size_t h, size_t np,
+                                const double *angles, const double *weights, const orc_fdk_geom *g,
+                                const size_t *mfirst, const size_t *msecond, double *vol, int kind, int cosw);
+
 /* FDK in double precision: geometry evaluated in double from the same
  * configuration, FDK filter in double, sampling/accumulation in double. */
 void orc_fdk_bp_f64(const float *proj, size_t w, size_t h, size_t np,
                     const double *angles, const double *weights, const orc_fdk_geom *g,
                     const size_t *mfirst, const size_t *msecond, double *vol)
+{
+    orc_fdk_bp_f64_impl(proj, w, h, np, angles, weights, g, mfirst, msecond, vol, 0, 0);
+}
+
+/* The same with the project's filter extension (no reference counterpart):
+ * kind 1 = Shepp-Logan ramp |k| sinc(k/P) / (P/2) instead of the reference's
+ * window; cosw = 1 multiplies every detector pixel by the Feldkamp cosine
+ * weight SDD / sqrt(SDD^2 + u^2 + v^2) (u, v in mm from the piercing point)
+ * before the filter (and before the reference's edge-row copies). */
+void orc_fdk_bp_ext_f64(const float *proj, size_t w, size_t h, size_t np,
+                        const double *angles, const double *weights, const orc_fdk_geom *g,
+                        const size_t *mfirst, const size_t *msecond, double *vol, int kind, int cosw)
+{
+    orc_fdk_bp_f64_impl(proj, w, h, np, angles, weights, g, mfirst, msecond, vol, kind, cosw);
+}
+
+static void orc_fdk_bp_f64_impl(const float *proj, size_t w, size_t h, size_t np,
+                                const double *angles, const double *weights, const orc_fdk_geom *g,
+                                const size_t *mfirst, const size_t *msecond, double *vol, int kind, int cosw)
 {
     const size_t nx = (size_t)(g->roi[2] - g->roi[0]), ny = nx;
     const size_t nz = (size_t)(g->roi[3] - g->roi[1]);
@@ -850,7 +881,13 @@ void orc_fdk_bp_f64(const float *proj, size_t w, size_t h, size_t np,
     for (size_t ip = 0; ip < np; ++ip) {
         const float *src = proj + ip * w * h;
         for (size_t i = 0; i < w * h; ++i) img[i] = (double)src[i] * weights[ip];
-        orc_fdk_filter_f64(img, w, h);
+        if (cosw)
+            for (size_t r = 0; r < h; ++r)
+                for (size_t c = 0; c < w; ++c) {
+                    const double u = ((double)c - ic[0]) * g->pix[0], v = ((double)r - ic[1]) * g->pix[1];
+                    img[r * w + c] *= sdd / sqrt(sdd * sdd + u * u + v * v);
+                }
+        orc_fdk_filter_impl(NULL, img, w, h, kind);
         for (size_t i = 0; i < w * h; ++i) img[i] *= sad_sid_2 * scale;
         /* projection matrix in closed form: rows -plt/pix_u, -pup/pix_v, -nrm/sdd */
         double a = (g->direction == 0 ? -angles[ip] : angles[ip]) * ORC_DPI / 180.0;
