@@ -18,6 +18,7 @@ MR_ERR_UNSUPPORTED = 3
 MR_ERR_INTERNAL = 4
 
 FILTER_TYPES = {"ram-lak": 0, "shepp-logan": 1, "hanning": 2, "hamming": 3, "butterworth": 4}
+ITER_METHODS = {"sirt": 0, "landweber": 1, "cgls": 2}
 
 # Every symbol include/muhrec_hip.h declares.
 EXPORTED_SYMBOLS = (
@@ -29,6 +30,7 @@ EXPORTED_SYMBOLS = (
     "mr_fbp_parallel",
     "mr_fdk_window", "mr_fdk_plan_create", "mr_fdk_plan_destroy", "mr_fdk_plan_window", "mr_fdk_bp",
     "mr_joseph_forward", "mr_joseph_adjoint",
+    "mr_iter_plan_create", "mr_iter_run", "mr_iter_plan_destroy",
     "mr_lognorm_reference", "mr_lognorm", "mr_lognorm_logdose",
     "mr_batch_plan_create", "mr_batch_plan_info", "mr_batch_run", "mr_batch_run_graph",
     "mr_batch_run_sequential", "mr_batch_plan_destroy",
@@ -120,6 +122,9 @@ def load():
     L.mr_fdk_bp.argtypes = [vp, vp, sz, sz, fp, fp, vp, vp]
     for f in (L.mr_joseph_forward, L.mr_joseph_adjoint):
         f.argtypes = [ctypes.POINTER(JosephGeom), vp, fp, sz, vp, vp]
+    L.mr_iter_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(JosephGeom), fp, sz, i32]
+    L.mr_iter_run.argtypes = [vp, vp, vp, i32, f32, i32, ctypes.POINTER(ctypes.c_double), vp]
+    L.mr_iter_plan_destroy.argtypes = [vp]
     L.mr_lognorm_reference.argtypes = [vp, vp, sz, f32, vp]
     L.mr_lognorm.argtypes = [vp, sz, sz, vp, vp, fp, f32, vp]
     L.mr_lognorm_logdose.argtypes = [vp, sz, sz, vp, vp, fp, vp]

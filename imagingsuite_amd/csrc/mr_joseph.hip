@@ -19,6 +19,7 @@
 // (slice fastest).  Each thread owns ZT slices held as float4 vectors.
 #include "mr_common.h"
 #include "mr_filter.h"
+#include "mr_joseph.h"
 
 #include <algorithm>
 #include <cmath>
@@ -278,29 +279,20 @@ std::vector<float> joseph_tables(const mr_joseph_geom &g, const float *angles, s
     return tab;
 }
 
-}  // namespace mr
-
-namespace {
-
-int joseph_common(const mr_joseph_geom *g, const float *src, const float *angles, size_t n_ang, float *dst,
-                  void *stream, bool forward)
+int joseph_check(const mr_joseph_geom *g, size_t n_ang)
 {
-    mr::clear_error();
     MR_REQUIRE(g != nullptr, "geometry is NULL");
-    MR_REQUIRE(src != nullptr && dst != nullptr && angles != nullptr, "NULL pointer");
     MR_REQUIRE(g->nx >= 2 && g->n_det >= 2 && g->nz >= 4, "sizes too small");
     MR_REQUIRE(g->nz % 4 == 0, "nz must be a multiple of 4");
     MR_REQUIRE(n_ang >= 1 && n_ang <= 65535 * 4, "angle count out of range");
     MR_REQUIRE(g->direction == 0 || g->direction == 1, "direction must be 0 (CW) or 1 (CCW)");
-    hipStream_t s = (hipStream_t)stream;
-    std::vector<float> tab = mr::joseph_tables(*g, angles, n_ang);
-    float4 *d_tab = nullptr;
-    MR_HIP(hipMallocAsync((void **)&d_tab, sizeof(float4) * n_ang, s));
-    hipError_t e = hipMemcpyAsync(d_tab, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice, s);
-    if (e != hipSuccess) {
-        (void)hipFreeAsync(d_tab, s);
-        return mr::hip_fail(e, "hipMemcpyAsync", __FILE__, __LINE__);
-    }
+    return MR_OK;
+}
+
+// Enqueues A (forward) or A^T on s with a resident angle table; no host sync.
+int joseph_launch(const mr_joseph_geom *g, const float *src, float *dst, const float4 *d_tab, size_t n_ang,
+                  bool forward, hipStream_t s)
+{
     mr::JosephArgs a{src, dst, d_tab, (int)g->nx, (int)g->nz, (int)g->n_det, (int)n_ang};
     constexpr int ZT = 4;
     // ZT = 16 when the slice count allows it
@@ -333,9 +325,33 @@ int joseph_common(const mr_joseph_geom *g, const float *src, const float *angles
             hipLaunchKernelGGL(mr::joseph_adjoint_kernel<ZT>, grid, dim3(256), 0, s, a);
         }
     }
-    e = hipGetLastError();
+    MR_HIP(hipGetLastError());
+    return MR_OK;
+}
+
+}  // namespace mr
+
+namespace {
+
+int joseph_common(const mr_joseph_geom *g, const float *src, const float *angles, size_t n_ang, float *dst,
+                  void *stream, bool forward)
+{
+    mr::clear_error();
+    int rc = mr::joseph_check(g, n_ang);
+    if (rc != MR_OK) return rc;
+    MR_REQUIRE(src != nullptr && dst != nullptr && angles != nullptr, "NULL pointer");
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<float> tab = mr::joseph_tables(*g, angles, n_ang);
+    float4 *d_tab = nullptr;
+    MR_HIP(hipMallocAsync((void **)&d_tab, sizeof(float4) * n_ang, s));
+    hipError_t e = hipMemcpyAsync(d_tab, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) {
+        (void)hipFreeAsync(d_tab, s);
+        return mr::hip_fail(e, "hipMemcpyAsync", __FILE__, __LINE__);
+    }
+    rc = mr::joseph_launch(g, src, dst, d_tab, n_ang, forward, s);
     hipError_t e2 = hipFreeAsync(d_tab, s);
-    if (e != hipSuccess) return mr::hip_fail(e, "joseph kernel launch", __FILE__, __LINE__);
+    if (rc != MR_OK) return rc;
     if (e2 != hipSuccess) return mr::hip_fail(e2, "hipFreeAsync", __FILE__, __LINE__);
     // the host table must outlive the asynchronous copy
     MR_HIP(hipStreamSynchronize(s));

@@ -412,3 +412,55 @@ class BatchReconstructor:
             self._plan = ctypes.c_void_p()
 
     __del__ = close
+
+
+class IterativeReconstructor:
+    """SIRT / Landweber / CGLS on the matched Joseph pair (mr_iter_*), the
+    methods of IterativeMeth.txt (frameworks/tomography/Backprojectors/
+    IterativeBackProj/src/IterativeMeth.txt:1-113).  Volume [nx, nx, nz]
+    (MatrixZXY), sinogram [n_ang, n_det, nz]; nz % 4 == 0."""
+
+    def __init__(self, nx: int, nz: int, n_det: int, angles, center: float, method: str = "sirt",
+                 direction: int = 0):
+        if method not in _lib.ITER_METHODS:
+            raise MrError(_lib.MR_ERR_INVALID, "method must be sirt, landweber or cgls")
+        L = _lib.load()
+        self.nx, self.nz, self.n_det, self.method = nx, nz, n_det, method
+        self.angles = np.ascontiguousarray(angles, np.float32)
+        self._g = _lib.JosephGeom(nx, nz, n_det, center, direction)
+        self._plan = ctypes.c_void_p()
+        check(L.mr_iter_plan_create(ctypes.byref(self._plan), ctypes.byref(self._g),
+                                    self.angles.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                    self.angles.size, _lib.ITER_METHODS[method]))
+
+    def run(self, sino, x=None, iterations: int = 10, tau: float = 0.0, nonneg: Optional[bool] = None,
+            residuals: bool = False, stream=None):
+        """Iterates on ``x`` (zeros if None) in place; returns x, or (x,
+        history of |b - A x|^2 before each update) with ``residuals``."""
+        import torch
+        _require_cuda_f32(sino, 3, "sino")
+        if tuple(sino.shape) != (self.angles.size, self.n_det, self.nz):
+            raise MrError(_lib.MR_ERR_INVALID, "sinogram shape does not match the plan")
+        if x is None:
+            x = torch.zeros((self.nx, self.nx, self.nz), dtype=torch.float32, device=sino.device)
+        _require_cuda_f32(x, 3, "x")
+        if tuple(x.shape) != (self.nx, self.nx, self.nz):
+            raise MrError(_lib.MR_ERR_INVALID, "volume shape does not match the plan")
+        hist = np.zeros(max(iterations, 1), np.float64) if residuals else None
+        nn = -1 if nonneg is None else int(bool(nonneg))
+        check(_lib.load().mr_iter_run(self._plan, _ptr(sino), _ptr(x), int(iterations), float(tau), nn,
+                                      hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if residuals else None,
+                                      _stream(stream)))
+        return (x, hist[:iterations]) if residuals else x
+
+    def close(self):
+        if getattr(self, "_plan", None) and self._plan.value:
+            try:
+                _lib.load().mr_iter_plan_destroy(self._plan)
+            except (TypeError, AttributeError):
+                pass
+            self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        self.close()
+

@@ -278,6 +278,31 @@ int mr_joseph_forward(const mr_joseph_geom *geom, const float *d_volume, const f
 int mr_joseph_adjoint(const mr_joseph_geom *geom, const float *d_sino, const float *angles,
                       size_t n_ang, float *d_volume, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Iterative reconstruction on the matched pair: SIRT, Landweber and CGLS as
+ * sketched in frameworks/tomography/Backprojectors/IterativeBackProj/src/IterativeMeth.txt
+ * (landweberOTF :5-17, SIRT_OTF :21-52, cgLS_OTF :56-88); replaces the
+ * reference's SIRTbp::Process (.../src/sirtbp.cpp:32-81), which is a stub.
+ * The plan holds the angle table and work buffers (SIRT: R = 1/A1 and
+ * C = 1/A^T 1, computed at creation).  mr_iter_run iterates on d_volume
+ * (initial guess in, result out; [nx][nx][nz]) against d_sino
+ * ([n_ang][n_det][nz]); it enqueues on `stream` without host synchronisation
+ * unless `residuals` (host, `iterations` values of |b - A x|^2 before each
+ * update) is requested.  tau: Landweber step (> 0; ignored otherwise);
+ * nonneg: -1 = method default (Landweber clamps at 0 like landweberOTF,
+ * SIRT does not), 0/1 = off/on (CGLS: off only).  Calls on one plan must not
+ * overlap.
+ * ------------------------------------------------------------------------ */
+#define MR_ITER_SIRT 0
+#define MR_ITER_LANDWEBER 1
+#define MR_ITER_CGLS 2
+typedef struct mr_iter_plan mr_iter_plan;
+int mr_iter_plan_create(mr_iter_plan **plan, const mr_joseph_geom *geom, const float *angles, size_t n_ang,
+                        int method);
+int mr_iter_run(mr_iter_plan *plan, const float *d_sino, float *d_volume, int iterations, float tau, int nonneg,
+                double *residuals, void *stream);
+int mr_iter_plan_destroy(mr_iter_plan *plan);
+
 #ifdef __cplusplus
 }
 #endif

@@ -447,6 +447,57 @@ def joseph_adjoint_f64(sino: np.ndarray, angles, nx: int, center: float,
     return out
 
 
+def iterative_f64(sino: np.ndarray, angles, nx: int, center: float, method: str = "sirt",
+                  iterations: int = 10, tau: float = 0.0, nonneg: Optional[bool] = None,
+                  x0: Optional[np.ndarray] = None, direction: int = 0):
+    """FP64 SIRT / Landweber / CGLS on the Joseph pair, following
+    IterativeMeth.txt (frameworks/tomography/Backprojectors/IterativeBackProj/
+    src/IterativeMeth.txt: landweberOTF :5-17, SIRT_OTF :21-52, cgLS_OTF
+    :56-88).  Returns (x, residual history |b - A x|^2 before each update)."""
+    b = np.ascontiguousarray(sino, np.float64)
+    na, nd, nz = b.shape
+    A = lambda v: joseph_forward_f64(v, angles, nd, center, direction)
+    AT = lambda r: joseph_adjoint_f64(r, angles, nx, center, direction)
+    x = np.zeros((nx, nx, nz)) if x0 is None else np.array(x0, np.float64)
+    if nonneg is None:
+        nonneg = method == "landweber"
+    hist = []
+    if method in ("sirt", "landweber"):
+        if method == "sirt":
+            rs = A(np.ones_like(x))
+            cs = AT(np.ones_like(b))
+            R = np.where(rs > 0, 1.0 / np.where(rs > 0, rs, 1.0), 0.0)
+            C = np.where(cs > 0, 1.0 / np.where(cs > 0, cs, 1.0), 0.0)
+        for _ in range(iterations):
+            r = b - A(x)
+            hist.append(float((r * r).sum()))
+            if method == "sirt":
+                x = x + C * AT(R * r)
+            else:
+                x = x + tau * AT(r)
+            if nonneg:
+                x = np.maximum(x, 0.0)
+    elif method == "cgls":
+        r = b - A(x)
+        d = AT(r)
+        normr2 = float((d * d).sum())
+        for _ in range(iterations):
+            hist.append(float((r * r).sum()))
+            Ad = A(d)
+            q2 = float((Ad * Ad).sum())
+            alpha = normr2 / q2 if q2 > 0 else 0.0
+            x = x + alpha * d
+            r = r - alpha * Ad
+            s = AT(r)
+            nn = float((s * s).sum())
+            beta = nn / normr2 if normr2 > 0 else 0.0
+            normr2 = nn
+            d = s + beta * d
+    else:
+        raise ValueError("method must be sirt, landweber or cgls")
+    return x, np.array(hist)
+
+
 # --------------------------------------------------------------------------
 # Angles and weights (BuildFileList + ComputeWeights + text transport)
 # --------------------------------------------------------------------------
