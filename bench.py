@@ -353,6 +353,35 @@ def bench_joseph(args):
                     {"workload": "matched Joseph pair 1024^2 x 64 slices x 900 angles"})
     r["adjoint"] = {"ms": round(ms_t, 3), "GUPS": round(upd / (ms_t / 1e3) / 1e9, 2)}
     r["gradient_descent_50"] = {"seconds": round(gd_s, 2), "loss_first": losses[0], "loss_last": losses[-1]}
+    # SIRT and CGLS (IterativeMeth.txt) on the same pair: 10 iterations each,
+    # one A and one A^T per iteration, all on one stream
+    from imagingsuite_amd.recon import IterativeReconstructor
+    for method in ("sirt", "cgls"):
+        it = IterativeReconstructor(nx, nz, nx, ang, nx / 2, method=method)
+        xi = torch.zeros((nx, nx, nz), dtype=torch.float32, device=dev)
+        it.run(y, x=xi, iterations=1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xi.zero_()
+        _, hist = it.run(y, x=xi, iterations=10, residuals=True)
+        torch.cuda.synchronize()
+        s_it = time.perf_counter() - t0
+        r[method + "_10"] = {"seconds": round(s_it, 3), "GUPS_per_operator": round(20 * upd / s_it / 1e9, 2),
+                             "residual_first": float(hist[0]), "residual_last": float(hist[-1])}
+        it.close()
+    # gradient through filtered back-projection (F^T BP^T) on a 1024^2 x 64 block
+    from recon_autograd import FBPOperator
+    n_f = 900
+    af = (np.arange(n_f) * (180.0 / n_f)).astype(np.float32)
+    wf = np.full(n_f, np.float32(1.0 / n_f))
+    op = FBPOperator(nx, nz, nx / 2 + 0.5, af, wf, buffer_size=0, fused=True)
+    p = torch.rand((n_f, nz, nx), device=dev)
+    g = torch.rand((nx, nx, nz), device=dev)
+    ms_f = timed(lambda: op.forward(p), max(1, args.steps), 1, "FBP")
+    ms_b = timed(lambda: op.adjoint(g), max(1, args.steps), 1, "FBP^T")
+    r["fbp_pair_1024x64x900"] = {"forward_ms": round(ms_f, 3), "adjoint_ms": round(ms_b, 3),
+                                 "forward_GUPS": round(upd / (ms_f / 1e3) / 1e9, 2),
+                                 "adjoint_GUPS": round(upd / (ms_b / 1e3) / 1e9, 2)}
     emit(r)
 
 

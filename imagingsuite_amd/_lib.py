@@ -31,6 +31,7 @@ EXPORTED_SYMBOLS = (
     "mr_fdk_window", "mr_fdk_plan_create", "mr_fdk_plan_destroy", "mr_fdk_plan_window", "mr_fdk_bp",
     "mr_joseph_forward", "mr_joseph_adjoint",
     "mr_iter_plan_create", "mr_iter_run", "mr_iter_plan_destroy",
+    "mr_fbp_stream_create", "mr_fbp_stream_run", "mr_fbp_stream_destroy", "mr_host_alloc", "mr_host_free",
     "mr_lognorm_reference", "mr_lognorm", "mr_lognorm_logdose",
     "mr_batch_plan_create", "mr_batch_plan_info", "mr_batch_run", "mr_batch_run_graph",
     "mr_batch_run_sequential", "mr_batch_plan_destroy",
@@ -125,6 +126,12 @@ def load():
     L.mr_iter_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(JosephGeom), fp, sz, i32]
     L.mr_iter_run.argtypes = [vp, vp, vp, i32, f32, i32, ctypes.POINTER(ctypes.c_double), vp]
     L.mr_iter_plan_destroy.argtypes = [vp]
+    L.mr_fbp_stream_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(BpParams), sz, sz, sz, i32, f32, f32,
+                                       i32, f32, sz]
+    L.mr_fbp_stream_run.argtypes = [vp, vp, fp, fp, vp, i32]
+    L.mr_fbp_stream_destroy.argtypes = [vp]
+    L.mr_host_alloc.argtypes = [ctypes.POINTER(vp), sz]
+    L.mr_host_free.argtypes = [vp]
     L.mr_lognorm_reference.argtypes = [vp, vp, sz, f32, vp]
     L.mr_lognorm.argtypes = [vp, sz, sz, vp, vp, fp, f32, vp]
     L.mr_lognorm_logdose.argtypes = [vp, sz, sz, vp, vp, fp, vp]

@@ -15,6 +15,7 @@
 // loop (MultiProjBPparallel.cpp:133-163) and the result is bit-identical.
 // Geometry tables are read with scalar loads; detector columns are read as
 // float4 vectors along the slice axis.
+#include "mr_bp.h"
 #include "mr_common.h"
 #include "mr_filter.h"
 #include "mr_host.h"
@@ -543,16 +544,19 @@ static int env_int(const char *name, int dflt)
 // float operations, so n and f are the forward's exactly.  Slices at or
 // beyond 4*(nv/4) and dropped projections get no gradient (the caller's
 // buffer is zero-filled first).
-template <int INTERP>
+template <int INTERP, int ZG>
 __global__ __launch_bounds__(256) void bp_adjoint_kernel(const float *__restrict__ g, float *__restrict__ out,
                                                          const float4 *__restrict__ tab, const int2 *__restrict__ mask,
                                                          const int *__restrict__ idx, const float *__restrict__ wts,
-                                                         int nu, int nv)
+                                                         int nu, int nv, int nz4)
 {
-    const int u = blockIdx.x * 256 + threadIdx.x;
+    // ZG groups of 4 slices side by side: the ZG threads of one detector bin
+    // read one voxel's ZG*16 contiguous bytes together
+    const int zq = threadIdx.x % ZG, ul = threadIdx.x / ZG;
+    const int u = blockIdx.x * (256 / ZG) + ul;
     const int k = blockIdx.y;
-    const int z0 = blockIdx.z * 4;
-    if (u >= nu) return;
+    const int z0 = (blockIdx.z * ZG + zq) * 4;
+    if (u >= nu || z0 >= nz4) return;
     const float4 t = tab[k];
     const float su = t.x, c = t.y, s = t.z;
     const int num2 = nu - 2;
@@ -581,11 +585,16 @@ __global__ __launch_bounds__(256) void bp_adjoint_kernel(const float *__restrict
         acc.z += wgt * v.z;
         acc.w += wgt * v.w;
     };
+    // candidates: u(x, y) in [u - 1, u + 1] (n = u - 1 or u), or in (-1, 2)
+    // for u <= 1 (truncation maps (-1, 0) to n = 0); one voxel of margin on
+    // each side for the rounding of the bounds; visit() re-tests with the
+    // forward's own operations
+    const float ulo = u <= 1 ? -1.0f : fu - 1.0f, uhi = fu + 1.0f;
     if (fabsf(s) >= fabsf(c)) {
         const float inv = 1.0f / s;
         for (int y = 1; y <= nu; ++y) {
             const float lsu = su + c * (float)y;
-            const float xa = (lsu - (fu + 2.0f)) * inv, xb = (lsu - (fu - 2.0f)) * inv;
+            const float xa = (lsu - uhi) * inv, xb = (lsu - ulo) * inv;
             const int lo = max(1, (int)floorf(fminf(xa, xb)) - 1);
             const int hi = min(nu, (int)ceilf(fmaxf(xa, xb)) + 1);
             for (int x = lo; x <= hi; ++x) visit(x, y);
@@ -594,7 +603,7 @@ __global__ __launch_bounds__(256) void bp_adjoint_kernel(const float *__restrict
         const float inv = 1.0f / c;
         for (int x = 1; x <= nu; ++x) {
             const float b = s * (float)x - su;
-            const float ya = (fu - 2.0f + b) * inv, yb = (fu + 2.0f + b) * inv;
+            const float ya = (ulo + b) * inv, yb = (uhi + b) * inv;
             const int lo = max(1, (int)floorf(fminf(ya, yb)) - 1);
             const int hi = min(nu, (int)ceilf(fmaxf(ya, yb)) + 1);
             for (int y = lo; y <= hi; ++y) visit(x, y);
@@ -746,6 +755,8 @@ struct mr_bp_plan {
     float *d_w = nullptr;
     size_t idx_cap = 0;
     int impl = 1;  // 1 LDS-staged kernel, 0 direct-gather kernel
+    size_t loaded_n = 0, loaded_nk = 0;   // projections / used projections of the resident tables
+    bool loaded = false;
 };
 
 namespace {
@@ -862,30 +873,42 @@ extern "C" int mr_bp_plan_mask(const mr_bp_plan *plan, size_t *first, size_t *se
     MR_GUARD_END
 }
 
-extern "C" int mr_bp_parallel(mr_bp_plan *plan, const float *d_proj, size_t n_proj,
-                              const float *angles, const float *weights, float *d_volume,
-                              int accumulate, void *stream)
+namespace mr {
+
+int bp_load_tables(mr_bp_plan *plan, size_t n_proj, const float *angles, const float *weights, hipStream_t s)
 {
-    MR_GUARD_BEGIN
-    mr::clear_error();
-    MR_REQUIRE(plan != nullptr, "plan is NULL");
-    MR_REQUIRE(d_volume != nullptr, "volume pointer is NULL");
-    MR_REQUIRE(n_proj == 0 || (d_proj && angles && weights), "NULL projection/angle/weight pointer");
-    hipStream_t s = (hipStream_t)stream;
     const mr_bp_params &q = plan->p;
+    MR_REQUIRE(n_proj == 0 || (angles && weights), "NULL angle/weight pointer");
     std::vector<size_t> used = mr::used_projections(n_proj, (size_t)q.buffer_size);
     const size_t nk = used.size();
     MR_REQUIRE(nk <= 65535, "at most 65535 projections per call");
+    plan->loaded = false;
     int rc = ensure_capacity(plan, n_proj, nk);
     if (rc != MR_OK) return rc;
-    std::vector<float> tab;
-    mr::bp_tables(angles, used.data(), nk, q.direction, q.center, q.width / 2, tab);
-    std::vector<int> idx(nk);
-    for (size_t k = 0; k < nk; ++k) idx[k] = (int)used[k];
     if (nk > 0) {
+        std::vector<float> tab;
+        mr::bp_tables(angles, used.data(), nk, q.direction, q.center, q.width / 2, tab);
+        std::vector<int> idx(nk);
+        for (size_t k = 0; k < nk; ++k) idx[k] = (int)used[k];
         MR_HIP(hipMemcpyAsync(plan->d_tab, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice, s));
         MR_HIP(hipMemcpyAsync(plan->d_idx, idx.data(), sizeof(int) * nk, hipMemcpyHostToDevice, s));
         MR_HIP(hipMemcpyAsync(plan->d_w, weights, sizeof(float) * n_proj, hipMemcpyHostToDevice, s));
+        // the host tables are temporaries
+        MR_HIP(hipStreamSynchronize(s));
+    }
+    plan->loaded_n = n_proj;
+    plan->loaded_nk = nk;
+    plan->loaded = true;
+    return MR_OK;
+}
+
+int bp_enqueue(mr_bp_plan *plan, const float *d_proj, float *d_volume, int accumulate, hipStream_t s)
+{
+    MR_REQUIRE(plan->loaded, "no projection tables loaded");
+    const mr_bp_params &q = plan->p;
+    const size_t nk = plan->loaded_nk;
+    MR_REQUIRE(nk == 0 || d_proj != nullptr, "NULL projection pointer");
+    if (nk > 0) {
         dim3 g((unsigned)((q.width + 31) / 32), (unsigned)((plan->vpad + 31) / 32), (unsigned)nk);
         mr::ProfScope prof("bp_prep", s);
         hipLaunchKernelGGL(mr::bp_prep_kernel, g, dim3(256), 0, s, d_proj, plan->d_sino,
@@ -898,9 +921,28 @@ extern "C" int mr_bp_parallel(mr_bp_plan *plan, const float *d_proj, size_t n_pr
     a.nz4 = (int)((q.slices / 4) * 4);
     a.bstride = q.multiprojbp ? a.nz4 : (int)plan->vpad;
     a.accumulate = accumulate ? 1 : 0;
-    rc = mr::run_bp(a, q.interp, plan->impl, q.fused ? 0 : 1, s);
+    return mr::run_bp(a, q.interp, plan->impl, q.fused ? 0 : 1, s);
+}
+
+size_t bp_plan_slices(const mr_bp_plan *plan) { return plan->p.slices; }
+size_t bp_plan_width(const mr_bp_plan *plan) { return plan->p.width; }
+
+}  // namespace mr
+
+extern "C" int mr_bp_parallel(mr_bp_plan *plan, const float *d_proj, size_t n_proj,
+                              const float *angles, const float *weights, float *d_volume,
+                              int accumulate, void *stream)
+{
+    MR_GUARD_BEGIN
+    mr::clear_error();
+    MR_REQUIRE(plan != nullptr, "plan is NULL");
+    MR_REQUIRE(d_volume != nullptr, "volume pointer is NULL");
+    MR_REQUIRE(n_proj == 0 || (d_proj && angles && weights), "NULL projection/angle/weight pointer");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = mr::bp_load_tables(plan, n_proj, angles, weights, s);
     if (rc != MR_OK) return rc;
-    // Host buffers (tables) must stay alive until the copies have been consumed.
+    rc = mr::bp_enqueue(plan, d_proj, d_volume, accumulate, s);
+    if (rc != MR_OK) return rc;
     MR_HIP(hipStreamSynchronize(s));
     return MR_OK;
     MR_GUARD_END
@@ -929,6 +971,7 @@ extern "C" int mr_bp_parallel_adjoint(mr_bp_plan *plan, const float *d_grad_volu
     }
     rc = ensure_capacity(plan, n_proj, nk);
     if (rc != MR_OK) return rc;
+    plan->loaded = false;   // the tables below replace the forward ones
     std::vector<float> tab;
     mr::bp_tables(angles, used.data(), nk, q.direction, q.center, q.width / 2, tab);
     std::vector<int> idx(nk);
@@ -936,15 +979,22 @@ extern "C" int mr_bp_parallel_adjoint(mr_bp_plan *plan, const float *d_grad_volu
     MR_HIP(hipMemcpyAsync(plan->d_tab, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice, s));
     MR_HIP(hipMemcpyAsync(plan->d_idx, idx.data(), sizeof(int) * nk, hipMemcpyHostToDevice, s));
     MR_HIP(hipMemcpyAsync(plan->d_w, weights, sizeof(float) * n_proj, hipMemcpyHostToDevice, s));
-    dim3 grid((unsigned)((q.width + 255) / 256), (unsigned)nk, (unsigned)(nz4 / 4));
     {
+        // ZG: groups of 4 slices per detector bin handled side by side
+        const int nq = (int)(nz4 / 4);
+        const int zg = nq >= 16 ? 16 : (nq >= 8 ? 8 : (nq >= 4 ? 4 : (nq >= 2 ? 2 : 1)));
+        dim3 grid((unsigned)((q.width + (256 / zg) - 1) / (256 / zg)), (unsigned)nk, (unsigned)((nq + zg - 1) / zg));
         mr::ProfScope prof("bp_adjoint", s);
-        if (q.interp == MR_INTERP_NEAREST)
-            hipLaunchKernelGGL(mr::bp_adjoint_kernel<1>, grid, dim3(256), 0, s, d_grad_volume, d_grad_proj,
-                               plan->d_tab, plan->d_mask, plan->d_idx, plan->d_w, (int)q.width, (int)q.slices);
-        else
-            hipLaunchKernelGGL(mr::bp_adjoint_kernel<0>, grid, dim3(256), 0, s, d_grad_volume, d_grad_proj,
-                               plan->d_tab, plan->d_mask, plan->d_idx, plan->d_w, (int)q.width, (int)q.slices);
+        const bool nn = q.interp == MR_INTERP_NEAREST;
+#define MR_ADJ_LAUNCH(I, Z)                                                                                        \
+    hipLaunchKernelGGL((mr::bp_adjoint_kernel<I, Z>), grid, dim3(256), 0, s, d_grad_volume, d_grad_proj,          \
+                       plan->d_tab, plan->d_mask, plan->d_idx, plan->d_w, (int)q.width, (int)q.slices, (int)nz4)
+        if (zg == 16) { if (nn) MR_ADJ_LAUNCH(1, 16); else MR_ADJ_LAUNCH(0, 16); }
+        else if (zg == 8) { if (nn) MR_ADJ_LAUNCH(1, 8); else MR_ADJ_LAUNCH(0, 8); }
+        else if (zg == 4) { if (nn) MR_ADJ_LAUNCH(1, 4); else MR_ADJ_LAUNCH(0, 4); }
+        else if (zg == 2) { if (nn) MR_ADJ_LAUNCH(1, 2); else MR_ADJ_LAUNCH(0, 2); }
+        else { if (nn) MR_ADJ_LAUNCH(1, 1); else MR_ADJ_LAUNCH(0, 1); }
+#undef MR_ADJ_LAUNCH
     }
     MR_HIP(hipGetLastError());
     MR_HIP(hipStreamSynchronize(s));

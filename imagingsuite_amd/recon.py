@@ -464,3 +464,90 @@ class IterativeReconstructor:
     def __del__(self):
         self.close()
 
+
+class _PinnedBuffer:
+    def __init__(self, nbytes: int):
+        self.ptr = ctypes.c_void_p()
+        check(_lib.load().mr_host_alloc(ctypes.byref(self.ptr), max(int(nbytes), 1)))
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and self.ptr.value:
+            try:
+                _lib.load().mr_host_free(self.ptr)
+            except (TypeError, AttributeError):
+                pass
+            self.ptr = ctypes.c_void_p()
+
+
+def host_empty(shape, dtype=np.float32) -> np.ndarray:
+    """A numpy array in pinned host memory (mr_host_alloc); FBPStream copies
+    such arrays by DMA without staging.  The memory is freed with the array."""
+    dt = np.dtype(dtype)
+    n = int(np.prod(shape)) * dt.itemsize
+    buf = _PinnedBuffer(n)
+    raw = (ctypes.c_char * max(n, 1)).from_address(buf.ptr.value)
+    a = np.frombuffer(raw, dtype=dt, count=int(np.prod(shape))).reshape(shape)
+    # keep the allocation alive as long as any view of the array
+    raw._mr_owner = buf
+    return a
+
+
+class FBPStream:
+    """Streaming FBP of a host-resident stack (mr_fbp_stream_*): the engine's
+    SliceBlock loop (ReconEngine::Run3DFull) with filter and back-projection on
+    the GPU and the host copies of neighbouring blocks overlapped with the
+    kernels.  ``run`` takes projections [n_proj, rows, width] (numpy float32)
+    and returns slices [rows, width, width]; equal bit for bit to filtering and
+    back-projecting each block on the device."""
+
+    def __init__(self, width: int, rows: int, n_proj: int, center: float, slice_block: int = 32,
+                 backprojector: str = "MultiProjBPparallel", direction: int = 0, buffer_size: int = 16,
+                 fused: bool = False, filtertype: str = "Hamming", cutoff: float = 0.5, order: float = 1.0,
+                 usebias: bool = True, biasweight: float = 0.1, paddingdoubler: int = 2):
+        if backprojector not in BP_MODULES:
+            raise MrError(_lib.MR_ERR_INVALID, f"unknown back-projector {backprojector}")
+        ft = _lib.FILTER_TYPES.get(filtertype.lower())
+        if ft is None:
+            raise MrError(_lib.MR_ERR_INVALID, "Could not convert filter string to enum.")
+        p = BpParams()
+        p.width, p.slices, p.center, p.direction = int(width), int(slice_block), float(center), int(direction)
+        p.interp = BP_MODULES[backprojector]["interp"]
+        p.multiprojbp = BP_MODULES[backprojector]["multiprojbp"]
+        p.buffer_size = int(buffer_size)
+        p.fused = int(bool(fused))
+        self.width, self.rows, self.n_proj = int(width), int(rows), int(n_proj)
+        self._s = ctypes.c_void_p()
+        check(_lib.load().mr_fbp_stream_create(ctypes.byref(self._s), ctypes.byref(p), self.rows, self.n_proj,
+                                               int(slice_block), ft, cutoff, order, int(bool(usebias)),
+                                               biasweight, paddingdoubler))
+
+    def run(self, proj: np.ndarray, angles, weights, out: Optional[np.ndarray] = None,
+            serial: bool = False) -> np.ndarray:
+        shape = (self.n_proj, self.rows, self.width)
+        if not isinstance(proj, np.ndarray) or proj.dtype != np.float32 or proj.shape != shape \
+                or not proj.flags.c_contiguous:
+            raise MrError(_lib.MR_ERR_INVALID, f"proj must be a contiguous float32 array of shape {shape}")
+        oshape = (self.rows, self.width, self.width)
+        if out is None:
+            out = np.empty(oshape, np.float32)
+        if out.dtype != np.float32 or out.shape != oshape or not out.flags.c_contiguous:
+            raise MrError(_lib.MR_ERR_INVALID, f"out must be a contiguous float32 array of shape {oshape}")
+        a = _f32arr(angles, self.n_proj)
+        w = _f32arr(weights, self.n_proj)
+        fp = ctypes.POINTER(ctypes.c_float)
+        check(_lib.load().mr_fbp_stream_run(self._s, proj.ctypes.data_as(ctypes.c_void_p), a.ctypes.data_as(fp),
+                                            w.ctypes.data_as(fp), out.ctypes.data_as(ctypes.c_void_p),
+                                            int(bool(serial))))
+        return out
+
+    def close(self):
+        if getattr(self, "_s", None) and self._s.value:
+            try:
+                _lib.load().mr_fbp_stream_destroy(self._s)
+            except (TypeError, AttributeError):
+                pass
+            self._s = ctypes.c_void_p()
+
+    def __del__(self):
+        self.close()
+

@@ -164,6 +164,34 @@ int mr_lognorm_logdose(float *d_proj, size_t n_proj, size_t n_pix, const float *
                        const float *dose_log, void *stream);
 
 /* ------------------------------------------------------------------------
+ * Streaming FBP of a host-resident stack (engine side).  Replaces the
+ * SliceBlock loop of ReconEngine::Run3DFull/Process3D
+ * (frameworks/tomography/Framework/ReconFramework/src/ReconEngine.cpp:838-1185,
+ * 1314-1714) for the ProjectionFilterSingle + parallel back-projector chain,
+ * and its slice transfer (TransferMatrix, :531-574): each block of
+ * slice_block detector rows is copied to the GPU, filtered, weighted and
+ * back-projected there, transposed to slices and copied back, with the
+ * copies of neighbouring blocks overlapped with the kernels.  The results
+ * equal, bit for bit, mr_filter_rows + mr_bp_parallel run block by block.
+ * bp->slices is ignored (the block height is used).  h_proj:
+ * [n_proj][rows][width]; h_volume: [rows][width][width] (slice, y, x: the
+ * order of GetSlice).  Host memory from mr_host_alloc is transferred by DMA
+ * directly; other host memory is staged through pinned buffers.
+ * serial = 1 waits for each block before starting the next (no overlap; a
+ * baseline for timing).
+ * ------------------------------------------------------------------------ */
+typedef struct mr_fbp_stream mr_fbp_stream;
+int mr_fbp_stream_create(mr_fbp_stream **stream, const mr_bp_params *bp, size_t rows, size_t n_proj,
+                         size_t slice_block, int filter_type, float cutoff, float order, int use_bias,
+                         float bias_weight, size_t padding_doubler);
+int mr_fbp_stream_run(mr_fbp_stream *stream, const float *h_proj, const float *angles, const float *weights,
+                      float *h_volume, int serial);
+int mr_fbp_stream_destroy(mr_fbp_stream *stream);
+/* Pinned (page-locked) host memory that the streaming path copies by DMA. */
+int mr_host_alloc(void **ptr, size_t bytes);
+int mr_host_free(void *ptr);
+
+/* ------------------------------------------------------------------------
  * FDK cone-beam back-projection (FDKbp_single and FDKbp).
  * Replaces FdkReconBase::SetROI/Process(3D)
  * (frameworks/tomography/Backprojectors/FDKBackProjectors/src/fdkreconbase.cpp:91-277),
