@@ -177,7 +177,20 @@ constexpr int kBpSpan = 26;
 // One 16 x 16 x ZT tile (bx, by, bz) of one volume; shared by the plain grid
 // kernel and the persistent batched kernel.  All 256 threads of the block
 // must call it together (it synchronises internally).
-template <int ZT, int INTERP, bool EXACT, int KC>
+// Projection table entry read through the constant address space: with a
+// block-uniform index this is a scalar load, so the per-projection geometry
+// costs no vector load and wait.
+__device__ __forceinline__ float4 tab_entry(const float4 *tab, int i)
+{
+    return ((const __attribute__((address_space(4))) float4 *)tab)[i];
+}
+
+// PIPE = 0: one projection at a time, slice groups beyond the tile's last
+// updated slice skipped (low register use, 8 waves per SIMD).
+// PIPE = 1: the projection loop unrolled over the chunk and every slice group
+// computed (zeros staged beyond 4*(nv/4)), so the LDS reads of several groups
+// and projections are in flight together (about 80 registers, 6 waves).
+template <int ZT, int INTERP, bool EXACT, int KC, int PIPE>
 __device__ __forceinline__ void bp_lds_tile(const BpArgs &a, int bx, int by, int bz,
                                             float (&sm)[KC][kBpSpan][ZT + 4], int (&sb0)[KC])
 {
@@ -226,15 +239,19 @@ __device__ __forceinline__ void bp_lds_tile(const BpArgs &a, int bx, int by, int
                 const int b = r / G, q = r - b * G;
                 const int col = sb0[kk] + b;
                 float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                if (col < a.nu && z0 + 4 * q < a.vpad)
+                // slices at or beyond 4*(nv/4) are staged as zeros: the
+                // accumulation below then runs over every group of the tile
+                // and leaves them unchanged
+                if (col < a.nu && z0 + 4 * q < a.nz4)
                     v = *reinterpret_cast<const float4 *>(
                         a.sino + ((size_t)(kb + kk) * a.nu + col) * a.vpad + z0 + 4 * q);
                 *reinterpret_cast<float4 *>(&sm[kk][b][4 * q]) = v;
             }
             __syncthreads();
+            if constexpr (PIPE == 0) {
             if (inside) {
                 for (int kk = 0; kk < kc; ++kk) {
-                    const float4 t = a.tab[kb + kk];
+                    const float4 t = tab_entry(a.tab, kb + kk);
                     const float lsu = t.x + t.y * yf;
                     const float u = lsu - t.z * xf;
                     const int n = (int)u;
@@ -277,6 +294,57 @@ __device__ __forceinline__ void bp_lds_tile(const BpArgs &a, int bx, int by, int
                     }
                 }
             }
+            } else {
+            if (inside) {
+                // the chunk's staging offsets in scalar registers; the
+                // projection loop is unrolled so neighbouring projections'
+                // LDS reads and FMAs interleave
+                int sbr[KC];
+#pragma unroll
+                for (int q = 0; q < KC; ++q) sbr[q] = __builtin_amdgcn_readfirstlane(sb0[q]);
+#pragma unroll
+                for (int kk = 0; kk < KC; ++kk) {
+                    if (kk >= kc) break;
+                    const float4 t = tab_entry(a.tab, kb + kk);
+                    const float lsu = t.x + t.y * yf;
+                    const float u = lsu - t.z * xf;
+                    const int n = (int)u;
+                    if (n < 0 || n > num2) continue;
+                    const int loc = min(max(n - sbr[kk], 0), SPAN - 2);
+                    const float *A = &sm[kk][loc][0];
+                    if (INTERP == 0) {
+                        const float wb = fabsf(u - (float)n);
+                        const float wa = 1.0f - wb;
+                        const float *B = A + RS;
+#pragma unroll
+                        for (int g = 0; g < G; ++g) {
+                            const float4 va = *reinterpret_cast<const float4 *>(A + 4 * g);
+                            const float4 vb = *reinterpret_cast<const float4 *>(B + 4 * g);
+                            if (EXACT) {
+                                acc[4 * g + 0] = acc[4 * g + 0] + (va.x * wa + vb.x * wb);
+                                acc[4 * g + 1] = acc[4 * g + 1] + (va.y * wa + vb.y * wb);
+                                acc[4 * g + 2] = acc[4 * g + 2] + (va.z * wa + vb.z * wb);
+                                acc[4 * g + 3] = acc[4 * g + 3] + (va.w * wa + vb.w * wb);
+                            } else {
+                                acc[4 * g + 0] = __builtin_fmaf(vb.x, wb, __builtin_fmaf(va.x, wa, acc[4 * g + 0]));
+                                acc[4 * g + 1] = __builtin_fmaf(vb.y, wb, __builtin_fmaf(va.y, wa, acc[4 * g + 1]));
+                                acc[4 * g + 2] = __builtin_fmaf(vb.z, wb, __builtin_fmaf(va.z, wa, acc[4 * g + 2]));
+                                acc[4 * g + 3] = __builtin_fmaf(vb.w, wb, __builtin_fmaf(va.w, wa, acc[4 * g + 3]));
+                            }
+                        }
+                    } else {
+#pragma unroll
+                        for (int g = 0; g < G; ++g) {
+                            const float4 va = *reinterpret_cast<const float4 *>(A + 4 * g);
+                            acc[4 * g + 0] = acc[4 * g + 0] + va.x;
+                            acc[4 * g + 1] = acc[4 * g + 1] + va.y;
+                            acc[4 * g + 2] = acc[4 * g + 2] + va.z;
+                            acc[4 * g + 3] = acc[4 * g + 3] + va.w;
+                        }
+                    }
+                }
+            }
+            }
             __syncthreads();
         }
     }
@@ -293,14 +361,14 @@ __device__ __forceinline__ void bp_lds_tile(const BpArgs &a, int bx, int by, int
     }
 }
 
-template <int ZT, int INTERP, bool EXACT, int KC>
+template <int ZT, int INTERP, bool EXACT, int KC, int PIPE>
 __global__ __launch_bounds__(256) void bp_lds_kernel(BpArgs a)
 {
     __shared__ float sm[KC][kBpSpan][ZT + 4];
     __shared__ int sb0[KC];
     int bx, by, bz;
     if (!tile_of_block(a, bx, by, bz)) return;
-    bp_lds_tile<ZT, INTERP, EXACT, KC>(a, bx, by, bz, sm, sb0);
+    bp_lds_tile<ZT, INTERP, EXACT, KC, PIPE>(a, bx, by, bz, sm, sb0);
 }
 
 // Quad variant (linear interpolation only).  A block owns a 32 x 32 voxel
@@ -504,7 +572,7 @@ __global__ __launch_bounds__(256) void bp_batch_kernel(const BpArgs *__restrict_
         if (it < 0) break;
         const int4 w = items[it];
         const BpArgs a = stacks[w.x];
-        bp_lds_tile<ZT, INTERP, EXACT, KC>(a, w.y, w.z, w.w, sm, sb0);
+        bp_lds_tile<ZT, INTERP, EXACT, KC, 0>(a, w.y, w.z, w.w, sm, sb0);
         __syncthreads();
     }
 }
@@ -631,21 +699,29 @@ static dim3 tiled_grid(BpArgs &a, int T, int ZT, int order)
     return dim3((unsigned)(8 * ((n + 7) / 8)));
 }
 
-template <int ZT, int INTERP, bool EXACT, int KC>
+template <int ZT, int INTERP, bool EXACT, int KC, int PIPE>
 static int launch_bp_lds(BpArgs a, int order, hipStream_t s)
 {
     const dim3 grid = tiled_grid(a, 16, ZT, order);
     ProfScope prof("bp_parallel", s);
-    hipLaunchKernelGGL((bp_lds_kernel<ZT, INTERP, EXACT, KC>), grid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL((bp_lds_kernel<ZT, INTERP, EXACT, KC, PIPE>), grid, dim3(256), 0, s, a);
     MR_HIP(hipGetLastError());
     return MR_OK;
 }
 
-template <int ZT, int KC>
+template <int ZT, int KC, int PIPE = 0>
 static int launch_bp_lds_mode(const BpArgs &a, int interp, int exact, int order, hipStream_t s)
 {
-    if (interp == 1) return launch_bp_lds<ZT, 1, true, KC>(a, order, s);
-    return exact ? launch_bp_lds<ZT, 0, true, KC>(a, order, s) : launch_bp_lds<ZT, 0, false, KC>(a, order, s);
+    if (interp == 1) return launch_bp_lds<ZT, 1, true, KC, PIPE>(a, order, s);
+    return exact ? launch_bp_lds<ZT, 0, true, KC, PIPE>(a, order, s)
+                 : launch_bp_lds<ZT, 0, false, KC, PIPE>(a, order, s);
+}
+
+// MR_BP_PIPE = 1 selects the unrolled form of the LDS kernel (slice tiles 32
+// and 16 with their default projection chunks).
+static int bp_pipe(int zt, int kc)
+{
+    return env_int("MR_BP_PIPE", 0) == 1 && ((zt == 32 && kc == 4) || (zt == 16 && kc == 8)) ? 1 : 0;
 }
 
 template <int ZT, int KC>
@@ -697,6 +773,9 @@ int run_bp(const BpArgs &a, int interp, int impl, int exact, hipStream_t s)
         lds_shape(a.nv, zt, kc);
         if (zt == 64) return kc == 2 ? launch_bp_lds_mode<64, 2>(a, interp, exact, order, s)
                                      : launch_bp_lds_mode<64, 4>(a, interp, exact, order, s);
+        if (bp_pipe(zt, kc))
+            return zt == 32 ? launch_bp_lds_mode<32, 4, 1>(a, interp, exact, order, s)
+                            : launch_bp_lds_mode<16, 8, 1>(a, interp, exact, order, s);
         if (zt == 32) return kc == 4 ? launch_bp_lds_mode<32, 4>(a, interp, exact, order, s)
                                      : launch_bp_lds_mode<32, 8>(a, interp, exact, order, s);
         if (zt == 16) {
@@ -733,7 +812,7 @@ std::string describe_bp(int nv, int interp, int impl, int exact)
         int zt, kc;
         lds_shape(nv, zt, kc);
         return "bp_lds_kernel<" + std::to_string(zt) + "," + std::to_string(interp) + "," +
-               (interp == 1 ? "true" : ex) + "," + std::to_string(kc) + ">" + ord;
+               (interp == 1 ? "true" : ex) + "," + std::to_string(kc) + (bp_pipe(zt, kc) ? ",1" : "") + ">" + ord;
     }
     const int zt = nv >= 16 ? 16 : (nv >= 8 ? 8 : 4);
     return "bp_kernel<" + std::to_string(zt) + "," + std::to_string(interp) + ">";

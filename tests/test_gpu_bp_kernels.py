@@ -3,8 +3,9 @@
 The library picks one kernel per plan; the environment variables
 MR_BP_IMPL (0 direct gathers, 1 LDS 16x16 tiles, 2 LDS 32x32 tiles with 2x2
 voxel groups), MR_BP_ORDER (0 grid order, 1 XCD-striped order), MR_BP_ZT and
-MR_BP_KC (slice tile, projection chunk) force the others, so each variant is
-checked here against the oracle:
+MR_BP_KC (slice tile, projection chunk) and MR_BP_PIPE (1: the unrolled form
+of the LDS kernel) force the others, so each variant is checked here against
+the oracle:
 
 * exact mode: bit-exact against the MuhRec-faithful float oracle
   (MultiProjBPparallel.cpp:115-167 operation order);
@@ -24,18 +25,20 @@ from conftest import rel_err
 pytestmark = pytest.mark.gpu
 
 VARIANTS = [
-    # impl, order, zt, kc
-    (1, 0, 16, 16), (1, 1, 16, 16), (1, 1, 16, 8), (1, 1, 16, 4), (1, 1, 8, 32), (1, 0, 32, 8),
-    (1, 1, 32, 4), (1, 1, 32, 8), (1, 1, 64, 2), (1, 1, 64, 4),
-    (2, 0, 16, 8), (2, 1, 16, 8), (2, 1, 16, 16), (2, 1, 8, 8), (2, 0, 8, 16),
+    # impl, order, zt, kc, pipe
+    (1, 0, 16, 16, 0), (1, 1, 16, 16, 0), (1, 1, 16, 8, 0), (1, 1, 16, 4, 0), (1, 1, 8, 32, 0), (1, 0, 32, 8, 0),
+    (1, 1, 32, 4, 0), (1, 1, 32, 8, 0), (1, 1, 64, 2, 0), (1, 1, 64, 4, 0),
+    (1, 1, 32, 4, 1), (1, 0, 16, 8, 1),
+    (2, 0, 16, 8, 0), (2, 1, 16, 8, 0), (2, 1, 16, 16, 0), (2, 1, 8, 8, 0), (2, 0, 8, 16, 0),
 ]
 
 
-def _set_variant(monkeypatch, impl, order, zt, kc):
+def _set_variant(monkeypatch, impl, order, zt, kc, pipe=0):
     monkeypatch.setenv("MR_BP_IMPL", str(impl))
     monkeypatch.setenv("MR_BP_ORDER", str(order))
     monkeypatch.setenv("MR_BP_ZT", str(zt))
     monkeypatch.setenv("MR_BP_KC", str(kc))
+    monkeypatch.setenv("MR_BP_PIPE", str(pipe))
 
 
 def _case(seed, nu, nv, n_p):
@@ -53,26 +56,28 @@ def _run(dev, proj, ang, w, nu, nv, center, fused, bufsize=16, direction=0):
     return bp.process(torch.from_numpy(proj).to(dev), ang, w).cpu().numpy()
 
 
-@pytest.mark.parametrize("impl,order,zt,kc", VARIANTS)
+@pytest.mark.parametrize("impl,order,zt,kc,pipe", VARIANTS)
 @pytest.mark.parametrize("nu,nv,n_p,center,direction", [(100, 32, 45, 49.5, 0), (96, 30, 37, 47.0, 1),
                                                         (64, 17, 20, 31.25, 0), (128, 64, 61, 66.0, 0)])
-def test_variant_exact_bitexact(orc, dev, monkeypatch, impl, order, zt, kc, nu, nv, n_p, center, direction):
-    _set_variant(monkeypatch, impl, order, zt, kc)
+def test_variant_exact_bitexact(orc, dev, monkeypatch, impl, order, zt, kc, pipe, nu, nv, n_p, center,
+                                direction):
+    _set_variant(monkeypatch, impl, order, zt, kc, pipe)
     proj, ang, w = _case(nu + nv + n_p, nu, nv, n_p)
     vol = _run(dev, proj, ang, w, nu, nv, center, fused=False, direction=direction)
     ref = orc.bp_parallel_f32(proj, ang, w, center, direction, "linear", 16, False)
     nbad = int(np.sum(vol != ref))
-    print(f"impl={impl} order={order} zt={zt} kc={kc} {nu}x{nv}x{n_p}: {nbad} voxels differ")
+    print(f"impl={impl} order={order} zt={zt} kc={kc} pipe={pipe} {nu}x{nv}x{n_p}: {nbad} voxels differ")
     assert nbad == 0
 
 
-@pytest.mark.parametrize("impl,order,zt,kc", [v for v in VARIANTS if v[0] == 2] + [(1, 1, 16, 16)])
-def test_variant_fused_matches_lds16(orc, dev, monkeypatch, impl, order, zt, kc):
+@pytest.mark.parametrize("impl,order,zt,kc,pipe", [v for v in VARIANTS if v[0] == 2 or v[4] == 1]
+                         + [(1, 1, 16, 16, 0)])
+def test_variant_fused_matches_lds16(orc, dev, monkeypatch, impl, order, zt, kc, pipe):
     nu, nv, n_p, center = 300, 32, 120, 151.5
     proj, ang, w = _case(7, nu, nv, n_p)
     _set_variant(monkeypatch, 1, 0, 16, 16)
     base = _run(dev, proj, ang, w, nu, nv, center, fused=True, bufsize=0)
-    _set_variant(monkeypatch, impl, order, zt, kc)
+    _set_variant(monkeypatch, impl, order, zt, kc, pipe)
     vol = _run(dev, proj, ang, w, nu, nv, center, fused=True, bufsize=0)
     ref = orc.bp_parallel_f32(proj, ang, w, center, 0, "linear", 0, False)
     err = rel_err(vol, ref)[0]

