@@ -182,7 +182,11 @@ constexpr int kBpSpan = 26;
 // costs no vector load and wait.
 __device__ __forceinline__ float4 tab_entry(const float4 *tab, int i)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
     return ((const __attribute__((address_space(4))) float4 *)tab)[i];
+#else
+    return tab[i];   // host pass of a device-only function: never executed
+#endif
 }
 
 // PIPE = 0: one projection at a time, slice groups beyond the tile's last
