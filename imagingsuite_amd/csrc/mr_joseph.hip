@@ -132,6 +132,125 @@ __global__ __launch_bounds__(256) void joseph_forward_kernel(JosephArgs a)
             make_float4(acc[4 * g] * inv_m, acc[4 * g + 1] * inv_m, acc[4 * g + 2] * inv_m, acc[4 * g + 3] * inv_m);
 }
 
+// LDS-staged forward over a chunk of angles that all step along the same
+// axis (class 0: |sin| >= |cos|, rows y; class 1: columns x).  A block owns
+// kFwdNB detector bins x (up to) kFwdKA angles x ZT slices, one ray per
+// thread.  It walks the lines L = 0..N-1 of its class kFwdR at a time: for
+// each line it stages the kFwdW voxels (x ZT slices) from the lowest crossing
+// of any of its rays, then every ray interpolates its two voxels from LDS.
+// Staged positions outside the volume are zeros and carry weight zero, so the
+// sums are those of the direct kernel, accumulated over the same lines in
+// the same order.  Neighbouring angles cross a line close together, so one
+// staged segment serves all the chunk's rays: the volume is read once per
+// chunk instead of once per angle.  The host builds the chunks so that every
+// line's crossings of a block fit in kFwdW - 2 voxels.
+constexpr int kFwdNB = 16, kFwdKA = 16, kFwdR = 4, kFwdW = 80;
+
+template <int ZT>
+__global__ __launch_bounds__(256) void joseph_forward_lds_kernel(JosephArgs a, const int4 *__restrict__ chunks,
+                                                                 const int *__restrict__ order, int n_chunks,
+                                                                 int nbt, int nzs)
+{
+    constexpr int NB = kFwdNB, R = kFwdR, W = kFwdW, RS = ZT + 4, G = ZT / 4;
+    __shared__ float seg[R][W][RS];
+    __shared__ int lo_s[R], hi_s[R];
+    const int total = n_chunks * nbt * nzs;
+    const int per = (total + 7) >> 3;
+    const int b = (int)blockIdx.x;
+    const int wi = (b & 7) * per + (b >> 3);   // XCD b % 8 takes the (b % 8)-th eighth
+    if (wi >= total) return;
+    const int bt = wi % nbt;
+    const int rest = wi / nbt;
+    const int ci = rest % n_chunks;
+    const int zs = rest / n_chunks;
+    const int4 ch = chunks[ci];                // first, count, class
+    const int ai = threadIdx.x / NB, bi = threadIdx.x % NB;
+    const int n = bt * NB + bi;
+    const bool active = ai < ch.y && n < a.nd;
+    const bool edge = active && (bi == 0 || bi == NB - 1 || n == a.nd - 1);
+    const int ang = order[ch.x + min(ai, ch.y - 1)];
+    const float4 t = a.tab[ang];
+    const float su = t.x, c = t.y, s = t.z, inv_m = t.w;
+    const int N = a.nx;
+    const int z0 = zs * ZT;
+    const bool rows = ch.z == 0;
+    // crossing of line L: ((a0 + b1 (L+1)) + cn) * inv - 1
+    const float a0 = rows ? su : -su, b1 = rows ? c : s;
+    const float fn = (float)n;
+    const float cn = rows ? -fn : fn;
+    const float inv = rows ? 1.0f / s : 1.0f / c;
+    const size_t strideL = rows ? (size_t)N * a.nz : (size_t)a.nz;
+    const size_t strideP = rows ? (size_t)a.nz : (size_t)N * a.nz;
+    float acc[ZT];
+#pragma unroll
+    for (int z = 0; z < ZT; ++z) acc[z] = 0.0f;
+    for (int L0 = 0; L0 < N; L0 += R) {
+        if (threadIdx.x < R) {
+            lo_s[threadIdx.x] = 0x7fffffff;
+            hi_s[threadIdx.x] = -0x7fffffff;
+        }
+        __syncthreads();
+        if (edge) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const float pf = ((a0 + b1 * (float)(L0 + r + 1)) + cn) * inv - 1.0f;
+                const int p0 = (int)floorf(pf);
+                atomicMin(&lo_s[r], p0);
+                atomicMax(&hi_s[r], p0);
+            }
+        }
+        __syncthreads();
+        // the group is skipped when no ray crosses the volume on its lines
+        bool any = false;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            any = any || (L0 + r < N && hi_s[r] >= -1 && lo_s[r] < N);
+        if (any) {
+            for (int e = threadIdx.x; e < R * W * G; e += 256) {
+                const int r = e / (W * G);
+                const int q = e - r * W * G;
+                const int j = q / G, g = q - j * G;
+                const int L = L0 + r, p = lo_s[r] + j;
+                float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (L < N && p >= 0 && p < N)
+                    v = *reinterpret_cast<const float4 *>(a.src + (size_t)L * strideL + (size_t)p * strideP + z0 + 4 * g);
+                *reinterpret_cast<float4 *>(&seg[r][j][4 * g]) = v;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (L0 + r < N) {
+                    const float pf = ((a0 + b1 * (float)(L0 + r + 1)) + cn) * inv - 1.0f;
+                    const float fl = floorf(pf);
+                    const int p0 = (int)fl;
+                    const float f = pf - fl;
+                    const float w0 = (p0 >= 0 && p0 < N) ? 1.0f - f : 0.0f;
+                    const float w1 = (p0 + 1 >= 0 && p0 + 1 < N) ? f : 0.0f;
+                    const int j = min(max(p0 - lo_s[r], 0), W - 2);
+                    const float *v0 = &seg[r][j][0];
+                    const float *v1 = v0 + RS;
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        const float4 x0 = *reinterpret_cast<const float4 *>(v0 + 4 * g);
+                        const float4 x1 = *reinterpret_cast<const float4 *>(v1 + 4 * g);
+                        acc[4 * g + 0] += w0 * x0.x + w1 * x1.x;
+                        acc[4 * g + 1] += w0 * x0.y + w1 * x1.y;
+                        acc[4 * g + 2] += w0 * x0.z + w1 * x1.z;
+                        acc[4 * g + 3] += w0 * x0.w + w1 * x1.w;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!active) return;
+    float *out = a.dst + ((size_t)ang * a.nd + n) * a.nz + z0;
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+        *reinterpret_cast<float4 *>(out + 4 * g) =
+            make_float4(acc[4 * g] * inv_m, acc[4 * g + 1] * inv_m, acc[4 * g + 2] * inv_m, acc[4 * g + 3] * inv_m);
+}
+
 // Adjoint: one thread per (x, y) voxel column and ZT slices.
 template <int ZT>
 __global__ __launch_bounds__(256) void joseph_adjoint_kernel(JosephArgs a)
@@ -231,7 +350,7 @@ __global__ __launch_bounds__(256) void joseph_adjoint_lds_kernel(JosephArgs a, i
         __syncthreads();
         if (in_vol) {
             for (int kk = 0; kk < kc; ++kk) {
-                const float4 t = a.tab[kb + kk];
+                const float4 t = tab_entry(a.tab, kb + kk);
                 const float u = (t.x + t.y * yf) - t.z * xf;
                 const float fl = floorf(u);
                 const int n1 = (int)fl;
@@ -279,6 +398,110 @@ std::vector<float> joseph_tables(const mr_joseph_geom &g, const float *angles, s
     return tab;
 }
 
+// Angle chunks of the LDS forward: angles of one stepping class in order of
+// direction, cut so that on every line the crossings of any block's rays
+// (kFwdNB bins x the chunk's angles) span at most kFwdW - 3 voxels (one voxel
+// of margin for float rounding; the interpolation needs p0 and p0 + 1).  The
+// span of affine functions of the line index is largest at the first or
+// last line, so those two are checked.
+JosephPacked joseph_pack(const mr_joseph_geom &g, const float *angles, size_t n_ang)
+{
+    JosephPacked pk;
+    pk.tab = joseph_tables(g, angles, n_ang);
+    const int N = (int)g.nx, nd = (int)g.n_det;
+    const int nbt = (nd + kFwdNB - 1) / kFwdNB;
+    for (int cls = 0; cls < 2; ++cls) {
+        std::vector<std::pair<double, int>> idx;
+        for (size_t i = 0; i < n_ang; ++i) {
+            const float c = pk.tab[4 * i + 1], s = pk.tab[4 * i + 2];
+            const int k = std::fabs(s) >= std::fabs(c) ? 0 : 1;
+            if (k != cls) continue;
+            double dir = std::atan2((double)s, (double)c);
+            if (dir < 0) dir += 2.0 * 3.14159265358979323846;
+            idx.push_back({dir, (int)i});
+        }
+        std::sort(idx.begin(), idx.end());
+        // per bin tile and end line: min / max crossing over the chunk
+        std::vector<double> mn(2 * nbt), mx(2 * nbt), tmn(2 * nbt), tmx(2 * nbt);
+        int first = (int)pk.order.size(), count = 0;
+        auto flush = [&]() {
+            if (count > 0) pk.chunks.push_back(make_int4(first, count, cls, 0));
+            first = (int)pk.order.size();
+            count = 0;
+        };
+        for (const auto &e : idx) {
+            const int i = e.second;
+            const double su = pk.tab[4 * i], c = pk.tab[4 * i + 1], s = pk.tab[4 * i + 2];
+            const double a0 = cls == 0 ? su : -su, b1 = cls == 0 ? c : s, inv = cls == 0 ? 1.0 / s : 1.0 / c;
+            double worst = 0.0;
+            for (int bt = 0; bt < nbt; ++bt) {
+                const int na = bt * kFwdNB, nb = std::min(na + kFwdNB - 1, nd - 1);
+                for (int end = 0; end < 2; ++end) {
+                    const double L1 = end == 0 ? 1.0 : (double)N;
+                    double lo = 1e300, hi = -1e300;
+                    for (int nn : {na, nb}) {
+                        const double cn = cls == 0 ? -(double)nn : (double)nn;
+                        const double pf = ((a0 + b1 * L1) + cn) * inv - 1.0;
+                        lo = std::min(lo, pf);
+                        hi = std::max(hi, pf);
+                    }
+                    const int k = 2 * bt + end;
+                    tmn[k] = count ? std::min(mn[k], lo) : lo;
+                    tmx[k] = count ? std::max(mx[k], hi) : hi;
+                    worst = std::max(worst, tmx[k] - tmn[k]);
+                }
+            }
+            if (count > 0 && (count == kFwdKA || std::ceil(worst) + 1.0 > (double)(kFwdW - 2))) {
+                flush();
+                // restart the extents with this angle alone
+                for (int bt = 0; bt < nbt; ++bt) {
+                    const int na = bt * kFwdNB, nb = std::min(na + kFwdNB - 1, nd - 1);
+                    for (int end = 0; end < 2; ++end) {
+                        const double L1 = end == 0 ? 1.0 : (double)N;
+                        double lo = 1e300, hi = -1e300;
+                        for (int nn : {na, nb}) {
+                            const double cn = cls == 0 ? -(double)nn : (double)nn;
+                            const double pf = ((a0 + b1 * L1) + cn) * inv - 1.0;
+                            lo = std::min(lo, pf);
+                            hi = std::max(hi, pf);
+                        }
+                        tmn[2 * bt + end] = lo;
+                        tmx[2 * bt + end] = hi;
+                    }
+                }
+            }
+            mn = tmn;
+            mx = tmx;
+            pk.order.push_back(i);
+            ++count;
+        }
+        flush();
+    }
+    return pk;
+}
+
+size_t JosephPacked::bytes() const
+{
+    return sizeof(float) * tab.size() + sizeof(int4) * chunks.size() + sizeof(int) * order.size();
+}
+
+// One buffer: tables, then chunks, then the angle order.
+int joseph_upload(const JosephPacked &pk, void *d_base, hipStream_t s, JosephDevTables &dt)
+{
+    char *b = static_cast<char *>(d_base);
+    const size_t o1 = sizeof(float) * pk.tab.size(), o2 = o1 + sizeof(int4) * pk.chunks.size();
+    MR_HIP(hipMemcpyAsync(b, pk.tab.data(), o1, hipMemcpyHostToDevice, s));
+    if (!pk.chunks.empty())
+        MR_HIP(hipMemcpyAsync(b + o1, pk.chunks.data(), o2 - o1, hipMemcpyHostToDevice, s));
+    if (!pk.order.empty())
+        MR_HIP(hipMemcpyAsync(b + o2, pk.order.data(), sizeof(int) * pk.order.size(), hipMemcpyHostToDevice, s));
+    dt.tab = reinterpret_cast<const float4 *>(b);
+    dt.chunks = reinterpret_cast<const int4 *>(b + o1);
+    dt.order = reinterpret_cast<const int *>(b + o2);
+    dt.n_chunks = (int)pk.chunks.size();
+    return MR_OK;
+}
+
 int joseph_check(const mr_joseph_geom *g, size_t n_ang)
 {
     MR_REQUIRE(g != nullptr, "geometry is NULL");
@@ -290,15 +513,22 @@ int joseph_check(const mr_joseph_geom *g, size_t n_ang)
 }
 
 // Enqueues A (forward) or A^T on s with a resident angle table; no host sync.
-int joseph_launch(const mr_joseph_geom *g, const float *src, float *dst, const float4 *d_tab, size_t n_ang,
+int joseph_launch(const mr_joseph_geom *g, const float *src, float *dst, const JosephDevTables &dt, size_t n_ang,
                   bool forward, hipStream_t s)
 {
-    mr::JosephArgs a{src, dst, d_tab, (int)g->nx, (int)g->nz, (int)g->n_det, (int)n_ang};
+    mr::JosephArgs a{src, dst, dt.tab, (int)g->nx, (int)g->nz, (int)g->n_det, (int)n_ang};
     constexpr int ZT = 4;
     // ZT = 16 when the slice count allows it
     if (forward) {
         mr::ProfScope prof("joseph_forward", s);
-        if (g->nz % 16 == 0) {
+        const char *ef = std::getenv("MR_JOSEPH_FWD");
+        if (g->nz % 16 == 0 && dt.n_chunks > 0 && !(ef && std::atoi(ef) == 0)) {
+            const int nbt = (int)((g->n_det + kFwdNB - 1) / kFwdNB), nzs = (int)(g->nz / 16);
+            const long total = (long)dt.n_chunks * nbt * nzs;
+            MR_REQUIRE(total < (1L << 31) - 8, "Joseph forward grid too large");
+            hipLaunchKernelGGL(mr::joseph_forward_lds_kernel<16>, dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256),
+                               0, s, a, dt.chunks, dt.order, dt.n_chunks, nbt, nzs);
+        } else if (g->nz % 16 == 0) {
             dim3 grid((unsigned)((g->n_det + 63) / 64), (unsigned)((n_ang + 3) / 4), (unsigned)(g->nz / 16));
             hipLaunchKernelGGL(mr::joseph_forward_kernel<16>, grid, dim3(256), 0, s, a);
         } else {
@@ -341,16 +571,13 @@ int joseph_common(const mr_joseph_geom *g, const float *src, const float *angles
     if (rc != MR_OK) return rc;
     MR_REQUIRE(src != nullptr && dst != nullptr && angles != nullptr, "NULL pointer");
     hipStream_t s = (hipStream_t)stream;
-    std::vector<float> tab = mr::joseph_tables(*g, angles, n_ang);
-    float4 *d_tab = nullptr;
-    MR_HIP(hipMallocAsync((void **)&d_tab, sizeof(float4) * n_ang, s));
-    hipError_t e = hipMemcpyAsync(d_tab, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice, s);
-    if (e != hipSuccess) {
-        (void)hipFreeAsync(d_tab, s);
-        return mr::hip_fail(e, "hipMemcpyAsync", __FILE__, __LINE__);
-    }
-    rc = mr::joseph_launch(g, src, dst, d_tab, n_ang, forward, s);
-    hipError_t e2 = hipFreeAsync(d_tab, s);
+    const mr::JosephPacked pk = mr::joseph_pack(*g, angles, n_ang);
+    void *d_buf = nullptr;
+    MR_HIP(hipMallocAsync(&d_buf, pk.bytes(), s));
+    mr::JosephDevTables dt;
+    rc = mr::joseph_upload(pk, d_buf, s, dt);
+    if (rc == MR_OK) rc = mr::joseph_launch(g, src, dst, dt, n_ang, forward, s);
+    hipError_t e2 = hipFreeAsync(d_buf, s);
     if (rc != MR_OK) return rc;
     if (e2 != hipSuccess) return mr::hip_fail(e2, "hipFreeAsync", __FILE__, __LINE__);
     // the host table must outlive the asynchronous copy
