@@ -29,7 +29,7 @@ EXPORTED_SYMBOLS = (
     "mr_bp_plan_create", "mr_bp_plan_destroy", "mr_bp_plan_mask", "mr_bp_plan_kernel", "mr_bp_parallel",
     "mr_fbp_parallel",
     "mr_fdk_window", "mr_fdk_plan_create", "mr_fdk_plan_destroy", "mr_fdk_plan_window", "mr_fdk_bp",
-    "mr_joseph_forward", "mr_joseph_adjoint",
+    "mr_joseph_forward", "mr_joseph_adjoint", "mr_linear_forward",
     "mr_iter_plan_create", "mr_iter_run", "mr_iter_plan_destroy",
     "mr_fbp_stream_create", "mr_fbp_stream_run", "mr_fbp_stream_destroy", "mr_host_alloc", "mr_host_free",
     "mr_lognorm_reference", "mr_lognorm", "mr_lognorm_logdose",
@@ -123,6 +123,7 @@ def load():
     L.mr_fdk_bp.argtypes = [vp, vp, sz, sz, fp, fp, vp, vp]
     for f in (L.mr_joseph_forward, L.mr_joseph_adjoint):
         f.argtypes = [ctypes.POINTER(JosephGeom), vp, fp, sz, vp, vp]
+    L.mr_linear_forward.argtypes = [vp, sz, sz, fp, sz, vp, vp]
     L.mr_iter_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(JosephGeom), fp, sz, i32]
     L.mr_iter_run.argtypes = [vp, vp, vp, i32, f32, i32, ctypes.POINTER(ctypes.c_double), vp]
     L.mr_iter_plan_destroy.argtypes = [vp]

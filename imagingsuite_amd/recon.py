@@ -414,6 +414,29 @@ class BatchReconstructor:
     __del__ = close
 
 
+class LinearForwardProjector:
+    """LinearForwardProjector::project (linearforwardprojector.cpp:15-143) on
+    the GPU: ``project(slice, angles)`` takes a [M, M] slice (slice[y, x], the
+    reference's TImage<float,2>) and returns proj [n_ang, M] (proj[t, r]); a
+    [M, M, nz] MatrixZXY stack returns [n_ang, M, nz].  Bit-identical to the
+    reference per slice."""
+
+    def project(self, vol, angles, stream=None):
+        import torch
+        if not isinstance(vol, torch.Tensor) or not vol.is_cuda or vol.dtype != torch.float32:
+            raise TypeError("vol must be a float32 CUDA tensor")
+        if vol.dim() not in (2, 3) or vol.shape[0] != vol.shape[1]:
+            raise MrError(_lib.MR_ERR_INVALID, "slice must be [M, M] or [M, M, nz]")
+        v = vol.contiguous()
+        M = int(v.shape[0])
+        nz = 1 if v.dim() == 2 else int(v.shape[2])
+        a = np.ascontiguousarray(angles, np.float32).ravel()
+        out = torch.empty((a.size, M, nz), dtype=torch.float32, device=v.device)
+        check(_lib.load().mr_linear_forward(_ptr(v), M, nz, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                            a.size, _ptr(out), _stream(stream)))
+        return out[..., 0] if v.dim() == 2 else out
+
+
 class IterativeReconstructor:
     """SIRT / Landweber / CGLS on the matched Joseph pair (mr_iter_*), the
     methods of IterativeMeth.txt (frameworks/tomography/Backprojectors/

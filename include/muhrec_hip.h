@@ -306,6 +306,18 @@ int mr_joseph_forward(const mr_joseph_geom *geom, const float *d_volume, const f
 int mr_joseph_adjoint(const mr_joseph_geom *geom, const float *d_sino, const float *angles,
                       size_t n_ang, float *d_volume, void *stream);
 
+/* LinearForwardProjector::project
+ * (frameworks/tomography/Framework/ReconAlgorithms/ReconAlgorithms/linearforwardprojector.cpp:15-143)
+ * for a stack of slices: d_volume [M(y)][M(x)][nz] (MatrixZXY; nz = 1 is the
+ * reference's single TImage<float,2> slice, slice(x, y) = d_volume[y M + x]),
+ * d_sino [n_ang][M][nz] (proj(r, t) of slice z at [(t M + r) nz + z]).  Each
+ * slice's projection equals the reference's bit for bit, including its
+ * quirks (bins whose position starts below 0 at 90 degrees lose the steps
+ * before it turns non-negative, the last bin of an axis-aligned view is 0).
+ * The reference throws for 3-D input; here slices are independent. */
+int mr_linear_forward(const float *d_volume, size_t M, size_t nz, const float *angles, size_t n_ang,
+                      float *d_sino, void *stream);
+
 /* ------------------------------------------------------------------------
  * Iterative reconstruction on the matched pair: SIRT, Landweber and CGLS as
  * sketched in frameworks/tomography/Backprojectors/IterativeBackProj/src/IterativeMeth.txt

@@ -90,6 +90,7 @@ def _setup(L):
     L.orc_fdk_filter_f32.argtypes = [_fp, _sz, _sz]
     L.orc_fdk_filter_f64.argtypes = [_dp, _sz, _sz]
     L.orc_fdk_filter_ext_f64.argtypes = [_dp, _sz, _sz, ctypes.c_int]
+    L.orc_linear_forward_f32.argtypes = [_fp, _sz, _fp, _sz, _fp]
     L.orc_fdk_bp_f32.argtypes = [_fp, _sz, _sz, _sz, _fp, _fp, ctypes.POINTER(FdkGeom), _sp, _sp, _fp]
     L.orc_fdk_bp_f64.argtypes = [_fp, _sz, _sz, _sz, _dp, _dp, ctypes.POINTER(FdkGeom), _sp, _sp, _dp]
     L.orc_fdk_bp_ext_f64.argtypes = [_fp, _sz, _sz, _sz, _dp, _dp, ctypes.POINTER(FdkGeom), _sp, _sp, _dp,
@@ -444,6 +445,17 @@ def joseph_adjoint_f64(sino: np.ndarray, angles, nx: int, center: float,
     a = np.ascontiguousarray(angles, np.float64)
     out = np.zeros((nx, nx, nz), np.float64)
     lib().orc_joseph_adjoint_f64(_d(s), nx, nz, nd, center, direction, _d(a), na, _d(out))
+    return out
+
+
+def linear_forward_f32(slice_: np.ndarray, angles) -> np.ndarray:
+    """LinearForwardProjector::project (linearforwardprojector.cpp:15-143):
+    slice [M, M] indexed slice[y, x] -> proj [T, M] (proj[t, r])."""
+    sl = np.ascontiguousarray(slice_, np.float32)
+    M = sl.shape[0]
+    a = np.ascontiguousarray(angles, np.float32)
+    out = np.zeros((a.size, M), np.float32)
+    lib().orc_linear_forward_f32(_f(sl), M, _f(a), a.size, _f(out))
     return out
 
 

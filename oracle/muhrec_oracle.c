@@ -1231,3 +1231,101 @@ void orc_joseph_adjoint_f64(const double *sino, size_t nx, size_t nz, size_t nd,
         }
     free(tab);
 }
+
+/* ------------------------------------------------------------------------
+ * LinearForwardProjector::project (2-D)
+ * (frameworks/tomography/Framework/ReconAlgorithms/ReconAlgorithms/
+ * linearforwardprojector.cpp:15-143) with the reference's types: float
+ * state, double dX*sinT/cosT and scale factors, the sequential float
+ * position update nfloat += alpha, truncation, the [0, N-1) validity test and
+ * the ceil/floor step range with eps = 1e-4.  slice(x, y) = slice[x + y*M];
+ * proj(r, t) = proj[r + t*M] (R = M).  The reference's trig is std::sin /
+ * std::cos of a float (sharedbuffer.h brings `using namespace std`).
+ * ------------------------------------------------------------------------ */
+void orc_linear_forward_f32(const float *slice, size_t Ms, const float *angles, size_t T, float *proj)
+{
+    const int M = (int)Ms, N = (int)Ms, R = (int)Ms;
+    const float eps = 1e-4f;
+    const float rho_min = (float)(-R / 2), x_min = (float)(-M / 2);
+    const float Delta_rho = 1.0f, Delta_x = 1.0f;
+    memset(proj, 0, sizeof(float) * (size_t)R * T);
+    for (size_t t = 0; t < T; ++t) {
+        float theta = angles[t] + 0.0f;
+        theta *= ORC_FPI / 180.0f;
+        const float sintheta = sinf(theta), costheta = cosf(theta);
+        const float rhooffset = x_min * (costheta + sintheta);
+        if ((double)fabsf(sintheta) > sqrt(0.5)) {
+            const float alpha = -costheta / sintheta;
+            const double dXsinT = (double)(Delta_x * sintheta);
+            const double dXdsinT = (double)(Delta_x / fabsf(sintheta));
+            float rho = 0.0f;
+            for (int r = 0; r < R; ++r, rho += Delta_rho) {
+                const float beta = (float)((double)(rho + rho_min - rhooffset) / dXsinT);
+                const float betap = (float)((double)beta + 0.5);
+                float sum = 0.0f;
+                int mmin, mmax;
+                if (alpha > 1e-6f) {
+                    mmin = (int)ceilf(-(betap - eps) / alpha);
+                    mmax = 1 + (int)floorf(((float)N - betap - eps) / alpha);
+                } else if (alpha < -1e-6f) {
+                    mmin = (int)ceilf(((float)N - betap - eps) / alpha);
+                    mmax = 1 + (int)floorf(-(betap - eps) / alpha);
+                } else {
+                    mmin = 0;
+                    mmax = M;
+                }
+                if (mmin < 0) mmin = 0;
+                if (mmin >= M) mmin = M - 1;
+                if (mmax > M) mmax = M;
+                float nfloat = beta + (float)mmin * alpha;
+                for (int m = mmin; m < mmax; ++m) {
+                    const float nf = (float)(int)nfloat;
+                    const float w = nfloat - nf;
+                    if (nfloat >= 0.0f && nfloat < (float)(N - 1)) {
+                        const float a = slice[(size_t)m + (size_t)nf * M];
+                        const float b = slice[(size_t)m + (size_t)(nf + 1.0f) * M];
+                        sum += a + w * (b - a);
+                    }
+                    nfloat += alpha;
+                }
+                proj[(size_t)r + t * R] = (float)((double)sum * dXdsinT);
+            }
+        } else {
+            const float alpha = -sintheta / costheta;
+            const double dXcosT = (double)(Delta_x * costheta);
+            const double dXdcosT = (double)(Delta_x / fabsf(costheta));
+            float rho = 0.0f;
+            for (int r = 0; r < R; ++r, rho += Delta_rho) {
+                const float beta = (float)((double)(rho + rho_min - rhooffset) / dXcosT);
+                const float betap = (float)((double)beta + 0.5);
+                float sum = 0.0f;
+                int nmin, nmax;
+                if (alpha > 1e-6f) {
+                    nmin = (int)ceilf(-(betap - eps) / alpha);
+                    nmax = 1 + (int)floorf(((float)M - betap - eps) / alpha);
+                } else if (alpha < -1e-6f) {
+                    nmin = (int)ceilf(((float)M - betap - eps) / alpha);
+                    nmax = 1 + (int)floorf(-(betap - eps) / alpha);
+                } else {
+                    nmin = 0;
+                    nmax = N;
+                }
+                if (nmin < 0) nmin = 0;
+                if (nmin >= N) nmin = N - 1;
+                if (nmax > N) nmax = N;
+                float mfloat = beta + (float)nmin * alpha;
+                for (int n = nmin; n < nmax; ++n) {
+                    const float mf = (float)(int)mfloat;
+                    const float w = mfloat - mf;
+                    if (mfloat >= 0.0f && mfloat < (float)(M - 1)) {
+                        const float a = slice[(size_t)mf + (size_t)n * M];
+                        const float b = slice[(size_t)(mf + 1.0f) + (size_t)n * M];
+                        sum += a + w * (b - a);
+                    }
+                    mfloat += alpha;
+                }
+                proj[(size_t)r + t * R] = (float)((double)sum * dXdcosT);
+            }
+        }
+    }
+}
