@@ -183,12 +183,48 @@ def bench_parallel(args):
         r["exact_mode"] = {"value": round(updates / (ex["ms"] / 1e3) / 1e9, 2), "ms_per_step": round(ex["ms"], 3),
                            "bp_kernel_ms": round(ex["bp"], 3), "kernel": ex["kernel"],
                            "note": "reference float order per voxel, bit-identical to MultiProjBPparallel"}
+    if not args.no_stream:
+        r["pcie_inclusive"] = stream_leg(pristine, angles, weights, cfg, center, buf, nu, nv, n_p, updates)
     if not args.no_cpu_baseline:
         host = pristine[:args.cpu_sample_proj, :args.cpu_sample_slices].cpu().numpy()
         log("[bench] timing CPU baseline (oracle) ...")
         r["cpu_baseline"] = cpu_baseline(host, angles, weights, cfg, min(args.cpu_sample_proj, n_p),
                                          min(args.cpu_sample_slices, nv))
     emit(r)
+
+
+def stream_leg(pristine, angles, weights, cfg, center, buf, nu, nv, n_p, updates):
+    """Host stack in, host slices out (mr_fbp_stream): the transfers of
+    neighbouring 32-slice blocks overlapped with the kernels, against the same
+    blocks run one after the other, for pinned and pageable host buffers."""
+    from imagingsuite_amd.recon import FBPStream, host_empty
+    log("[bench] streaming leg: host-resident stack, 32-slice blocks ...")
+    hp = host_empty(tuple(pristine.shape))
+    hp[...] = pristine.cpu().numpy()
+    ho = host_empty((nv, nu, nu))
+    st = FBPStream(nu, nv, n_p, center, slice_block=32, buffer_size=buf, fused=True, filtertype=cfg["filtertype"])
+    out = {}
+    for name, src, dst, serial in [("overlapped_pinned", hp, ho, False), ("serial_pinned", hp, ho, True)]:
+        st.run(src, angles, weights, out=dst, serial=serial)
+        t0 = time.perf_counter()
+        st.run(src, angles, weights, out=dst, serial=serial)
+        ms = (time.perf_counter() - t0) * 1e3
+        out[name] = {"ms": round(ms, 2), "GUPS": round(updates / (ms / 1e3) / 1e9, 1)}
+        log(f"[bench] stream {name}: {ms:.1f} ms")
+    hq = np.array(hp)          # pageable copies
+    del hp
+    hv = np.empty((nv, nu, nu), np.float32)
+    st.run(hq, angles, weights, out=hv)
+    t0 = time.perf_counter()
+    st.run(hq, angles, weights, out=hv)
+    ms = (time.perf_counter() - t0) * 1e3
+    out["overlapped_pageable"] = {"ms": round(ms, 2), "GUPS": round(updates / (ms / 1e3) / 1e9, 1)}
+    log(f"[bench] stream overlapped_pageable: {ms:.1f} ms")
+    st.close()
+    out["bytes_h2d"] = 4 * n_p * nv * nu
+    out["bytes_d2h"] = 4 * nv * nu * nu
+    out["note"] = "whole FBP with the host-to-device copy of the stack and the device-to-host copy of the slices"
+    return out
 
 
 # --------------------------------------------------------------------------
@@ -212,8 +248,12 @@ def bench_fdk(args):
     sod, sdd, pix, vox = 500.0, 1000.0, (0.2, 0.2), (0.1, 0.1, 0.1)
     ppoint = (n / 2, n / 2)
     roi = (0, 0, n, n)
-    fdk = FDKbp_single(sod, sdd, pix, vox, ppoint, n / 2, roi, n - 1, fused=True)
+    # the configuration's filter: Shepp-Logan with cosine pre-weighting (an
+    # extension; positions and sums in the reference's float order)
+    fdk = FDKbp_single(sod, sdd, pix, vox, ppoint, n / 2, roi, n - 1, filter="shepp-logan", cosine_weight=True)
+    # MuhRec's own filter (ramp_filter_tuned), the parity path
     fdk_exact = FDKbp_single(sod, sdd, pix, vox, ppoint, n / 2, roi, n - 1)
+    fdk_fused = FDKbp_single(sod, sdd, pix, vox, ppoint, n / 2, roi, n - 1, fused=True)
     cb = fdk.window
     log(f"[bench] FDK 1024^3, window {cb}")
     angles = np.arange(n_p, dtype=np.float32) * np.float32(360.0 / n_p)
@@ -247,10 +287,11 @@ def bench_fdk(args):
     r = base_result("GUPS (voxel*proj updates/s) FDK 1024^3 x 720 proj", round(updates / (ms / 1e3) / 1e9, 2),
                     "GUPS", args.steps, 1, ms, "f32",
                     {"workload": "FDK 1024^3 from 720 projections, 1024^2 detector, SOD 500, SDD 1000",
-                     "window_rows": cb[3] - cb[1], "precision": "fused"})
+                     "window_rows": cb[3] - cb[1], "filter": "Shepp-Logan + cosine pre-weighting",
+                     "precision": "exact positions"})
     r["roofline"] = {"bound": "mfma", "pipe": "fp32 VALU", "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                     "traffic": None, "kernel": "fdk_kernel<%s,true>" % os.environ.get("MR_FDK_KT", "16"),
+                     "traffic": None, "kernel": "fdk_kernel<%s,false>" % os.environ.get("MR_FDK_KT", "16"),
                      "kernel_ms": round(kms, 3),
                      "flop_per_update": FDK_FLOP_PER_UPDATE}
     r["kernels_ms"] = {"row_filter_fdk": round(fl_ms / max(fl_n, 1), 3), "fdk_bp": round(kms, 3)}
@@ -266,9 +307,26 @@ def bench_fdk(args):
     mse = timed(step_e, max(1, args.steps // 2), 0, "fdk exact")
     bpe_ms, bpe_n = _lib.profile_query("fdk_bp")
     _lib.profile(False)
-    r["exact_mode"] = {"ms_per_step": round(mse, 3), "GUPS": round(updates / (mse / 1e3) / 1e9, 2),
-                       "kernel_ms": round(bpe_ms / max(bpe_n, 1), 3),
-                       "note": "reference float operation order: positions and sums bit-identical to FDKbp_single"}
+    r["muhrec_filter"] = {"ms_per_step": round(mse, 3), "GUPS": round(updates / (mse / 1e3) / 1e9, 2),
+                          "kernel_ms": round(bpe_ms / max(bpe_n, 1), 3),
+                          "note": "ramp_filter_tuned as the reference; positions and sums bit-identical to "
+                                  "FDKbp_single (the parity path)"}
+
+    def step_f():
+        work.copy_(proj)
+        fdk_fused.process(work, angles, weights, vol)
+
+    step_f()
+    torch.cuda.synchronize()
+    _lib.profile(True)
+    _lib.profile_reset()
+    msf = timed(step_f, max(1, args.steps // 2), 0, "fdk fused")
+    bpf_ms, bpf_n = _lib.profile_query("fdk_bp")
+    _lib.profile(False)
+    r["fused_positions"] = {"ms_per_step": round(msf, 3), "GUPS": round(updates / (msf / 1e3) / 1e9, 2),
+                            "kernel_ms": round(bpf_ms / max(bpf_n, 1), 3),
+                            "note": "reference filter, row position folded into FMAs (relRMSE up to 4e-4 "
+                                    "against the reference on random data)"}
     # the double-geometry module on the same slab (FDKbp, fdkbp.cpp)
     from imagingsuite_amd.recon import FDKbp
     fdkd = FDKbp(sod, sdd, pix, vox, ppoint, n / 2, roi, n - 1)
@@ -410,6 +468,8 @@ def main():
                     help="fused (headline): interpolation with FMAs; exact: the reference's "
                          "float order (bit-identical); both: report both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-stream", action="store_true",
+                    help="skip the host-resident streaming leg (PCIe-inclusive timing)")
     ap.add_argument("--cpu-sample-proj", type=int, default=1500)
     ap.add_argument("--cpu-sample-slices", type=int, default=96)
     ap.add_argument("--calibrate-copy", action="store_true")

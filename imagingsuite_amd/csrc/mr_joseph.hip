@@ -206,10 +206,12 @@ __global__ __launch_bounds__(256) void joseph_forward_lds_kernel(JosephArgs a, c
         for (int r = 0; r < R; ++r)
             any = any || (L0 + r < N && hi_s[r] >= -1 && lo_s[r] < N);
         if (any) {
+            // only the voxels a line's rays read: lo .. hi + 1
             for (int e = threadIdx.x; e < R * W * G; e += 256) {
                 const int r = e / (W * G);
                 const int q = e - r * W * G;
                 const int j = q / G, g = q - j * G;
+                if (j > hi_s[r] - lo_s[r] + 1) continue;
                 const int L = L0 + r, p = lo_s[r] + j;
                 float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 if (L < N && p >= 0 && p < N)
