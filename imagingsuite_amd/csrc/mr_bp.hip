@@ -21,6 +21,7 @@
 #include "mr_host.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -594,6 +595,154 @@ static int env_int(const char *name, int dflt)
     return e ? std::atoi(e) : dflt;
 }
 
+// Chunked adjoint: the used projections are sorted by stepping class (rows
+// when |sin| >= |cos|, else columns) and direction and cut into chunks whose
+// candidate voxels, for every block of kAdjNB detector bins, lie within
+// kAdjW voxels on every line.  A block (kAdjNB bins x one chunk of up to
+// kAdjKA projections x ZT slices, one (bin, projection) per thread) walks the
+// lines kAdjR at a time: it stages each line's candidate voxels of the volume
+// gradient into LDS (zeros outside the circle mask, so masked voxels add
+// nothing) and every thread tests its candidates exactly as the direct
+// kernel does (u recomputed with the forward's float operations), so the sums
+// are the direct kernel's, in the same line order.  The volume is read once
+// per chunk instead of once per projection.
+constexpr int kAdjNB = 16, kAdjKA = 16, kAdjR = 4, kAdjW = 80;
+
+template <int INTERP, int ZT>
+__global__ __launch_bounds__(256) void bp_adjoint_lds_kernel(const float *__restrict__ g, float *__restrict__ out,
+                                                             const float4 *__restrict__ tab,
+                                                             const int2 *__restrict__ mask,
+                                                             const int *__restrict__ idx,
+                                                             const float *__restrict__ wts,
+                                                             const int4 *__restrict__ chunks,
+                                                             const int *__restrict__ order, int n_chunks, int nbt,
+                                                             int nzs, int nu, int nv)
+{
+    constexpr int NB = kAdjNB, R = kAdjR, W = kAdjW, RS = ZT + 4, G = ZT / 4;
+    __shared__ float seg[R][W][RS];
+    __shared__ int lo_s[R], hi_s[R];
+    const int total = n_chunks * nbt * nzs;
+    const int per = (total + 7) >> 3;
+    const int b = (int)blockIdx.x;
+    const int wi = (b & 7) * per + (b >> 3);
+    if (wi >= total) return;
+    const int bt = wi % nbt;
+    const int rest = wi / nbt;
+    const int ci = rest % n_chunks;
+    const int zs = rest / n_chunks;
+    const int4 ch = chunks[ci];                 // first, count, class
+    const int ai = threadIdx.x / NB, bi = threadIdx.x % NB;
+    const int u = bt * NB + bi;
+    const bool active = ai < ch.y && u < nu;
+    const bool edge = active && (bi == 0 || bi == NB - 1 || u == nu - 1);
+    const int k = order[ch.x + min(ai, ch.y - 1)];   // used-projection index
+    const float4 t = tab[k];
+    const float su = t.x, c = t.y, s = t.z;
+    const int num2 = nu - 2;
+    const float fu = (float)u;
+    const bool rows = ch.z == 0;
+    const int z0 = zs * ZT;
+    // candidates on line L (1-based): u(x, y) in [ulo, uhi]
+    const float ulo = u <= 1 ? -1.0f : fu - 1.0f, uhi = fu + 1.0f;
+    const float inv = rows ? 1.0f / s : 1.0f / c;
+    float acc[ZT];
+#pragma unroll
+    for (int z = 0; z < ZT; ++z) acc[z] = 0.0f;
+    for (int L0 = 1; L0 <= nu; L0 += R) {
+        if (threadIdx.x < R) {
+            lo_s[threadIdx.x] = 0x7fffffff;
+            hi_s[threadIdx.x] = -0x7fffffff;
+        }
+        __syncthreads();
+        int clo[R], chi[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int L = L0 + r;
+            float pa, pb;
+            if (rows) {
+                const float lsu = su + c * (float)L;
+                pa = (lsu - uhi) * inv;
+                pb = (lsu - ulo) * inv;
+            } else {
+                const float bb = s * (float)L - su;
+                pa = (ulo + bb) * inv;
+                pb = (uhi + bb) * inv;
+            }
+            // the edge rays' unclamped ranges bound every ray's range of the
+            // block (the range moves monotonically with the bin)
+            const int rlo = (int)floorf(fminf(pa, pb)) - 1, rhi = (int)ceilf(fmaxf(pa, pb)) + 1;
+            clo[r] = max(1, rlo);
+            chi[r] = min(nu, rhi);
+            if (edge) {
+                atomicMin(&lo_s[r], rlo);
+                atomicMax(&hi_s[r], rhi);
+            }
+        }
+        __syncthreads();
+        bool any = false;
+#pragma unroll
+        for (int r = 0; r < R; ++r) any = any || (L0 + r <= nu && lo_s[r] <= nu && hi_s[r] >= 1);
+        if (any) {
+            for (int e = threadIdx.x; e < R * W * G; e += 256) {
+                const int r = e / (W * G);
+                const int q = e - r * W * G;
+                const int j = q / G, gq = q - j * G;
+                const int L = L0 + r, p = lo_s[r] + j;   // 1-based position on the line
+                if (j > hi_s[r] - lo_s[r]) continue;
+                float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (L <= nu && p >= 1 && p <= nu) {
+                    const int x = rows ? p : L, y = rows ? L : p;
+                    const int2 m = mask[y - 1];
+                    if (x - 1 >= m.x && x - 1 < m.y)
+                        v = *reinterpret_cast<const float4 *>(g + ((size_t)(y - 1) * nu + (x - 1)) * nv + z0 + 4 * gq);
+                }
+                *reinterpret_cast<float4 *>(&seg[r][j][4 * gq]) = v;
+            }
+            __syncthreads();
+            if (active) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int L = L0 + r;
+                    if (L > nu) break;
+                    const int base = lo_s[r];
+                    for (int p = clo[r]; p <= chi[r]; ++p) {
+                        const int x = rows ? p : L, y = rows ? L : p;
+                        const float lsu = su + c * (float)y;
+                        const float uu = lsu - s * (float)x;
+                        const int n = (int)uu;
+                        if (n < 0 || n > num2) continue;
+                        float wgt;
+                        if (INTERP == 1) {
+                            if (n != u) continue;
+                            wgt = 1.0f;
+                        } else {
+                            const float wb = fabsf(uu - (float)n);
+                            if (n == u) wgt = 1.0f - wb;
+                            else if (n == u - 1) wgt = wb;
+                            else continue;
+                        }
+                        const float *v = &seg[r][min(max(p - base, 0), W - 1)][0];
+#pragma unroll
+                        for (int gq = 0; gq < G; ++gq) {
+                            const float4 x4 = *reinterpret_cast<const float4 *>(v + 4 * gq);
+                            acc[4 * gq + 0] += wgt * x4.x;
+                            acc[4 * gq + 1] += wgt * x4.y;
+                            acc[4 * gq + 2] += wgt * x4.z;
+                            acc[4 * gq + 3] += wgt * x4.w;
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!active) return;
+    const float w = wts[idx[k]];
+    float *o = out + (size_t)idx[k] * nv * nu + u;
+#pragma unroll
+    for (int z = 0; z < ZT; ++z) o[(size_t)(z0 + z) * nu] = w * acc[z];
+}
+
 // Adjoint of the parallel back-projection (the project's own operator for
 // gradients through FBP; no reference counterpart).  The forward adds, for
 // every used projection k and voxel (x, y) inside the mask, w_k ((1-f) S[n] +
@@ -830,6 +979,8 @@ struct mr_bp_plan {
     int impl = 1;  // 1 LDS-staged kernel, 0 direct-gather kernel
     size_t loaded_n = 0, loaded_nk = 0;   // projections / used projections of the resident tables
     bool loaded = false;
+    void *d_adj = nullptr;                // adjoint chunks (int4) and projection order (int)
+    size_t adj_cap = 0;                   // bytes
 };
 
 namespace {
@@ -919,6 +1070,7 @@ extern "C" int mr_bp_plan_destroy(mr_bp_plan *plan)
     if (plan->d_tab) (void)hipFree(plan->d_tab);
     if (plan->d_idx) (void)hipFree(plan->d_idx);
     if (plan->d_w) (void)hipFree(plan->d_w);
+    if (plan->d_adj) (void)hipFree(plan->d_adj);
     delete plan;
     return MR_OK;
     MR_GUARD_END
@@ -1021,6 +1173,75 @@ extern "C" int mr_bp_parallel(mr_bp_plan *plan, const float *d_proj, size_t n_pr
     MR_GUARD_END
 }
 
+namespace {
+
+// Chunks of the used projections for bp_adjoint_lds_kernel: one stepping
+// class per chunk, projections in order of direction, cut where the candidate
+// positions of some block of kAdjNB bins on the first or last line (the span
+// of affine functions of the line is largest at an end) would spread over
+// more than kAdjW - 6 voxels (floor/ceil and one voxel of margin each side).
+void adj_chunks(const std::vector<float> &tab, int nu, std::vector<int4> &chunks, std::vector<int> &order)
+{
+    const int nk = (int)(tab.size() / 4);
+    const int nbt = (nu + mr::kAdjNB - 1) / mr::kAdjNB;
+    for (int cls = 0; cls < 2; ++cls) {
+        std::vector<std::pair<double, int>> ks;
+        for (int k = 0; k < nk; ++k) {
+            const float c = tab[4 * k + 1], sn = tab[4 * k + 2];
+            if ((std::fabs(sn) >= std::fabs(c) ? 0 : 1) != cls) continue;
+            double dir = std::atan2((double)sn, (double)c);
+            if (dir < 0) dir += 2.0 * 3.14159265358979323846;
+            ks.push_back({dir, k});
+        }
+        std::sort(ks.begin(), ks.end());
+        std::vector<double> mn(2 * nbt), mx(2 * nbt), tmn(2 * nbt), tmx(2 * nbt);
+        int first = (int)order.size(), count = 0;
+        auto extents = [&](int k, int bt, int end, double &lo, double &hi) {
+            const double su = tab[4 * k], c = tab[4 * k + 1], sn = tab[4 * k + 2];
+            const int n0 = bt * mr::kAdjNB, n1 = std::min(n0 + mr::kAdjNB - 1, nu - 1);
+            const double ulo = n0 <= 1 ? -1.0 : (double)n0 - 1.0, uhi = (double)n1 + 1.0;
+            const double L = end == 0 ? 1.0 : (double)nu;
+            double a, b;
+            if (cls == 0) {
+                a = (su + c * L - uhi) / sn;
+                b = (su + c * L - ulo) / sn;
+            } else {
+                a = (ulo + sn * L - su) / c;
+                b = (uhi + sn * L - su) / c;
+            }
+            lo = std::min(a, b);
+            hi = std::max(a, b);
+        };
+        for (const auto &e : ks) {
+            const int k = e.second;
+            double worst = 0.0;
+            for (int bt = 0; bt < nbt; ++bt)
+                for (int end = 0; end < 2; ++end) {
+                    double lo, hi;
+                    extents(k, bt, end, lo, hi);
+                    const int i = 2 * bt + end;
+                    tmn[i] = count ? std::min(mn[i], lo) : lo;
+                    tmx[i] = count ? std::max(mx[i], hi) : hi;
+                    worst = std::max(worst, tmx[i] - tmn[i]);
+                }
+            if (count > 0 && (count == mr::kAdjKA || worst > (double)(mr::kAdjW - 6))) {
+                chunks.push_back(make_int4(first, count, cls, 0));
+                first = (int)order.size();
+                count = 0;
+                for (int bt = 0; bt < nbt; ++bt)
+                    for (int end = 0; end < 2; ++end) extents(k, bt, end, tmn[2 * bt + end], tmx[2 * bt + end]);
+            }
+            mn = tmn;
+            mx = tmx;
+            order.push_back(k);
+            ++count;
+        }
+        if (count > 0) chunks.push_back(make_int4(first, count, cls, 0));
+    }
+}
+
+}  // namespace
+
 extern "C" int mr_bp_parallel_adjoint(mr_bp_plan *plan, const float *d_grad_volume, size_t n_proj,
                                       const float *angles, const float *weights, float *d_grad_proj, void *stream)
 {
@@ -1052,7 +1273,40 @@ extern "C" int mr_bp_parallel_adjoint(mr_bp_plan *plan, const float *d_grad_volu
     MR_HIP(hipMemcpyAsync(plan->d_tab, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice, s));
     MR_HIP(hipMemcpyAsync(plan->d_idx, idx.data(), sizeof(int) * nk, hipMemcpyHostToDevice, s));
     MR_HIP(hipMemcpyAsync(plan->d_w, weights, sizeof(float) * n_proj, hipMemcpyHostToDevice, s));
-    {
+    const int adj_impl = mr::env_int("MR_BP_ADJ", 1);
+    // the chunked kernel needs every span of one projection's 16 bins to fit
+    // (16 / sqrt(1/2) + 6 < kAdjW always holds) and at least one block of bins
+    std::vector<int4> chunks;
+    std::vector<int> order;
+    if (adj_impl == 1) adj_chunks(tab, (int)q.width, chunks, order);
+    if (!chunks.empty()) {
+        const size_t bytes = sizeof(int4) * chunks.size() + sizeof(int) * order.size();
+        if (bytes > plan->adj_cap) {
+            if (plan->d_adj) (void)hipFree(plan->d_adj);
+            plan->d_adj = nullptr;
+            plan->adj_cap = 0;
+            MR_HIP(hipMalloc(&plan->d_adj, bytes));
+            plan->adj_cap = bytes;
+        }
+        int4 *d_chunks = static_cast<int4 *>(plan->d_adj);
+        int *d_order = reinterpret_cast<int *>(d_chunks + chunks.size());
+        MR_HIP(hipMemcpyAsync(d_chunks, chunks.data(), sizeof(int4) * chunks.size(), hipMemcpyHostToDevice, s));
+        MR_HIP(hipMemcpyAsync(d_order, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, s));
+        const int zt = nz4 % 16 == 0 ? 16 : 4;
+        const int nzs = (int)(nz4 / zt), nbt = (int)((q.width + mr::kAdjNB - 1) / mr::kAdjNB);
+        const long total = (long)chunks.size() * nbt * nzs;
+        MR_REQUIRE(total < (1L << 31) - 8, "adjoint grid too large");
+        const dim3 grid((unsigned)(8 * ((total + 7) / 8)));
+        const bool nn = q.interp == MR_INTERP_NEAREST;
+        mr::ProfScope prof("bp_adjoint", s);
+#define MR_ADJ2_LAUNCH(I, Z)                                                                                       \
+    hipLaunchKernelGGL((mr::bp_adjoint_lds_kernel<I, Z>), grid, dim3(256), 0, s, d_grad_volume, d_grad_proj,      \
+                       plan->d_tab, plan->d_mask, plan->d_idx, plan->d_w, d_chunks, d_order, (int)chunks.size(),   \
+                       nbt, nzs, (int)q.width, (int)q.slices)
+        if (zt == 16) { if (nn) MR_ADJ2_LAUNCH(1, 16); else MR_ADJ2_LAUNCH(0, 16); }
+        else { if (nn) MR_ADJ2_LAUNCH(1, 4); else MR_ADJ2_LAUNCH(0, 4); }
+#undef MR_ADJ2_LAUNCH
+    } else {
         // ZG: groups of 4 slices per detector bin handled side by side
         const int nq = (int)(nz4 / 4);
         const int zg = nq >= 16 ? 16 : (nq >= 8 ? 8 : (nq >= 4 ? 4 : (nq >= 2 ? 2 : 1)));

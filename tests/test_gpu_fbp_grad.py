@@ -25,9 +25,14 @@ def _case(seed, nu=64, nv=8, n_p=45):
     return rng, ang, w
 
 
+@pytest.mark.parametrize("impl", [1, 0])
 @pytest.mark.parametrize("nu,nv,bufsize,interp", [(64, 8, 16, "linear"), (97, 12, 0, "linear"),
-                                                  (64, 4, 16, "nearest"), (150, 6, 16, "linear")])
-def test_bp_adjoint_matches_oracle(orc, dev, nu, nv, bufsize, interp):
+                                                  (64, 4, 16, "nearest"), (150, 6, 16, "linear"),
+                                                  (200, 32, 0, "linear"), (120, 19, 16, "nearest")])
+def test_bp_adjoint_matches_oracle(orc, dev, monkeypatch, impl, nu, nv, bufsize, interp):
+    """impl 1: the chunked LDS kernel (default; 16-slice and 4-slice groups);
+    impl 0: the direct kernel."""
+    monkeypatch.setenv("MR_BP_ADJ", str(impl))
     from imagingsuite_amd.recon import BackProjector
     rng, ang, w = _case(nu + nv, nu, nv)
     n_p = ang.size
@@ -42,7 +47,7 @@ def test_bp_adjoint_matches_oracle(orc, dev, nu, nv, bufsize, interp):
     bps = bp.process(torch.from_numpy(s).to(dev), ang, w).cpu().numpy().astype(np.float64)
     lhs = float((bps * G).sum())
     rhs = float((s.astype(np.float64) * got.astype(np.float64)).sum())
-    print(f"BP^T {name} nu={nu} nv={nv}: vs FP64 {e[0]:.2e}/{e[1]:.2e}, dot {abs(lhs - rhs) / abs(lhs):.2e}")
+    print(f"BP^T impl={impl} {name} nu={nu} nv={nv}: vs FP64 {e[0]:.2e}/{e[1]:.2e}, dot {abs(lhs - rhs) / abs(lhs):.2e}")
     assert e[0] <= 1e-6
     assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
 
