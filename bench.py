@@ -289,9 +289,17 @@ def bench_fdk(args):
                     {"workload": "FDK 1024^3 from 720 projections, 1024^2 detector, SOD 500, SDD 1000",
                      "window_rows": cb[3] - cb[1], "filter": "Shepp-Logan + cosine pre-weighting",
                      "precision": "exact positions"})
+    kname = "fdk_kernel<%s,false>" % os.environ.get("MR_FDK_KT", "16")
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "fdk_traffic_c3.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tj = json.load(f)
+        if tj.get("kernel") == kname:
+            traffic = tj.get("hbm_bytes_per_launch")
     r["roofline"] = {"bound": "mfma", "pipe": "fp32 VALU", "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                     "traffic": None, "kernel": "fdk_kernel<%s,false>" % os.environ.get("MR_FDK_KT", "16"),
+                     "traffic": traffic, "kernel": kname,
                      "kernel_ms": round(kms, 3),
                      "flop_per_update": FDK_FLOP_PER_UPDATE}
     r["kernels_ms"] = {"row_filter_fdk": round(fl_ms / max(fl_n, 1), 3), "fdk_bp": round(kms, 3)}
