@@ -69,6 +69,19 @@ __device__ __forceinline__ bool fdk_tile(int order, int ntx, int nty, int ntz, i
 // The KT row terms are fetched first, then all KT samples are requested
 // before any of them is used, so the loads of one projection are in flight
 // together.
+// (int)floor(v) in one instruction for finite v (the clamped row position):
+// the compiler keeps floor and convert apart because they differ for NaN.
+__device__ __forceinline__ int cvt_floor(float v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+#else
+    return (int)floorf(v);   // host pass of a device-only function: never executed
+#endif
+}
+
 template <int KT, bool FULL, bool FUSED>
 __device__ __forceinline__ void fdk_column(float (&acc)[KT], const float *__restrict__ zt, int kn,
                                            __amdgpu_buffer_rsrc_t img, float yip1, float xip1, float dw,
@@ -90,7 +103,7 @@ __device__ __forceinline__ void fdk_column(float (&acc)[KT], const float *__rest
             r = dw * (acc21 + xip1);
             r = r + 0.5f;
         }
-        const int rr = (int)floorf(__builtin_amdgcn_fmed3f(r, -1.0f, fh));
+        const int rr = cvt_floor(__builtin_amdgcn_fmed3f(r, -1.0f, fh));
         val[kk] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(img, __mul24(rr, w4) + cc4, 0, 0));
     }
 #pragma unroll
