@@ -725,10 +725,10 @@ __global__ __launch_bounds__(256) void bp_adjoint_lds_kernel(const float *__rest
 #pragma unroll
                         for (int gq = 0; gq < G; ++gq) {
                             const float4 x4 = *reinterpret_cast<const float4 *>(v + 4 * gq);
-                            acc[4 * gq + 0] += wgt * x4.x;
-                            acc[4 * gq + 1] += wgt * x4.y;
-                            acc[4 * gq + 2] += wgt * x4.z;
-                            acc[4 * gq + 3] += wgt * x4.w;
+                            acc[4 * gq + 0] = __builtin_fmaf(wgt, x4.x, acc[4 * gq + 0]);
+                            acc[4 * gq + 1] = __builtin_fmaf(wgt, x4.y, acc[4 * gq + 1]);
+                            acc[4 * gq + 2] = __builtin_fmaf(wgt, x4.z, acc[4 * gq + 2]);
+                            acc[4 * gq + 3] = __builtin_fmaf(wgt, x4.w, acc[4 * gq + 3]);
                         }
                     }
                 }
@@ -790,10 +790,10 @@ __global__ __launch_bounds__(256) void bp_adjoint_kernel(const float *__restrict
             else return;
         }
         const float4 v = *reinterpret_cast<const float4 *>(g + ((size_t)(y - 1) * nu + (x - 1)) * nv + z0);
-        acc.x += wgt * v.x;
-        acc.y += wgt * v.y;
-        acc.z += wgt * v.z;
-        acc.w += wgt * v.w;
+        acc.x = __builtin_fmaf(wgt, v.x, acc.x);
+        acc.y = __builtin_fmaf(wgt, v.y, acc.y);
+        acc.z = __builtin_fmaf(wgt, v.z, acc.z);
+        acc.w = __builtin_fmaf(wgt, v.w, acc.w);
     };
     // candidates: u(x, y) in [u - 1, u + 1] (n = u - 1 or u), or in (-1, 2)
     // for u <= 1 (truncation maps (-1, 0) to n = 0); one voxel of margin on

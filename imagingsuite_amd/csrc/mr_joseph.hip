@@ -46,10 +46,10 @@ __device__ __forceinline__ void lerp_accumulate(float (&acc)[ZT], const float *p
     for (int g = 0; g < ZT / 4; ++g) {
         const float4 v0 = *reinterpret_cast<const float4 *>(p0 + 4 * g);
         const float4 v1 = *reinterpret_cast<const float4 *>(p1 + 4 * g);
-        acc[4 * g + 0] += w0 * v0.x + w1 * v1.x;
-        acc[4 * g + 1] += w0 * v0.y + w1 * v1.y;
-        acc[4 * g + 2] += w0 * v0.z + w1 * v1.z;
-        acc[4 * g + 3] += w0 * v0.w + w1 * v1.w;
+        acc[4 * g + 0] = __builtin_fmaf(w1, v1.x, __builtin_fmaf(w0, v0.x, acc[4 * g + 0]));
+        acc[4 * g + 1] = __builtin_fmaf(w1, v1.y, __builtin_fmaf(w0, v0.y, acc[4 * g + 1]));
+        acc[4 * g + 2] = __builtin_fmaf(w1, v1.z, __builtin_fmaf(w0, v0.z, acc[4 * g + 2]));
+        acc[4 * g + 3] = __builtin_fmaf(w1, v1.w, __builtin_fmaf(w0, v0.w, acc[4 * g + 3]));
     }
 }
 
@@ -207,16 +207,18 @@ __global__ __launch_bounds__(256) void joseph_forward_lds_kernel(JosephArgs a, c
             any = any || (L0 + r < N && hi_s[r] >= -1 && lo_s[r] < N);
         if (any) {
             // only the voxels a line's rays read: lo .. hi + 1
-            for (int e = threadIdx.x; e < R * W * G; e += 256) {
-                const int r = e / (W * G);
-                const int q = e - r * W * G;
-                const int j = q / G, g = q - j * G;
-                if (j > hi_s[r] - lo_s[r] + 1) continue;
-                const int L = L0 + r, p = lo_s[r] + j;
-                float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                if (L < N && p >= 0 && p < N)
-                    v = *reinterpret_cast<const float4 *>(a.src + (size_t)L * strideL + (size_t)p * strideP + z0 + 4 * g);
-                *reinterpret_cast<float4 *>(&seg[r][j][4 * g]) = v;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int L = L0 + r, lo = lo_s[r], nj = min(hi_s[r] - lo + 2, W);
+                const float *line = a.src + (size_t)min(L, N - 1) * strideL + z0;
+                for (int e = threadIdx.x; e < nj * G; e += 256) {
+                    const int j = e / G, g = e - j * G;
+                    const int p = lo + j;
+                    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    if (L < N && p >= 0 && p < N)
+                        v = *reinterpret_cast<const float4 *>(line + (size_t)p * strideP + 4 * g);
+                    *reinterpret_cast<float4 *>(&seg[r][j][4 * g]) = v;
+                }
             }
             __syncthreads();
 #pragma unroll
@@ -235,10 +237,10 @@ __global__ __launch_bounds__(256) void joseph_forward_lds_kernel(JosephArgs a, c
                     for (int g = 0; g < G; ++g) {
                         const float4 x0 = *reinterpret_cast<const float4 *>(v0 + 4 * g);
                         const float4 x1 = *reinterpret_cast<const float4 *>(v1 + 4 * g);
-                        acc[4 * g + 0] += w0 * x0.x + w1 * x1.x;
-                        acc[4 * g + 1] += w0 * x0.y + w1 * x1.y;
-                        acc[4 * g + 2] += w0 * x0.z + w1 * x1.z;
-                        acc[4 * g + 3] += w0 * x0.w + w1 * x1.w;
+                        acc[4 * g + 0] = __builtin_fmaf(w1, x1.x, __builtin_fmaf(w0, x0.x, acc[4 * g + 0]));
+                        acc[4 * g + 1] = __builtin_fmaf(w1, x1.y, __builtin_fmaf(w0, x0.y, acc[4 * g + 1]));
+                        acc[4 * g + 2] = __builtin_fmaf(w1, x1.z, __builtin_fmaf(w0, x0.z, acc[4 * g + 2]));
+                        acc[4 * g + 3] = __builtin_fmaf(w1, x1.w, __builtin_fmaf(w0, x0.w, acc[4 * g + 3]));
                     }
                 }
             }
@@ -367,10 +369,10 @@ __global__ __launch_bounds__(256) void joseph_adjoint_lds_kernel(JosephArgs a, i
                 for (int g = 0; g < G; ++g) {
                     const float4 va = *reinterpret_cast<const float4 *>(A + 4 * g);
                     const float4 vb = *reinterpret_cast<const float4 *>(B + 4 * g);
-                    acc[4 * g + 0] += w1 * va.x + w2 * vb.x;
-                    acc[4 * g + 1] += w1 * va.y + w2 * vb.y;
-                    acc[4 * g + 2] += w1 * va.z + w2 * vb.z;
-                    acc[4 * g + 3] += w1 * va.w + w2 * vb.w;
+                    acc[4 * g + 0] = __builtin_fmaf(w2, vb.x, __builtin_fmaf(w1, va.x, acc[4 * g + 0]));
+                    acc[4 * g + 1] = __builtin_fmaf(w2, vb.y, __builtin_fmaf(w1, va.y, acc[4 * g + 1]));
+                    acc[4 * g + 2] = __builtin_fmaf(w2, vb.z, __builtin_fmaf(w1, va.z, acc[4 * g + 2]));
+                    acc[4 * g + 3] = __builtin_fmaf(w2, vb.w, __builtin_fmaf(w1, va.w, acc[4 * g + 3]));
                 }
             }
         }
