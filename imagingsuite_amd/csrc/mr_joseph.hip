@@ -146,12 +146,12 @@ __global__ __launch_bounds__(256) void joseph_forward_kernel(JosephArgs a)
 // line's crossings of a block fit in kFwdW - 2 voxels.
 constexpr int kFwdNB = 16, kFwdKA = 16, kFwdR = 4, kFwdW = 80;
 
-template <int ZT>
+template <int ZT, int R>
 __global__ __launch_bounds__(256) void joseph_forward_lds_kernel(JosephArgs a, const int4 *__restrict__ chunks,
                                                                  const int *__restrict__ order, int n_chunks,
                                                                  int nbt, int nzs)
 {
-    constexpr int NB = kFwdNB, R = kFwdR, W = kFwdW, RS = ZT + 4, G = ZT / 4;
+    constexpr int NB = kFwdNB, W = kFwdW, RS = ZT + 4, G = ZT / 4;
     __shared__ float seg[R][W][RS];
     __shared__ int lo_s[R], hi_s[R];
     const int total = n_chunks * nbt * nzs;
@@ -527,11 +527,26 @@ int joseph_launch(const mr_joseph_geom *g, const float *src, float *dst, const J
         mr::ProfScope prof("joseph_forward", s);
         const char *ef = std::getenv("MR_JOSEPH_FWD");
         if (g->nz % 16 == 0 && dt.n_chunks > 0 && !(ef && std::atoi(ef) == 0)) {
-            const int nbt = (int)((g->n_det + kFwdNB - 1) / kFwdNB), nzs = (int)(g->nz / 16);
+            // slices per block and lines per staging group (MR_JOSEPH_ZT, MR_JOSEPH_R for tuning)
+            const char *ez = std::getenv("MR_JOSEPH_ZT"), *er = std::getenv("MR_JOSEPH_R");
+            const int zt = (ez && std::atoi(ez) == 32 && g->nz % 32 == 0) ? 32 : 16;
+            const int rr = er ? std::atoi(er) : kFwdR;
+            const int nbt = (int)((g->n_det + kFwdNB - 1) / kFwdNB), nzs = (int)(g->nz / zt);
             const long total = (long)dt.n_chunks * nbt * nzs;
             MR_REQUIRE(total < (1L << 31) - 8, "Joseph forward grid too large");
-            hipLaunchKernelGGL(mr::joseph_forward_lds_kernel<16>, dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256),
-                               0, s, a, dt.chunks, dt.order, dt.n_chunks, nbt, nzs);
+            const dim3 grid((unsigned)(8 * ((total + 7) / 8)));
+#define MR_JF_LAUNCH(Z, RR)                                                                                     \
+    hipLaunchKernelGGL((mr::joseph_forward_lds_kernel<Z, RR>), grid, dim3(256), 0, s, a, dt.chunks, dt.order, \
+                       dt.n_chunks, nbt, nzs)
+            if (zt == 32) {
+                if (rr == 4) MR_JF_LAUNCH(32, 4);
+                else MR_JF_LAUNCH(32, 2);
+            } else {
+                if (rr == 8) MR_JF_LAUNCH(16, 8);
+                else if (rr == 2) MR_JF_LAUNCH(16, 2);
+                else MR_JF_LAUNCH(16, 4);
+            }
+#undef MR_JF_LAUNCH
         } else if (g->nz % 16 == 0) {
             dim3 grid((unsigned)((g->n_det + 63) / 64), (unsigned)((n_ang + 3) / 4), (unsigned)(g->nz / 16));
             hipLaunchKernelGGL(mr::joseph_forward_kernel<16>, grid, dim3(256), 0, s, a);
