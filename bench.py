@@ -484,7 +484,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
-              "(the path shards by z-slab with no exchange; see DESIGN.md)", file=sys.stderr)
+              "(several GPUs would each take a z-slab of the stack, with no exchange)", file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy:
         return calibrate_copy()

@@ -600,15 +600,15 @@ static int env_int(const char *name, int dflt)
 // candidate voxels, for every block of kAdjNB detector bins, lie within
 // kAdjW voxels on every line.  A block (kAdjNB bins x one chunk of up to
 // kAdjKA projections x ZT slices, one (bin, projection) per thread) walks the
-// lines kAdjR at a time: it stages each line's candidate voxels of the volume
+// lines R at a time: it stages each line's candidate voxels of the volume
 // gradient into LDS (zeros outside the circle mask, so masked voxels add
 // nothing) and every thread tests its candidates exactly as the direct
 // kernel does (u recomputed with the forward's float operations), so the sums
 // are the direct kernel's, in the same line order.  The volume is read once
 // per chunk instead of once per projection.
-constexpr int kAdjNB = 16, kAdjKA = 16, kAdjR = 4, kAdjW = 80;
+constexpr int kAdjNB = 16, kAdjKA = 16, kAdjW = 80;
 
-template <int INTERP, int ZT>
+template <int INTERP, int ZT, int R>
 __global__ __launch_bounds__(256) void bp_adjoint_lds_kernel(const float *__restrict__ g, float *__restrict__ out,
                                                              const float4 *__restrict__ tab,
                                                              const int2 *__restrict__ mask,
@@ -618,7 +618,7 @@ __global__ __launch_bounds__(256) void bp_adjoint_lds_kernel(const float *__rest
                                                              const int *__restrict__ order, int n_chunks, int nbt,
                                                              int nzs, int nu, int nv)
 {
-    constexpr int NB = kAdjNB, R = kAdjR, W = kAdjW, RS = ZT + 4, G = ZT / 4;
+    constexpr int NB = kAdjNB, W = kAdjW, RS = ZT + 4, G = ZT / 4;
     __shared__ float seg[R][W][RS];
     __shared__ int lo_s[R], hi_s[R];
     const int total = n_chunks * nbt * nzs;
@@ -1292,7 +1292,10 @@ extern "C" int mr_bp_parallel_adjoint(mr_bp_plan *plan, const float *d_grad_volu
         int *d_order = reinterpret_cast<int *>(d_chunks + chunks.size());
         MR_HIP(hipMemcpyAsync(d_chunks, chunks.data(), sizeof(int4) * chunks.size(), hipMemcpyHostToDevice, s));
         MR_HIP(hipMemcpyAsync(d_order, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, s));
-        const int zt = nz4 % 16 == 0 ? 16 : 4;
+        // slices per block and lines per staging group (MR_BP_ADJ_ZT, MR_BP_ADJ_R for tuning)
+        const int ezt = mr::env_int("MR_BP_ADJ_ZT", 32), er = mr::env_int("MR_BP_ADJ_R", 2);
+        const int zt = (ezt == 32 && nz4 % 32 == 0) ? 32 : (nz4 % 16 == 0 ? 16 : 4);
+        const int rr = er == 4 ? 4 : (er == 1 ? 1 : 2);
         const int nzs = (int)(nz4 / zt), nbt = (int)((q.width + mr::kAdjNB - 1) / mr::kAdjNB);
         const long total = (long)chunks.size() * nbt * nzs;
         MR_REQUIRE(total < (1L << 31) - 8, "adjoint grid too large");
@@ -1300,11 +1303,19 @@ extern "C" int mr_bp_parallel_adjoint(mr_bp_plan *plan, const float *d_grad_volu
         const bool nn = q.interp == MR_INTERP_NEAREST;
         mr::ProfScope prof("bp_adjoint", s);
 #define MR_ADJ2_LAUNCH(I, Z)                                                                                       \
-    hipLaunchKernelGGL((mr::bp_adjoint_lds_kernel<I, Z>), grid, dim3(256), 0, s, d_grad_volume, d_grad_proj,      \
+    do {                                                                                                           \
+        if (rr == 4) MR_ADJ2_LAUNCH_R(I, Z, 4);                                                                    \
+        else if (rr == 1) MR_ADJ2_LAUNCH_R(I, Z, 1);                                                               \
+        else MR_ADJ2_LAUNCH_R(I, Z, 2);                                                                            \
+    } while (0)
+#define MR_ADJ2_LAUNCH_R(I, Z, RR)                                                                                 \
+    hipLaunchKernelGGL((mr::bp_adjoint_lds_kernel<I, Z, RR>), grid, dim3(256), 0, s, d_grad_volume, d_grad_proj,  \
                        plan->d_tab, plan->d_mask, plan->d_idx, plan->d_w, d_chunks, d_order, (int)chunks.size(),   \
                        nbt, nzs, (int)q.width, (int)q.slices)
-        if (zt == 16) { if (nn) MR_ADJ2_LAUNCH(1, 16); else MR_ADJ2_LAUNCH(0, 16); }
+        if (zt == 32) { if (nn) MR_ADJ2_LAUNCH(1, 32); else MR_ADJ2_LAUNCH(0, 32); }
+        else if (zt == 16) { if (nn) MR_ADJ2_LAUNCH(1, 16); else MR_ADJ2_LAUNCH(0, 16); }
         else { if (nn) MR_ADJ2_LAUNCH(1, 4); else MR_ADJ2_LAUNCH(0, 4); }
+#undef MR_ADJ2_LAUNCH_R
 #undef MR_ADJ2_LAUNCH
     } else {
         // ZG: groups of 4 slices per detector bin handled side by side

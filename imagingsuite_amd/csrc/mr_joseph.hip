@@ -135,7 +135,7 @@ __global__ __launch_bounds__(256) void joseph_forward_kernel(JosephArgs a)
 // LDS-staged forward over a chunk of angles that all step along the same
 // axis (class 0: |sin| >= |cos|, rows y; class 1: columns x).  A block owns
 // kFwdNB detector bins x (up to) kFwdKA angles x ZT slices, one ray per
-// thread.  It walks the lines L = 0..N-1 of its class kFwdR at a time: for
+// thread.  It walks the lines L = 0..N-1 of its class R at a time: for
 // each line it stages the kFwdW voxels (x ZT slices) from the lowest crossing
 // of any of its rays, then every ray interpolates its two voxels from LDS.
 // Staged positions outside the volume are zeros and carry weight zero, so the
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(256) void joseph_forward_kernel(JosephArgs a)
 // staged segment serves all the chunk's rays: the volume is read once per
 // chunk instead of once per angle.  The host builds the chunks so that every
 // line's crossings of a block fit in kFwdW - 2 voxels.
-constexpr int kFwdNB = 16, kFwdKA = 16, kFwdR = 4, kFwdW = 80;
+constexpr int kFwdNB = 16, kFwdKA = 16, kFwdR = 2, kFwdW = 80;
 
 template <int ZT, int R>
 __global__ __launch_bounds__(256) void joseph_forward_lds_kernel(JosephArgs a, const int4 *__restrict__ chunks,
@@ -528,8 +528,10 @@ int joseph_launch(const mr_joseph_geom *g, const float *src, float *dst, const J
         const char *ef = std::getenv("MR_JOSEPH_FWD");
         if (g->nz % 16 == 0 && dt.n_chunks > 0 && !(ef && std::atoi(ef) == 0)) {
             // slices per block and lines per staging group (MR_JOSEPH_ZT, MR_JOSEPH_R for tuning)
+            // (measured at 1024^2 x 64 x 900: 32 slices and 2 lines 10.7 ms, 16 and 4 16.2 ms,
+            // 16 and 8 26.7 ms: the staging buffer's size sets how many blocks share a CU)
             const char *ez = std::getenv("MR_JOSEPH_ZT"), *er = std::getenv("MR_JOSEPH_R");
-            const int zt = (ez && std::atoi(ez) == 32 && g->nz % 32 == 0) ? 32 : 16;
+            const int zt = (g->nz % 32 == 0 && !(ez && std::atoi(ez) == 16)) ? 32 : 16;
             const int rr = er ? std::atoi(er) : kFwdR;
             const int nbt = (int)((g->n_det + kFwdNB - 1) / kFwdNB), nzs = (int)(g->nz / zt);
             const long total = (long)dt.n_chunks * nbt * nzs;
@@ -540,11 +542,12 @@ int joseph_launch(const mr_joseph_geom *g, const float *src, float *dst, const J
                        dt.n_chunks, nbt, nzs)
             if (zt == 32) {
                 if (rr == 4) MR_JF_LAUNCH(32, 4);
+                else if (rr == 1) MR_JF_LAUNCH(32, 1);
                 else MR_JF_LAUNCH(32, 2);
             } else {
-                if (rr == 8) MR_JF_LAUNCH(16, 8);
-                else if (rr == 2) MR_JF_LAUNCH(16, 2);
-                else MR_JF_LAUNCH(16, 4);
+                if (rr == 4) MR_JF_LAUNCH(16, 4);
+                else if (rr == 1) MR_JF_LAUNCH(16, 1);
+                else MR_JF_LAUNCH(16, 2);
             }
 #undef MR_JF_LAUNCH
         } else if (g->nz % 16 == 0) {
