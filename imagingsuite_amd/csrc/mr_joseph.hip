@@ -144,7 +144,7 @@ __global__ __launch_bounds__(256) void joseph_forward_kernel(JosephArgs a)
 // staged segment serves all the chunk's rays: the volume is read once per
 // chunk instead of once per angle.  The host builds the chunks so that every
 // line's crossings of a block fit in kFwdW - 2 voxels.
-constexpr int kFwdNB = 16, kFwdKA = 16, kFwdR = 2, kFwdW = 80;
+constexpr int kFwdNB = 16, kFwdKA = 16, kFwdR = 1, kFwdW = 80;
 
 template <int ZT, int R>
 __global__ __launch_bounds__(256) void joseph_forward_lds_kernel(JosephArgs a, const int4 *__restrict__ chunks,
@@ -528,7 +528,7 @@ int joseph_launch(const mr_joseph_geom *g, const float *src, float *dst, const J
         const char *ef = std::getenv("MR_JOSEPH_FWD");
         if (g->nz % 16 == 0 && dt.n_chunks > 0 && !(ef && std::atoi(ef) == 0)) {
             // slices per block and lines per staging group (MR_JOSEPH_ZT, MR_JOSEPH_R for tuning)
-            // (measured at 1024^2 x 64 x 900: 32 slices and 2 lines 10.7 ms, 16 and 4 16.2 ms,
+            // (measured at 1024^2 x 64 x 900: 32 slices and 1 line 9.8 ms, 2 lines 10.7 ms, 16 and 4 16.2 ms,
             // 16 and 8 26.7 ms: the staging buffer's size sets how many blocks share a CU)
             const char *ez = std::getenv("MR_JOSEPH_ZT"), *er = std::getenv("MR_JOSEPH_R");
             const int zt = (g->nz % 32 == 0 && !(ez && std::atoi(ez) == 16)) ? 32 : 16;
