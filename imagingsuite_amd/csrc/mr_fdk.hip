@@ -170,7 +170,9 @@ __global__ __launch_bounds__(256) void fdk_kernel(FdkArgs a)
 // read from LDS; rows outside the detector give 0 as before.  Positions and
 // sums are those of fdk_kernel, so the result is bit-identical to it.  A
 // window larger than kFdkCap samples falls back to the direct gathers for
-// that projection.
+// that projection.  Not the default: with 16 samples per thread and
+// projection, the two barriers per projection cost more than the gathers
+// they replace (measured 505 against 213 ms at 1024^3 x 720).
 constexpr int kFdkCap = 4096;
 
 template <bool FUSED>
@@ -671,8 +673,10 @@ extern "C" int mr_fdk_bp(mr_fdk_plan *plan, float *d_proj, size_t n_proj, size_t
                                : dim3((unsigned)a.ntx, (unsigned)a.nty, (unsigned)a.ntz);
         mr::ProfScope prof("fdk_bp", s);
         const bool fused = plan->p.fused != 0;
+        // MR_FDK_LDS=1 selects the LDS-staged kernel (bit-identical, but measured slower at
+        // 1024^3 x 720: 505 against 213 ms, two barriers per projection for 16 samples per thread)
         const char *el = std::getenv("MR_FDK_LDS");
-        if (kt == 16 && !(el && std::atoi(el) == 0)) {
+        if (kt == 16 && el && std::atoi(el) == 1) {
             if (fused) hipLaunchKernelGGL((mr::fdk_lds_kernel<16, true>), g, dim3(256), 0, s, a);
             else hipLaunchKernelGGL((mr::fdk_lds_kernel<16, false>), g, dim3(256), 0, s, a);
         } else if (kt == 32) {

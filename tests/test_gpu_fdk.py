@@ -181,9 +181,9 @@ def test_fdk_filter_extension_vs_fp64(orc, dev, shepp_logan, cosine, fused):
 @pytest.mark.parametrize("fused", [False, True])
 @pytest.mark.parametrize("slab,n", [((0, 48, 128, 80), 128), ((0, 0, 96, 19), 96), ((0, 100, 160, 160), 160)])
 def test_fdk_lds_kernel_matches_direct(orc, dev, monkeypatch, fused, slab, n):
-    """The LDS-staged kernel (default) reads the same samples as the direct
-    gathers (MR_FDK_LDS=0): bit-identical volumes, including slabs at the
-    detector's top and bottom rows and widths that leave partial blocks."""
+    """The LDS-staged kernel (MR_FDK_LDS=1) reads the same samples as the
+    direct gathers (the default): bit-identical volumes, including slabs at
+    the detector's top and bottom rows and widths that leave partial blocks."""
     from imagingsuite_amd.recon import FDKbp_single
     n_p = 45
     g = _geom(orc, slab, n, 0)
@@ -194,6 +194,7 @@ def test_fdk_lds_kernel_matches_direct(orc, dev, monkeypatch, fused, slab, n):
     win = np.ascontiguousarray(rng.random((n_p, cb[3] - cb[1], n)).astype(np.float32))
     angles = np.arange(n_p, dtype=np.float32) * np.float32(8.0)
     weights = np.full(n_p, 1.0 / n_p, np.float32)
+    monkeypatch.setenv("MR_FDK_LDS", "1")
     lds = fdk.process(torch.from_numpy(win).to(dev), angles, weights).cpu().numpy()
     monkeypatch.setenv("MR_FDK_LDS", "0")
     direct = fdk.process(torch.from_numpy(win).to(dev), angles, weights).cpu().numpy()
