@@ -35,7 +35,22 @@ struct FdkArgs {
     int nx, ny, nz, w, h, np;
     float ox, oy, sx, sy;
     int order, ntx, nty, ntz;
+    int wx;             // voxel columns along x per wave (a power of two up to 64)
+    int bwx;            // waves along x per block (1, 2 or 4)
 };
+
+// Thread -> voxel column.  A wave covers wx x (64 / wx) columns and a block
+// bwx x (4 / bwx) waves.  A compact wave footprint keeps a gather's 64
+// samples on fewer detector rows (the row of slice k moves with the depth,
+// through 1/depth, which changes along the ray direction), and a compact
+// block footprint lets its waves share the detector lines they fetch.
+__device__ __forceinline__ void fdk_voxel(int wx, int bwx, int bx, int by, int &i, int &j)
+{
+    const int l = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+    const int wy = 64 / wx, bwy = 4 / bwx;
+    i = bx * bwx * wx + (w & (bwx - 1)) * wx + (l & (wx - 1));
+    j = by * bwy * wy + (w / bwx) * wy + l / wx;
+}
 
 // Block -> tile mapping: order 0 is the 3-D grid; order 1 a 1-D grid whose
 // consecutive blocks go to consecutive XCDs (round robin over 8), each XCD
@@ -120,8 +135,8 @@ __global__ __launch_bounds__(256) void fdk_kernel(FdkArgs a)
 {
     int bx, by, bz;
     if (!fdk_tile(a.order, a.ntx, a.nty, a.ntz, bx, by, bz)) return;
-    const int i = bx * 64 + (threadIdx.x & 63);
-    const int j = by * 4 + (threadIdx.x >> 6);
+    int i, j;
+    fdk_voxel(a.wx, a.bwx, bx, by, i, j);
     const int k0 = bz * KT;
     if (i >= a.nx || j >= a.ny) return;
     const int kn = min(KT, a.nz - k0);
@@ -316,6 +331,7 @@ struct FdkdArgs {
     int nx, ny, nz, w, h, np;
     float ox, oy, sx, sy;
     int order, ntx, nty, ntz;
+    int wx, bwx;
 };
 
 template <int KT, bool FULL>
@@ -343,8 +359,8 @@ __global__ __launch_bounds__(256) void fdkd_kernel(FdkdArgs a)
 {
     int bx, by, bz;
     if (!fdk_tile(a.order, a.ntx, a.nty, a.ntz, bx, by, bz)) return;
-    const int i = bx * 64 + (threadIdx.x & 63);
-    const int j = by * 4 + (threadIdx.x >> 6);
+    int i, j;
+    fdk_voxel(a.wx, a.bwx, bx, by, i, j);
     const int k0 = bz * KT;
     if (i >= a.nx || j >= a.ny) return;
     const int kn = min(KT, a.nz - k0);
@@ -383,6 +399,26 @@ __global__ __launch_bounds__(256) void fdkd_kernel(FdkdArgs a)
     for (int kk = 0; kk < KT; ++kk)
         if (kk < kn)
             a.vol[((size_t)(a.nz - 1 - (k0 + kk)) * a.ny + j) * a.nx + (a.nx - 1 - i)] = acc[kk];
+}
+
+// Voxel columns along x per wave (MR_FDK_WX: a power of two up to 64).
+// Default 4 x 16: measured at 1024^3 x 720 (fdk_kernel<16, false>) 64 x 1
+// 215 ms, 16 x 4 217, 8 x 8 163-168, 4 x 16 145, 2 x 32 160, 1 x 64 229.
+static int fdk_wave_x()
+{
+    const char *e = std::getenv("MR_FDK_WX");
+    const int v = e ? std::atoi(e) : 4;
+    return (v >= 1 && v <= 64 && (v & (v - 1)) == 0) ? v : 4;
+}
+
+// Waves along x per block (MR_FDK_BWX: 1, 2 or 4); a 64-wide wave stacks
+// its block's waves along y.
+static int fdk_block_waves_x(int wx)
+{
+    if (wx >= 64) return 1;
+    const char *e = std::getenv("MR_FDK_BWX");
+    const int v = e ? std::atoi(e) : 2;
+    return (v == 1 || v == 2 || v == 4) ? v : 2;
 }
 
 }  // namespace mr
@@ -655,7 +691,6 @@ extern "C" int mr_fdk_bp(mr_fdk_plan *plan, float *d_proj, size_t n_proj, size_t
     rc = mr::run_fdk_margin_rows(d_proj, (int)w, (int)h, (long)n_proj, s);
     if (rc != MR_OK) return rc;
     constexpr int KT = 16;
-    dim3 grid((unsigned)((nx + 63) / 64), (unsigned)((nx + 3) / 4), (unsigned)((nz + KT - 1) / KT));
     if (!dbl) {
         MR_REQUIRE(w * h * 4 < (size_t)1 << 31, "FDK window larger than 2^29 samples");
         mr::FdkArgs a{};
@@ -665,7 +700,13 @@ extern "C" int mr_fdk_bp(mr_fdk_plan *plan, float *d_proj, size_t n_proj, size_t
         const char *eo = std::getenv("MR_FDK_ORDER");
         const char *ek = std::getenv("MR_FDK_KT");
         const int kt = (ek && std::atoi(ek) == 32) ? 32 : 16;
-        a.ntx = (int)((nx + 63) / 64); a.nty = (int)((nx + 3) / 4); a.ntz = (int)((nz + kt - 1) / kt);
+        const char *el = std::getenv("MR_FDK_LDS");
+        const bool lds = kt == 16 && el && std::atoi(el) == 1;
+        // the LDS-staged kernel's corner search assumes 64 x 4 blocks
+        a.wx = lds ? 64 : mr::fdk_wave_x();
+        a.bwx = mr::fdk_block_waves_x(a.wx);
+        const int bw = a.bwx * a.wx, bh = 256 / bw;
+        a.ntx = (int)((nx + bw - 1) / bw); a.nty = (int)((nx + bh - 1) / bh); a.ntz = (int)((nz + kt - 1) / kt);
         const long ntiles = (long)a.ntx * a.nty * a.ntz;
         // grid order measured faster for the float kernel (1024^3: 218 vs 236 ms)
         a.order = (eo && std::atoi(eo) == 1) && ntiles < (1L << 30) ? 1 : 0;
@@ -675,8 +716,7 @@ extern "C" int mr_fdk_bp(mr_fdk_plan *plan, float *d_proj, size_t n_proj, size_t
         const bool fused = plan->p.fused != 0;
         // MR_FDK_LDS=1 selects the LDS-staged kernel (bit-identical, but measured slower at
         // 1024^3 x 720: 505 against 213 ms, two barriers per projection for 16 samples per thread)
-        const char *el = std::getenv("MR_FDK_LDS");
-        if (kt == 16 && el && std::atoi(el) == 1) {
+        if (lds) {
             if (fused) hipLaunchKernelGGL((mr::fdk_lds_kernel<16, true>), g, dim3(256), 0, s, a);
             else hipLaunchKernelGGL((mr::fdk_lds_kernel<16, false>), g, dim3(256), 0, s, a);
         } else if (kt == 32) {
@@ -692,12 +732,16 @@ extern "C" int mr_fdk_bp(mr_fdk_plan *plan, float *d_proj, size_t n_proj, size_t
         a.img = d_proj; a.vol = d_volume; a.ptab = plan->d_ptabd; a.zt1 = plan->d_zt1d; a.mask = plan->d_mask;
         a.nx = (int)nx; a.ny = (int)nx; a.nz = (int)nz; a.w = (int)w; a.h = (int)h; a.np = (int)n_proj;
         a.ox = plan->origin[0]; a.oy = plan->origin[1]; a.sx = plan->p.voxel[0]; a.sy = plan->p.voxel[1];
-        a.ntx = (int)((nx + 63) / 64); a.nty = (int)((nx + 3) / 4); a.ntz = (int)((nz + KT - 1) / KT);
+        a.wx = mr::fdk_wave_x();
+        a.bwx = mr::fdk_block_waves_x(a.wx);
+        const int bw = a.bwx * a.wx, bh = 256 / bw;
+        a.ntx = (int)((nx + bw - 1) / bw); a.nty = (int)((nx + bh - 1) / bh); a.ntz = (int)((nz + KT - 1) / KT);
         const long ntiles = (long)a.ntx * a.nty * a.ntz;
         const char *eo = std::getenv("MR_FDK_ORDER");
         // XCD order measured faster for the double kernel (1024^3: 258 vs 302 ms)
         a.order = (eo && std::atoi(eo) == 0) || ntiles >= (1L << 30) ? 0 : 1;
-        const dim3 g = a.order ? dim3((unsigned)(8 * ((ntiles + 7) / 8))) : grid;
+        const dim3 g = a.order ? dim3((unsigned)(8 * ((ntiles + 7) / 8)))
+                               : dim3((unsigned)a.ntx, (unsigned)a.nty, (unsigned)a.ntz);
         mr::ProfScope prof("fdkd_bp", s);
         hipLaunchKernelGGL(mr::fdkd_kernel<KT>, g, dim3(256), 0, s, a);
     }

@@ -203,3 +203,33 @@ def test_fdk_lds_kernel_matches_direct(orc, dev, monkeypatch, fused, slab, n):
     if not fused:
         ref = orc.fdk_bp(win, angles, weights, g)
         assert rel_err(lds, ref)[0] <= 1e-5
+
+
+@pytest.mark.parametrize("cls,fused", [("FDKbp_single", False), ("FDKbp_single", True), ("FDKbp", False)])
+@pytest.mark.parametrize("slab,n", [((0, 48, 128, 80), 128), ((0, 40, 100, 59), 100)])
+def test_fdk_wave_footprints_identical(dev, monkeypatch, cls, fused, slab, n):
+    """The voxel columns a wave covers (MR_FDK_WX: wx x 64/wx patches) and
+    the arrangement of a block's four waves (MR_FDK_BWX: bwx x 4/bwx) change
+    only which thread owns which column: bit-identical volumes, also for
+    widths that leave partial blocks."""
+    from imagingsuite_amd import recon
+    n_p = 40
+    kw = {} if cls == "FDKbp" else {"fused": fused}
+    fdk = getattr(recon, cls)(500.0, 1000.0, (0.2, 0.2), (0.1, 0.1, 0.1), (n / 2 + 0.3, n / 2 - 0.7), n / 2 + 0.3,
+                              slab, n - 1, **kw)
+    cb = fdk.window
+    rng = np.random.default_rng(n + slab[1] + 7)
+    win = np.ascontiguousarray(rng.random((n_p, cb[3] - cb[1], n)).astype(np.float32))
+    angles = np.arange(n_p, dtype=np.float32) * np.float32(9.0)
+    weights = np.full(n_p, 1.0 / n_p, np.float32)
+    shapes = [(wx, 2) for wx in (64, 32, 16, 8, 4, 2, 1)] + [(wx, b) for wx in (8, 4, 2) for b in (1, 4)]
+    vols = {}
+    for wx, bwx in shapes:
+        monkeypatch.setenv("MR_FDK_WX", str(wx))
+        monkeypatch.setenv("MR_FDK_BWX", str(bwx))
+        vols[wx, bwx] = fdk.process(torch.from_numpy(win).to(dev), angles, weights).cpu().numpy()
+    ref = vols[64, 2]
+    assert np.count_nonzero(ref) > 0
+    for key, v in vols.items():
+        print(f"{cls} fused={fused} slab {slab} n={n} wx, bwx = {key}: {int(np.sum(v != ref))} voxels differ")
+        assert np.array_equal(v, ref)
