@@ -204,26 +204,31 @@ def stream_leg(pristine, angles, weights, cfg, center, buf, nu, nv, n_p, updates
     ho = host_empty((nv, nu, nu))
     st = FBPStream(nu, nv, n_p, center, slice_block=32, buffer_size=buf, fused=True, filtertype=cfg["filtertype"])
     out = {}
-    for name, src, dst, serial in [("overlapped_pinned", hp, ho, False), ("serial_pinned", hp, ho, True)]:
+    reps = 5
+
+    def leg(name, src, dst, serial):
         st.run(src, angles, weights, out=dst, serial=serial)
-        t0 = time.perf_counter()
-        st.run(src, angles, weights, out=dst, serial=serial)
-        ms = (time.perf_counter() - t0) * 1e3
-        out[name] = {"ms": round(ms, 2), "GUPS": round(updates / (ms / 1e3) / 1e9, 1)}
-        log(f"[bench] stream {name}: {ms:.1f} ms")
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            st.run(src, angles, weights, out=dst, serial=serial)
+            times.append((time.perf_counter() - t0) * 1e3)
+        ms = float(np.median(times))
+        out[name] = {"ms": round(ms, 2), "ms_min": round(min(times), 2),
+                     "GUPS": round(updates / (ms / 1e3) / 1e9, 1)}
+        log(f"[bench] stream {name}: median {ms:.1f} ms of {reps} (min {min(times):.1f})")
+
+    leg("overlapped_pinned", hp, ho, False)
+    leg("serial_pinned", hp, ho, True)
     hq = np.array(hp)          # pageable copies
     del hp
     hv = np.empty((nv, nu, nu), np.float32)
-    st.run(hq, angles, weights, out=hv)
-    t0 = time.perf_counter()
-    st.run(hq, angles, weights, out=hv)
-    ms = (time.perf_counter() - t0) * 1e3
-    out["overlapped_pageable"] = {"ms": round(ms, 2), "GUPS": round(updates / (ms / 1e3) / 1e9, 1)}
-    log(f"[bench] stream overlapped_pageable: {ms:.1f} ms")
+    leg("overlapped_pageable", hq, hv, False)
     st.close()
     out["bytes_h2d"] = 4 * n_p * nv * nu
     out["bytes_d2h"] = 4 * nv * nu * nu
-    out["note"] = "whole FBP with the host-to-device copy of the stack and the device-to-host copy of the slices"
+    out["note"] = ("whole FBP with the host-to-device copy of the stack and the device-to-host copy of the slices; "
+                   f"median of {reps} runs after one warm-up")
     return out
 
 
@@ -375,22 +380,40 @@ def bench_batch(args):
         stacks.append(dict(proj=s, angles=a, weights=w, center=nu / 2 + off, filtertype="Hamming",
                            buffer_size=0))
     torch.cuda.synchronize()
-    log("[bench] 32 stacks generated; building batch plan (autotuner) ...")
-    br = BatchReconstructor(stacks, tune=True)
+    log("[bench] 32 stacks generated; building batch plans (autotuner) ...")
+    # headline: interpolation folded into FMAs, as the single-stack headline;
+    # the reference operation order (bit-identical per stack) beside it
+    br = BatchReconstructor(stacks, tune=True, fused=True)
     stream = torch.cuda.Stream()
-    res = {}
-    for name, fn in [("graph", br.run_graph), ("persistent", br.run), ("sequential", br.run_sequential)]:
-        def step(fn=fn):
-            with torch.cuda.stream(stream):
-                fn(stream=stream)
-            stream.synchronize()
-        res[name] = timed(step, args.steps, args.warmup, name)
+
+    def measure(rec, legs):
+        res = {}
+        for name in legs:
+            fn = {"graph": rec.run_graph, "persistent": rec.run, "sequential": rec.run_sequential}[name]
+
+            def step(fn=fn):
+                with torch.cuda.stream(stream):
+                    fn(stream=stream)
+                stream.synchronize()
+            res[name] = timed(step, args.steps, args.warmup, name)
+        return res
+
+    res = measure(br, ["graph", "persistent", "sequential"])
     upd = br.updates
     r = base_result("GUPS (voxel*proj updates/s) batched 32 uneven stacks", round(upd / (res["graph"] / 1e3) / 1e9, 2),
                     "GUPS", args.steps, args.warmup, res["graph"], "f32",
                     {"workload": "32 uneven parallel-beam FBP stacks (width 512-2048, 360-1800 proj, 8-128 slices)",
-                     "slice_tile": br.slice_tile, "proj_chunk": br.proj_chunk, "work_items": br.n_items})
+                     "precision": "fused", "slice_tile": br.slice_tile, "proj_chunk": br.proj_chunk,
+                     "work_items": br.n_items})
     r["batched_vs_plain"] = {k: {"ms": round(v, 3), "GUPS": round(upd / (v / 1e3) / 1e9, 2)} for k, v in res.items()}
+    br.close()
+    del br
+    bx = BatchReconstructor(stacks, tune=True, fused=False)
+    rx = measure(bx, ["graph", "sequential"])
+    r["exact_mode"] = {k: {"ms": round(v, 3), "GUPS": round(upd / (v / 1e3) / 1e9, 2)} for k, v in rx.items()}
+    r["exact_mode"]["slice_tile"], r["exact_mode"]["proj_chunk"] = bx.slice_tile, bx.proj_chunk
+    r["exact_mode"]["note"] = "MultiProjBPparallel operation order, bit-identical per stack"
+    bx.close()
     emit(r)
 
 

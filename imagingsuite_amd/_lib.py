@@ -19,6 +19,9 @@ MR_ERR_INTERNAL = 4
 
 FILTER_TYPES = {"ram-lak": 0, "shepp-logan": 1, "hanning": 2, "hamming": 3, "butterworth": 4}
 ITER_METHODS = {"sirt": 0, "landweber": 1, "cgls": 2}
+# mr_batch_plan_create flags
+BATCH_TUNE = 1
+BATCH_FUSED = 2
 
 # Every symbol include/muhrec_hip.h declares.
 EXPORTED_SYMBOLS = (

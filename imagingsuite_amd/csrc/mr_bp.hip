@@ -1396,6 +1396,7 @@ struct mr_batch_plan {
     int n_items[2] = {0, 0};
     int *d_counter = nullptr;                // 8 partition counters
     int variant = 0;
+    int exact = 1;                           // 0: interpolation folded into FMAs (MR_BATCH_FUSED)
     int grid[kNumVariants] = {0};
     double updates = 0.0;
     hipGraph_t graph = nullptr;
@@ -1414,9 +1415,14 @@ int batch_launch_bp(mr_batch_plan *p, int variant, hipStream_t s)
     mr::ProfScope prof("bp_batch", s);
     dim3 grid((unsigned)p->grid[variant]), block(256);
 #define MR_BATCH_CASE(ZT, KC)                                                                         \
-    if (v.zt == ZT && v.kc == KC)                                                                     \
-        hipLaunchKernelGGL((mr::bp_batch_kernel<ZT, 0, true, KC>), grid, block, 0, s, p->d_args,       \
-                           p->d_items[li], p->d_parts[li], p->d_counter);
+    if (v.zt == ZT && v.kc == KC) {                                                                   \
+        if (p->exact)                                                                                 \
+            hipLaunchKernelGGL((mr::bp_batch_kernel<ZT, 0, true, KC>), grid, block, 0, s, p->d_args,   \
+                               p->d_items[li], p->d_parts[li], p->d_counter);                         \
+        else                                                                                          \
+            hipLaunchKernelGGL((mr::bp_batch_kernel<ZT, 0, false, KC>), grid, block, 0, s, p->d_args,  \
+                               p->d_items[li], p->d_parts[li], p->d_counter);                         \
+    }
     MR_BATCH_CASE(32, 4)
     MR_BATCH_CASE(32, 8)
     MR_BATCH_CASE(16, 8)
@@ -1460,12 +1466,15 @@ void batch_free(mr_batch_plan *p)
 }
 
 template <int ZT, int KC>
-int occupancy_grid(int &grid)
+int occupancy_grid(int &grid, int exact)
 {
     int dev = 0, ncu = 0, nb = 0;
     MR_HIP(hipGetDevice(&dev));
     MR_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    MR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mr::bp_batch_kernel<ZT, 0, true, KC>, 256, 0));
+    if (exact)
+        MR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mr::bp_batch_kernel<ZT, 0, true, KC>, 256, 0));
+    else
+        MR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mr::bp_batch_kernel<ZT, 0, false, KC>, 256, 0));
     grid = std::max(1, nb) * std::max(1, ncu);
     return MR_OK;
 }
@@ -1473,13 +1482,16 @@ int occupancy_grid(int &grid)
 }  // namespace
 
 extern "C" int mr_batch_plan_create(mr_batch_plan **plan, const mr_batch_stack *stacks, size_t n_stacks,
-                                    int tune)
+                                    int flags)
 {
     MR_GUARD_BEGIN
     mr::clear_error();
     MR_REQUIRE(plan != nullptr && stacks != nullptr && n_stacks > 0, "NULL argument or empty batch");
+    MR_REQUIRE((flags & ~(MR_BATCH_TUNE | MR_BATCH_FUSED)) == 0, "unknown batch flags");
     *plan = nullptr;
+    const bool tune = (flags & MR_BATCH_TUNE) != 0;
     auto *p = new mr_batch_plan();
+    p->exact = (flags & MR_BATCH_FUSED) ? 0 : 1;
     p->stacks.resize(n_stacks);
     std::vector<std::pair<double, int4>> items[2];   // ZT = 32, ZT = 16
     for (size_t i = 0; i < n_stacks; ++i) {
@@ -1589,11 +1601,11 @@ extern "C" int mr_batch_plan_create(mr_batch_plan **plan, const mr_batch_stack *
         return mr::hip_fail(e, "mr_batch_plan_create args", __FILE__, __LINE__);
     }
     int rc = MR_OK;
-    if (rc == MR_OK) rc = occupancy_grid<32, 4>(p->grid[0]);
-    if (rc == MR_OK) rc = occupancy_grid<32, 8>(p->grid[1]);
-    if (rc == MR_OK) rc = occupancy_grid<16, 8>(p->grid[2]);
-    if (rc == MR_OK) rc = occupancy_grid<16, 4>(p->grid[3]);
-    if (rc == MR_OK) rc = occupancy_grid<16, 16>(p->grid[4]);
+    if (rc == MR_OK) rc = occupancy_grid<32, 4>(p->grid[0], p->exact);
+    if (rc == MR_OK) rc = occupancy_grid<32, 8>(p->grid[1], p->exact);
+    if (rc == MR_OK) rc = occupancy_grid<16, 8>(p->grid[2], p->exact);
+    if (rc == MR_OK) rc = occupancy_grid<16, 4>(p->grid[3], p->exact);
+    if (rc == MR_OK) rc = occupancy_grid<16, 16>(p->grid[4], p->exact);
     if (rc != MR_OK) {
         batch_free(p);
         return rc;
@@ -1696,7 +1708,7 @@ extern "C" int mr_batch_run_sequential(mr_batch_plan *plan, void *stream)
     int rc = batch_prepare(plan, s);
     if (rc != MR_OK) return rc;
     for (auto &b : plan->stacks) {
-        rc = mr::run_bp(b.args, 0, 1, 1, s);
+        rc = mr::run_bp(b.args, 0, 1, plan->exact, s);
         if (rc != MR_OK) return rc;
     }
     return MR_OK;

@@ -357,9 +357,11 @@ class BatchReconstructor:
     """Many parallel-beam stacks of uneven geometry in one persistent launch
     (mr_batch_*).  ``stacks`` is a list of dicts with keys proj (CUDA
     [n_proj, slices, width]), angles, weights, center and optionally
-    filtertype, cutoff, direction, buffer_size; volumes are allocated here."""
+    filtertype, cutoff, direction, buffer_size; volumes are allocated here.
+    ``fused`` folds the interpolation into FMAs (float rounding only);
+    otherwise every stack is bit-identical to MultiProjBPparallel."""
 
-    def __init__(self, stacks, tune: bool = True):
+    def __init__(self, stacks, tune: bool = True, fused: bool = False):
         import torch
         L = _lib.load()
         self._keep = []
@@ -388,7 +390,8 @@ class BatchReconstructor:
             d.weights = w.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
             d.d_volume = vol.data_ptr()
         self._plan = ctypes.c_void_p()
-        check(L.mr_batch_plan_create(ctypes.byref(self._plan), arr, len(stacks), int(bool(tune))))
+        flags = (_lib.BATCH_TUNE if tune else 0) | (_lib.BATCH_FUSED if fused else 0)
+        check(L.mr_batch_plan_create(ctypes.byref(self._plan), arr, len(stacks), flags))
         zt, kc, ni, upd = ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t(), ctypes.c_double()
         check(L.mr_batch_plan_info(self._plan, ctypes.byref(zt), ctypes.byref(kc), ctypes.byref(ni),
                                    ctypes.byref(upd)))

@@ -273,9 +273,14 @@ typedef struct {
 
 typedef struct mr_batch_plan mr_batch_plan;
 
-/* tune != 0: the autotuner times every (slice tile, projection chunk) variant
- * of the persistent kernel on this batch and keeps the fastest. */
-int mr_batch_plan_create(mr_batch_plan **plan, const mr_batch_stack *stacks, size_t n_stacks, int tune);
+/* flags: MR_BATCH_TUNE - the autotuner times every (slice tile, projection
+ * chunk) variant of the persistent kernel on this batch and keeps the
+ * fastest; MR_BATCH_FUSED - the interpolation is folded into two FMAs per
+ * update (as mr_bp_params.fused; float rounding only), otherwise every stack
+ * keeps MultiProjBPparallel's operation order.  Unknown bits: MR_ERR_INVALID. */
+#define MR_BATCH_TUNE 1
+#define MR_BATCH_FUSED 2
+int mr_batch_plan_create(mr_batch_plan **plan, const mr_batch_stack *stacks, size_t n_stacks, int flags);
 int mr_batch_plan_info(const mr_batch_plan *plan, int *slice_tile, int *proj_chunk, size_t *n_items,
                        double *updates);
 int mr_batch_run(mr_batch_plan *plan, void *stream);

@@ -67,3 +67,13 @@ def test_filter_type_string_rejected():
     from imagingsuite_amd.recon import ProjectionFilterSingle
     with pytest.raises(MrError):
         ProjectionFilterSingle(256, filtertype="gaussian")
+
+
+def test_batch_rejects_unknown_flags():
+    import imagingsuite_amd._lib as L
+    lib = L.load()
+    plan = ctypes.c_void_p()
+    arr = (L.BatchStack * 1)()
+    assert lib.mr_batch_plan_create(ctypes.byref(plan), arr, 1, 4) == L.MR_ERR_INVALID
+    assert not plan.value
+    assert b"flags" in lib.mr_last_error()
