@@ -211,6 +211,33 @@ __device__ __forceinline__ void bp_lds_tile(const BpArgs &a, int bx, int by, int
     if (any_inside && zl > 0) {
         const float yf = (float)(y + 1), xf = (float)(x + 1);
         const int num2 = a.nu - 2;
+        // Staging: element e = threadIdx.x + 256 i of a chunk is (projection
+        // kk, column b, slice group q).  Its LDS slot and its byte offset in
+        // the chunk (less the column term) do not change between chunks, so
+        // they are computed once here; a chunk then costs one multiply-add
+        // per element.  Loads go through a buffer resource over the chunk:
+        // slices at or beyond 4*(nv/4) get an offset past its end and read as
+        // zeros (the accumulation below runs over every group of the tile and
+        // leaves them unchanged), and columns past the detector edge, which no
+        // voxel reads, stay inside the buffer or read as zeros.  Chunks too
+        // large for 32-bit offsets take the general loop.
+        constexpr int NE = (KC * SPAN * G + 255) / 256;
+        const int vpad4 = 4 * a.vpad;
+        const bool off32 = (long)KC * (a.nu + SPAN) * vpad4 < 0x7FFFFFF0L;
+        int st_kk[NE], st_b[NE], st_lds[NE];
+        unsigned st_off[NE];
+#pragma unroll
+        for (int i = 0; i < NE; ++i) {
+            const int e = (int)threadIdx.x + 256 * i;
+            const int kk = e / (SPAN * G);
+            const int r = e - kk * SPAN * G;
+            const int b = r / G, q = r - b * G;
+            st_kk[i] = e < KC * SPAN * G ? kk : KC;
+            st_b[i] = b;
+            st_lds[i] = (kk * SPAN + b) * RS + 4 * q;
+            st_off[i] = z0 + 4 * q < a.nz4 ? (unsigned)(kk * a.nu * vpad4 + 4 * (z0 + 4 * q)) : 0x80000000u;
+        }
+        float *smf = &sm[0][0][0];
         for (int kb = 0; kb < a.nk; kb += KC) {
             const int kc = min(KC, a.nk - kb);
             if (threadIdx.x < kc) {
@@ -226,19 +253,39 @@ __device__ __forceinline__ void bp_lds_tile(const BpArgs &a, int bx, int by, int
                 sb0[threadIdx.x] = max(n0 - 1, 0);
             }
             __syncthreads();
-            for (int e = threadIdx.x; e < kc * SPAN * G; e += 256) {
-                const int kk = e / (SPAN * G);
-                const int r = e - kk * SPAN * G;
-                const int b = r / G, q = r - b * G;
-                const int col = sb0[kk] + b;
-                float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                // slices at or beyond 4*(nv/4) are staged as zeros: the
-                // accumulation below then runs over every group of the tile
-                // and leaves them unchanged
-                if (col < a.nu && z0 + 4 * q < a.nz4)
-                    v = *reinterpret_cast<const float4 *>(
-                        a.sino + ((size_t)(kb + kk) * a.nu + col) * a.vpad + z0 + 4 * q);
-                *reinterpret_cast<float4 *>(&sm[kk][b][4 * q]) = v;
+            if (off32) {
+                const float *cb = a.sino + (size_t)kb * a.nu * a.vpad;
+                const long rem = (long)(a.nk - kb) * a.nu * vpad4;
+                const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                    (void *)cb, (short)0, (int)min(rem, 0x7FFFFFF0L), 0x00020000);
+                // all loads first, then the LDS writes, so the loads are in
+                // flight together; elements of absent projections (last
+                // chunk) load zeros from an offset past the buffer, and any
+                // offset is safe with a buffer resource
+                float4 v[NE];
+#pragma unroll
+                for (int i = 0; i < NE; ++i) {
+                    const unsigned col = (unsigned)(sb0[min(st_kk[i], KC - 1)] + st_b[i]);
+                    const unsigned off = st_kk[i] < kc ? col * (unsigned)vpad4 + st_off[i] : 0x80000000u;
+                    // the whole 16-byte result is reinterpreted: indexing its
+                    // elements makes this compiler load one dword only
+                    v[i] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+                }
+#pragma unroll
+                for (int i = 0; i < NE; ++i)
+                    if (st_kk[i] < KC) *reinterpret_cast<float4 *>(smf + st_lds[i]) = v[i];
+            } else {
+                for (int e = threadIdx.x; e < kc * SPAN * G; e += 256) {
+                    const int kk = e / (SPAN * G);
+                    const int r = e - kk * SPAN * G;
+                    const int b = r / G, q = r - b * G;
+                    const int col = sb0[kk] + b;
+                    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    if (col < a.nu && z0 + 4 * q < a.nz4)
+                        v = *reinterpret_cast<const float4 *>(
+                            a.sino + ((size_t)(kb + kk) * a.nu + col) * a.vpad + z0 + 4 * q);
+                    *reinterpret_cast<float4 *>(&sm[kk][b][4 * q]) = v;
+                }
             }
             __syncthreads();
             if constexpr (PIPE == 0) {
@@ -526,6 +573,188 @@ __global__ __launch_bounds__(256) void bp_quad_kernel(BpArgs a)
     int bx, by, bz;
     if (!tile_of_block(a, bx, by, bz)) return;
     bp_quad_tile<ZT, EXACT, KC>(a, bx, by, bz, sm, sb0);
+}
+
+// Pair variant (linear interpolation only).  A block owns a 32 x 16 voxel
+// tile (x by y) and ZT slices; each thread owns two neighbouring columns
+// (x, y) and (x + 1, y), whose positions differ by fSin, so their bins n
+// usually lie in {nb, nb + 1} and the pair reads three staged columns
+// nb..nb+2 where two single columns need four: 3G 16-byte LDS reads serve
+// 8G updates.  Each voxel weighs its columns n, n+1 with 1-w, w and the
+// third with zero (the same argument as the quad variant: the zero terms
+// are exact, so both modes are bit-identical to the one-column kernel).  A
+// pair whose bins are two or more apart (|fSin| within rounding of 1) takes
+// its two voxels one at a time.  Lanes 0-15 of a wave cover 16 x 2 voxels,
+// so the columns one 16-lane group reads stay within about 16 bins.
+// Staged span: the tile's bins span at most sqrt(31^2 + 15^2) + 1 < 36, one
+// rounding-margin column below and up to two above the highest pair base.
+constexpr int kPairSpan = 40;
+
+template <int ZT, bool EXACT, int KC>
+__device__ __forceinline__ void bp_pair_tile(const BpArgs &a, int bx, int by, int bz,
+                                             float (&sm)[KC][kPairSpan][ZT + 4], int (&sb0)[KC])
+{
+    constexpr int TX = 32, TY = 16;
+    constexpr int SPAN = kPairSpan;
+    constexpr int RS = ZT + 4;
+    constexpr int G = ZT / 4;
+    const int tid = (int)threadIdx.x;
+    const int px = (tid & 7) | ((tid >> 1) & 8);          // pair 0..15 along x
+    const int yl = ((tid >> 3) & 1) | ((tid >> 4) & 14);  // row 0..15
+    const int x0 = bx * TX, y0 = by * TY;
+    const int xa = x0 + 2 * px, y = y0 + yl;
+    const int z0 = bz * ZT;
+    const int zn = min(ZT, a.nv - z0);
+    const int zl = max(0, min(ZT, a.nz4 - z0));
+    float acc[2][ZT];
+    bool in_vol[2], inside[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int x = xa + q;
+        in_vol[q] = x < a.nu && y < a.nu;
+        const float *out = a.vol + ((size_t)min(y, a.nu - 1) * a.nu + min(x, a.nu - 1)) * a.nv + z0;
+#pragma unroll
+        for (int z = 0; z < ZT; ++z) acc[q][z] = (in_vol[q] && a.accumulate && z < zn) ? out[z] : 0.0f;
+        inside[q] = false;
+        if (in_vol[q]) {
+            const int2 m = a.mask[y];
+            inside[q] = x >= m.x && x < m.y;
+        }
+    }
+    const bool any = inside[0] || inside[1];
+    const bool any_inside = __syncthreads_or(any ? 1 : 0) != 0;
+    if (any_inside && zl > 0) {
+        const float yf = (float)(y + 1);
+        const float xf0 = (float)(xa + 1), xf1 = (float)(xa + 2);
+        const int num2 = a.nu - 2;
+        for (int kb = 0; kb < a.nk; kb += KC) {
+            const int kc = min(KC, a.nk - kb);
+            if (tid < kc) {
+                // lowest detector column any voxel of the tile reaches
+                const float4 t = a.tab[kb + tid];
+                const float ya = (float)(y0 + 1), yb = (float)(y0 + TY);
+                const float xa1 = (float)(x0 + 1), xb = (float)(x0 + TX);
+                const float la = t.x + t.y * ya, lb = t.x + t.y * yb;
+                const int n0 = min(min((int)(la - t.z * xa1), (int)(la - t.z * xb)),
+                                   min((int)(lb - t.z * xa1), (int)(lb - t.z * xb)));
+                sb0[tid] = max(n0 - 1, 0);
+            }
+            __syncthreads();
+            for (int e = tid; e < kc * SPAN * G; e += 256) {
+                const int kk = e / (SPAN * G);
+                const int r = e - kk * SPAN * G;
+                const int b = r / G, q = r - b * G;
+                const int col = sb0[kk] + b;
+                float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                // slices at or beyond 4*(nv/4) are staged as zeros
+                if (col < a.nu && z0 + 4 * q < a.nz4)
+                    v = *reinterpret_cast<const float4 *>(
+                        a.sino + ((size_t)(kb + kk) * a.nu + col) * a.vpad + z0 + 4 * q);
+                *reinterpret_cast<float4 *>(&sm[kk][b][4 * q]) = v;
+            }
+            __syncthreads();
+            if (any) {
+                int sbr[KC];
+#pragma unroll
+                for (int q = 0; q < KC; ++q) sbr[q] = __builtin_amdgcn_readfirstlane(sb0[q]);
+#pragma unroll
+                for (int kk = 0; kk < KC; ++kk) {
+                    if (kk >= kc) break;
+                    const float4 t = tab_entry(a.tab, kb + kk);
+                    const float lsu = t.x + t.y * yf;
+                    const float u0 = lsu - t.z * xf0, u1 = lsu - t.z * xf1;
+                    const int n0 = (int)u0, n1 = (int)u1;
+                    const bool ok0 = inside[0] && n0 >= 0 && n0 <= num2;
+                    const bool ok1 = inside[1] && n1 >= 0 && n1 <= num2;
+                    if (!ok0 && !ok1) continue;
+                    const float wb0 = fabsf(u0 - (float)n0), wb1 = fabsf(u1 - (float)n1);
+                    const float wa0 = 1.0f - wb0, wa1 = 1.0f - wb1;
+                    if (ok0 && ok1 && (n0 - n1 > 1 || n1 - n0 > 1)) {
+                        // bins two apart: one voxel at a time, two columns each
+#pragma unroll
+                        for (int q = 0; q < 2; ++q) {
+                            const int n = q ? n1 : n0;
+                            const float wa = q ? wa1 : wa0, wb = q ? wb1 : wb0;
+                            const float *A = &sm[kk][min(max(n - sbr[kk], 0), SPAN - 2)][0];
+#pragma unroll
+                            for (int g = 0; g < G; ++g) {
+                                const float4 va = *reinterpret_cast<const float4 *>(A + 4 * g);
+                                const float4 vb = *reinterpret_cast<const float4 *>(A + RS + 4 * g);
+                                const float ra[4] = {va.x, va.y, va.z, va.w}, rb[4] = {vb.x, vb.y, vb.z, vb.w};
+#pragma unroll
+                                for (int c = 0; c < 4; ++c) {
+                                    float &sacc = acc[q][4 * g + c];
+                                    sacc = EXACT ? sacc + (ra[c] * wa + rb[c] * wb)
+                                                 : __builtin_fmaf(rb[c], wb, __builtin_fmaf(ra[c], wa, sacc));
+                                }
+                            }
+                        }
+                        continue;
+                    }
+                    const int nb = ok0 ? (ok1 ? min(n0, n1) : n0) : n1;
+                    const int loc = min(max(nb - sbr[kk], 0), SPAN - 3);
+                    const int base = sbr[kk] + loc;
+                    // weights of columns base, base+1, base+2 per voxel (zero when not used)
+                    float w[2][3];
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        const bool ok = q ? ok1 : ok0;
+                        const int d = (q ? n1 : n0) - base;
+                        const float wa = q ? wa1 : wa0, wb = q ? wb1 : wb0;
+                        w[q][0] = (ok && d == 0) ? wa : 0.0f;
+                        w[q][1] = ok ? (d == 0 ? wb : wa) : 0.0f;
+                        w[q][2] = (ok && d == 1) ? wb : 0.0f;
+                    }
+                    const float *A = &sm[kk][loc][0];
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        const float4 c0 = *reinterpret_cast<const float4 *>(A + 4 * g);
+                        const float4 c1 = *reinterpret_cast<const float4 *>(A + RS + 4 * g);
+                        const float4 c2 = *reinterpret_cast<const float4 *>(A + 2 * RS + 4 * g);
+                        const float r0[4] = {c0.x, c0.y, c0.z, c0.w}, r1[4] = {c1.x, c1.y, c1.z, c1.w};
+                        const float r2[4] = {c2.x, c2.y, c2.z, c2.w};
+#pragma unroll
+                        for (int q = 0; q < 2; ++q)
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) {
+                                float &sacc = acc[q][4 * g + c];
+                                if (EXACT)
+                                    sacc = sacc + ((r0[c] * w[q][0] + r1[c] * w[q][1]) + r2[c] * w[q][2]);
+                                else
+                                    sacc = __builtin_fmaf(r2[c], w[q][2],
+                                                          __builtin_fmaf(r1[c], w[q][1], __builtin_fmaf(r0[c], w[q][0], sacc)));
+                            }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        if (!in_vol[q]) continue;
+        float *out = a.vol + ((size_t)y * a.nu + xa + q) * a.nv + z0;
+        if ((a.nv & 3) == 0 && zn == ZT) {
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+                *reinterpret_cast<float4 *>(out + 4 * g) =
+                    make_float4(acc[q][4 * g], acc[q][4 * g + 1], acc[q][4 * g + 2], acc[q][4 * g + 3]);
+        } else {
+#pragma unroll
+            for (int z = 0; z < ZT; ++z)
+                if (z < zn) out[z] = acc[q][z];
+        }
+    }
+}
+
+template <int ZT, bool EXACT, int KC>
+__global__ __launch_bounds__(256) void bp_pair_kernel(BpArgs a)
+{
+    __shared__ float sm[KC][kPairSpan][ZT + 4];
+    __shared__ int sb0[KC];
+    int bx, by, bz;
+    if (!tile_of_block(a, bx, by, bz)) return;
+    bp_pair_tile<ZT, EXACT, KC>(a, bx, by, bz, sm, sb0);
 }
 
 // Persistent batched back-projection: a grid sized to the device (every
@@ -880,6 +1109,26 @@ static int launch_bp_quad(BpArgs a, int exact, int order, hipStream_t s)
     return MR_OK;
 }
 
+template <int ZT, int KC>
+static int launch_bp_pair(BpArgs a, int exact, int order, hipStream_t s)
+{
+    a.ntx = (a.nu + 31) / 32;
+    a.nty = (a.nu + 15) / 16;
+    a.ntz = (a.nv + ZT - 1) / ZT;
+    const long n = (long)a.ntx * a.nty * a.ntz;
+    a.order = n < (1L << 30) ? order : 0;
+    const int sw = env_int("MR_BP_SW", 4);
+    a.sw = (sw == 2 || sw == 8 || sw == 16) ? sw : 4;
+    const dim3 grid = a.order ? dim3((unsigned)(8 * ((n + 7) / 8))) : dim3((unsigned)a.ntx, (unsigned)a.nty, (unsigned)a.ntz);
+    ProfScope prof("bp_parallel", s);
+    if (exact)
+        hipLaunchKernelGGL((bp_pair_kernel<ZT, true, KC>), grid, dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((bp_pair_kernel<ZT, false, KC>), grid, dim3(256), 0, s, a);
+    MR_HIP(hipGetLastError());
+    return MR_OK;
+}
+
 // Slice tile / projection chunk of the LDS kernels (MR_BP_ZT / MR_BP_KC
 // override the defaults for tuning experiments).
 static void lds_shape(int nv, int &zt, int &kc)
@@ -896,13 +1145,21 @@ static void lds_shape(int nv, int &zt, int &kc)
 }
 
 // impl: 0 = direct global gathers, 1 = LDS-staged 16 x 16 tiles, 2 = LDS-staged
-// 32 x 32 tiles with 2 x 2 voxel groups (linear interpolation; nearest
-// neighbour falls back to impl 1).  exact: bit-faithful float order
+// 32 x 32 tiles with 2 x 2 voxel groups, 3 = LDS-staged 32 x 16 tiles with
+// voxel pairs (2 and 3: linear interpolation; nearest neighbour falls back
+// to impl 1).  exact: bit-faithful float order
 // (otherwise fused multiply-adds).  MR_BP_ORDER selects the block order of
 // the tiled kernels (0 grid order, 1 XCD stripes).
 int run_bp(const BpArgs &a, int interp, int impl, int exact, hipStream_t s)
 {
     const int order = env_int("MR_BP_ORDER", 1) ? 1 : 0;
+    if (impl == 3 && interp == 0) {
+        const int ez = env_int("MR_BP_ZT", 0), ek = env_int("MR_BP_KC", 0);
+        if (ez == 32 || (ez == 0 && a.nv >= 32)) return ek == 8 ? launch_bp_pair<32, 8>(a, exact, order, s)
+                                                                : launch_bp_pair<32, 4>(a, exact, order, s);
+        if (ez == 8 || a.nv < 16) return launch_bp_pair<8, 8>(a, exact, order, s);
+        return ek == 4 ? launch_bp_pair<16, 4>(a, exact, order, s) : launch_bp_pair<16, 8>(a, exact, order, s);
+    }
     if (impl == 2 && interp == 0) {
         const int ez = env_int("MR_BP_ZT", 0), ek = env_int("MR_BP_KC", 0);
         const int zt = (ez == 8 || a.nv < 16) ? 8 : 16;
@@ -945,6 +1202,21 @@ std::string describe_bp(int nv, int interp, int impl, int exact)
     const int sw = (swe == 2 || swe == 8 || swe == 16) ? swe : 4;
     const std::string ord = order ? " order=1 sw=" + std::to_string(sw) : " order=0";
     const char *ex = exact ? "true" : "false";
+    if (impl == 3 && interp == 0) {
+        const int ez = env_int("MR_BP_ZT", 0), ek = env_int("MR_BP_KC", 0);
+        int zt, kc;
+        if (ez == 32 || (ez == 0 && nv >= 32)) {
+            zt = 32;
+            kc = ek == 8 ? 8 : 4;
+        } else if (ez == 8 || nv < 16) {
+            zt = 8;
+            kc = 8;
+        } else {
+            zt = 16;
+            kc = ek == 4 ? 4 : 8;
+        }
+        return "bp_pair_kernel<" + std::to_string(zt) + "," + ex + "," + std::to_string(kc) + ">" + ord;
+    }
     if (impl == 2 && interp == 0) {
         const int ez = env_int("MR_BP_ZT", 0), ek = env_int("MR_BP_KC", 0);
         const int zt = (ez == 8 || nv < 16) ? 8 : 16;
@@ -1041,7 +1313,7 @@ extern "C" int mr_bp_plan_create(mr_bp_plan **plan, const mr_bp_params *params)
     p->impl = (q.multiprojbp && (q.slices & 3)) ? 0 : 1;
     if (const char *e = std::getenv("MR_BP_IMPL")) {
         const int v = std::atoi(e);
-        p->impl = v == 0 ? 0 : (p->impl == 0 ? 0 : (v == 2 ? 2 : 1));
+        p->impl = v == 0 ? 0 : (p->impl == 0 ? 0 : (v == 2 || v == 3 ? v : 1));
     }
     mr::circle_mask(q.width, q.width, q.use_roi != 0, q.roi, false, p->first, p->second);
     std::vector<int2> m(q.width);

@@ -2,7 +2,7 @@
 
 The library picks one kernel per plan; the environment variables
 MR_BP_IMPL (0 direct gathers, 1 LDS 16x16 tiles, 2 LDS 32x32 tiles with 2x2
-voxel groups), MR_BP_ORDER (0 grid order, 1 XCD-striped order), MR_BP_ZT and
+voxel groups, 3 LDS 32x16 tiles with voxel pairs), MR_BP_ORDER (0 grid order, 1 XCD-striped order), MR_BP_ZT and
 MR_BP_KC (slice tile, projection chunk) and MR_BP_PIPE (1: the unrolled form
 of the LDS kernel) force the others, so each variant is checked here against
 the oracle:
@@ -10,7 +10,7 @@ the oracle:
 * exact mode: bit-exact against the MuhRec-faithful float oracle
   (MultiProjBPparallel.cpp:115-167 operation order);
 * fused mode: every variant gives the same bits as the 16x16 fused kernel
-  (the 2x2-group kernel only adds exact zero terms), and relRMSE <= 1e-6 of the
+  (the 2x2-group and pair kernels only add exact zero terms), and relRMSE <= 1e-6 of the
   range against the oracle.
 
 Sizes are chosen to leave partial tiles on every edge (width 100 is not a
@@ -30,6 +30,7 @@ VARIANTS = [
     (1, 1, 32, 4, 0), (1, 1, 32, 8, 0), (1, 1, 64, 2, 0), (1, 1, 64, 4, 0),
     (1, 1, 32, 4, 1), (1, 0, 16, 8, 1), (1, 1, 16, 8, 0),
     (2, 0, 16, 8, 0), (2, 1, 16, 8, 0), (2, 1, 16, 16, 0), (2, 1, 8, 8, 0), (2, 0, 8, 16, 0),
+    (3, 1, 16, 8, 0), (3, 0, 16, 4, 0), (3, 1, 8, 8, 0), (3, 1, 32, 4, 0),
 ]
 
 
@@ -70,7 +71,7 @@ def test_variant_exact_bitexact(orc, dev, monkeypatch, impl, order, zt, kc, pipe
     assert nbad == 0
 
 
-@pytest.mark.parametrize("impl,order,zt,kc,pipe", [v for v in VARIANTS if v[0] == 2 or v[4] == 1]
+@pytest.mark.parametrize("impl,order,zt,kc,pipe", [v for v in VARIANTS if v[0] in (2, 3) or v[4] == 1]
                          + [(1, 1, 16, 16, 0)])
 def test_variant_fused_matches_lds16(orc, dev, monkeypatch, impl, order, zt, kc, pipe):
     nu, nv, n_p, center = 300, 32, 120, 151.5
