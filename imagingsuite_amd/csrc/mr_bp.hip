@@ -1093,7 +1093,8 @@ static int launch_bp_lds_mode(const BpArgs &a, int interp, int exact, int order,
 // at 2048^2 x 256 x 1500); MR_BP_PIPE = 0 selects the rolled form.
 static int bp_pipe(int zt, int kc)
 {
-    return env_int("MR_BP_PIPE", 1) == 1 && ((zt == 32 && kc == 4) || (zt == 16 && kc == 8)) ? 1 : 0;
+    return env_int("MR_BP_PIPE", 1) == 1 && ((zt == 32 && kc == 4) || (zt == 16 && kc == 8) || (zt == 64 && kc == 2))
+               ? 1 : 0;
 }
 
 template <int ZT, int KC>
@@ -1171,6 +1172,7 @@ int run_bp(const BpArgs &a, int interp, int impl, int exact, hipStream_t s)
     if (impl >= 1) {
         int zt, kc;
         lds_shape(a.nv, zt, kc);
+        if (zt == 64 && bp_pipe(zt, kc)) return launch_bp_lds_mode<64, 2, 1>(a, interp, exact, order, s);
         if (zt == 64) return kc == 2 ? launch_bp_lds_mode<64, 2>(a, interp, exact, order, s)
                                      : launch_bp_lds_mode<64, 4>(a, interp, exact, order, s);
         if (bp_pipe(zt, kc))

@@ -28,7 +28,7 @@ VARIANTS = [
     # impl, order, zt, kc, pipe
     (1, 0, 16, 16, 0), (1, 1, 16, 16, 0), (1, 1, 16, 8, 0), (1, 1, 16, 4, 0), (1, 1, 8, 32, 0), (1, 0, 32, 8, 0),
     (1, 1, 32, 4, 0), (1, 1, 32, 8, 0), (1, 1, 64, 2, 0), (1, 1, 64, 4, 0),
-    (1, 1, 32, 4, 1), (1, 0, 16, 8, 1), (1, 1, 16, 8, 0),
+    (1, 1, 32, 4, 1), (1, 0, 16, 8, 1), (1, 1, 16, 8, 0), (1, 1, 64, 2, 1),
     (2, 0, 16, 8, 0), (2, 1, 16, 8, 0), (2, 1, 16, 16, 0), (2, 1, 8, 8, 0), (2, 0, 8, 16, 0),
     (3, 1, 16, 8, 0), (3, 0, 16, 4, 0), (3, 1, 8, 8, 0), (3, 1, 32, 4, 0),
 ]
