@@ -235,7 +235,10 @@ __device__ __forceinline__ void bp_lds_tile(const BpArgs &a, int bx, int by, int
             st_kk[i] = e < KC * SPAN * G ? kk : KC;
             st_b[i] = b;
             st_lds[i] = (kk * SPAN + b) * RS + 4 * q;
-            st_off[i] = z0 + 4 * q < a.nz4 ? (unsigned)(kk * a.nu * vpad4 + 4 * (z0 + 4 * q)) : 0x80000000u;
+            // unsigned: only used when the chunk fits 32-bit offsets (off32)
+            st_off[i] = z0 + 4 * q < a.nz4
+                            ? (unsigned)kk * (unsigned)a.nu * (unsigned)vpad4 + 4u * (unsigned)(z0 + 4 * q)
+                            : 0x80000000u;
         }
         float *smf = &sm[0][0][0];
         for (int kb = 0; kb < a.nk; kb += KC) {

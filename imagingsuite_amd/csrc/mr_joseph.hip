@@ -255,6 +255,126 @@ __global__ __launch_bounds__(256) void joseph_forward_lds_kernel(JosephArgs a, c
             make_float4(acc[4 * g] * inv_m, acc[4 * g + 1] * inv_m, acc[4 * g + 2] * inv_m, acc[4 * g + 3] * inv_m);
 }
 
+// The same forward with one barrier per line.  The block first computes,
+// for every line, the span of its rays' crossings (the minimum and maximum
+// floor(crossing) over the edge rays, the first and last valid bin of each
+// of its angles, with the same float expression) into LDS; the per-line
+// loop then stages line L + 1 into one buffer while its rays interpolate
+// line L from the other, and a single barrier separates the lines.  Staged
+// values, offsets and the order of the sums are those of
+// joseph_forward_lds_kernel, so the projections are bit-identical to it.
+// Dynamic LDS: 2 N ints (the spans).
+template <int ZT>
+__global__ __launch_bounds__(256) void joseph_forward_lds2_kernel(JosephArgs a, const int4 *__restrict__ chunks,
+                                                                  const int *__restrict__ order, int n_chunks,
+                                                                  int nbt, int nzs)
+{
+    constexpr int NB = kFwdNB, KA = kFwdKA, W = kFwdW, RS = ZT + 4, G = ZT / 4;
+    __shared__ float seg[2][W][RS];
+    __shared__ float4 ep[2 * KA];   // edge rays: (a0, b1, cn, inv)
+    extern __shared__ int spans[];  // lo of line L at [L], hi at [N + L]
+    const int total = n_chunks * nbt * nzs;
+    const int per = (total + 7) >> 3;
+    const int b = (int)blockIdx.x;
+    const int wi = (b & 7) * per + (b >> 3);   // XCD b % 8 takes the (b % 8)-th eighth
+    if (wi >= total) return;
+    const int bt = wi % nbt;
+    const int rest = wi / nbt;
+    const int ci = rest % n_chunks;
+    const int zs = rest / n_chunks;
+    const int4 ch = chunks[ci];                // first, count, class
+    const int tid = (int)threadIdx.x;
+    const int ai = tid / NB, bi = tid % NB;
+    const int n = bt * NB + bi;
+    const bool active = ai < ch.y && n < a.nd;
+    const int ang = order[ch.x + min(ai, ch.y - 1)];
+    const float4 t = a.tab[ang];
+    const float su = t.x, c = t.y, s = t.z, inv_m = t.w;
+    const int N = a.nx;
+    const int z0 = zs * ZT;
+    const bool rows = ch.z == 0;
+    // crossing of line L: ((a0 + b1 (L+1)) + cn) * inv - 1
+    const float a0 = rows ? su : -su, b1 = rows ? c : s;
+    const float fn = (float)n;
+    const float cn = rows ? -fn : fn;
+    const float inv = rows ? 1.0f / s : 1.0f / c;
+    const size_t strideL = rows ? (size_t)N * a.nz : (size_t)a.nz;
+    const size_t strideP = rows ? (size_t)a.nz : (size_t)N * a.nz;
+    if (tid < 2 * KA) {
+        // edge ray tid: angle tid / 2, the block's first (even) or last valid (odd) bin
+        const int ea = tid >> 1;
+        const int en = min(bt * NB + ((tid & 1) ? NB - 1 : 0), a.nd - 1);
+        const float4 et = a.tab[order[ch.x + min(ea, ch.y - 1)]];
+        const float efn = (float)en;
+        ep[tid] = make_float4(rows ? et.x : -et.x, rows ? et.y : et.z, rows ? -efn : efn,
+                              rows ? 1.0f / et.z : 1.0f / et.y);
+    }
+    __syncthreads();
+    const int ne = 2 * ch.y;
+    for (int L = tid; L < N; L += 256) {
+        int lo = 0x7fffffff, hi = -0x7fffffff;
+        for (int e = 0; e < ne; ++e) {
+            const float4 q = ep[e];
+            const int p0 = (int)floorf(((q.x + q.y * (float)(L + 1)) + q.z) * q.w - 1.0f);
+            lo = min(lo, p0);
+            hi = max(hi, p0);
+        }
+        spans[L] = lo;
+        spans[N + L] = hi;
+    }
+    __syncthreads();
+    float acc[ZT];
+#pragma unroll
+    for (int z = 0; z < ZT; ++z) acc[z] = 0.0f;
+    // a line is skipped when no ray of the block crosses the volume on it
+    auto live = [&](int L) { return spans[N + L] >= -1 && spans[L] < N; };
+    // only the voxels a line's rays read: lo .. hi + 1
+    auto stage = [&](int L, int buf) {
+        const int lo = spans[L], nj = min(spans[N + L] - lo + 2, W);
+        const float *line = a.src + (size_t)L * strideL + z0;
+        for (int e = tid; e < nj * G; e += 256) {
+            const int j = e / G, g = e - j * G;
+            const int p = lo + j;
+            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (p >= 0 && p < N) v = *reinterpret_cast<const float4 *>(line + (size_t)p * strideP + 4 * g);
+            *reinterpret_cast<float4 *>(&seg[buf][j][4 * g]) = v;
+        }
+    };
+    if (live(0)) stage(0, 0);
+    __syncthreads();
+    for (int L = 0; L < N; ++L) {
+        const int buf = L & 1;
+        if (L + 1 < N && live(L + 1)) stage(L + 1, buf ^ 1);
+        if (live(L)) {
+            const float pf = ((a0 + b1 * (float)(L + 1)) + cn) * inv - 1.0f;
+            const float fl = floorf(pf);
+            const int p0 = (int)fl;
+            const float f = pf - fl;
+            const float w0 = (p0 >= 0 && p0 < N) ? 1.0f - f : 0.0f;
+            const float w1 = (p0 + 1 >= 0 && p0 + 1 < N) ? f : 0.0f;
+            const int j = min(max(p0 - spans[L], 0), W - 2);
+            const float *v0 = &seg[buf][j][0];
+            const float *v1 = v0 + RS;
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const float4 x0 = *reinterpret_cast<const float4 *>(v0 + 4 * g);
+                const float4 x1 = *reinterpret_cast<const float4 *>(v1 + 4 * g);
+                acc[4 * g + 0] = __builtin_fmaf(w1, x1.x, __builtin_fmaf(w0, x0.x, acc[4 * g + 0]));
+                acc[4 * g + 1] = __builtin_fmaf(w1, x1.y, __builtin_fmaf(w0, x0.y, acc[4 * g + 1]));
+                acc[4 * g + 2] = __builtin_fmaf(w1, x1.z, __builtin_fmaf(w0, x0.z, acc[4 * g + 2]));
+                acc[4 * g + 3] = __builtin_fmaf(w1, x1.w, __builtin_fmaf(w0, x0.w, acc[4 * g + 3]));
+            }
+        }
+        __syncthreads();
+    }
+    if (!active) return;
+    float *out = a.dst + ((size_t)ang * a.nd + n) * a.nz + z0;
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+        *reinterpret_cast<float4 *>(out + 4 * g) =
+            make_float4(acc[4 * g] * inv_m, acc[4 * g + 1] * inv_m, acc[4 * g + 2] * inv_m, acc[4 * g + 3] * inv_m);
+}
+
 // Adjoint: one thread per (x, y) voxel column and ZT slices.
 template <int ZT>
 __global__ __launch_bounds__(256) void joseph_adjoint_kernel(JosephArgs a)
@@ -537,6 +657,19 @@ int joseph_launch(const mr_joseph_geom *g, const float *src, float *dst, const J
             const long total = (long)dt.n_chunks * nbt * nzs;
             MR_REQUIRE(total < (1L << 31) - 8, "Joseph forward grid too large");
             const dim3 grid((unsigned)(8 * ((total + 7) / 8)));
+            // one barrier per line (spans precomputed, double-buffered staging) unless MR_JOSEPH_FWD=1
+            // or the span table would not fit beside the staging buffers
+            if (!(ef && std::atoi(ef) == 1) && g->nx <= 4096) {
+                const size_t dyn = sizeof(int) * 2 * g->nx;
+                if (zt == 32)
+                    hipLaunchKernelGGL((mr::joseph_forward_lds2_kernel<32>), grid, dim3(256), dyn, s, a, dt.chunks,
+                                       dt.order, dt.n_chunks, nbt, nzs);
+                else
+                    hipLaunchKernelGGL((mr::joseph_forward_lds2_kernel<16>), grid, dim3(256), dyn, s, a, dt.chunks,
+                                       dt.order, dt.n_chunks, nbt, nzs);
+                MR_HIP(hipGetLastError());
+                return MR_OK;
+            }
 #define MR_JF_LAUNCH(Z, RR)                                                                                     \
     hipLaunchKernelGGL((mr::joseph_forward_lds_kernel<Z, RR>), grid, dim3(256), 0, s, a, dt.chunks, dt.order, \
                        dt.n_chunks, nbt, nzs)
