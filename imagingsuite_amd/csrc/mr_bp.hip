@@ -975,6 +975,160 @@ __global__ __launch_bounds__(256) void bp_adjoint_lds_kernel(const float *__rest
     for (int z = 0; z < ZT; ++z) o[(size_t)(z0 + z) * nu] = w * acc[z];
 }
 
+// The chunked adjoint with one barrier per line.  The block first computes
+// every line's candidate span (the edge rays' unclamped ranges, same float
+// expressions) into LDS; the per-line loop then stages line L + 1 into one
+// buffer while its threads test the candidates of line L in the other, and
+// a single barrier separates the lines.  Staged values, offsets and the
+// order of the sums are those of bp_adjoint_lds_kernel, so the gradients
+// are bit-identical to it.  Dynamic LDS: 2 nu ints (the spans).
+template <int INTERP, int ZT>
+__global__ __launch_bounds__(256) void bp_adjoint_lds2_kernel(const float *__restrict__ g, float *__restrict__ out,
+                                                              const float4 *__restrict__ tab,
+                                                              const int2 *__restrict__ mask,
+                                                              const int *__restrict__ idx,
+                                                              const float *__restrict__ wts,
+                                                              const int4 *__restrict__ chunks,
+                                                              const int *__restrict__ order, int n_chunks, int nbt,
+                                                              int nzs, int nu, int nv)
+{
+    constexpr int NB = kAdjNB, KA = kAdjKA, W = kAdjW, RS = ZT + 4, G = ZT / 4;
+    __shared__ float seg[2][W][RS];
+    __shared__ float4 ep[2 * KA];   // edge rays: (su, c, s, ulo); uhi = ulo of the bin + 2 (see below)
+    __shared__ float eu[2 * KA];    // edge rays: uhi
+    extern __shared__ int spans[];  // lo of line L (1-based) at [L - 1], hi at [nu + L - 1]
+    const int total = n_chunks * nbt * nzs;
+    const int per = (total + 7) >> 3;
+    const int b = (int)blockIdx.x;
+    const int wi = (b & 7) * per + (b >> 3);
+    if (wi >= total) return;
+    const int bt = wi % nbt;
+    const int rest = wi / nbt;
+    const int ci = rest % n_chunks;
+    const int zs = rest / n_chunks;
+    const int4 ch = chunks[ci];                 // first, count, class
+    const int tid = (int)threadIdx.x;
+    const int ai = tid / NB, bi = tid % NB;
+    const int u = bt * NB + bi;
+    const bool active = ai < ch.y && u < nu;
+    const int k = order[ch.x + min(ai, ch.y - 1)];   // used-projection index
+    const float4 t = tab[k];
+    const float su = t.x, c = t.y, s = t.z;
+    const int num2 = nu - 2;
+    const float fu = (float)u;
+    const bool rows = ch.z == 0;
+    const int z0 = zs * ZT;
+    // candidates on line L (1-based): u(x, y) in [ulo, uhi]
+    const float ulo = u <= 1 ? -1.0f : fu - 1.0f, uhi = fu + 1.0f;
+    const float inv = rows ? 1.0f / s : 1.0f / c;
+    if (tid < 2 * KA) {
+        // edge ray tid: projection tid / 2, the block's first (even) or last valid (odd) bin
+        const int eu_bin = min(bt * NB + ((tid & 1) ? NB - 1 : 0), nu - 1);
+        const float4 et = tab[order[ch.x + min(tid >> 1, ch.y - 1)]];
+        const float efu = (float)eu_bin;
+        ep[tid] = make_float4(et.x, et.y, et.z, eu_bin <= 1 ? -1.0f : efu - 1.0f);
+        eu[tid] = efu + 1.0f;
+    }
+    __syncthreads();
+    const int ne = 2 * ch.y;
+    for (int L = tid + 1; L <= nu; L += 256) {
+        int lo = 0x7fffffff, hi = -0x7fffffff;
+        for (int e = 0; e < ne; ++e) {
+            const float4 q = ep[e];
+            const float ehi = eu[e];
+            const float einv = rows ? 1.0f / q.z : 1.0f / q.y;
+            float pa, pb;
+            if (rows) {
+                const float lsu = q.x + q.y * (float)L;
+                pa = (lsu - ehi) * einv;
+                pb = (lsu - q.w) * einv;
+            } else {
+                const float bb = q.z * (float)L - q.x;
+                pa = (q.w + bb) * einv;
+                pb = (ehi + bb) * einv;
+            }
+            lo = min(lo, (int)floorf(fminf(pa, pb)) - 1);
+            hi = max(hi, (int)ceilf(fmaxf(pa, pb)) + 1);
+        }
+        spans[L - 1] = lo;
+        spans[nu + L - 1] = hi;
+    }
+    __syncthreads();
+    float acc[ZT];
+#pragma unroll
+    for (int z = 0; z < ZT; ++z) acc[z] = 0.0f;
+    auto live = [&](int L) { return spans[L - 1] <= nu && spans[nu + L - 1] >= 1; };
+    // the line's candidate voxels of the volume gradient, zeros outside the mask
+    auto stage = [&](int L, int buf) {
+        const int lo = spans[L - 1], span = spans[nu + L - 1] - lo;
+        for (int e = tid; e < W * G; e += 256) {
+            const int j = e / G, gq = e - j * G;
+            const int p = lo + j;   // 1-based position on the line
+            if (j > span) continue;
+            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (p >= 1 && p <= nu) {
+                const int x = rows ? p : L, y = rows ? L : p;
+                const int2 m = mask[y - 1];
+                if (x - 1 >= m.x && x - 1 < m.y)
+                    v = *reinterpret_cast<const float4 *>(g + ((size_t)(y - 1) * nu + (x - 1)) * nv + z0 + 4 * gq);
+            }
+            *reinterpret_cast<float4 *>(&seg[buf][j][4 * gq]) = v;
+        }
+    };
+    if (live(1)) stage(1, 0);
+    __syncthreads();
+    for (int L = 1; L <= nu; ++L) {
+        const int buf = L & 1 ? 0 : 1;   // line L in buffer (L - 1) & 1
+        if (L < nu && live(L + 1)) stage(L + 1, buf ^ 1);
+        if (active && live(L)) {
+            float pa, pb;
+            if (rows) {
+                const float lsu = su + c * (float)L;
+                pa = (lsu - uhi) * inv;
+                pb = (lsu - ulo) * inv;
+            } else {
+                const float bb = s * (float)L - su;
+                pa = (ulo + bb) * inv;
+                pb = (uhi + bb) * inv;
+            }
+            const int clo = max(1, (int)floorf(fminf(pa, pb)) - 1), chi = min(nu, (int)ceilf(fmaxf(pa, pb)) + 1);
+            const int base = spans[L - 1];
+            for (int p = clo; p <= chi; ++p) {
+                const int x = rows ? p : L, y = rows ? L : p;
+                const float lsu = su + c * (float)y;
+                const float uu = lsu - s * (float)x;
+                const int n = (int)uu;
+                if (n < 0 || n > num2) continue;
+                float wgt;
+                if (INTERP == 1) {
+                    if (n != u) continue;
+                    wgt = 1.0f;
+                } else {
+                    const float wb = fabsf(uu - (float)n);
+                    if (n == u) wgt = 1.0f - wb;
+                    else if (n == u - 1) wgt = wb;
+                    else continue;
+                }
+                const float *v = &seg[buf][min(max(p - base, 0), W - 1)][0];
+#pragma unroll
+                for (int gq = 0; gq < G; ++gq) {
+                    const float4 x4 = *reinterpret_cast<const float4 *>(v + 4 * gq);
+                    acc[4 * gq + 0] = __builtin_fmaf(wgt, x4.x, acc[4 * gq + 0]);
+                    acc[4 * gq + 1] = __builtin_fmaf(wgt, x4.y, acc[4 * gq + 1]);
+                    acc[4 * gq + 2] = __builtin_fmaf(wgt, x4.z, acc[4 * gq + 2]);
+                    acc[4 * gq + 3] = __builtin_fmaf(wgt, x4.w, acc[4 * gq + 3]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!active) return;
+    const float w = wts[idx[k]];
+    float *o = out + (size_t)idx[k] * nv * nu + u;
+#pragma unroll
+    for (int z = 0; z < ZT; ++z) o[(size_t)(z0 + z) * nu] = w * acc[z];
+}
+
 // Adjoint of the parallel back-projection (the project's own operator for
 // gradients through FBP; no reference counterpart).  The forward adds, for
 // every used projection k and voxel (x, y) inside the mask, w_k ((1-f) S[n] +
@@ -1560,7 +1714,7 @@ extern "C" int mr_bp_parallel_adjoint(mr_bp_plan *plan, const float *d_grad_volu
     // (16 / sqrt(1/2) + 6 < kAdjW always holds) and at least one block of bins
     std::vector<int4> chunks;
     std::vector<int> order;
-    if (adj_impl == 1) adj_chunks(tab, (int)q.width, chunks, order);
+    if (adj_impl >= 1) adj_chunks(tab, (int)q.width, chunks, order);
     if (!chunks.empty()) {
         const size_t bytes = sizeof(int4) * chunks.size() + sizeof(int) * order.size();
         if (bytes > plan->adj_cap) {
@@ -1584,6 +1738,19 @@ extern "C" int mr_bp_parallel_adjoint(mr_bp_plan *plan, const float *d_grad_volu
         const dim3 grid((unsigned)(8 * ((total + 7) / 8)));
         const bool nn = q.interp == MR_INTERP_NEAREST;
         mr::ProfScope prof("bp_adjoint", s);
+        if (adj_impl == 1 && q.width <= 4096) {
+            // one barrier per line (spans precomputed, double-buffered staging); MR_BP_ADJ=2 selects the
+            // per-line-group form
+            const size_t dyn = sizeof(int) * 2 * q.width;
+#define MR_ADJ3_LAUNCH(I, Z)                                                                                       \
+    hipLaunchKernelGGL((mr::bp_adjoint_lds2_kernel<I, Z>), grid, dim3(256), dyn, s, d_grad_volume, d_grad_proj,   \
+                       plan->d_tab, plan->d_mask, plan->d_idx, plan->d_w, d_chunks, d_order, (int)chunks.size(),   \
+                       nbt, nzs, (int)q.width, (int)q.slices)
+            if (zt == 32) { if (nn) MR_ADJ3_LAUNCH(1, 32); else MR_ADJ3_LAUNCH(0, 32); }
+            else if (zt == 16) { if (nn) MR_ADJ3_LAUNCH(1, 16); else MR_ADJ3_LAUNCH(0, 16); }
+            else { if (nn) MR_ADJ3_LAUNCH(1, 4); else MR_ADJ3_LAUNCH(0, 4); }
+#undef MR_ADJ3_LAUNCH
+        } else {
 #define MR_ADJ2_LAUNCH(I, Z)                                                                                       \
     do {                                                                                                           \
         if (rr == 4) MR_ADJ2_LAUNCH_R(I, Z, 4);                                                                    \
@@ -1599,6 +1766,7 @@ extern "C" int mr_bp_parallel_adjoint(mr_bp_plan *plan, const float *d_grad_volu
         else { if (nn) MR_ADJ2_LAUNCH(1, 4); else MR_ADJ2_LAUNCH(0, 4); }
 #undef MR_ADJ2_LAUNCH_R
 #undef MR_ADJ2_LAUNCH
+        }
     } else {
         // ZG: groups of 4 slices per detector bin handled side by side
         const int nq = (int)(nz4 / 4);

@@ -25,13 +25,14 @@ def _case(seed, nu=64, nv=8, n_p=45):
     return rng, ang, w
 
 
-@pytest.mark.parametrize("impl", [1, 0])
+@pytest.mark.parametrize("impl", [1, 2, 0])
 @pytest.mark.parametrize("nu,nv,bufsize,interp", [(64, 8, 16, "linear"), (97, 12, 0, "linear"),
                                                   (64, 4, 16, "nearest"), (150, 6, 16, "linear"),
                                                   (200, 32, 0, "linear"), (120, 19, 16, "nearest")])
 def test_bp_adjoint_matches_oracle(orc, dev, monkeypatch, impl, nu, nv, bufsize, interp):
-    """impl 1: the chunked LDS kernel (default; 16-slice and 4-slice groups);
-    impl 0: the direct kernel."""
+    """impl 1: the chunked LDS kernel with one barrier per line (default;
+    32-, 16- and 4-slice groups); impl 2: the chunked kernel with per-line
+    barriers; impl 0: the direct kernel."""
     monkeypatch.setenv("MR_BP_ADJ", str(impl))
     from imagingsuite_amd.recon import BackProjector
     rng, ang, w = _case(nu + nv, nu, nv)
@@ -50,6 +51,24 @@ def test_bp_adjoint_matches_oracle(orc, dev, monkeypatch, impl, nu, nv, bufsize,
     print(f"BP^T impl={impl} {name} nu={nu} nv={nv}: vs FP64 {e[0]:.2e}/{e[1]:.2e}, dot {abs(lhs - rhs) / abs(lhs):.2e}")
     assert e[0] <= 1e-6
     assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+
+
+@pytest.mark.parametrize("nu,nv,bufsize,interp", [(97, 12, 0, "linear"), (200, 32, 16, "linear"),
+                                                  (120, 19, 16, "nearest"), (256, 64, 0, "linear")])
+def test_bp_adjoint_one_barrier_matches_chunked(dev, monkeypatch, nu, nv, bufsize, interp):
+    """The one-barrier-per-line adjoint (precomputed spans, double-buffered
+    staging) stages the same voxels at the same offsets and sums in the same
+    order as the per-line-group form (MR_BP_ADJ=2): identical bits."""
+    from imagingsuite_amd.recon import BackProjector
+    rng, ang, w = _case(nu + 3 * nv, nu, nv, 61)
+    name = "NearestNeighborBP" if interp == "nearest" else "MultiProjBPparallel"
+    bp = BackProjector(name, nu, nv, nu / 2 + 0.25, buffer_size=bufsize)
+    G = torch.from_numpy(rng.normal(size=(nu, nu, nv)).astype(np.float32)).to(dev)
+    fast = bp.adjoint(G, ang, w).cpu().numpy()
+    monkeypatch.setenv("MR_BP_ADJ", "2")
+    chunked = bp.adjoint(G, ang, w).cpu().numpy()
+    print(f"BP^T one-barrier vs chunked {name} nu={nu} nv={nv}: {int(np.sum(fast != chunked))} values differ")
+    assert np.array_equal(fast, chunked)
 
 
 @pytest.mark.parametrize("nu,ftype", [(64, "hamming"), (577, "hanning"), (2048, "ram-lak"), (30, "shepp-logan")])
