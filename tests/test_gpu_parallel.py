@@ -44,6 +44,31 @@ def test_filter_matches_oracle(orc, dev, width, ftype):
     assert err <= 2e-5 and rms <= 3e-6
 
 
+@pytest.mark.parametrize("mode", ["pb", "adjoint"])
+@pytest.mark.parametrize("width", [64, 300, 1024, 2048, 4096])
+def test_filter_thread_counts_identical(dev, monkeypatch, mode, width):
+    """The radix-8 filter with one butterfly per thread (P / 8 threads per
+    block, the default) performs the same FFT operations per element as with
+    P / 16 threads (MR_FILTER_NT=half): identical bits for the filter; the
+    adjoint folds its taper sums back onto the row ends with a block
+    reduction whose order follows the thread count, so there the two agree
+    to float rounding (1e-6 of the largest value)."""
+    from imagingsuite_amd.recon import ProjectionFilterSingle
+    rng = np.random.default_rng(width)
+    f = ProjectionFilterSingle(width, filtertype="hanning")
+    rows = rng.normal(size=(9, width)).astype(np.float32)
+    outs = []
+    for nt in ("half", "default"):
+        monkeypatch.setenv("MR_FILTER_NT", nt)
+        t = _gpu(rows, dev)
+        outs.append((f.process(t) if mode == "pb" else f.adjoint(t)).cpu().numpy())
+    print(f"filter P={f.fft_size} ({mode}), P/8 vs P/16 threads: {int(np.sum(outs[0] != outs[1]))} values differ")
+    if mode == "pb":
+        assert np.array_equal(outs[0], outs[1])
+    else:
+        assert np.abs(outs[0] - outs[1]).max() <= 1e-6 * np.abs(outs[0]).max()
+
+
 def _random_stack(rng, n_p, nv, nu):
     return rng.normal(size=(n_p, nv, nu)).astype(np.float32)
 
