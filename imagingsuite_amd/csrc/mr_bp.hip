@@ -55,6 +55,59 @@ __global__ __launch_bounds__(256) void bp_prep_kernel(const float *__restrict__ 
     }
 }
 
+// The same weighting and transposition with 64 x 64 tiles and 16-byte
+// global loads and stores (rows of a projection and of the sinogram are
+// read and written four floats at a time); width % 4 == 0.
+__global__ __launch_bounds__(256) void bp_prep4_kernel(const float *__restrict__ proj,
+                                                       float *__restrict__ sino,
+                                                       const int *__restrict__ idx,
+                                                       const float *__restrict__ wts, int nu, int nv,
+                                                       int vpad)
+{
+    __shared__ float tile[64][65];
+    const int k = blockIdx.z;
+    const int p = idx[k];
+    const float w = wts[p];
+    const int u0 = blockIdx.x * 64, v0 = blockIdx.y * 64;
+    const int c4 = threadIdx.x & 15, rr = threadIdx.x >> 4;   // 16 x 16
+    const float *src = proj + (size_t)p * nu * nv;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int r = rr + 16 * i, v = v0 + r, u = u0 + 4 * c4;
+        float4 x = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (v < nv && u < nu) x = *reinterpret_cast<const float4 *>(src + (size_t)v * nu + u);
+        tile[r][4 * c4 + 0] = x.x * w;
+        tile[r][4 * c4 + 1] = x.y * w;
+        tile[r][4 * c4 + 2] = x.z * w;
+        tile[r][4 * c4 + 3] = x.w * w;
+    }
+    __syncthreads();
+    float *dst = sino + (size_t)k * nu * vpad;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int r = rr + 16 * i, u = u0 + r, v = v0 + 4 * c4;
+        if (u < nu && v < vpad)
+            *reinterpret_cast<float4 *>(dst + (size_t)u * vpad + v) =
+                make_float4(tile[4 * c4][r], tile[4 * c4 + 1][r], tile[4 * c4 + 2][r], tile[4 * c4 + 3][r]);
+    }
+}
+
+// Weighting and transposition of the used projections into the sinogram.
+inline int launch_bp_prep(const float *proj, float *sino, const int *idx, const float *wts, int nu, int nv,
+                          int vpad, int nk, hipStream_t s)
+{
+    ProfScope prof("bp_prep", s);
+    if (nu % 4 == 0) {
+        dim3 g((unsigned)((nu + 63) / 64), (unsigned)((vpad + 63) / 64), (unsigned)nk);
+        hipLaunchKernelGGL(bp_prep4_kernel, g, dim3(256), 0, s, proj, sino, idx, wts, nu, nv, vpad);
+    } else {
+        dim3 g((unsigned)((nu + 31) / 32), (unsigned)((vpad + 31) / 32), (unsigned)nk);
+        hipLaunchKernelGGL(bp_prep_kernel, g, dim3(256), 0, s, proj, sino, idx, wts, nu, nv, vpad);
+    }
+    MR_HIP(hipGetLastError());
+    return MR_OK;
+}
+
 struct BpArgs {
     const float *sino;
     float *vol;
@@ -1570,11 +1623,9 @@ int bp_enqueue(mr_bp_plan *plan, const float *d_proj, float *d_volume, int accum
     const size_t nk = plan->loaded_nk;
     MR_REQUIRE(nk == 0 || d_proj != nullptr, "NULL projection pointer");
     if (nk > 0) {
-        dim3 g((unsigned)((q.width + 31) / 32), (unsigned)((plan->vpad + 31) / 32), (unsigned)nk);
-        mr::ProfScope prof("bp_prep", s);
-        hipLaunchKernelGGL(mr::bp_prep_kernel, g, dim3(256), 0, s, d_proj, plan->d_sino,
-                           plan->d_idx, plan->d_w, (int)q.width, (int)q.slices, (int)plan->vpad);
-        MR_HIP(hipGetLastError());
+        const int rc = mr::launch_bp_prep(d_proj, plan->d_sino, plan->d_idx, plan->d_w, (int)q.width,
+                                          (int)q.slices, (int)plan->vpad, (int)nk, s);
+        if (rc != MR_OK) return rc;
     }
     mr::BpArgs a{};
     a.sino = plan->d_sino; a.vol = d_volume; a.tab = plan->d_tab; a.mask = plan->d_mask;
@@ -1890,11 +1941,9 @@ int batch_prepare(mr_batch_plan *p, hipStream_t s)
         int rc = mr::filter_rows_to(b.fplan, d.d_proj, b.d_filt, d.n_proj * d.slices, s);
         if (rc != MR_OK) return rc;
         if (b.n_used == 0) continue;
-        dim3 g((unsigned)((d.width + 31) / 32), (unsigned)((b.vpad + 31) / 32), (unsigned)b.n_used);
-        mr::ProfScope prof("bp_prep", s);
-        hipLaunchKernelGGL(mr::bp_prep_kernel, g, dim3(256), 0, s, b.d_filt, b.d_sino, b.d_idx, b.d_w,
-                           (int)d.width, (int)d.slices, (int)b.vpad);
-        MR_HIP(hipGetLastError());
+        rc = mr::launch_bp_prep(b.d_filt, b.d_sino, b.d_idx, b.d_w, (int)d.width, (int)d.slices, (int)b.vpad,
+                                (int)b.n_used, s);
+        if (rc != MR_OK) return rc;
     }
     return MR_OK;
 }
