@@ -1061,7 +1061,10 @@ __global__ __launch_bounds__(256) void bp_adjoint_lds2_kernel(const float *__res
     const int zs = rest / n_chunks;
     const int4 ch = chunks[ci];                 // first, count, class
     const int tid = (int)threadIdx.x;
-    const int ai = tid / NB, bi = tid % NB;
+    // lanes 16 q .. 16 q + 15 of a wave take 4 projections x 4 neighbouring
+    // bins, whose candidates lie within a few staged rows (one projection's
+    // 16 bins spread over about 20 rows, and rows 16 apart share LDS banks)
+    const int ai = ((tid >> 2) & 3) | ((tid >> 4) & 12), bi = (tid & 3) | ((tid >> 2) & 12);
     const int u = bt * NB + bi;
     const bool active = ai < ch.y && u < nu;
     const int k = order[ch.x + min(ai, ch.y - 1)];   // used-projection index

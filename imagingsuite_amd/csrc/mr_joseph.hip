@@ -284,7 +284,10 @@ __global__ __launch_bounds__(256) void joseph_forward_lds2_kernel(JosephArgs a, 
     const int zs = rest / n_chunks;
     const int4 ch = chunks[ci];                // first, count, class
     const int tid = (int)threadIdx.x;
-    const int ai = tid / NB, bi = tid % NB;
+    // lanes 16 q .. 16 q + 15 of a wave take 4 angles x 4 neighbouring bins,
+    // whose crossings of a line lie within a few staged rows: one angle's 16
+    // bins spread over about 20 rows, and rows 16 apart share LDS banks
+    const int ai = ((tid >> 2) & 3) | ((tid >> 4) & 12), bi = (tid & 3) | ((tid >> 2) & 12);
     const int n = bt * NB + bi;
     const bool active = ai < ch.y && n < a.nd;
     const int ang = order[ch.x + min(ai, ch.y - 1)];
