@@ -117,6 +117,7 @@ struct BpArgs {
     int order;           // block -> tile mapping (tile_of_block)
     int ntx, nty, ntz;   // tiles along x, y and slice chunks
     int sw;              // stripe width (tiles) of the XCD-striped order
+    int stage64;         // 1: stage with 64-bit addresses even when 32-bit offsets fit (tests)
 };
 
 // Block -> tile mapping of the tiled kernels.  order 0: the 3-D grid itself.
@@ -276,7 +277,7 @@ __device__ __forceinline__ void bp_lds_tile(const BpArgs &a, int bx, int by, int
         // large for 32-bit offsets take the general loop.
         constexpr int NE = (KC * SPAN * G + 255) / 256;
         const int vpad4 = 4 * a.vpad;
-        const bool off32 = (long)KC * (a.nu + SPAN) * vpad4 < 0x7FFFFFF0L;
+        const bool off32 = !a.stage64 && (long)KC * (a.nu + SPAN) * vpad4 < 0x7FFFFFF0L;
         int st_kk[NE], st_b[NE], st_lds[NE];
         unsigned st_off[NE];
 #pragma unroll
@@ -1636,6 +1637,8 @@ int bp_enqueue(mr_bp_plan *plan, const float *d_proj, float *d_volume, int accum
     a.nz4 = (int)((q.slices / 4) * 4);
     a.bstride = q.multiprojbp ? a.nz4 : (int)plan->vpad;
     a.accumulate = accumulate ? 1 : 0;
+    // MR_BP_STAGE64=1: the 64-bit staging path that chunks beyond 32-bit offsets take
+    a.stage64 = mr::env_int("MR_BP_STAGE64", 0) == 1 ? 1 : 0;
     return mr::run_bp(a, q.interp, plan->impl, q.fused ? 0 : 1, s);
 }
 

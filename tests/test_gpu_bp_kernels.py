@@ -86,3 +86,19 @@ def test_variant_fused_matches_lds16(orc, dev, monkeypatch, impl, order, zt, kc,
           f"{int(np.sum(vol != base))} voxels differ from the 16x16 kernel")
     assert err <= 1e-6
     assert np.array_equal(vol, base)
+
+
+@pytest.mark.parametrize("zt,kc,pipe", [(32, 4, 1), (16, 8, 1), (16, 16, 0)])
+@pytest.mark.parametrize("nu,nv,n_p,center", [(100, 30, 45, 49.5), (64, 17, 20, 31.25)])
+def test_stage64_path_bitexact(orc, dev, monkeypatch, zt, kc, pipe, nu, nv, n_p, center):
+    """MR_BP_STAGE64=1 forces the staging with 64-bit addresses that chunks
+    too large for 32-bit offsets take (never needed at sizes that fit one
+    GPU's memory): bit-exact against the oracle like the default path."""
+    _set_variant(monkeypatch, 1, 1, zt, kc, pipe)
+    monkeypatch.setenv("MR_BP_STAGE64", "1")
+    proj, ang, w = _case(nu + nv + n_p + 1, nu, nv, n_p)
+    vol = _run(dev, proj, ang, w, nu, nv, center, fused=False)
+    ref = orc.bp_parallel_f32(proj, ang, w, center, 0, "linear", 16, False)
+    nbad = int(np.sum(vol != ref))
+    print(f"stage64 zt={zt} kc={kc} pipe={pipe} {nu}x{nv}x{n_p}: {nbad} voxels differ")
+    assert nbad == 0
