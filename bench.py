@@ -249,7 +249,8 @@ def stream_leg(pristine, angles, weights, cfg, center, buf, nu, nv, n_p, updates
     hp = host_empty(tuple(pristine.shape))
     hp[...] = pristine.cpu().numpy()
     ho = host_empty((nv, nu, nu))
-    st = FBPStream(nu, nv, n_p, center, slice_block=32, buffer_size=buf, fused=True, filtertype=cfg["filtertype"])
+    sb = int(os.environ.get("MR_BENCH_STREAM_SB", "32"))
+    st = FBPStream(nu, nv, n_p, center, slice_block=sb, buffer_size=buf, fused=True, filtertype=cfg["filtertype"])
     out = {}
     reps = 5
 
@@ -274,6 +275,7 @@ def stream_leg(pristine, angles, weights, cfg, center, buf, nu, nv, n_p, updates
     st.close()
     out["bytes_h2d"] = 4 * n_p * nv * nu
     out["bytes_d2h"] = 4 * nv * nu * nu
+    out["slice_block"] = sb
     out["note"] = ("whole FBP with the host-to-device copy of the stack and the device-to-host copy of the slices; "
                    f"median of {reps} runs after one warm-up")
     return out
