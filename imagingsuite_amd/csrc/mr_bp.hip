@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -1473,6 +1474,9 @@ struct mr_bp_plan {
     bool loaded = false;
     void *d_adj = nullptr;                // adjoint chunks (int4) and projection order (int)
     size_t adj_cap = 0;                   // bytes
+    void *h_stage = nullptr;              // pinned host staging of the tables: tab | idx | w
+    size_t h_stage_cap = 0;               // bytes
+    hipEvent_t stage_ev = nullptr;        // recorded after the last upload from h_stage
 };
 
 namespace {
@@ -1563,6 +1567,11 @@ extern "C" int mr_bp_plan_destroy(mr_bp_plan *plan)
     if (plan->d_idx) (void)hipFree(plan->d_idx);
     if (plan->d_w) (void)hipFree(plan->d_w);
     if (plan->d_adj) (void)hipFree(plan->d_adj);
+    if (plan->stage_ev) {
+        (void)hipEventSynchronize(plan->stage_ev);
+        (void)hipEventDestroy(plan->stage_ev);
+    }
+    if (plan->h_stage) (void)hipHostFree(plan->h_stage);
     delete plan;
     return MR_OK;
     MR_GUARD_END
@@ -1606,13 +1615,29 @@ int bp_load_tables(mr_bp_plan *plan, size_t n_proj, const float *angles, const f
     if (nk > 0) {
         std::vector<float> tab;
         mr::bp_tables(angles, used.data(), nk, q.direction, q.center, q.width / 2, tab);
-        std::vector<int> idx(nk);
-        for (size_t k = 0; k < nk; ++k) idx[k] = (int)used[k];
-        MR_HIP(hipMemcpyAsync(plan->d_tab, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice, s));
-        MR_HIP(hipMemcpyAsync(plan->d_idx, idx.data(), sizeof(int) * nk, hipMemcpyHostToDevice, s));
-        MR_HIP(hipMemcpyAsync(plan->d_w, weights, sizeof(float) * n_proj, hipMemcpyHostToDevice, s));
-        // the host tables are temporaries
-        MR_HIP(hipStreamSynchronize(s));
+        // The tables go through the plan's pinned staging buffer, so the
+        // copies are asynchronous and the stream is not drained here (the
+        // filter before and the back-projection after stay back to back);
+        // the previous upload from that buffer must have finished reading it.
+        const size_t bt = sizeof(float) * tab.size(), bi = sizeof(int) * nk, bw = sizeof(float) * n_proj;
+        if (plan->stage_ev) MR_HIP(hipEventSynchronize(plan->stage_ev));
+        else MR_HIP(hipEventCreateWithFlags(&plan->stage_ev, hipEventDisableTiming));
+        if (bt + bi + bw > plan->h_stage_cap) {
+            if (plan->h_stage) (void)hipHostFree(plan->h_stage);
+            plan->h_stage = nullptr;
+            plan->h_stage_cap = 0;
+            MR_HIP(hipHostMalloc(&plan->h_stage, bt + bi + bw, hipHostMallocDefault));
+            plan->h_stage_cap = bt + bi + bw;
+        }
+        char *h = static_cast<char *>(plan->h_stage);
+        std::memcpy(h, tab.data(), bt);
+        int *hi = reinterpret_cast<int *>(h + bt);
+        for (size_t k = 0; k < nk; ++k) hi[k] = (int)used[k];
+        std::memcpy(h + bt + bi, weights, bw);
+        MR_HIP(hipMemcpyAsync(plan->d_tab, h, bt, hipMemcpyHostToDevice, s));
+        MR_HIP(hipMemcpyAsync(plan->d_idx, h + bt, bi, hipMemcpyHostToDevice, s));
+        MR_HIP(hipMemcpyAsync(plan->d_w, h + bt + bi, bw, hipMemcpyHostToDevice, s));
+        MR_HIP(hipEventRecord(plan->stage_ev, s));
     }
     plan->loaded_n = n_proj;
     plan->loaded_nk = nk;
