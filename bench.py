@@ -190,6 +190,8 @@ def bench_parallel(args):
         pr_ms, pr_n = _lib.profile_query("bp_prep")
         _lib.profile(False)
         assert torch.isfinite(vol).all().item()
+        first, second = bp.mask
+        mask_voxels = int(np.maximum(second - first, 0).sum())
         res[fused] = dict(ms=ms, bp=bp_ms / max(bp_n, 1), filt=fl_ms / max(fl_n, 1), prep=pr_ms / max(pr_n, 1),
                           kernel=bp.kernel)
         del bp
@@ -214,6 +216,10 @@ def bench_parallel(args):
                     round(gups, 2), "GUPS", args.steps, args.warmup, head["ms"], "f32",
                     {"workload": cfg["name"], "width": nu, "slices": nv, "n_proj": n_p,
                      "n_proj_backprojected": n_used, "mode": args.mode, "center": center,
+                     # MuhRec's ProjectionBufferSize 16 drops every 16th projection (the compat mode)
+                     "muhrec_used_projections": n_p - n_p // 16,
+                     "updates_in_mask": mask_voxels * nv * n_used,
+                     "in_mask_fraction": round(mask_voxels / float(nu * nu), 4),
                      "filter": cfg["filtertype"], "slice_block": nv, "input": input_note,
                      "precision": "fused" if True in res else "exact"})
     r["roofline"] = {"bound": "mfma", "pipe": "fp32 VALU (no MFMA: not a contraction)",
