@@ -431,17 +431,25 @@ void orc_bp_parallel_f32(const float *proj, size_t nu, size_t nv, size_t np,
                 buf[((size_t)k * nu + u) * vpad + v] = p[v * nu + u] * w;
     }
     memset(vol, 0, sizeof(float) * nu * nu * nv);
+    /* projections in chunks of 16, each back-projected over the whole slice
+     * block before the next (the reference's ProjectionBufferSize loop,
+     * StdBackProjectorBase.cpp:104-114, MultiProjBPparallel.cpp:115-167):
+     * the chunk's columns stay in cache; every voxel still adds its
+     * projections in increasing order */
+    const size_t CH = 16;
+    for (size_t k0 = 0; k0 < nused; k0 += CH) {
+    const size_t k1 = nused < k0 + CH ? nused : k0 + CH;
 #pragma omp parallel
     {
-        float *lsu = (float *)malloc(sizeof(float) * (nused + 1));
+        float lsu[16];
 #pragma omp for schedule(dynamic, 4)
         for (long y = 1; y <= (long)nu; ++y) {
             const size_t x0 = mfirst[y - 1], x1 = msecond[y - 1];
-            for (size_t k = 0; k < nused; ++k) lsu[k] = fsu[used[k]] + fcos[used[k]] * (float)y;
+            for (size_t k = k0; k < k1; ++k) lsu[k - k0] = fsu[used[k]] + fcos[used[k]] * (float)y;
             for (size_t x = x0 + 1; x <= x1; ++x) {
                 float *col = vol + ((size_t)(y - 1) * nu + (x - 1)) * nv;
-                for (size_t k = 0; k < nused; ++k) {
-                    float u = lsu[k] - fsin[used[k]] * (float)x;
+                for (size_t k = k0; k < k1; ++k) {
+                    float u = lsu[k - k0] - fsin[used[k]] * (float)x;
                     int n = (int)u;
                     if ((n < 0) || (sizeUm2 < n)) continue;
                     const float *A = buf + ((size_t)k * nu + (size_t)n) * vpad;
@@ -461,7 +469,7 @@ void orc_bp_parallel_f32(const float *proj, size_t nu, size_t nv, size_t np,
                 }
             }
         }
-        free(lsu);
+    }
     }
     free(buf); free(used); free(fsin); free(fcos); free(fsu);
 }
