@@ -563,7 +563,7 @@ def main():
     ap.add_argument("--no-stream", action="store_true",
                     help="skip the host-resident streaming leg (PCIe-inclusive timing)")
     ap.add_argument("--cpu-sample-proj", type=int, default=1500)
-    ap.add_argument("--cpu-sample-slices", type=int, default=96)
+    ap.add_argument("--cpu-sample-slices", type=int, default=192)
     ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
