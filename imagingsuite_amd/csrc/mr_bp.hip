@@ -1477,6 +1477,7 @@ struct mr_bp_plan {
     void *h_stage = nullptr;              // pinned host staging of the tables: tab | idx | w
     size_t h_stage_cap = 0;               // bytes
     hipEvent_t stage_ev = nullptr;        // recorded after the last upload from h_stage
+    hipEvent_t done_ev = nullptr;         // recorded after the last mr_bp_parallel's kernels
 };
 
 namespace {
@@ -1484,6 +1485,10 @@ namespace {
 int ensure_capacity(mr_bp_plan *p, size_t n_proj, size_t n_used)
 {
     const size_t need = std::max<size_t>(n_used, 1) * p->p.width * p->vpad;
+    // a buffer about to be replaced may still be read by the last call's kernels
+    if (p->done_ev && (need > p->sino_cap || std::max<size_t>(n_used, 1) > p->tab_cap ||
+                       std::max<size_t>(n_proj, 1) > p->idx_cap))
+        MR_HIP(hipEventSynchronize(p->done_ev));
     if (need > p->sino_cap) {
         if (p->d_sino) (void)hipFree(p->d_sino);
         p->d_sino = nullptr;
@@ -1561,6 +1566,7 @@ extern "C" int mr_bp_plan_destroy(mr_bp_plan *plan)
 {
     MR_GUARD_BEGIN
     if (!plan) return MR_OK;
+    if (plan->done_ev) (void)hipEventSynchronize(plan->done_ev);
     if (plan->d_mask) (void)hipFree(plan->d_mask);
     if (plan->d_sino) (void)hipFree(plan->d_sino);
     if (plan->d_tab) (void)hipFree(plan->d_tab);
@@ -1571,6 +1577,7 @@ extern "C" int mr_bp_plan_destroy(mr_bp_plan *plan)
         (void)hipEventSynchronize(plan->stage_ev);
         (void)hipEventDestroy(plan->stage_ev);
     }
+    if (plan->done_ev) (void)hipEventDestroy(plan->done_ev);
     if (plan->h_stage) (void)hipHostFree(plan->h_stage);
     delete plan;
     return MR_OK;
@@ -1612,6 +1619,9 @@ int bp_load_tables(mr_bp_plan *plan, size_t n_proj, const float *angles, const f
     plan->loaded = false;
     int rc = ensure_capacity(plan, n_proj, nk);
     if (rc != MR_OK) return rc;
+    // the tables and the staged sinogram are overwritten below: on a stream
+    // other than the last call's, wait for that call's kernels first
+    if (plan->done_ev) MR_HIP(hipStreamWaitEvent(s, plan->done_ev, 0));
     if (nk > 0) {
         std::vector<float> tab;
         mr::bp_tables(angles, used.data(), nk, q.direction, q.center, q.width / 2, tab);
@@ -1686,7 +1696,9 @@ extern "C" int mr_bp_parallel(mr_bp_plan *plan, const float *d_proj, size_t n_pr
     if (rc != MR_OK) return rc;
     rc = mr::bp_enqueue(plan, d_proj, d_volume, accumulate, s);
     if (rc != MR_OK) return rc;
-    MR_HIP(hipStreamSynchronize(s));
+    // no host wait: the next call on this plan (any stream) waits for this event
+    if (!plan->done_ev) MR_HIP(hipEventCreateWithFlags(&plan->done_ev, hipEventDisableTiming));
+    MR_HIP(hipEventRecord(plan->done_ev, s));
     return MR_OK;
     MR_GUARD_END
 }
@@ -1783,6 +1795,7 @@ extern "C" int mr_bp_parallel_adjoint(mr_bp_plan *plan, const float *d_grad_volu
     }
     rc = ensure_capacity(plan, n_proj, nk);
     if (rc != MR_OK) return rc;
+    if (plan->done_ev) MR_HIP(hipStreamWaitEvent(s, plan->done_ev, 0));
     plan->loaded = false;   // the tables below replace the forward ones
     std::vector<float> tab;
     mr::bp_tables(angles, used.data(), nk, q.direction, q.center, q.width / 2, tab);

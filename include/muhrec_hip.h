@@ -126,7 +126,11 @@ int mr_bp_plan_mask(const mr_bp_plan *plan, size_t *first, size_t *second);
 /* Back-projects n_proj filtered projections d_proj [n_proj][slices][width]
  * (the reference's TImage<float,3> layout) with host angles (degrees) and
  * weights into d_volume [width(y)][width(x)][slices] (MatrixZXY).
- * accumulate = 0 overwrites the volume (SetROI zeroes it); 1 adds to it. */
+ * accumulate = 0 overwrites the volume (SetROI zeroes it); 1 adds to it.
+ * The kernels are enqueued on `stream` and the call returns without waiting
+ * for them (angles and weights are copied before it returns); the next call
+ * on the same plan waits for them on the device, whatever its stream.
+ * mr_fbp_parallel behaves the same way. */
 int mr_bp_parallel(mr_bp_plan *plan, const float *d_proj, size_t n_proj, const float *angles,
                    const float *weights, float *d_volume, int accumulate, void *stream);
 /* Adjoint of mr_bp_parallel (the project's own operator for gradients; no
