@@ -1464,11 +1464,13 @@ struct mr_bp_plan {
     int2 *d_mask = nullptr;
     float *d_sino = nullptr;
     size_t sino_cap = 0;   // floats
+    // d_tab | d_idx | d_w live in one allocation, laid out as the pinned
+    // staging buffer, so one copy uploads all three (table_layout)
+    void *d_tables = nullptr;
+    size_t tables_cap = 0; // bytes
     float4 *d_tab = nullptr;
-    size_t tab_cap = 0;    // entries
     int *d_idx = nullptr;
     float *d_w = nullptr;
-    size_t idx_cap = 0;
     int impl = 1;  // 1 LDS-staged kernel, 0 direct-gather kernel
     size_t loaded_n = 0, loaded_nk = 0;   // projections / used projections of the resident tables
     bool loaded = false;
@@ -1482,12 +1484,27 @@ struct mr_bp_plan {
 
 namespace {
 
+// Byte offsets of the used-projection indices and of the weights after the
+// nt table entries (float4), and the total size for n_proj weights.
+struct TableLayout {
+    size_t idx, w, bytes;
+};
+inline TableLayout table_layout(size_t nt, size_t n_proj)
+{
+    TableLayout l;
+    l.idx = sizeof(float4) * nt;
+    l.w = l.idx + sizeof(int) * ((nt + 3) & ~size_t(3));
+    l.bytes = l.w + sizeof(float) * n_proj;
+    return l;
+}
+
 int ensure_capacity(mr_bp_plan *p, size_t n_proj, size_t n_used)
 {
     const size_t need = std::max<size_t>(n_used, 1) * p->p.width * p->vpad;
+    const size_t nt = std::max<size_t>(n_used, 1);
+    const TableLayout tl = table_layout(nt, std::max<size_t>(n_proj, 1));
     // a buffer about to be replaced may still be read by the last call's kernels
-    if (p->done_ev && (need > p->sino_cap || std::max<size_t>(n_used, 1) > p->tab_cap ||
-                       std::max<size_t>(n_proj, 1) > p->idx_cap))
+    if (p->done_ev && (need > p->sino_cap || tl.bytes > p->tables_cap))
         MR_HIP(hipEventSynchronize(p->done_ev));
     if (need > p->sino_cap) {
         if (p->d_sino) (void)hipFree(p->d_sino);
@@ -1496,25 +1513,17 @@ int ensure_capacity(mr_bp_plan *p, size_t n_proj, size_t n_used)
         MR_HIP(hipMalloc(&p->d_sino, need * sizeof(float)));
         p->sino_cap = need;
     }
-    const size_t nt = std::max<size_t>(n_used, 1);
-    if (nt > p->tab_cap) {
-        if (p->d_tab) (void)hipFree(p->d_tab);
-        p->d_tab = nullptr;
-        p->tab_cap = 0;
-        MR_HIP(hipMalloc(&p->d_tab, nt * sizeof(float4)));
-        p->tab_cap = nt;
+    if (tl.bytes > p->tables_cap) {
+        if (p->d_tables) (void)hipFree(p->d_tables);
+        p->d_tables = nullptr;
+        p->tables_cap = 0;
+        MR_HIP(hipMalloc(&p->d_tables, tl.bytes));
+        p->tables_cap = tl.bytes;
     }
-    const size_t ni = std::max<size_t>(n_proj, 1);
-    if (ni > p->idx_cap) {
-        if (p->d_idx) (void)hipFree(p->d_idx);
-        if (p->d_w) (void)hipFree(p->d_w);
-        p->d_idx = nullptr;
-        p->d_w = nullptr;
-        p->idx_cap = 0;
-        MR_HIP(hipMalloc(&p->d_idx, ni * sizeof(int)));
-        MR_HIP(hipMalloc(&p->d_w, ni * sizeof(float)));
-        p->idx_cap = ni;
-    }
+    char *b = static_cast<char *>(p->d_tables);
+    p->d_tab = reinterpret_cast<float4 *>(b);
+    p->d_idx = reinterpret_cast<int *>(b + tl.idx);
+    p->d_w = reinterpret_cast<float *>(b + tl.w);
     return MR_OK;
 }
 
@@ -1569,9 +1578,7 @@ extern "C" int mr_bp_plan_destroy(mr_bp_plan *plan)
     if (plan->done_ev) (void)hipEventSynchronize(plan->done_ev);
     if (plan->d_mask) (void)hipFree(plan->d_mask);
     if (plan->d_sino) (void)hipFree(plan->d_sino);
-    if (plan->d_tab) (void)hipFree(plan->d_tab);
-    if (plan->d_idx) (void)hipFree(plan->d_idx);
-    if (plan->d_w) (void)hipFree(plan->d_w);
+    if (plan->d_tables) (void)hipFree(plan->d_tables);
     if (plan->d_adj) (void)hipFree(plan->d_adj);
     if (plan->stage_ev) {
         (void)hipEventSynchronize(plan->stage_ev);
@@ -1629,24 +1636,25 @@ int bp_load_tables(mr_bp_plan *plan, size_t n_proj, const float *angles, const f
         // copies are asynchronous and the stream is not drained here (the
         // filter before and the back-projection after stay back to back);
         // the previous upload from that buffer must have finished reading it.
-        const size_t bt = sizeof(float) * tab.size(), bi = sizeof(int) * nk, bw = sizeof(float) * n_proj;
+        // Staging and device buffer share one layout (table_layout), so a
+        // single copy uploads the table, the indices and the weights.
+        const TableLayout tl = table_layout(nk, n_proj);
         if (plan->stage_ev) MR_HIP(hipEventSynchronize(plan->stage_ev));
         else MR_HIP(hipEventCreateWithFlags(&plan->stage_ev, hipEventDisableTiming));
-        if (bt + bi + bw > plan->h_stage_cap) {
+        if (tl.bytes > plan->h_stage_cap) {
             if (plan->h_stage) (void)hipHostFree(plan->h_stage);
             plan->h_stage = nullptr;
             plan->h_stage_cap = 0;
-            MR_HIP(hipHostMalloc(&plan->h_stage, bt + bi + bw, hipHostMallocDefault));
-            plan->h_stage_cap = bt + bi + bw;
+            MR_HIP(hipHostMalloc(&plan->h_stage, tl.bytes, hipHostMallocDefault));
+            plan->h_stage_cap = tl.bytes;
         }
         char *h = static_cast<char *>(plan->h_stage);
-        std::memcpy(h, tab.data(), bt);
-        int *hi = reinterpret_cast<int *>(h + bt);
+        std::memcpy(h, tab.data(), sizeof(float) * tab.size());
+        int *hi = reinterpret_cast<int *>(h + tl.idx);
         for (size_t k = 0; k < nk; ++k) hi[k] = (int)used[k];
-        std::memcpy(h + bt + bi, weights, bw);
-        MR_HIP(hipMemcpyAsync(plan->d_tab, h, bt, hipMemcpyHostToDevice, s));
-        MR_HIP(hipMemcpyAsync(plan->d_idx, h + bt, bi, hipMemcpyHostToDevice, s));
-        MR_HIP(hipMemcpyAsync(plan->d_w, h + bt + bi, bw, hipMemcpyHostToDevice, s));
+        for (size_t k = nk; k < (tl.w - tl.idx) / sizeof(int); ++k) hi[k] = 0;
+        std::memcpy(h + tl.w, weights, sizeof(float) * n_proj);
+        MR_HIP(hipMemcpyAsync(plan->d_tables, h, tl.bytes, hipMemcpyHostToDevice, s));
         MR_HIP(hipEventRecord(plan->stage_ev, s));
     }
     plan->loaded_n = n_proj;
