@@ -45,6 +45,29 @@ def test_used_projections_drop(orc):
     assert orc.used_projections(7, 0).size == 7
 
 
+def _process_loop_used(n_p, buf):
+    """Which projections reach BackProject, by stepping the slot counter of
+    StdBackProjectorBase::Process(2D) (SBP/src/StdBackProjectorBase.cpp:
+    104-114) one projection at a time: slot c takes the projection, c++;
+    on the last projection or c == buf, c == buf first discards slot buf-1's
+    projection (c--), then the c slots are back-projected and c = 0."""
+    used, slots = [], []
+    for i in range(n_p):
+        slots.append(i)
+        if i == n_p - 1 or len(slots) == buf:
+            if len(slots) == buf:
+                slots.pop()
+            used += slots
+            slots = []
+    return used
+
+
+@pytest.mark.parametrize("n_p", [1, 2, 15, 16, 17, 31, 32, 33, 48, 180])
+def test_used_projections_match_process_loop(orc, n_p):
+    assert orc.used_projections(n_p, 16).tolist() == _process_loop_used(n_p, 16)
+    assert orc.used_projections(n_p, 0).tolist() == list(range(n_p))
+
+
 @pytest.mark.parametrize("nv", [8, 6])
 def test_constant_sinogram(orc, nv):
     nu, n_p = 32, 40
