@@ -288,17 +288,6 @@ def stream_leg(pristine, angles, weights, cfg, center, buf, nu, nv, n_p, updates
 
 
 # --------------------------------------------------------------------------
-def fdk_matrix(sod, sdd, pix, angle, off_center=0.0, direction=0):
-    """FDKbp_single projection matrix in closed form (double), for input synthesis."""
-    a = (-angle if direction == 0 else angle) * math.pi / 180.0
-    nrm = np.array([math.cos(a), math.sin(a), 0.0])
-    plt = np.array([nrm[1], -nrm[0], 0.0])
-    pup = np.cross(plt, nrm)
-    return np.array([[*(-plt / pix[0]), -off_center / pix[0]],
-                     [*(-pup / pix[1]), 0.0],
-                     [*(-nrm / sdd), sod / sdd]])
-
-
 def bench_fdk(args):
     import torch
     from imagingsuite_amd import _lib, phantom
@@ -322,7 +311,7 @@ def bench_fdk(args):
     ic = (ppoint[0] - cb[0], ppoint[1] - cb[1])
     t = time.perf_counter()
     proj = phantom.cone_projections(ells, n, cb[3] - cb[1], angles,
-                                    lambda a: fdk_matrix(sod, sdd, pix, a), ic, n * vox[0] * 0.8, device=dev)
+                                    lambda a: phantom.fdk_matrix(sod, sdd, pix, a), ic, n * vox[0] * 0.8, device=dev)
     proj = phantom.add_poisson_log(proj, 2e4, seed=3)
     torch.cuda.synchronize()
     log(f"[bench] cone-beam projections {tuple(proj.shape)} in {time.perf_counter() - t:.1f} s")

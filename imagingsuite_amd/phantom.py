@@ -139,6 +139,23 @@ def voxelize(ells: Sequence[Ellipsoid], width: int, slices: int, device="cpu"):
     return vol.to(torch.float32)
 
 
+def fdk_matrix(sod: float, sdd: float, pix, angle: float, off_center: float = 0.0,
+               direction: int = 0) -> np.ndarray:
+    """FDKbp_single's 3x4 projection matrix (fdkbp_single.cpp:83-255) in closed
+    form (double): source at SOD (cos a, sin a, 0) with a = -angle for CW and
+    +angle for CCW, nrm = source direction, plt = nrm x z, pup = plt x nrm;
+    rows -plt/pix_x (translation -off_center/pix_x), -pup/pix_y, -nrm/SDD
+    (translation SOD/SDD).  A world point X maps to detector column
+    ic0 + p0/p2 and row ic1 + p1/p2 with (p0, p1, p2) = P (X, 1)."""
+    a = (-angle if direction == 0 else angle) * math.pi / 180.0
+    nrm = np.array([math.cos(a), math.sin(a), 0.0])
+    plt = np.array([nrm[1], -nrm[0], 0.0])
+    pup = np.cross(plt, nrm)
+    return np.array([[*(-plt / pix[0]), -off_center / pix[0]],
+                     [*(-pup / pix[1]), 0.0],
+                     [*(-nrm / sdd), sod / sdd]])
+
+
 def cone_projections(ells: Sequence[Ellipsoid], n_det_u: int, n_det_v: int, angles_deg, pm_fn,
                      ic, fov_mm: float, device="cpu", dtype=None):
     """Noise-free cone-beam line integrals [n_proj, n_det_v, n_det_u] (mu in

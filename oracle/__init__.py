@@ -523,27 +523,51 @@ def _text6(x: float) -> float:
 
 
 def sequential_angles(first_index: int, last_index: int, arc0: float, arc1: float,
-                      step: int = 1):
+                      step: int = 1, skip_list=()):
     """SequentialScan branch of BuildFileList (ReconHelpers.cpp:77-98): float
-    angles, keyed by fmod(angle, 180)."""
-    astep = _f32(_f32(arc1 - arc0) / _f32(last_index - first_index))
+    angles, keyed by fmod(angle, 180).  An index on the skip list advances
+    the loop index before the angle is formed (:83-93)."""
+    with np.errstate(divide="ignore", invalid="ignore"):   # one projection: x/0 = inf, 0*inf = NaN
+        astep = float(np.float32(_f32(arc1) - _f32(arc0)) / np.float32(last_index - first_index))
+    skip = set(skip_list)
     out = []
-    for i in range(first_index, last_index + 1, step):
-        ang = _f32(_f32((i - first_index) * astep) + _f32(arc0))
+    i = first_index
+    while i <= last_index:
+        while i in skip:
+            i += 1
+        ang = _f32(_f32((i - first_index) * astep) + _f32(arc0)) if math.isfinite(astep) else math.nan
         out.append(ang)
+        i += step
     return out
 
 
-def golden_angles(first_index: int, last_index: int, arc: float, golden_start: int = 0):
-    """GoldenSectionScan branch of BuildFileList (ReconHelpers.cpp:100-155),
-    no skip list."""
+def golden_angles(first_index: int, last_index: int, arc: float, golden_start: int = 0,
+                  skip_list=()):
+    """GoldenSectionScan branch of BuildFileList (ReconHelpers.cpp:100-155):
+    fmod((i - nGoldenStartIdx - skip) * phi * arc, arc) in float for
+    i = first .. last + skip, skip counting the skip-list hits; the index
+    difference is size_t arithmetic (wraps below zero)."""
     if arc not in (180.0, 360.0):
         raise ValueError("The golden ratio reconstruction requires arc to be 180 or 360 degrees")
     phi = np.float32(0.5) * (np.float32(1.0) + np.sqrt(np.float32(5.0), dtype=np.float32))
+    sl = set(skip_list)
+    skip = 0
+    i = 0
+    while i < first_index:
+        if i in sl:
+            i += 1
+            skip += 1
+        i += 1
     out = []
-    for i in range(first_index, last_index + 1):
-        x = np.float32(i - golden_start) * phi * np.float32(arc)
+    i = first_index
+    while i <= last_index + skip:
+        while i in sl:
+            i += 1
+            skip += 1
+        idx = (i - golden_start - skip) % (1 << 64)
+        x = np.float32(idx) * phi * np.float32(arc)
         out.append(float(np.float32(math.fmod(float(x), arc))))
+        i += 1
     return out
 
 
@@ -559,10 +583,12 @@ def compute_weights(angles, arc0: float, arc1: float, golden: bool = False):
     W = [0.0] * len(K)
     n = len(K)
     if n < 3:
+        # inserted into the output map under the fmod key (ReconHelpers.cpp:366-372):
+        # the first of two equal keys stays, and the reader walks the keys in order
         res = {}
         for i in range(n):
-            res[A[i]] = _f32(1.0 / n)
-        items = sorted(res.items())
+            res.setdefault(K[i], (A[i], _f32(1.0 / n)))
+        items = [res[k] for k in sorted(res)]
         return [a for a, _ in items], [w for _, w in items]
     for q1 in range(1, n - 1):
         q0, q2 = q1 - 1, q1 + 1
@@ -604,18 +630,26 @@ def compute_weights(angles, arc0: float, arc1: float, golden: bool = False):
     for i in range(n):
         res[A[i]] = W[i]
         tot = _f32(tot + W[i])
-    if abs(tot - 0.5) < 0.001:
+    if abs(_f32(tot - 0.5)) < _f32(0.001):
         res = {a: _f32(w * 2) for a, w in res.items()}
     items = sorted(res.items())
     return [a for a, _ in items], [w for _, w in items]
 
 
 def scan_angles_weights(n_proj: int, arc0: float, arc1: float, resolution_mm: float = 0.1,
-                        rotation: float = 0.0, first_index: int = 0):
+                        rotation: float = 0.0, first_index: int = 0, scan: str = "sequential",
+                        golden_start: int = 0, step: int = 1, skip_list=(), angle_list=None):
     """Angles and weights exactly as the back-projector receives them: sorted
-    by angle, weight x 1/(0.1*resolution), both through 6-digit text."""
-    ang = sequential_angles(first_index, first_index + n_proj - 1, arc0, arc1)
-    a, w = compute_weights(ang, arc0, arc1)
+    by angle, weight x 1/(0.1*resolution), both through 6-digit text.
+    ``angle_list`` stands for a list file's angles (acquisition order)."""
+    last = first_index + n_proj - 1
+    if angle_list is not None:
+        ang = [_f32(a) for a in angle_list]
+    elif scan == "golden":
+        ang = golden_angles(first_index, last, arc1, golden_start, skip_list)
+    else:
+        ang = sequential_angles(first_index, last, arc0, arc1, step, skip_list)
+    a, w = compute_weights(ang, arc0, arc1, golden=(scan == "golden" and angle_list is None))
     rw = float(np.float32(1.0) / (np.float32(resolution_mm) * np.float32(0.1) if resolution_mm > 0 else np.float32(1.0)))
     angles = np.array([_text6(float(np.float32(x) + np.float32(rotation))) for x in a], np.float32)
     weights = np.array([_text6(float(np.float32(x) * np.float32(rw))) for x in w], np.float32)

@@ -83,9 +83,6 @@ def _fdk_setup(orc, dev, slab, shepp_logan=False):
 
 
 def test_config3_slab_parity(orc, dev):
-    import sys, os
-    sys.path.insert(0, os.path.join(os.path.dirname(__file__), ".."))
-    from bench import fdk_matrix
     from imagingsuite_amd import phantom
     n = 1024
     slab = (0, 510, 1024, 514)
@@ -93,8 +90,9 @@ def test_config3_slab_parity(orc, dev):
     ells = phantom.random_ellipsoids(3)
     ic = (n / 2 - cb[0], n / 2 - cb[1])
     proj = phantom.cone_projections(ells, n, cb[3] - cb[1], angles,
-                                    lambda a: fdk_matrix(500.0, 1000.0, (0.2, 0.2), a), ic, n * 0.1 * 0.8,
+                                    lambda a: phantom.fdk_matrix(500.0, 1000.0, (0.2, 0.2), a), ic, n * 0.1 * 0.8,
                                     device=dev)
+    clean = fdk.process(proj.clone(), angles, weights).cpu().numpy()
     proj = phantom.add_poisson_log(proj, 2e4, seed=3)
     host = proj.cpu().numpy()
     vol = fdk.process(proj.clone(), angles, weights).cpu().numpy()
@@ -106,6 +104,25 @@ def test_config3_slab_parity(orc, dev):
     print(f"config 3 slab (reference filter): vs MuhRec CPU order relRMSE {e32[0]:.2e} max {e32[1]:.2e}; "
           f"vs FP64 relRMSE {e64[0]:.2e} max {e64[1]:.2e} (the float oracle itself: {eref[0]:.2e})")
     assert e32[0] <= 1e-5 and e64[0] <= 1.05 * eref[0]
+    # the reconstruction of the noise-free projections reproduces the phantom
+    # at the voxels the reference's grid, x-mirror and z-flip assign to it, at
+    # the closed-form scale of a centred cylinder through the same chain
+    # (fdk_pins.axis_scale)
+    from fdk_pins import Geometry, axis_scale
+    G = Geometry(n, (slab[1], slab[3]))
+    fov = n * 0.1 * 0.8
+    truth = G.voxelize_output(ells, fov)
+    a, b = clean[m].astype(np.float64), truth[m]
+    corr = float(np.corrcoef(a, b)[0, 1])
+    corr_unmirrored = float(np.corrcoef(a, G.voxelize_output(ells, fov, mirror=False)[m])[0, 1])
+    scale = float((a * b).sum() / (b * b).sum())
+    u = (np.arange(n) - n / 2) * 0.2 * 500.0 / 1000.0
+    cyl = 2.0 * np.sqrt(np.clip(20.0 ** 2 - u * u, 0, None)) * 0.1   # radius 20 mm, 1/cm
+    K, _ = axis_scale(G, cyl, weights)
+    print(f"config 3 slab vs voxelized phantom: correlation {corr:.4f} (unmirrored {corr_unmirrored:.4f}), "
+          f"fitted scale {scale:.4e} vs closed form {K:.4e} per 1/cm")
+    assert corr > 0.99 and corr_unmirrored < 0.9
+    assert abs(scale / K - 1.0) < 0.05
     fdk2, g2, _, _, _ = _fdk_setup(orc, dev, slab, shepp_logan=True)
     vol2 = fdk2.process(proj.clone(), angles, weights).cpu().numpy()
     ext64 = orc.fdk_bp_ext_f64(host, angles, weights, g2, shepp_logan=True, cosine=True)

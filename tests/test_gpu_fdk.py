@@ -51,25 +51,65 @@ def test_fdk_matches_oracle(orc, dev, slab, direction):
     assert e64[0] <= 1e-4
 
 
-def test_fdk_uniform_cylinder(orc, dev):
-    """Projections of a centred uniform cylinder (radius 4 mm, mu = 1/cm) at
-    the magnified detector: the centre slice reconstructs to about mu."""
-    from imagingsuite_amd.recon import FDKbp_single
-    n, n_p, mu, r = 128, 180, 0.1, 4.0   # mu in 1/mm
-    slab = (0, 48, 128, 80)
-    ppoint = (64.0, 64.0)
-    fdk = FDKbp_single(500.0, 1000.0, (0.2, 0.2), (0.1, 0.1, 0.1), ppoint, 64.0, slab, n - 1)
+def _module(cls, G, **kw):
+    from imagingsuite_amd import recon
+    return getattr(recon, cls)(G.sod, G.sdd, G.pix, G.vox, G.ppoint, G.center, G.roi, G.n - 1,
+                               direction=G.direction, **kw)
+
+
+@pytest.mark.parametrize("cls,fused", [("FDKbp_single", False), ("FDKbp_single", True), ("FDKbp", False)])
+@pytest.mark.parametrize("direction,shift", [(0, 0.0), (1, 0.0), (0, 2.5)])
+def test_fdk_ball_lands_on_predicted_voxel(orc, dev, cls, fused, direction, shift):
+    """Geometry pin on the HIP path: a small ball at the world position of
+    internal voxel (i, j, k) reconstructs with its maximum at output
+    [V-1-k][j][U-1-i] and its centroid within 0.1 voxel (the reference's
+    origin, fdkbp_single.cpp:314-320, x-mirror and z-flip,
+    fdkreconbase.cpp:259-270).  Dropping the mirror, the flip or half a voxel
+    of origin in the kernel fails this test."""
+    from fdk_pins import Geometry, ball, centroid
+    from test_oracle_fdk_geometry import BALLS
+    G = Geometry(64, (16, 48), ppoint=(32.0, 32.0), center=32.0 + shift, direction=direction)
+    kw = {} if cls == "FDKbp" else {"fused": fused}
+    fdk = _module(cls, G, **kw)
     cb = fdk.window
-    u = (np.arange(n) - ppoint[0]) * 0.2 / 2.0            # detector column -> iso-centre (mm)
-    chord = 2.0 * np.sqrt(np.clip(r * r - u * u, 0, None)) * mu
-    full = np.broadcast_to(chord.astype(np.float32), (n_p, n, n)).copy()
-    win = np.ascontiguousarray(full[:, cb[1]:cb[3], :])
+    n_p = 240
+    angles = np.arange(n_p, dtype=np.float32) * np.float32(360.0 / n_p)
+    weights = np.full(n_p, 1.0 / n_p, np.float32)
+    for (i0, j0, k0) in BALLS:
+        proj = G.project([ball(G.world_of_internal(i0, j0, k0), 0.25)], angles, cb, device=dev)
+        vol = fdk.process(proj, angles, weights).cpu().numpy()
+        pred = G.output_index(i0, j0, k0)
+        am = tuple(int(a) for a in np.unravel_index(np.argmax(vol), vol.shape))
+        cen = centroid(vol, am)
+        print(f"{cls} fused={fused} dir={direction} shift={shift}: ball {(i0, j0, k0)} -> max {am}, "
+              f"centroid {np.round(cen, 3)}, predicted {pred}")
+        assert am == pred
+        assert np.abs(cen - np.array(pred)).max() < 0.1
+
+
+@pytest.mark.parametrize("cls", ["FDKbp_single", "FDKbp"])
+def test_fdk_uniform_cylinder(orc, dev, cls):
+    """Scale pin on the HIP path: a centred uniform cylinder (radius 2 mm,
+    1/cm) reconstructs to (SOD/SDD) sum(w) q(ic0) in its core, q being the
+    reference's filter evaluated with an explicit DFT (FDKbp: / n_proj),
+    within 0.5 %, flat to 0.1 %, centred on the rotation axis."""
+    from fdk_pins import Geometry, axis_scale, cylinder
+    G = Geometry(64, (16, 48), ppoint=(32.0, 32.0), center=32.0)
+    fdk = _module(cls, G)
+    cb = fdk.window
+    n_p = 180
     angles = np.arange(n_p, dtype=np.float32) * np.float32(2.0)
-    weights = np.full(n_p, np.float32(2.0 / 360.0), np.float32)
-    vol = fdk.process(torch.from_numpy(win).to(dev), angles, weights).cpu().numpy()
-    core = vol[16, 64 - 20:64 + 20, 64 - 20:64 + 20]
-    print("cylinder core mean", core.mean(), "(scale-free check of shape: flat core)")
-    assert core.std() < 0.1 * abs(core.mean())
+    weights = np.full(n_p, np.float32(2.0 / n_p), np.float32)
+    proj = G.project([cylinder(2.0, 1.0)], angles, cb, device=dev)
+    row = int(round(G.ppoint[1] - cb[1]))
+    K, _ = axis_scale(G, proj[0, row].cpu().numpy(), weights, cls)
+    vol = fdk.process(proj, angles, weights).cpu().numpy()
+    ax, ay = G.axis_output_xy()
+    yy, xx = np.meshgrid(np.arange(G.n), np.arange(G.n), indexing="ij")
+    core = vol[G.nz // 2 - 4:G.nz // 2 + 4][:, (xx - ax) ** 2 + (yy - ay) ** 2 <= 36]
+    print(f"{cls} cylinder core {core.mean():.6e} (std {core.std():.2e}), closed form {K:.6e}")
+    assert abs(core.mean() / K - 1.0) < 5e-3
+    assert core.std() < 1e-3 * abs(core.mean())
 
 
 @pytest.mark.parametrize("slab,direction", [((0, 48, 128, 80), 0), ((0, 16, 128, 48), 1),

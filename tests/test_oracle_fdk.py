@@ -8,10 +8,14 @@ What pins the oracle:
   tiny image (zero padding, 5-row/5-column edge copies, the 0.54+0.46(cos+1)
   roll-off, 1/P); the float flavour within float rounding of it.
 * a12/a15: the double back-projection against the float restatement, and
-  the window (CBCT_roi) against hand-evaluated values of its formula.
+  the window (CBCT_roi) against hand-evaluated values of its formula; the
+  voxel grid, x-mirror, z-flip and output scale by closed-form ball positions
+  and cylinder values (test_oracle_fdk_geometry.py).
 """
 import numpy as np
 import pytest
+
+from fdk_pins import explicit_fdk_filter as _explicit_fdk_filter
 
 
 def _g(orc, slab=(0, 48, 128, 80), direction=0, ppoint=(64.0, 64.0), center=64.0):
@@ -33,9 +37,9 @@ def test_projection_matrix_geometry(orc):
     pm = orc.fdk_projmatrix_f32(g, 0.0)
     # CW at 0 deg: source on +x; a point 2 mm along y lands 2*SDD/SOD/pix = 20 px off
     c, r, _ = _project(pm, (0.0, 2.0, 0.0))
-    assert abs(abs(c) - 20.0) < 1e-4 and abs(r) < 1e-5
+    assert abs(c - 20.0) < 1e-4 and abs(r) < 1e-5
     c, r, _ = _project(pm, (0.0, 0.0, 1.5))
-    assert abs(abs(r) - 15.0) < 1e-4 and abs(c) < 1e-5
+    assert abs(r + 15.0) < 1e-4 and abs(c) < 1e-5
     g2 = _g(orc, direction=1)
     for ang in [10.0, 75.0]:
         a = _project(orc.fdk_projmatrix_f32(g, ang), (1.0, 2.0, 0.0))
@@ -52,26 +56,6 @@ def test_cbct_window_formula(orc):
     v2 = 64 + (80 - 64) * 0.1 * 1000 / (500 - radius) / 0.2
     assert cb[1] == int(v1) - 8 and cb[3] == int(v2) + 8
     assert cb[0] == 0 and cb[2] == 128
-
-
-def _explicit_fdk_filter(img):
-    h, w = img.shape
-    P = int(2 * 2 ** np.ceil(np.log2(w)))
-    pf = (P - w) // 2
-    pad = np.zeros((h, P))
-    pad[:, pf:pf + w] = img
-    for r in range(5):
-        pad[r] = pad[5]
-    for r in range(h - 5, h):
-        pad[r] = pad[h - 6]
-    pad[:, :5] = pad[:, 5:6]
-    pad[:, P - 5:] = pad[:, P - 6:P - 5]
-    i = np.arange(P)
-    ramp = np.where(i < P // 2, i, P - i).astype(np.float64)
-    ramp *= (0.54 + 0.46 * (np.cos(2 * np.pi * i / P) + 1)) / (P * 0.5)
-    F = np.exp(-2j * np.pi * np.outer(i, i) / P)
-    out = ((F.conj() @ ((F @ pad.T) * ramp[:, None])).real / P).T
-    return out[:, pf:pf + w]
 
 
 def test_fdk_filter_explicit_dft(orc):
@@ -106,5 +90,6 @@ def test_fdk_f32_vs_f64(orc):
     d = v32 - v64
     rng_ = v64.max() - v64.min()
     assert np.sqrt(np.mean(d * d)) / rng_ < 1e-4
-    # mirror/flip: output slice 0 is the top (last) slab row
-    assert v32.shape == (32, 64, 64)
+    # the grid, mirror, flip and scale the two flavours share are pinned by
+    # closed-form positions and values in test_oracle_fdk_geometry.py
+    assert v32.shape == v64.shape == (32, 64, 64)
