@@ -30,7 +30,7 @@ EXPORTED_SYMBOLS = (
     "mr_filter_plan_create", "mr_filter_plan_destroy", "mr_filter_plan_info", "mr_filter_rows",
     "mr_filter_rows_adjoint", "mr_bp_parallel_adjoint",
     "mr_bp_plan_create", "mr_bp_plan_destroy", "mr_bp_plan_mask", "mr_bp_plan_kernel", "mr_bp_parallel",
-    "mr_fbp_parallel",
+    "mr_bp_parallel_noalloc", "mr_fbp_parallel",
     "mr_fdk_window", "mr_fdk_plan_create", "mr_fdk_plan_destroy", "mr_fdk_plan_window", "mr_fdk_bp",
     "mr_joseph_forward", "mr_joseph_adjoint", "mr_linear_forward",
     "mr_iter_plan_create", "mr_iter_run", "mr_iter_plan_destroy",
@@ -118,6 +118,7 @@ def load():
     L.mr_bp_plan_mask.argtypes = [vp, szp, szp]
     L.mr_bp_plan_kernel.argtypes = [vp, ctypes.c_char_p, sz]
     L.mr_bp_parallel.argtypes = [vp, vp, sz, fp, fp, vp, i32, vp]
+    L.mr_bp_parallel_noalloc.argtypes = [ctypes.POINTER(BpParams), vp, sz, fp, fp, vp, i32, vp]
     L.mr_fbp_parallel.argtypes = [vp, vp, vp, sz, fp, fp, vp, vp]
     L.mr_fdk_window.argtypes = [ctypes.POINTER(FdkParams), szp]
     L.mr_fdk_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(FdkParams)]

@@ -94,7 +94,7 @@ __global__ __launch_bounds__(256) void bp_prep4_kernel(const float *__restrict__
 }
 
 // Weighting and transposition of the used projections into the sinogram.
-inline int launch_bp_prep(const float *proj, float *sino, const int *idx, const float *wts, int nu, int nv,
+int launch_bp_prep(const float *proj, float *sino, const int *idx, const float *wts, int nu, int nv,
                           int vpad, int nk, hipStream_t s)
 {
     ProfScope prof("bp_prep", s);
@@ -108,18 +108,6 @@ inline int launch_bp_prep(const float *proj, float *sino, const int *idx, const 
     MR_HIP(hipGetLastError());
     return MR_OK;
 }
-
-struct BpArgs {
-    const float *sino;
-    float *vol;
-    const float4 *tab;   // (fStartU, fCos, fSin, 0) per used projection
-    const int2 *mask;    // [first, second) per y, 0-based x
-    int nu, nv, vpad, nk, nz4, bstride, accumulate;
-    int order;           // block -> tile mapping (tile_of_block)
-    int ntx, nty, ntz;   // tiles along x, y and slice chunks
-    int sw;              // stripe width (tiles) of the XCD-striped order
-    int stage64;         // 1: stage with 64-bit addresses even when 32-bit offsets fit (tests)
-};
 
 // Block -> tile mapping of the tiled kernels.  order 0: the 3-D grid itself.
 // order 1: a 1-D grid.  Consecutive blocks are dispatched to consecutive XCDs

@@ -133,6 +133,22 @@ int mr_bp_plan_mask(const mr_bp_plan *plan, size_t *first, size_t *second);
  * mr_fbp_parallel behaves the same way. */
 int mr_bp_parallel(mr_bp_plan *plan, const float *d_proj, size_t n_proj, const float *angles,
                    const float *weights, float *d_volume, int accumulate, void *stream);
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  mr_bp_parallel without a plan and without any allocation:
+ * the circle mask, the projection tables and the weighted, transposed
+ * sinogram live in a fixed device array of MR_NOALLOC_SCRATCH_BYTES per GPU,
+ * held until the process ends.  The used projections are back-projected in
+ * chunks of as many as fit in that array; the result equals mr_bp_parallel's
+ * for the same params bit for bit.  Fails with MR_ERR_INVALID when one
+ * projection's staged size (width x slices rounded up to 4, float) exceeds
+ * the array's 60 MiB sinogram part, and for more than 65535 projections.
+ * Enqueues on `stream` (of the current device) and returns without waiting.
+ * All calls on one GPU share the array: calls that overlap on one GPU
+ * (other streams, other threads) race, and must be ordered by the caller. */
+#define MR_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int mr_bp_parallel_noalloc(const mr_bp_params *params, const float *d_proj, size_t n_proj, const float *angles,
+                           const float *weights, float *d_volume, int accumulate, void *stream);
 /* Adjoint of mr_bp_parallel (the project's own operator for gradients; no
  * reference counterpart): d_grad_volume [width(y)][width(x)][slices] ->
  * d_grad_proj [n_proj][slices][width] (overwritten; zero for dropped
