@@ -11,7 +11,17 @@ Prints one JSON line.  GUPS = voxel*projection updates per second over the
 bounding box with every configured projection.  The roofline entry is the
 dominant kernel (the back-projector), timed with HIP events around that
 kernel on its stream; the CPU baseline is the oracle restatement of the same
-two stages (OpenMP on this host's cores) on a bounded sample.
+two stages on a bounded sample (the back-projection with OpenMP on this
+host's cores, the filter on one core with single-precision FFTs, as MuhRec
+runs them).
+
+The default line also carries the other parts of the headline metric, each
+with its own kernel time and roofline fraction:
+  "fdk"               config 3 (FDK 1024^3 from 720 projections)
+  "batched_vs_plain"  config 4 (32 uneven stacks: graph / persistent / sequential)
+  "adjoint"           config 5 (Joseph A and A^T, FBP and FBP^T at 1024^2 x 64 x 900)
+  "plugin_path"       config 2 through the MuhRec plugin boundary (block driver +
+                      module quartet; host stack in, host slices out)
 
 Other configurations (reported by their own JSON line):
   --config 1  parallel FBP 256^2 x 64, 180 projections, Ram-Lak
@@ -66,22 +76,68 @@ def base_result(metric, value, unit, steps, warmup, ms, dtype, config, higher=Tr
             "config": config}
 
 
-def cpu_baseline(host_sino, angles, weights, cfg, n_sample_proj, n_sample_slices):
-    """Oracle (float restatement, OpenMP) on a bounded sample."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _omp_threads(n):
+    """Sets the OpenMP thread count of the oracle's runtime (libgomp, loaded
+    with liboracle.so)."""
+    import ctypes
+    ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(n))
+
+
+def cpu_baseline(host_sino, angles, weights, cfg, n_sample_proj, n_sample_slices, filter_rows=8192):
+    """The MuhRec CPU path on a bounded sample of the same stack, on this host:
+    ProjectionFilterSingle as MuhRec runs it (one core, single-precision FFTs,
+    one row after the other: ``oracle.filter_rows_fft32_serial``) on
+    ``filter_rows`` rows, scaled to the sample, and MultiProjBPparallel (the
+    float oracle, the reference's operation order, OpenMP over every core, in
+    16-projection chunks like ProjectionBufferSize) on the whole sample; a
+    single-thread back-projection figure on a smaller part beside it."""
     import oracle
     nu = cfg["width"]
+    cores = len(os.sched_getaffinity(0))
     sample = np.ascontiguousarray(host_sino[:n_sample_proj, :n_sample_slices, :])
+    rows = sample.reshape(-1, nu)
+    nf = min(filter_rows, rows.shape[0])
     t0 = time.perf_counter()
-    filt = oracle.filter_rows_f32(sample.reshape(-1, nu), cfg["filtertype"]).reshape(sample.shape)
+    oracle.filter_rows_fft32_serial(rows[:nf], cfg["filtertype"])
+    t_filter = (time.perf_counter() - t0) * rows.shape[0] / nf
+    # the back-projection input: the oracle's filter (same values, all cores)
+    filt = oracle.filter_rows_f32(rows, cfg["filtertype"]).reshape(sample.shape)
+    _omp_threads(cores)
     t1 = time.perf_counter()
     oracle.bp_parallel_f32(filt, angles[:n_sample_proj], weights[:n_sample_proj],
                            nu / 2 + cfg["center_off"], 0, "linear", 0)
-    t2 = time.perf_counter()
+    t_bp = time.perf_counter() - t1
     upd = nu * nu * n_sample_slices * n_sample_proj
-    return dict(value=round(upd / (t2 - t0) / 1e9, 3), unit="GUPS", cores=len(os.sched_getaffinity(0)),
-                kind="port",
-                sample=f"{nu}^2 x {n_sample_slices} slices x {n_sample_proj} proj "
-                       f"(filter {t1 - t0:.2f} s + BP {t2 - t1:.2f} s)")
+    # one thread: 8 slices x 96 projections of the same sample
+    n1p, n1s = min(96, n_sample_proj), min(8, n_sample_slices)
+    _omp_threads(1)
+    t2 = time.perf_counter()
+    oracle.bp_parallel_f32(np.ascontiguousarray(filt[:n1p, :n1s]), angles[:n1p], weights[:n1p],
+                           nu / 2 + cfg["center_off"], 0, "linear", 0)
+    t_bp1 = time.perf_counter() - t2
+    _omp_threads(cores)
+    upd1 = nu * nu * n1s * n1p
+    return dict(value=round(upd / (t_filter + t_bp) / 1e9, 3), unit="GUPS", cores=cores, kind="port",
+                cpu_model=cpu_model(),
+                sample=f"{nu}^2 x {n_sample_slices} slices x {n_sample_proj} proj: filter {t_filter:.2f} s "
+                       f"(1 thread, float FFT, timed on {nf} of {rows.shape[0]} rows and scaled) + "
+                       f"BP {t_bp:.2f} s ({cores} threads)",
+                filter_threads=1, bp_threads=cores,
+                bp_GUPS=round(upd / t_bp / 1e9, 3),
+                filter_rows_per_s=round(rows.shape[0] / t_filter, 1),
+                single_thread_bp_GUPS=round(upd1 / t_bp1 / 1e9, 4),
+                single_thread_sample=f"{nu}^2 x {n1s} slices x {n1p} proj in {t_bp1:.2f} s")
 
 
 def timed(fn, steps, warmup, log_name):
@@ -142,6 +198,55 @@ def timed_staged(staged, run, copy_then_run, steps, log_name):
 
 
 # --------------------------------------------------------------------------
+def roofline(kernel, kernel_ms, units, flop_per_unit, alg_bytes=None, traffic=None, l2_bytes=None):
+    """Roofline entry of one kernel: algorithmic FLOPs over its measured
+    launch time against MI355X's FP32 vector peak (none of these kernels is a
+    dense contraction, so no MFMA)."""
+    achieved = flop_per_unit * units / (kernel_ms / 1e3) / 1e12
+    d = {"bound": "valu", "pipe": "fp32 VALU (no MFMA: not a contraction)", "achieved": round(achieved, 3),
+         "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+         "traffic": traffic, "kernel": kernel, "kernel_ms": round(kernel_ms, 3), "flop_per_update": flop_per_unit}
+    if alg_bytes is not None:
+        d["hbm_algorithmic_bytes"] = alg_bytes
+        d["hbm_frac"] = round(alg_bytes / (kernel_ms / 1e3) / 1e9 / PEAK_HBM_GBS, 5)
+    if traffic is not None:
+        d["hbm_GBs"] = round(traffic / (kernel_ms / 1e3) / 1e9, 1)
+    if l2_bytes is not None:
+        d["l2_GBs"] = round(l2_bytes / (kernel_ms / 1e3) / 1e9, 1)
+    return d
+
+
+def profile_counters(name, kernel, **match):
+    """Counter figures of profiles/<name>.json (HBM bytes and L2 bytes per
+    launch from rocprofv3 --pmc), if they were taken on this kernel variant."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        tj = json.load(f)
+    if tj.get("kernel") != kernel or any(tj.get(k) != v for k, v in match.items()):
+        return None, None
+    return tj.get("hbm_bytes_per_launch"), tj.get("l2_bytes_per_launch")
+
+
+def profiled(fn, names):
+    """Runs fn() with kernel timing on; returns {name: (ms per launch, launches)}."""
+    from imagingsuite_amd import _lib
+    import torch
+    torch.cuda.synchronize()
+    _lib.profile(True)
+    _lib.profile_reset()
+    out = fn()
+    torch.cuda.synchronize()
+    res = {}
+    for n in names:
+        ms, cnt = _lib.profile_query(n)
+        res[n] = (ms / max(cnt, 1), cnt)
+    _lib.profile(False)
+    return out, res
+
+
+# --------------------------------------------------------------------------
 def bench_parallel(args):
     import torch
     from imagingsuite_amd import _lib, phantom, scan
@@ -197,20 +302,12 @@ def bench_parallel(args):
         del bp
     input_note = staged.note()
     staged.free()
+    del work, vol
     head = res[True] if True in res else res[False]
     updates = nu * nu * nv * n_p
     gups = updates / (head["ms"] / 1e3) / 1e9
-    flops = BP_FLOP_PER_UPDATE * nu * nu * nv * n_used
-    achieved = flops / (head["bp"] / 1e3) / 1e12
     bp_bytes = 4.0 * (nu * nv * n_used + nu * nu * nv)
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"bp_parallel_traffic_c{args.config}.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            tj = json.load(f)
-        # counters are only valid for the kernel variant they were taken on
-        if tj.get("kernel") == head["kernel"]:
-            traffic = tj.get("hbm_bytes_per_launch")
+    traffic, l2 = profile_counters(f"bp_parallel_traffic_c{args.config}.json", head["kernel"])
     r = base_result("GUPS (voxel*proj updates/s) FBP 2048^2x1500 proj" if args.config == 2 else
                     f"GUPS (voxel*proj updates/s) FBP config {args.config}",
                     round(gups, 2), "GUPS", args.steps, args.warmup, head["ms"], "f32",
@@ -222,13 +319,13 @@ def bench_parallel(args):
                      "in_mask_fraction": round(mask_voxels / float(nu * nu), 4),
                      "filter": cfg["filtertype"], "slice_block": nv, "input": input_note,
                      "precision": "fused" if True in res else "exact"})
-    r["roofline"] = {"bound": "mfma", "pipe": "fp32 VALU (no MFMA: not a contraction)",
-                     "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
-                     "kernel": head["kernel"],
-                     "kernel_ms": round(head["bp"], 3), "flop_per_update": BP_FLOP_PER_UPDATE,
-                     "hbm_algorithmic_bytes": bp_bytes,
-                     "hbm_frac": round(bp_bytes / (head["bp"] / 1e3) / 1e9 / PEAK_HBM_GBS, 5)}
+    r["roofline"] = roofline(head["kernel"], head["bp"], nu * nu * nv * n_used, BP_FLOP_PER_UPDATE,
+                             alg_bytes=bp_bytes, traffic=traffic, l2_bytes=l2)
+    # the ramp filter beside it: FFT FLOPs (5 P log2 P + P per row, two rows per complex FFT)
+    P = 1 << (int(math.ceil(math.log2(nu))) + 2)
+    f_rows = n_p * nv
+    r["filter_roofline"] = roofline("row_filter_kernel2", head["filt"], f_rows, 5 * P * math.log2(P) + P,
+                                    alg_bytes=8.0 * nu * f_rows)
     r["kernels_ms"] = {"row_filter": round(head["filt"], 3), "bp_prep": round(head["prep"], 3),
                        "bp_parallel": round(head["bp"], 3)}
     if False in res and True in res:
@@ -238,11 +335,25 @@ def bench_parallel(args):
                            "note": "reference float order per voxel, bit-identical to MultiProjBPparallel"}
     if not args.no_stream:
         r["pcie_inclusive"] = stream_leg(pristine, angles, weights, cfg, center, buf, nu, nv, n_p, updates)
+    host = None
+    if not args.no_cpu_baseline or (args.config == 2 and not args.no_legs):
+        host = pristine.cpu().numpy()
+    del pristine
+    torch.cuda.empty_cache()
+    if args.config == 2 and not args.no_legs:
+        # the rest of the headline metric, each leg on its own inputs
+        sub = max(2, min(args.steps, 5))
+        r["plugin_path"] = plugin_leg(host, angles, weights, cfg, center, buf, updates)
+        r["fdk"] = fdk_leg(sub, full=False)
+        torch.cuda.empty_cache()
+        r["batched_vs_plain"] = batch_leg(sub, 1, full=False)
+        torch.cuda.empty_cache()
+        r["adjoint"] = joseph_leg(sub, full=False)
+        torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
-        host = pristine[:args.cpu_sample_proj, :args.cpu_sample_slices].cpu().numpy()
         log("[bench] timing CPU baseline (oracle) ...")
-        r["cpu_baseline"] = cpu_baseline(host, angles, weights, cfg, min(args.cpu_sample_proj, n_p),
-                                         min(args.cpu_sample_slices, nv))
+        r["cpu_baseline"] = cpu_baseline(host[:args.cpu_sample_proj, :args.cpu_sample_slices], angles, weights, cfg,
+                                         min(args.cpu_sample_proj, n_p), min(args.cpu_sample_slices, nv))
     emit(r)
 
 
@@ -255,7 +366,7 @@ def stream_leg(pristine, angles, weights, cfg, center, buf, nu, nv, n_p, updates
     hp = host_empty(tuple(pristine.shape))
     hp[...] = pristine.cpu().numpy()
     ho = host_empty((nv, nu, nu))
-    sb = int(os.environ.get("MR_BENCH_STREAM_SB", "32"))
+    sb = 32
     st = FBPStream(nu, nv, n_p, center, slice_block=sb, buffer_size=buf, fused=True, filtertype=cfg["filtertype"])
     out = {}
     reps = 5
@@ -287,22 +398,57 @@ def stream_leg(pristine, angles, weights, cfg, center, buf, nu, nv, n_p, updates
     return out
 
 
+def plugin_leg(host, angles, weights, cfg, center, buf, updates):
+    """Config 2 through the MuhRec plugin boundary: the block driver
+    (ReconEngine's SliceBlock loop) loads libMuhRecHIPModules.so with
+    dlopen/GetModule and runs ProjectionFilterSingle + MultiProjBPparallel
+    module objects on host TImages (a host stack in, host slices out, angles
+    and weights as 6-digit text), at SliceBlock 256 and 32."""
+    from imagingsuite_amd import engine
+    nu, nv = cfg["width"], cfg["slices"]
+    flt = {"filtertype": cfg["filtertype"], "cutoff": 0.5, "order": 1, "usebias": "true", "biasweight": 0.1,
+           "paddingdoubler": 2}
+    out = {}
+    for sb in (256, 32):
+        log(f"[bench] plugin path, SliceBlock {sb} ...")
+        bpp = {"ProjectionBufferSize": 16, "SliceBlock": sb, "SubVolume": "1 1",
+               "BufferDrop": "true" if buf else "false", "fused": "true"}
+        e = engine.Engine("MultiProjBPparallel", bpp,
+                          {"geometry": "parallel", "center": center, "roi": f"0 0 {nu} {nv}",
+                           "resolution": f"{cfg['pixel_mm']} {cfg['pixel_mm']}"},
+                          "ProjectionFilterSingle", flt)
+        vol = np.empty((nv, nu, nu), np.float32)
+        e.run(host, angles, weights, nv, out=vol)
+        times = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            e.run(host, angles, weights, nv, out=vol)
+            times.append((time.perf_counter() - t0) * 1e3)
+        e.close()
+        ms = min(times)
+        out[f"slice_block_{sb}"] = {"ms": round(ms, 1), "GUPS": round(updates / (ms / 1e3) / 1e9, 1)}
+        log(f"[bench] plugin path SliceBlock {sb}: {ms:.1f} ms")
+    out["note"] = ("host stack in, host slices out through GetModule/Configure/SetROI/Process/GetSlice; "
+                   "best of 2 runs after one warm-up")
+    return out
+
+
 # --------------------------------------------------------------------------
-def bench_fdk(args):
+def fdk_leg(steps, full):
+    """Config 3: FDK 1024^3 from 720 cone-beam projections.  Headline: the
+    configuration's filter (Shepp-Logan with cosine pre-weighting, an
+    extension) with positions and sums in FDKbp_single's float order; beside
+    it MuhRec's own filter (ramp_filter_tuned, the parity path).  full adds
+    the FMA-position variant and the double-geometry FDKbp."""
     import torch
-    from imagingsuite_amd import _lib, phantom
-    from imagingsuite_amd.recon import FDKbp_single
+    from imagingsuite_amd import phantom
+    from imagingsuite_amd.recon import FDKbp, FDKbp_single
     dev = torch.device("cuda:0")
     n, n_p = 1024, 720
     sod, sdd, pix, vox = 500.0, 1000.0, (0.2, 0.2), (0.1, 0.1, 0.1)
     ppoint = (n / 2, n / 2)
     roi = (0, 0, n, n)
-    # the configuration's filter: Shepp-Logan with cosine pre-weighting (an
-    # extension; positions and sums in the reference's float order)
     fdk = FDKbp_single(sod, sdd, pix, vox, ppoint, n / 2, roi, n - 1, filter="shepp-logan", cosine_weight=True)
-    # MuhRec's own filter (ramp_filter_tuned), the parity path
-    fdk_exact = FDKbp_single(sod, sdd, pix, vox, ppoint, n / 2, roi, n - 1)
-    fdk_fused = FDKbp_single(sod, sdd, pix, vox, ppoint, n / 2, roi, n - 1, fused=True)
     cb = fdk.window
     log(f"[bench] FDK 1024^3, window {cb}")
     angles = np.arange(n_p, dtype=np.float32) * np.float32(360.0 / n_p)
@@ -317,103 +463,69 @@ def bench_fdk(args):
     log(f"[bench] cone-beam projections {tuple(proj.shape)} in {time.perf_counter() - t:.1f} s")
     work = torch.empty_like(proj)
     vol = torch.empty((n, n, n), dtype=torch.float32, device=dev)
-    staged = StagedInputs(proj, args.steps)
-
-    def step():
-        work.copy_(proj)
-        fdk.process(work, angles, weights, vol)
-
-    step()
-    torch.cuda.synchronize()
-    _lib.profile(True)
-    _lib.profile_reset()
-    ms = timed_staged(staged, lambda w: fdk.process(w, angles, weights, vol), step, args.steps, "fdk")
-    bp_ms, bp_n = _lib.profile_query("fdk_bp")
-    fl_ms, fl_n = _lib.profile_query("row_filter")
-    _lib.profile(False)
+    staged = StagedInputs(proj, steps)
     updates = float(n) ** 3 * n_p
-    kms = bp_ms / max(bp_n, 1)
-    achieved = FDK_FLOP_PER_UPDATE * updates / (kms / 1e3) / 1e12
-    r = base_result("GUPS (voxel*proj updates/s) FDK 1024^3 x 720 proj", round(updates / (ms / 1e3) / 1e9, 2),
-                    "GUPS", args.steps, 1, ms, "f32",
-                    {"workload": "FDK 1024^3 from 720 projections, 1024^2 detector, SOD 500, SDD 1000",
-                     "window_rows": cb[3] - cb[1], "filter": "Shepp-Logan + cosine pre-weighting",
-                     "precision": "exact positions", "input": staged.note()})
-    kname = "fdk_kernel<%s,false>" % os.environ.get("MR_FDK_KT", "16")
-    layout = "wx=%s bwx=%s" % (os.environ.get("MR_FDK_WX", "4"), os.environ.get("MR_FDK_BWX", "2"))
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "fdk_traffic_c3.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            tj = json.load(f)
-        # counters are only valid for the kernel and thread layout they were taken on
-        if tj.get("kernel") == kname and tj.get("layout") == layout:
-            traffic = tj.get("hbm_bytes_per_launch")
-    r["roofline"] = {"bound": "mfma", "pipe": "fp32 VALU", "achieved": round(achieved, 3),
-                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                     "traffic": traffic, "kernel": kname,
-                     "kernel_ms": round(kms, 3),
-                     "flop_per_update": FDK_FLOP_PER_UPDATE}
-    r["kernels_ms"] = {"row_filter_fdk": round(fl_ms / max(fl_n, 1), 3), "fdk_bp": round(kms, 3)}
-    # the reference float operation order (exact positions and sums)
-    def step_e():
-        work.copy_(proj)
-        fdk_exact.process(work, angles, weights, vol)
 
-    step_e()
-    torch.cuda.synchronize()
-    _lib.profile(True)
-    _lib.profile_reset()
-    mse = timed_staged(staged, lambda w: fdk_exact.process(w, angles, weights, vol), step_e,
-                       max(1, args.steps // 2), "fdk exact")
-    bpe_ms, bpe_n = _lib.profile_query("fdk_bp")
-    _lib.profile(False)
-    r["muhrec_filter"] = {"ms_per_step": round(mse, 3), "GUPS": round(updates / (mse / 1e3) / 1e9, 2),
-                          "kernel_ms": round(bpe_ms / max(bpe_n, 1), 3),
-                          "note": "ramp_filter_tuned as the reference; positions and sums bit-identical to "
-                                  "FDKbp_single (the parity path)"}
+    def run_module(m, name, kname, nsteps):
+        def step():
+            work.copy_(proj)
+            m.process(work, angles, weights, vol)
+        step()
+        ms, k = profiled(lambda: timed_staged(staged, lambda w: m.process(w, angles, weights, vol), step, nsteps,
+                                              name), [kname, "row_filter"])
+        return ms, k
 
-    def step_f():
-        work.copy_(proj)
-        fdk_fused.process(work, angles, weights, vol)
+    ms, k = run_module(fdk, "fdk", "fdk_bp", steps)
+    kms = k["fdk_bp"][0]
+    kname = "fdk_kernel<16,false>"
+    traffic, l2 = profile_counters("fdk_traffic_c3.json", kname)
+    out = {"workload": "FDK 1024^3 from 720 projections, 1024^2 detector at 0.2 mm, SOD 500, SDD 1000",
+           "filter": "Shepp-Logan + cosine pre-weighting", "precision": "exact positions",
+           "ms_per_step": round(ms, 3), "GUPS": round(updates / (ms / 1e3) / 1e9, 2),
+           "kernels_ms": {"row_filter_fdk": round(k["row_filter"][0], 3), "fdk_bp": round(kms, 3)},
+           "input": staged.note()}
+    out["roofline"] = roofline(kname, kms, updates, FDK_FLOP_PER_UPDATE,
+                               alg_bytes=4.0 * (n * (cb[3] - cb[1]) * n_p + n ** 3), traffic=traffic, l2_bytes=l2)
+    fdk_ref = FDKbp_single(sod, sdd, pix, vox, ppoint, n / 2, roi, n - 1)
+    mse, ke = run_module(fdk_ref, "fdk muhrec filter", "fdk_bp", max(1, steps // 2))
+    out["muhrec_filter"] = {"ms_per_step": round(mse, 3), "GUPS": round(updates / (mse / 1e3) / 1e9, 2),
+                            "kernel_ms": round(ke["fdk_bp"][0], 3),
+                            "note": "ramp_filter_tuned as the reference; positions and sums bit-identical to "
+                                    "FDKbp_single (the parity path)"}
+    if full:
+        fdk_fused = FDKbp_single(sod, sdd, pix, vox, ppoint, n / 2, roi, n - 1, fused=True)
+        msf, kf = run_module(fdk_fused, "fdk fused", "fdk_bp", max(1, steps // 2))
+        out["fused_positions"] = {"ms_per_step": round(msf, 3), "GUPS": round(updates / (msf / 1e3) / 1e9, 2),
+                                  "kernel_ms": round(kf["fdk_bp"][0], 3),
+                                  "note": "reference filter, row position folded into FMAs (relRMSE up to 4e-4 "
+                                          "against the reference on random data)"}
+        fdkd = FDKbp(sod, sdd, pix, vox, ppoint, n / 2, roi, n - 1)
+        msd, kd = run_module(fdkd, "fdkbp", "fdkd_bp", max(1, steps // 2))
+        out["fdkbp_double"] = {"ms_per_step": round(msd, 3), "GUPS": round(updates / (msd / 1e3) / 1e9, 2),
+                               "kernel": "fdkd_kernel<16>", "kernel_ms": round(kd["fdkd_bp"][0], 3)}
+    staged.free()
+    return out
 
-    step_f()
-    torch.cuda.synchronize()
-    _lib.profile(True)
-    _lib.profile_reset()
-    msf = timed_staged(staged, lambda w: fdk_fused.process(w, angles, weights, vol), step_f,
-                       max(1, args.steps // 2), "fdk fused")
-    bpf_ms, bpf_n = _lib.profile_query("fdk_bp")
-    _lib.profile(False)
-    r["fused_positions"] = {"ms_per_step": round(msf, 3), "GUPS": round(updates / (msf / 1e3) / 1e9, 2),
-                            "kernel_ms": round(bpf_ms / max(bpf_n, 1), 3),
-                            "note": "reference filter, row position folded into FMAs (relRMSE up to 4e-4 "
-                                    "against the reference on random data)"}
-    # the double-geometry module on the same slab (FDKbp, fdkbp.cpp)
-    from imagingsuite_amd.recon import FDKbp
-    fdkd = FDKbp(sod, sdd, pix, vox, ppoint, n / 2, roi, n - 1)
 
-    def step_d():
-        work.copy_(proj)
-        fdkd.process(work, angles, weights, vol)
-
-    step_d()
-    torch.cuda.synchronize()
-    _lib.profile(True)
-    _lib.profile_reset()
-    msd = timed_staged(staged, lambda w: fdkd.process(w, angles, weights, vol), step_d,
-                       max(1, args.steps // 2), "fdkbp")
-    bpd_ms, bpd_n = _lib.profile_query("fdkd_bp")
-    _lib.profile(False)
-    r["fdkbp_double"] = {"ms_per_step": round(msd, 3), "GUPS": round(updates / (msd / 1e3) / 1e9, 2),
-                         "kernel": "fdkd_kernel<16>", "kernel_ms": round(bpd_ms / max(bpd_n, 1), 3)}
+def bench_fdk(args):
+    d = fdk_leg(args.steps, full=True)
+    r = base_result("GUPS (voxel*proj updates/s) FDK 1024^3 x 720 proj", d.pop("GUPS"), "GUPS", args.steps, 1,
+                    d.pop("ms_per_step"), "f32",
+                    {"workload": d.pop("workload"), "filter": d.pop("filter"), "precision": d.pop("precision"),
+                     "input": d.pop("input")})
+    r.update(d)
     emit(r)
 
 
 # --------------------------------------------------------------------------
-def bench_batch(args):
+def batch_leg(steps, warmup, full):
+    """Config 4: 32 uneven parallel-beam stacks.  One persistent launch over
+    a work queue of all stacks' tiles (autotuned slice tile / projection
+    chunk), replayed from a hipGraph, against the same launch without the
+    graph and 32 sequential plain launches.  full adds the reference-order
+    variant."""
     import torch
-    from imagingsuite_amd import _lib, phantom, scan
+    from imagingsuite_amd import phantom, scan
     from imagingsuite_amd.recon import BatchReconstructor
     dev = torch.device("cuda:0")
     rng = np.random.default_rng(4)
@@ -431,8 +543,6 @@ def bench_batch(args):
                            buffer_size=0))
     torch.cuda.synchronize()
     log("[bench] 32 stacks generated; building batch plans (autotuner) ...")
-    # headline: interpolation folded into FMAs, as the single-stack headline;
-    # the reference operation order (bit-identical per stack) beside it
     br = BatchReconstructor(stacks, tune=True, fused=True)
     stream = torch.cuda.Stream()
 
@@ -445,34 +555,56 @@ def bench_batch(args):
                 with torch.cuda.stream(stream):
                     fn(stream=stream)
                 stream.synchronize()
-            res[name] = timed(step, args.steps, args.warmup, name)
+            res[name] = timed(step, steps, warmup, name)
         return res
 
     res = measure(br, ["graph", "persistent", "sequential"])
+    # the persistent kernel's own launch time
+    _, k = profiled(lambda: [br.run(stream=stream) for _ in range(steps)], ["bp_batch"])
     upd = br.updates
-    r = base_result("GUPS (voxel*proj updates/s) batched 32 uneven stacks", round(upd / (res["graph"] / 1e3) / 1e9, 2),
-                    "GUPS", args.steps, args.warmup, res["graph"], "f32",
-                    {"workload": "32 uneven parallel-beam FBP stacks (width 512-2048, 360-1800 proj, 8-128 slices)",
-                     "precision": "fused", "slice_tile": br.slice_tile, "proj_chunk": br.proj_chunk,
-                     "work_items": br.n_items})
-    r["batched_vs_plain"] = {k: {"ms": round(v, 3), "GUPS": round(upd / (v / 1e3) / 1e9, 2)} for k, v in res.items()}
+    out = {"workload": "32 uneven parallel-beam FBP stacks (width 512-2048, 360-1800 proj, 8-128 slices)",
+           "precision": "fused", "slice_tile": br.slice_tile, "proj_chunk": br.proj_chunk,
+           "work_items": br.n_items}
+    out.update({kk: {"ms": round(v, 3), "GUPS": round(upd / (v / 1e3) / 1e9, 2)} for kk, v in res.items()})
+    out["batched_over_sequential"] = round(res["sequential"] / res["graph"], 3)
+    kname = f"bp_batch_kernel<{br.slice_tile},0,false,{br.proj_chunk}>"
+    out["roofline"] = roofline(kname, k["bp_batch"][0], upd, BP_FLOP_PER_UPDATE)
     br.close()
     del br
-    bx = BatchReconstructor(stacks, tune=True, fused=False)
-    rx = measure(bx, ["graph", "sequential"])
-    r["exact_mode"] = {k: {"ms": round(v, 3), "GUPS": round(upd / (v / 1e3) / 1e9, 2)} for k, v in rx.items()}
-    r["exact_mode"]["slice_tile"], r["exact_mode"]["proj_chunk"] = bx.slice_tile, bx.proj_chunk
-    r["exact_mode"]["note"] = "MultiProjBPparallel operation order, bit-identical per stack"
-    bx.close()
+    if full:
+        bx = BatchReconstructor(stacks, tune=True, fused=False)
+        rx = measure(bx, ["graph", "sequential"])
+        out["exact_mode"] = {kk: {"ms": round(v, 3), "GUPS": round(upd / (v / 1e3) / 1e9, 2)} for kk, v in rx.items()}
+        out["exact_mode"]["slice_tile"], out["exact_mode"]["proj_chunk"] = bx.slice_tile, bx.proj_chunk
+        out["exact_mode"]["note"] = "MultiProjBPparallel operation order, bit-identical per stack"
+        bx.close()
+    return out
+
+
+def bench_batch(args):
+    d = batch_leg(args.steps, args.warmup, full=True)
+    r = base_result("GUPS (voxel*proj updates/s) batched 32 uneven stacks", d["graph"]["GUPS"], "GUPS",
+                    args.steps, args.warmup, d["graph"]["ms"], "f32",
+                    {"workload": d.pop("workload"), "precision": d.pop("precision"),
+                     "slice_tile": d.pop("slice_tile"), "proj_chunk": d.pop("proj_chunk"),
+                     "work_items": d.pop("work_items")})
+    r["batched_vs_plain"] = {kk: d.pop(kk) for kk in ("graph", "persistent", "sequential")}
+    r.update(d)
     emit(r)
 
 
 # --------------------------------------------------------------------------
-def bench_joseph(args):
+def joseph_leg(steps, full, fp64_gd=False):
+    """Config 5: the matched Joseph pair A / A^T and the FBP / FBP^T pair
+    (filter + MultiProjBPparallel and their adjoints) at 1024^2 x 64 x 900,
+    and 50 iterations of gradient descent on ||Ax - y||^2 + 1e-3 TV(x).
+    full adds SIRT and CGLS; fp64_gd the 50 iterations on a 4-slice slab of
+    the same data against the same iterations in FP64 on the CPU (minutes of
+    CPU work)."""
     import torch
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from imagingsuite_amd import _lib, phantom
-    from recon_autograd import Geometry, gradient_descent
+    from imagingsuite_amd import phantom
+    from recon_autograd import FBPOperator, Geometry, gradient_descent
     dev = torch.device("cuda:0")
     nx, nz, n_a = 1024, 64, 900
     ang = np.arange(n_a, dtype=np.float32) * np.float32(180.0 / n_a)
@@ -480,47 +612,91 @@ def bench_joseph(args):
     x = phantom.voxelize(phantom.random_ellipsoids(5), nx, nz, device=dev)
     y = geom.forward(x)
     torch.cuda.synchronize()
-    ms_a = timed(lambda: geom.forward(x), args.steps, 1, "A")
-    ms_t = timed(lambda: geom.adjoint(y), args.steps, 1, "AT")
     upd = float(nx) * nx * nz * n_a
+    ms_a, ka = profiled(lambda: timed(lambda: geom.forward(x), steps, 1, "A"), ["joseph_forward"])
+    ms_t, kt = profiled(lambda: timed(lambda: geom.adjoint(y), steps, 1, "AT"), ["joseph_adjoint"])
+    out = {"workload": "matched Joseph pair and FBP pair, 1024^2 x 64 slices x 900 angles",
+           "A": {"ms": round(ms_a, 3), "GUPS": round(upd / (ms_a / 1e3) / 1e9, 2),
+                 "roofline": roofline("joseph_forward_lds2_kernel", ka["joseph_forward"][0], upd, 4)},
+           "AT": {"ms": round(ms_t, 3), "GUPS": round(upd / (ms_t / 1e3) / 1e9, 2),
+                  "roofline": roofline("joseph_adjoint_lds_kernel", kt["joseph_adjoint"][0], upd, 4)}}
+    # gradient through filtered back-projection: F^T BP^T
+    af = (np.arange(n_a) * (180.0 / n_a)).astype(np.float32)
+    wf = np.full(n_a, np.float32(1.0 / n_a))
+    op = FBPOperator(nx, nz, nx / 2 + 0.5, af, wf, buffer_size=0, fused=True)
+    p = torch.rand((n_a, nz, nx), device=dev)
+    g = torch.rand((nx, nx, nz), device=dev)
+    ms_f, kf = profiled(lambda: timed(lambda: op.forward(p), steps, 1, "FBP"), ["bp_parallel"])
+    ms_b, kb = profiled(lambda: timed(lambda: op.adjoint(g), steps, 1, "FBP^T"), ["bp_adjoint"])
+    out["FBP"] = {"ms": round(ms_f, 3), "GUPS": round(upd / (ms_f / 1e3) / 1e9, 2),
+                  "roofline": roofline("bp_lds_kernel", kf["bp_parallel"][0], upd, 4)}
+    out["FBP_T"] = {"ms": round(ms_b, 3), "GUPS": round(upd / (ms_b / 1e3) / 1e9, 2),
+                    "roofline": roofline("bp_adjoint_lds2_kernel", kb["bp_adjoint"][0], upd, 4)}
     t0 = time.perf_counter()
     xr, losses, _ = gradient_descent(y, geom, iters=50, lam=1e-3)
     torch.cuda.synchronize()
-    gd_s = time.perf_counter() - t0
-    r = base_result("GUPS (voxel*angle updates/s) Joseph A and A^T, 1024^2x64x900",
-                    round(upd / (ms_a / 1e3) / 1e9, 2), "GUPS", args.steps, 1, ms_a, "f32",
-                    {"workload": "matched Joseph pair 1024^2 x 64 slices x 900 angles"})
-    r["adjoint"] = {"ms": round(ms_t, 3), "GUPS": round(upd / (ms_t / 1e3) / 1e9, 2)}
-    r["gradient_descent_50"] = {"seconds": round(gd_s, 2), "loss_first": losses[0], "loss_last": losses[-1]}
-    # SIRT and CGLS (IterativeMeth.txt) on the same pair: 10 iterations each,
-    # one A and one A^T per iteration, all on one stream
-    from imagingsuite_amd.recon import IterativeReconstructor
-    for method in ("sirt", "cgls"):
-        it = IterativeReconstructor(nx, nz, nx, ang, nx / 2, method=method)
-        xi = torch.zeros((nx, nx, nz), dtype=torch.float32, device=dev)
-        it.run(y, x=xi, iterations=1)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        xi.zero_()
-        _, hist = it.run(y, x=xi, iterations=10, residuals=True)
-        torch.cuda.synchronize()
-        s_it = time.perf_counter() - t0
-        r[method + "_10"] = {"seconds": round(s_it, 3), "GUPS_per_operator": round(20 * upd / s_it / 1e9, 2),
-                             "residual_first": float(hist[0]), "residual_last": float(hist[-1])}
-        it.close()
-    # gradient through filtered back-projection (F^T BP^T) on a 1024^2 x 64 block
-    from recon_autograd import FBPOperator
-    n_f = 900
-    af = (np.arange(n_f) * (180.0 / n_f)).astype(np.float32)
-    wf = np.full(n_f, np.float32(1.0 / n_f))
-    op = FBPOperator(nx, nz, nx / 2 + 0.5, af, wf, buffer_size=0, fused=True)
-    p = torch.rand((n_f, nz, nx), device=dev)
-    g = torch.rand((nx, nx, nz), device=dev)
-    ms_f = timed(lambda: op.forward(p), max(1, args.steps), 1, "FBP")
-    ms_b = timed(lambda: op.adjoint(g), max(1, args.steps), 1, "FBP^T")
-    r["fbp_pair_1024x64x900"] = {"forward_ms": round(ms_f, 3), "adjoint_ms": round(ms_b, 3),
-                                 "forward_GUPS": round(upd / (ms_f / 1e3) / 1e9, 2),
-                                 "adjoint_GUPS": round(upd / (ms_b / 1e3) / 1e9, 2)}
+    out["gradient_descent_50"] = {"seconds": round(time.perf_counter() - t0, 2), "loss_first": losses[0],
+                                  "loss_last": losses[-1]}
+    if full:
+        from imagingsuite_amd.recon import IterativeReconstructor
+        for method in ("sirt", "cgls"):
+            it = IterativeReconstructor(nx, nz, nx, ang, nx / 2, method=method)
+            xi = torch.zeros((nx, nx, nz), dtype=torch.float32, device=dev)
+            it.run(y, x=xi, iterations=1)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            xi.zero_()
+            _, hist = it.run(y, x=xi, iterations=10, residuals=True)
+            torch.cuda.synchronize()
+            s_it = time.perf_counter() - t0
+            out[method + "_10"] = {"seconds": round(s_it, 3), "GUPS_per_operator": round(20 * upd / s_it / 1e9, 2),
+                                   "residual_first": float(hist[0]), "residual_last": float(hist[-1])}
+            it.close()
+    if fp64_gd:
+        out["gradient_descent_50_vs_fp64"] = gd_vs_fp64(x, ang)
+    return out
+
+
+def gd_vs_fp64(x, ang, nzs=4, iters=50, lam=1e-3):
+    """50 gradient-descent iterations on a 4-slice slab of the config 5 data
+    (1024^2 x 4 x 900) on the GPU, against the same iterations in FP64 on the
+    CPU (oracle Joseph pair, torch double TV, the GPU run's step)."""
+    import torch
+    import oracle
+    from recon_autograd import Geometry, gradient_descent, tv
+    nx = x.shape[0]
+    xs = x[:, :, :nzs].contiguous()
+    geom = Geometry(nx, nzs, nx, ang, center=nx / 2, direction=0)
+    xt = xs.cpu().numpy().astype(np.float64)
+    a64 = ang.astype(np.float64)
+    log("[bench] FP64 data for the gradient-descent comparison ...")
+    y64 = oracle.joseph_forward_f64(xt, a64, nx, nx / 2)
+    yg = torch.from_numpy(y64.astype(np.float32)).to(x.device)
+    xg, losses, step = gradient_descent(yg, geom, iters=iters, lam=lam)
+    xg = xg.cpu().numpy()
+    xd = np.zeros_like(xt)
+    t0 = time.perf_counter()
+    for i in range(iters):
+        res = oracle.joseph_forward_f64(xd, a64, nx, nx / 2) - y64
+        gr = 2.0 * oracle.joseph_adjoint_f64(res, a64, nx, nx / 2)
+        tt = torch.from_numpy(xd).requires_grad_(True)
+        (gt,) = torch.autograd.grad(tv(tt, 1e-2), tt)
+        xd = xd - step * (gr + lam * gt.numpy())
+        if (i + 1) % 10 == 0:
+            log(f"[bench] FP64 gradient descent {i + 1}/{iters}")
+    rng_ = float(xd.max() - xd.min()) or 1.0
+    d = xg.astype(np.float64) - xd
+    return {"slab": f"{nx}^2 x {nzs} x {len(ang)}", "iterations": iters,
+            "relRMSE": float(np.sqrt(np.mean(d * d)) / rng_), "max_rel": float(np.abs(d).max() / rng_),
+            "loss_first": losses[0], "loss_last": losses[-1], "fp64_seconds": round(time.perf_counter() - t0, 1)}
+
+
+def bench_joseph(args):
+    d = joseph_leg(args.steps, full=True, fp64_gd=args.fp64_gd)
+    r = base_result("GUPS (voxel*angle updates/s) Joseph A and A^T, 1024^2x64x900", d["A"]["GUPS"], "GUPS",
+                    args.steps, 1, d["A"]["ms"], "f32", {"workload": d.pop("workload")})
+    r["roofline"] = d["A"]["roofline"]
+    r["adjoint"] = d
     emit(r)
 
 
@@ -551,6 +727,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-stream", action="store_true",
                     help="skip the host-resident streaming leg (PCIe-inclusive timing)")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="config 2 only: skip the FDK, batched, adjoint and plugin-path legs")
+    ap.add_argument("--fp64-gd", action="store_true",
+                    help="config 5: also run the 50 gradient-descent iterations on a 4-slice slab in FP64 "
+                         "on the CPU and report the GPU run's error against them")
     ap.add_argument("--cpu-sample-proj", type=int, default=1500)
     ap.add_argument("--cpu-sample-slices", type=int, default=192)
     ap.add_argument("--calibrate-copy", action="store_true")

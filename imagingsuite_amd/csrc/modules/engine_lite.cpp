@@ -96,6 +96,11 @@ struct mre_engine {
     std::map<std::string, std::string> pre_params, bp_params;
     ReconConfig cfg;
     size_t slice_block = 32;
+    // 0: Run3DFull's SliceBlock loop (Process(3D)); 1: ReconEngine::Run, one
+    // projection at a time through Process(2D) with its last flag
+    // (never set, ReconEngine.cpp:505); 2: the same with the last flag on the
+    // final projection
+    int projection_at_a_time = 0;
 };
 
 extern "C" const char *mre_last_error(void) { return g_err.c_str(); }
@@ -145,6 +150,14 @@ extern "C" int mre_engine_create(mre_engine **engine, const char *module_so, con
         e->bp->Configure(e->cfg, e->bp_params);
         e->slice_block = (size_t)GetIntParameter(e->bp_params, "SliceBlock");
         if (e->slice_block == 0) throw ReconException("SliceBlock must be positive", __FILE__, __LINE__);
+        const auto kv = parse_kv(config);
+        const auto pa = kv.find("projection_at_a_time");
+        if (pa != kv.end()) {
+            if (pa->second == "reference") e->projection_at_a_time = 1;
+            else if (pa->second == "final") e->projection_at_a_time = 2;
+            else if (pa->second != "false") throw ReconException("projection_at_a_time must be false, reference or final",
+                                                                 __FILE__, __LINE__);
+        }
     } catch (const std::exception &ex) {
         g_err = ex.what();
         mre_engine_destroy(e);
@@ -170,6 +183,30 @@ extern "C" int mre_engine_run(mre_engine *e, const float *projections, size_t n_
         if (width != roi[2] - roi[0]) throw ReconException("width does not match roi", __FILE__, __LINE__);
         if (!cone && rows != roi[3] - roi[1]) throw ReconException("rows do not match roi", __FILE__, __LINE__);
         const size_t total = roi[3] - roi[1];
+        if (e->projection_at_a_time) {
+            // ReconEngine::Run (ReconEngine.cpp:455-520): the whole ROI at once,
+            // per projection the pre-processing modules' Process(2D) and the
+            // back-projector's Process(2D) with the float angle and weight
+            if (cone) throw ReconException("projection-at-a-time runs parallel beam only", __FILE__, __LINE__);
+            size_t block[4] = {roi[0], roi[1], roi[2], roi[3]};
+            e->bp->SetROI(block);
+            std::map<std::string, std::string> params;
+            params["dose"] = "1";
+            size_t dims[2] = {width, rows};
+            for (size_t p = 0; p < n_proj; ++p) {
+                kipl::base::TImage<float, 2> img(dims);
+                std::memcpy(img.GetDataPtr(), projections + p * rows * width, sizeof(float) * rows * width);
+                if (e->pre) e->pre->Process(img, params);
+                const bool last = e->projection_at_a_time == 2 ? p + 1 == n_proj : n_proj < p + 1;
+                e->bp->Process(img, angles[p], weights[p], last);
+            }
+            for (size_t i = 0; i < total; ++i) {
+                kipl::base::TImage<float, 2> slice = e->bp->GetSlice(i);
+                std::memcpy(volume + i * width * width, slice.GetDataPtr(),
+                            sizeof(float) * std::min(slice.Size(), width * width));
+            }
+            return 0;
+        }
         const size_t sb = e->slice_block;
         const size_t nblocks = total / sb + (total % sb ? 1 : 0);
         std::map<std::string, std::string> params;

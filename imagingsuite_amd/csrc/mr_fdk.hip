@@ -175,145 +175,6 @@ __global__ __launch_bounds__(256) void fdk_kernel(FdkArgs a)
             a.vol[((size_t)(a.nz - 1 - (k0 + kk)) * a.ny + j) * a.nx + (a.nx - 1 - i)] = acc[kk];
 }
 
-// LDS-staged form of fdk_kernel.  For each projection the block first finds
-// the detector window its voxels can sample: the column and row positions
-// are linear-fractional in x and y and linear in the slice term, so over the
-// block's box they are extreme at its eight corners, which lanes 0-7 evaluate
-// with the per-voxel formulas (one pixel of margin on each side covers the
-// rounding of interior points).  The window (clipped to the detector) is
-// copied row by row into LDS with coalesced loads, and every sample is then
-// read from LDS; rows outside the detector give 0 as before.  Positions and
-// sums are those of fdk_kernel, so the result is bit-identical to it.  A
-// window larger than kFdkCap samples falls back to the direct gathers for
-// that projection.  Not the default: with 16 samples per thread and
-// projection, the two barriers per projection cost more than the gathers
-// they replace (measured 505 against 213 ms at 1024^3 x 720).
-constexpr int kFdkCap = 4096;
-
-template <bool FUSED>
-__device__ __forceinline__ float fdk_row_pos(float zt, float yip1, float xip1, float dw, float rb)
-{
-    if (FUSED) return __builtin_fmaf(dw, zt, rb);
-    const float acc21 = zt + yip1;
-    const float r = dw * (acc21 + xip1);
-    return r + 0.5f;
-}
-
-template <int KT, bool FUSED>
-__global__ __launch_bounds__(256) void fdk_lds_kernel(FdkArgs a)
-{
-    __shared__ float win[kFdkCap];
-    __shared__ int wpar[4];   // column lo, hi, row lo, hi of the current projection
-    int bx, by, bz;
-    if (!fdk_tile(a.order, a.ntx, a.nty, a.ntz, bx, by, bz)) return;
-    const int tid = threadIdx.x;
-    const int i = bx * 64 + (tid & 63);
-    const int j = by * 4 + (tid >> 6);
-    const int k0 = bz * KT;
-    const bool in_vol = i < a.nx && j < a.ny;
-    const int kn = min(KT, a.nz - k0);
-    float acc[KT];
-#pragma unroll
-    for (int kk = 0; kk < KT; ++kk) acc[kk] = 0.0f;
-    bool active = false;
-    if (in_vol) {
-        const int2 m = a.mask[j];
-        active = i > m.x && i <= m.y;
-    }
-    const float x = a.ox + (float)i * a.sx;
-    const float y = a.oy + (float)j * a.sy;
-    // corner q of the block's box: x end q & 1, y end q & 2, slice end q & 4
-    const int qi = (tid & 1) ? min(bx * 64 + 63, a.nx - 1) : bx * 64;
-    const int qj = (tid & 2) ? min(by * 4 + 3, a.ny - 1) : by * 4;
-    const int qk = (tid & 4) ? kn - 1 : 0;
-    const float qx = a.ox + (float)qi * a.sx, qy = a.oy + (float)qj * a.sy;
-    const float fw = (float)a.w, fh = (float)a.h;
-    const int w4 = 4 * a.w;
-    const int pbytes = 4 * a.w * a.h;
-    for (int p = 0; p < a.np; ++p) {
-        const float *t = a.ptab + 8 * p;
-        const float *zt = a.zt1 + (size_t)p * a.nz + k0;
-        if (tid < 8) {
-            const float xip0 = qx * t[0], xip1 = qx * t[1], xip2 = qx * t[2];
-            const float yip0 = qy * t[3], yip1 = qy * t[4], yip2 = qy * t[5];
-            const float acc20 = t[6] + yip0;
-            const float acc22 = t[7] + yip2;
-            const float dw = 1.0f / (acc22 + xip2);
-            float c = dw * (acc20 + xip0);
-            c = c + 0.5f;
-            const float rb = FUSED ? __builtin_fmaf(dw, yip1 + xip1, 0.5f) : 0.0f;
-            const float r = fdk_row_pos<FUSED>(zt[qk], yip1, xip1, dw, rb);
-            // clamp before converting: beyond the detector the window is clipped anyway
-            int clo = (int)floorf(__builtin_amdgcn_fmed3f(c, -2.0f, fw + 1.0f)), chi = clo;
-            int rlo = (int)floorf(__builtin_amdgcn_fmed3f(r, -2.0f, fh + 1.0f)), rhi = rlo;
-#pragma unroll
-            for (int o = 1; o < 8; o <<= 1) {
-                clo = min(clo, __shfl_xor(clo, o, 8));
-                chi = max(chi, __shfl_xor(chi, o, 8));
-                rlo = min(rlo, __shfl_xor(rlo, o, 8));
-                rhi = max(rhi, __shfl_xor(rhi, o, 8));
-            }
-            if (tid == 0) {
-                wpar[0] = max(clo - 1, 0);
-                wpar[1] = min(chi + 1, a.w - 1);
-                wpar[2] = max(rlo - 1, 0);
-                wpar[3] = min(rhi + 1, a.h - 1);
-            }
-        }
-        __syncthreads();
-        const int c_lo = wpar[0], r_lo = wpar[2];
-        const int wc = wpar[1] - c_lo + 1, wr = wpar[3] - r_lo + 1;
-        const bool staged = wc > 0 && wr > 0 && wc * wr <= kFdkCap;
-        const float *src = a.img + (size_t)p * a.h * a.w;
-        if (staged) {
-            for (int rr = tid >> 6; rr < wr; rr += 4) {
-                const float *line = src + (size_t)(r_lo + rr) * a.w + c_lo;
-                for (int cc = tid & 63; cc < wc; cc += 64) win[rr * wc + cc] = line[cc];
-            }
-        }
-        __syncthreads();
-        if (active) {
-            const float xip0 = x * t[0], xip1 = x * t[1], xip2 = x * t[2];
-            const float yip0 = y * t[3], yip1 = y * t[4], yip2 = y * t[5];
-            const float acc20 = t[6] + yip0;
-            const float acc22 = t[7] + yip2;
-            const float dw = 1.0f / (acc22 + xip2);
-            float c = dw * (acc20 + xip0);
-            c = c + 0.5f;
-            if (c >= 0.0f && c < fw) {
-                const float dw2 = dw * dw;
-                if (staged) {
-                    const int col = (int)c - c_lo;
-                    const int last = wc * wr - 1;
-                    const float rb = FUSED ? __builtin_fmaf(dw, yip1 + xip1, 0.5f) : 0.0f;
-                    float val[KT];
-#pragma unroll
-                    for (int kk = 0; kk < KT; ++kk) {
-                        const float r = fdk_row_pos<FUSED>((kk < kn) ? zt[kk] : 0.0f, yip1, xip1, dw, rb);
-                        const int rr = cvt_floor(__builtin_amdgcn_fmed3f(r, -1.0f, fh));
-                        const int idx = min(max((rr - r_lo) * wc + col, 0), last);
-                        val[kk] = (rr >= 0 && rr < a.h) ? win[idx] : 0.0f;
-                    }
-#pragma unroll
-                    for (int kk = 0; kk < KT; ++kk)
-                        if (kk < kn) acc[kk] = FUSED ? __builtin_fmaf(dw2, val[kk], acc[kk]) : acc[kk] + dw2 * val[kk];
-                } else {
-                    const int cc4 = 4 * (int)c;
-                    const __amdgpu_buffer_rsrc_t img =
-                        __builtin_amdgcn_make_buffer_rsrc((void *)src, (short)0, pbytes, 0x00020000);
-                    if (kn == KT) fdk_column<KT, true, FUSED>(acc, zt, kn, img, yip1, xip1, dw, dw2, fh, w4, cc4);
-                    else fdk_column<KT, false, FUSED>(acc, zt, kn, img, yip1, xip1, dw, dw2, fh, w4, cc4);
-                }
-            }
-        }
-    }
-    if (!in_vol) return;
-#pragma unroll
-    for (int kk = 0; kk < KT; ++kk)
-        if (kk < kn)
-            a.vol[((size_t)(a.nz - 1 - (k0 + kk)) * a.ny + j) * a.nx + (a.nx - 1 - i)] = acc[kk];
-}
-
 // FDKbp (fdkbp.cpp:324-510, 612-631): the same loop with the geometry in
 // double.  Voxel coordinates are float expressions widened to double
 // (origin + i*spacing in float), the per-projection coefficients and the
@@ -401,25 +262,12 @@ __global__ __launch_bounds__(256) void fdkd_kernel(FdkdArgs a)
             a.vol[((size_t)(a.nz - 1 - (k0 + kk)) * a.ny + j) * a.nx + (a.nx - 1 - i)] = acc[kk];
 }
 
-// Voxel columns along x per wave (MR_FDK_WX: a power of two up to 64).
-// Default 4 x 16: measured at 1024^3 x 720 (fdk_kernel<16, false>) 64 x 1
-// 215 ms, 16 x 4 217, 8 x 8 163-168, 4 x 16 145, 2 x 32 160, 1 x 64 229.
-static int fdk_wave_x()
-{
-    const char *e = std::getenv("MR_FDK_WX");
-    const int v = e ? std::atoi(e) : 4;
-    return (v >= 1 && v <= 64 && (v & (v - 1)) == 0) ? v : 4;
-}
-
-// Waves along x per block (MR_FDK_BWX: 1, 2 or 4); a 64-wide wave stacks
-// its block's waves along y.
-static int fdk_block_waves_x(int wx)
-{
-    if (wx >= 64) return 1;
-    const char *e = std::getenv("MR_FDK_BWX");
-    const int v = e ? std::atoi(e) : 2;
-    return (v == 1 || v == 2 || v == 4) ? v : 2;
-}
+// Thread layout of the FDK kernels: a wave covers 4 x 16 voxel columns (x by
+// y) and a block 2 x 2 waves.  Measured at 1024^3 x 720 (fdk_kernel<16,
+// false>), columns per wave along x: 64 -> 215 ms, 16 -> 217, 8 -> 163-168,
+// 4 -> 142-145, 2 -> 160, 1 -> 229; the mapping only changes which thread
+// owns which column (bit-identical volumes).
+constexpr int kFdkWaveX = 4, kFdkBlockWavesX = 2;
 
 }  // namespace mr
 
@@ -697,49 +545,30 @@ extern "C" int mr_fdk_bp(mr_fdk_plan *plan, float *d_proj, size_t n_proj, size_t
         a.img = d_proj; a.vol = d_volume; a.ptab = plan->d_ptab; a.zt1 = plan->d_zt1; a.mask = plan->d_mask;
         a.nx = (int)nx; a.ny = (int)nx; a.nz = (int)nz; a.w = (int)w; a.h = (int)h; a.np = (int)n_proj;
         a.ox = plan->origin[0]; a.oy = plan->origin[1]; a.sx = plan->p.voxel[0]; a.sy = plan->p.voxel[1];
-        const char *eo = std::getenv("MR_FDK_ORDER");
-        const char *ek = std::getenv("MR_FDK_KT");
-        const int kt = (ek && std::atoi(ek) == 32) ? 32 : 16;
-        const char *el = std::getenv("MR_FDK_LDS");
-        const bool lds = kt == 16 && el && std::atoi(el) == 1;
-        // the LDS-staged kernel's corner search assumes 64 x 4 blocks
-        a.wx = lds ? 64 : mr::fdk_wave_x();
-        a.bwx = mr::fdk_block_waves_x(a.wx);
+        a.wx = mr::kFdkWaveX;
+        a.bwx = mr::kFdkBlockWavesX;
         const int bw = a.bwx * a.wx, bh = 256 / bw;
-        a.ntx = (int)((nx + bw - 1) / bw); a.nty = (int)((nx + bh - 1) / bh); a.ntz = (int)((nz + kt - 1) / kt);
-        const long ntiles = (long)a.ntx * a.nty * a.ntz;
-        // grid order measured faster for the float kernel (1024^3: 218 vs 236 ms)
-        a.order = (eo && std::atoi(eo) == 1) && ntiles < (1L << 30) ? 1 : 0;
-        const dim3 g = a.order ? dim3((unsigned)(8 * ((ntiles + 7) / 8)))
-                               : dim3((unsigned)a.ntx, (unsigned)a.nty, (unsigned)a.ntz);
+        a.ntx = (int)((nx + bw - 1) / bw); a.nty = (int)((nx + bh - 1) / bh); a.ntz = (int)((nz + KT - 1) / KT);
+        // grid order (blocks in x, y, slice order; measured faster than the
+        // XCD-striped order for this kernel)
+        a.order = 0;
+        const dim3 g((unsigned)a.ntx, (unsigned)a.nty, (unsigned)a.ntz);
         mr::ProfScope prof("fdk_bp", s);
-        const bool fused = plan->p.fused != 0;
-        // MR_FDK_LDS=1 selects the LDS-staged kernel (bit-identical, but measured slower at
-        // 1024^3 x 720: 505 against 213 ms, two barriers per projection for 16 samples per thread)
-        if (lds) {
-            if (fused) hipLaunchKernelGGL((mr::fdk_lds_kernel<16, true>), g, dim3(256), 0, s, a);
-            else hipLaunchKernelGGL((mr::fdk_lds_kernel<16, false>), g, dim3(256), 0, s, a);
-        } else if (kt == 32) {
-            if (fused) hipLaunchKernelGGL((mr::fdk_kernel<32, true>), g, dim3(256), 0, s, a);
-            else hipLaunchKernelGGL((mr::fdk_kernel<32, false>), g, dim3(256), 0, s, a);
-        } else {
-            if (fused) hipLaunchKernelGGL((mr::fdk_kernel<16, true>), g, dim3(256), 0, s, a);
-            else hipLaunchKernelGGL((mr::fdk_kernel<16, false>), g, dim3(256), 0, s, a);
-        }
+        if (plan->p.fused) hipLaunchKernelGGL((mr::fdk_kernel<KT, true>), g, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((mr::fdk_kernel<KT, false>), g, dim3(256), 0, s, a);
     } else {
         MR_REQUIRE(w * h * 4 < (size_t)1 << 31, "FDK window larger than 2^29 samples");
         mr::FdkdArgs a{};
         a.img = d_proj; a.vol = d_volume; a.ptab = plan->d_ptabd; a.zt1 = plan->d_zt1d; a.mask = plan->d_mask;
         a.nx = (int)nx; a.ny = (int)nx; a.nz = (int)nz; a.w = (int)w; a.h = (int)h; a.np = (int)n_proj;
         a.ox = plan->origin[0]; a.oy = plan->origin[1]; a.sx = plan->p.voxel[0]; a.sy = plan->p.voxel[1];
-        a.wx = mr::fdk_wave_x();
-        a.bwx = mr::fdk_block_waves_x(a.wx);
+        a.wx = mr::kFdkWaveX;
+        a.bwx = mr::kFdkBlockWavesX;
         const int bw = a.bwx * a.wx, bh = 256 / bw;
         a.ntx = (int)((nx + bw - 1) / bw); a.nty = (int)((nx + bh - 1) / bh); a.ntz = (int)((nz + KT - 1) / KT);
         const long ntiles = (long)a.ntx * a.nty * a.ntz;
-        const char *eo = std::getenv("MR_FDK_ORDER");
-        // XCD order measured faster for the double kernel (1024^3: 258 vs 302 ms)
-        a.order = (eo && std::atoi(eo) == 0) || ntiles >= (1L << 30) ? 0 : 1;
+        // XCD-striped order (measured faster for the double kernel, 1024^3: 258 vs 302 ms)
+        a.order = ntiles < (1L << 30) ? 1 : 0;
         const dim3 g = a.order ? dim3((unsigned)(8 * ((ntiles + 7) / 8)))
                                : dim3((unsigned)a.ntx, (unsigned)a.nty, (unsigned)a.ntz);
         mr::ProfScope prof("fdkd_bp", s);

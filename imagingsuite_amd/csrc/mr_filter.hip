@@ -59,61 +59,7 @@ __device__ __forceinline__ float fdk_row_vv2(const RowFilterArgs &a, long r)
     return vv * vv;
 }
 
-template <int P, int NT>
-__global__ __launch_bounds__(NT) void row_filter_kernel(RowFilterArgs a)
-{
-    extern __shared__ float2 line[];
-    const long r0 = 2 * (long)blockIdx.x;
-    const bool has1 = r0 + 1 < a.n_rows;
-    float *row0 = a.rows + r0 * a.n;
-    float *row1 = has1 ? row0 + a.n : row0;
-    const int t = threadIdx.x;
-
-    if (a.mode == 0) {
-        const float f0 = row0[0], l0 = row0[a.n - 1];
-        const float f1 = row1[0], l1 = row1[a.n - 1];
-        for (int i = t; i < P; i += NT) {
-            const float x0 = pb_pad(row0, a.n, i, a.insert, P, a.taper, f0, l0);
-            const float x1 = has1 ? pb_pad(row1, a.n, i, a.insert, P, a.taper, f1, l1) : 0.0f;
-            line[i] = make_float2(x0, x1);
-        }
-    } else {
-        const float w0 = a.pre_w ? a.pre_w[r0 / a.rows_per_proj] : 1.0f;
-        const float w1 = (a.pre_w && has1) ? a.pre_w[(r0 + 1) / a.rows_per_proj] : 1.0f;
-        const float v0 = fdk_row_vv2(a, r0), v1 = fdk_row_vv2(a, r0 + 1);
-        for (int i = t; i < P; i += NT) {
-            const float x0 = fdk_pad(row0, a.n, i, a.insert, P, w0, a, v0);
-            const float x1 = has1 ? fdk_pad(row1, a.n, i, a.insert, P, w1, a, v1) : 0.0f;
-            line[i] = make_float2(x0, x1);
-        }
-    }
-    __syncthreads();
-    fft_lds<P, NT, false>(line, a.tw);
-    for (int i = t; i < P; i += NT) {
-        const float h = a.hs[i];
-        float2 v = line[i];
-        line[i] = make_float2(v.x * h, v.y * h);
-    }
-    __syncthreads();
-    fft_lds<P, NT, true>(line, a.tw);
-    float *o0 = a.out ? a.out + r0 * a.n : row0;
-    float *o1 = o0 + a.n;
-    if (a.mode == 0) {
-        for (int i = t; i < a.n; i += NT) {
-            const float2 v = line[i + a.insert];
-            o0[i] = v.x * a.post0;
-            if (has1) o1[i] = v.y * a.post0;
-        }
-    } else {
-        for (int i = t; i < a.n; i += NT) {
-            const float2 v = line[i + a.insert];
-            o0[i] = ((v.x * a.post0) * a.post1) * a.post2;
-            if (has1) o1[i] = ((v.y * a.post0) * a.post1) * a.post2;
-        }
-    }
-}
-
-// Same filter on the radix-8 padded FFT (fft_lds2): dynamic LDS holds the
+// The filter on the radix-8 padded FFT (fft_lds2): dynamic LDS holds the
 // padded line (lds_len(P) float2) followed by the P/8 twiddles it uses.
 template <int P, int NT>
 __global__ __launch_bounds__(NT) void row_filter_kernel2(RowFilterArgs a)
@@ -242,11 +188,9 @@ __global__ void fdk_margin_rows_kernel(float *img, int n, int h, long n_proj)
 template <int P>
 static int launch_row_filter(const RowFilterArgs &a, hipStream_t s)
 {
-    // radix-4 form: P / 16 threads; radix-8 form: one radix-8 butterfly per
-    // thread and pass (P / 8 threads, at most 1024), which doubles the waves
-    // that hide the LDS latency (P = 8192: 8.1 instead of 9.9 ms for 384 000
-    // rows of 2048)
-    constexpr int NT = (P / 16) < 64 ? 64 : ((P / 16) > 512 ? 512 : (P / 16));
+    // one radix-8 butterfly per thread and pass: P / 8 threads (at most
+    // 1024); measured 7.8-8.1 ms for the 384 000 rows of 2048 at P = 8192,
+    // against 9.3-9.8 ms with P / 16 threads and 19.75 ms for a radix-4 FFT
     constexpr int NT2 = (P / 8) < 64 ? 64 : ((P / 8) > 1024 ? 1024 : (P / 8));
     const long pairs = (a.n_rows + 1) / 2;
     if (pairs == 0) return MR_OK;
@@ -254,32 +198,12 @@ static int launch_row_filter(const RowFilterArgs &a, hipStream_t s)
         set_error("too many rows for one launch");
         return MR_ERR_INVALID;
     }
-    // MR_FILTER_IMPL=1 selects the radix-4 FFT (comparison runs)
-    const char *e = std::getenv("MR_FILTER_IMPL");
-    const bool radix4 = e && std::atoi(e) == 1;
     ProfScope prof(a.mode == 0 ? "row_filter_pb" : (a.mode == 1 ? "row_filter_fdk" : "row_filter_adjoint"), s);
-    if (radix4) {
-        const size_t lds = sizeof(float2) * P;
-        if (lds > 65536)
-            MR_HIP(hipFuncSetAttribute((const void *)row_filter_kernel<P, NT>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL((row_filter_kernel<P, NT>), dim3((unsigned)pairs), dim3(NT), lds, s, a);
-    } else {
-        const size_t lds = sizeof(float2) * (lds_len(P) + P / 8);
-        // MR_FILTER_NT=half: the previous P / 16 threads (comparison runs)
-        const char *en = std::getenv("MR_FILTER_NT");
-        if (en && std::strcmp(en, "half") == 0) {
-            if (lds > 65536)
-                MR_HIP(hipFuncSetAttribute((const void *)row_filter_kernel2<P, NT>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            hipLaunchKernelGGL((row_filter_kernel2<P, NT>), dim3((unsigned)pairs), dim3(NT), lds, s, a);
-        } else {
-            if (lds > 65536)
-                MR_HIP(hipFuncSetAttribute((const void *)row_filter_kernel2<P, NT2>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            hipLaunchKernelGGL((row_filter_kernel2<P, NT2>), dim3((unsigned)pairs), dim3(NT2), lds, s, a);
-        }
-    }
+    const size_t lds = sizeof(float2) * (lds_len(P) + P / 8);
+    if (lds > 65536)
+        MR_HIP(hipFuncSetAttribute((const void *)row_filter_kernel2<P, NT2>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((row_filter_kernel2<P, NT2>), dim3((unsigned)pairs), dim3(NT2), lds, s, a);
     MR_HIP(hipGetLastError());
     return MR_OK;
 }
@@ -430,8 +354,6 @@ extern "C" int mr_filter_rows_adjoint(mr_filter_plan *plan, float *d_rows, size_
     MR_REQUIRE(plan != nullptr, "plan is NULL");
     MR_REQUIRE(d_rows != nullptr || n_rows == 0, "rows pointer is NULL");
     if (n_rows == 0) return MR_OK;
-    MR_REQUIRE(!(std::getenv("MR_FILTER_IMPL") && std::atoi(std::getenv("MR_FILTER_IMPL")) == 1),
-               "the adjoint filter runs on the radix-8 FFT only");
     mr::RowFilterArgs a{};
     a.rows = d_rows; a.n_rows = (long)n_rows; a.n = (int)plan->width; a.mode = 2;
     a.insert = (int)plan->n_insert; a.taper = plan->d_taper; a.hs = plan->d_hs; a.tw = plan->d_tw;

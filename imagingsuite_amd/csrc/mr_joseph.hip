@@ -648,21 +648,16 @@ int joseph_launch(const mr_joseph_geom *g, const float *src, float *dst, const J
     // ZT = 16 when the slice count allows it
     if (forward) {
         mr::ProfScope prof("joseph_forward", s);
-        const char *ef = std::getenv("MR_JOSEPH_FWD");
-        if (g->nz % 16 == 0 && dt.n_chunks > 0 && !(ef && std::atoi(ef) == 0)) {
-            // slices per block and lines per staging group (MR_JOSEPH_ZT, MR_JOSEPH_R for tuning)
-            // (measured at 1024^2 x 64 x 900: 32 slices and 1 line 9.8 ms, 2 lines 10.7 ms, 16 and 4 16.2 ms,
-            // 16 and 8 26.7 ms: the staging buffer's size sets how many blocks share a CU)
-            const char *ez = std::getenv("MR_JOSEPH_ZT"), *er = std::getenv("MR_JOSEPH_R");
-            const int zt = (g->nz % 32 == 0 && !(ez && std::atoi(ez) == 16)) ? 32 : 16;
-            const int rr = er ? std::atoi(er) : kFwdR;
+        if (g->nz % 16 == 0 && dt.n_chunks > 0) {
+            // 32 slices per block when the slice count allows (measured at
+            // 1024^2 x 64 x 900: 7.7 ms against 8.4 with 16)
+            const int zt = g->nz % 32 == 0 ? 32 : 16;
             const int nbt = (int)((g->n_det + kFwdNB - 1) / kFwdNB), nzs = (int)(g->nz / zt);
             const long total = (long)dt.n_chunks * nbt * nzs;
             MR_REQUIRE(total < (1L << 31) - 8, "Joseph forward grid too large");
             const dim3 grid((unsigned)(8 * ((total + 7) / 8)));
-            // one barrier per line (spans precomputed, double-buffered staging) unless MR_JOSEPH_FWD=1
-            // or the span table would not fit beside the staging buffers
-            if (!(ef && std::atoi(ef) == 1) && g->nx <= 4096) {
+            if (g->nx <= 4096) {
+                // one barrier per line (spans precomputed in LDS, 8 bytes per line)
                 const size_t dyn = sizeof(int) * 2 * g->nx;
                 if (zt == 32)
                     hipLaunchKernelGGL((mr::joseph_forward_lds2_kernel<32>), grid, dim3(256), dyn, s, a, dt.chunks,
@@ -670,45 +665,33 @@ int joseph_launch(const mr_joseph_geom *g, const float *src, float *dst, const J
                 else
                     hipLaunchKernelGGL((mr::joseph_forward_lds2_kernel<16>), grid, dim3(256), dyn, s, a, dt.chunks,
                                        dt.order, dt.n_chunks, nbt, nzs);
-                MR_HIP(hipGetLastError());
-                return MR_OK;
-            }
-#define MR_JF_LAUNCH(Z, RR)                                                                                     \
-    hipLaunchKernelGGL((mr::joseph_forward_lds_kernel<Z, RR>), grid, dim3(256), 0, s, a, dt.chunks, dt.order, \
-                       dt.n_chunks, nbt, nzs)
-            if (zt == 32) {
-                if (rr == 4) MR_JF_LAUNCH(32, 4);
-                else if (rr == 1) MR_JF_LAUNCH(32, 1);
-                else MR_JF_LAUNCH(32, 2);
+            } else if (zt == 32) {
+                // wider volumes: the span table would not fit beside the staging
+                // buffers; the per-line form gives the same bits
+                hipLaunchKernelGGL((mr::joseph_forward_lds_kernel<32, kFwdR>), grid, dim3(256), 0, s, a, dt.chunks,
+                                   dt.order, dt.n_chunks, nbt, nzs);
             } else {
-                if (rr == 4) MR_JF_LAUNCH(16, 4);
-                else if (rr == 1) MR_JF_LAUNCH(16, 1);
-                else MR_JF_LAUNCH(16, 2);
+                hipLaunchKernelGGL((mr::joseph_forward_lds_kernel<16, kFwdR>), grid, dim3(256), 0, s, a, dt.chunks,
+                                   dt.order, dt.n_chunks, nbt, nzs);
             }
-#undef MR_JF_LAUNCH
-        } else if (g->nz % 16 == 0) {
-            dim3 grid((unsigned)((g->n_det + 63) / 64), (unsigned)((n_ang + 3) / 4), (unsigned)(g->nz / 16));
-            hipLaunchKernelGGL(mr::joseph_forward_kernel<16>, grid, dim3(256), 0, s, a);
         } else {
+            // slice counts that are not a multiple of 16: direct ray-driven gathers
             dim3 grid((unsigned)((g->n_det + 63) / 64), (unsigned)((n_ang + 3) / 4), (unsigned)(g->nz / ZT));
             hipLaunchKernelGGL(mr::joseph_forward_kernel<ZT>, grid, dim3(256), 0, s, a);
         }
     } else {
         mr::ProfScope prof("joseph_adjoint", s);
-        const char *ei = std::getenv("MR_JOSEPH_IMPL");
         const int ntx = (int)((g->nx + 15) / 16);
-        if (g->nz % 32 == 0 && !(ei && std::atoi(ei) == 0)) {
+        if (g->nz % 32 == 0) {
             const long nt = (long)ntx * ntx * (long)(g->nz / 32);
             hipLaunchKernelGGL((mr::joseph_adjoint_lds_kernel<32, 4>), dim3((unsigned)(8 * ((nt + 7) / 8))), dim3(256),
                                0, s, a, ntx, ntx, (int)(g->nz / 32));
-        } else if (g->nz % 16 == 0 && !(ei && std::atoi(ei) == 0)) {
+        } else if (g->nz % 16 == 0) {
             const long nt = (long)ntx * ntx * (long)(g->nz / 16);
             hipLaunchKernelGGL((mr::joseph_adjoint_lds_kernel<16, 8>), dim3((unsigned)(8 * ((nt + 7) / 8))), dim3(256),
                                0, s, a, ntx, ntx, (int)(g->nz / 16));
-        } else if (g->nz % 16 == 0) {
-            dim3 grid((unsigned)((g->nx + 15) / 16), (unsigned)((g->nx + 15) / 16), (unsigned)(g->nz / 16));
-            hipLaunchKernelGGL(mr::joseph_adjoint_kernel<16>, grid, dim3(256), 0, s, a);
         } else {
+            // slice counts that are not a multiple of 16: voxel-driven gathers
             dim3 grid((unsigned)((g->nx + 15) / 16), (unsigned)((g->nx + 15) / 16), (unsigned)(g->nz / ZT));
             hipLaunchKernelGGL(mr::joseph_adjoint_kernel<ZT>, grid, dim3(256), 0, s, a);
         }

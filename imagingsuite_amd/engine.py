@@ -75,13 +75,19 @@ class Engine:
             raise EngineError(L.mre_last_error().decode())
         self.config = config
 
-    def run(self, projections: np.ndarray, angles, weights, slices: int) -> np.ndarray:
-        """projections [n_proj, rows, width] -> volume [slices, width, width] (x fastest)."""
+    def run(self, projections: np.ndarray, angles, weights, slices: int, out: Optional[np.ndarray] = None):
+        """projections [n_proj, rows, width] -> volume [slices, width, width] (x fastest),
+        written into ``out`` when given (C-contiguous float32 of that shape)."""
         p = np.ascontiguousarray(projections, np.float32)
         n, rows, width = p.shape
         a = np.ascontiguousarray(angles, np.float32)
         w = np.ascontiguousarray(weights, np.float32)
-        vol = np.zeros((slices, width, width), np.float32)
+        if out is None:
+            vol = np.zeros((slices, width, width), np.float32)
+        else:
+            if out.shape != (slices, width, width) or out.dtype != np.float32 or not out.flags.c_contiguous:
+                raise ValueError("out must be a C-contiguous float32 array of shape (slices, width, width)")
+            vol = out
         fp = ctypes.POINTER(ctypes.c_float)
         rc = load().mre_engine_run(self._h, p.ctypes.data_as(fp), n, rows, width, a.ctypes.data_as(fp),
                                    w.ctypes.data_as(fp), vol.ctypes.data_as(fp))

@@ -43,6 +43,24 @@ def _stream(stream) -> ctypes.c_void_p:
     return ctypes.c_void_p(stream.cuda_stream)
 
 
+def _order_after(stream, *tensors):
+    """The back-projection calls return while their kernels still run on
+    ``stream``.  When that is not the current stream, the tensors they read
+    and write are marked as used on it (torch's caching allocator then keeps
+    their memory until that work is done) and the current stream waits for
+    it, so what the caller queues next (``.cpu()``, further kernels) sees the
+    finished result."""
+    if stream is None:
+        return
+    import torch
+    cur = torch.cuda.current_stream()
+    if stream == cur:
+        return
+    for t in tensors:
+        t.record_stream(stream)
+    cur.wait_stream(stream)
+
+
 def _f32arr(x, n: int) -> np.ndarray:
     a = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
     if a.size != n:
@@ -161,7 +179,11 @@ class BackProjector:
 
     def process(self, proj, angles, weights, volume=None, accumulate: bool = False, stream=None):
         """Back-projects ``proj`` [n_proj, slices, width] (CUDA) into a
-        [width, width, slices] ZXY volume (allocated when not given)."""
+        [width, width, slices] ZXY volume (allocated when not given).  The
+        kernels are enqueued on ``stream`` (default: the current stream) and
+        the call returns without waiting; with another stream than the
+        current one, the current stream is made to wait for them and the
+        tensors are recorded on ``stream``."""
         import torch
         _require_cuda_f32(proj, 3, "proj")
         n_proj = proj.shape[0]
@@ -178,6 +200,7 @@ class BackProjector:
         check(_lib.load().mr_bp_parallel(self._plan, _ptr(proj), n_proj, a.ctypes.data_as(fp),
                                          w.ctypes.data_as(fp), _ptr(volume), int(bool(accumulate)),
                                          _stream(stream)))
+        _order_after(stream, proj, volume)
         return volume
 
     def adjoint(self, grad_volume, angles, weights, out=None, stream=None):
@@ -211,7 +234,9 @@ class BackProjector:
 
 def fbp_parallel(filt: ProjectionFilterSingle, bp: BackProjector, proj, angles, weights,
                  volume, stream=None):
-    """Filter ``proj`` in place and back-project it (one engine block)."""
+    """Filter ``proj`` in place and back-project it (one engine block).
+    Enqueued on ``stream`` without a host wait, ordered before later work on
+    the current stream (see BackProjector.process)."""
     _require_cuda_f32(proj, 3, "proj")
     _require_cuda_f32(volume, 3, "volume")
     n_proj = proj.shape[0]
@@ -221,6 +246,7 @@ def fbp_parallel(filt: ProjectionFilterSingle, bp: BackProjector, proj, angles, 
     check(_lib.load().mr_fbp_parallel(filt._plan, bp._plan, _ptr(proj), n_proj,
                                       a.ctypes.data_as(fp), w.ctypes.data_as(fp), _ptr(volume),
                                       _stream(stream)))
+    _order_after(stream, proj, volume)
     return volume
 
 

@@ -35,7 +35,14 @@ int GetModuleList(const char *application, void *listptr);
  * modules' own keys; the configuration keys are
  *   geometry=parallel|cone, center, direction=cw|ccw, resolution="px py",
  *   voxelsize="vx vy vz", sod, sdd, ppoint="px py", roi="x0 y0 x1 y1",
- *   projection_roi3, slice_block, usematrixroi=true|false, matrixroi="x0 y0 x1 y1".
+ *   projection_roi3, slice_block, usematrixroi=true|false, matrixroi="x0 y0 x1 y1",
+ *   projection_at_a_time=false|reference|final: instead of the SliceBlock loop,
+ *   ReconEngine::Run's projection-at-a-time path (ReconEngine.cpp:455-520):
+ *   each projection goes through the pre-processing module's Process(2D) and
+ *   the back-projector's Process(2D, angle, weight, last).  "reference" passes
+ *   Run's last flag (n_proj < i+1, never set, so the projections after the
+ *   last full ProjectionBufferSize buffer are not back-projected); "final"
+ *   sets it on the final projection, as StdBackProjectorBase::Process(3D) does.
  * Errors return a non-zero code; mre_last_error() describes them.
  * ------------------------------------------------------------------------ */
 typedef struct mre_engine mre_engine;

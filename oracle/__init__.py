@@ -174,6 +174,52 @@ def filter_rows_f32(rows: np.ndarray, ftype="ram-lak", cutoff=0.5, order=1.0, us
     return a
 
 
+def filter_rows_fft32_serial(rows: np.ndarray, ftype="ram-lak", cutoff=0.5, order=1.0, usebias=True,
+                             biasweight=0.1, paddingdoubler=2, batch: int = 512) -> np.ndarray:
+    """ProjectionFilterSingle with single-precision FFTs on one thread: the
+    CPU baseline's timing port of MuhRec's filter loop, which runs FFTW's
+    float r2c/c2r one row after the other on one core
+    (SPP/src/ProjectionFilter.cpp:304-344, KIPL/src/fft/fftbasef.cpp:156-251).
+    scipy's pocketfft (float32, workers=1) stands in for FFTW; padding,
+    filter table, Nyquist pass-through and the pi/(2P) scale are those of
+    ``filter_rows_f32`` (rows must be float32; a copy is returned).  It agrees
+    with ``filter_rows_f32`` to float FFT rounding
+    (tests/test_oracle_filter.py)."""
+    import scipy.fft as sfft
+    a = np.ascontiguousarray(rows, dtype=np.float32)
+    shape = a.shape
+    n = shape[-1]
+    a = a.reshape(-1, n)
+    P = filter_size(n, paddingdoubler)
+    N2 = P // 2
+    H = filter_kernel(ftype, P, cutoff, order, usebias, biasweight)
+    Hn = np.ones(N2 + 1, np.float32)
+    Hn[:N2] = H
+    tap = filter_taper(n, P)
+    ni = tap.size
+    scale = np.float32(np.float32(math.pi) / (np.float32(4.0) * np.float32(N2)))
+    out = np.empty_like(a)
+    pad = np.zeros((batch, P), np.float32)
+    # right taper: dest[P+15-i] = src[n-1]*w[i] for P+15-i < P, i.e. i >= 16
+    jr = np.arange(16, ni)
+    for r0 in range(0, a.shape[0], batch):
+        r1 = min(a.shape[0], r0 + batch)
+        b = pad[:r1 - r0]
+        b[:] = 0.0
+        src = a[r0:r1]
+        b[:, ni:ni + n] = src
+        b[:, :ni] = src[:, :1] * tap[None, :]
+        b[:, P + 15 - jr] = src[:, n - 1:n] * tap[None, jr]
+        X = sfft.rfft(b, axis=1, workers=1)
+        X *= Hn[None, :]
+        X[:, 0].imag = 0.0
+        X[:, N2].imag = 0.0
+        y = sfft.irfft(X, n=P, axis=1, workers=1)
+        # pocketfft's inverse carries 1/P; FFTW's c2r does not
+        out[r0:r1] = (y[:, ni:ni + n] * np.float32(P)) * scale
+    return out.reshape(shape)
+
+
 def filter_rows_f64(rows: np.ndarray, ftype="ram-lak", cutoff=0.5, order=1.0, usebias=True,
                     biasweight=0.1, paddingdoubler=2) -> np.ndarray:
     a = np.ascontiguousarray(rows, dtype=np.float32)

@@ -1,20 +1,18 @@
-"""GPU parity of every parallel-beam back-projection kernel variant.
+"""GPU parity of the parallel-beam back-projection kernel shapes.
 
-The library picks one kernel per plan; the environment variables
-MR_BP_IMPL (0 direct gathers, 1 LDS 16x16 tiles, 2 LDS 32x32 tiles with 2x2
-voxel groups, 3 LDS 32x16 tiles with voxel pairs), MR_BP_ORDER (0 grid order, 1 XCD-striped order), MR_BP_ZT and
-MR_BP_KC (slice tile, projection chunk) and MR_BP_PIPE (1: the unrolled form
-of the LDS kernel) force the others, so each variant is checked here against
-the oracle:
+A plan picks its kernel from the stack's shape: LDS-staged 16 x 16 voxel
+tiles with 32 slices and 4-projection chunks (at least 32 slices and 1024
+tiles), 16 slices and 8-projection chunks (16 to 31 slices, or fewer tiles),
+8 slices and 16-projection chunks (fewer than 16 slices); MultiProjBP with
+slices % 4 != 0 reads its second column at its own offset and takes the
+direct-gather kernel.  Each shape is checked here against the oracle:
 
 * exact mode: bit-exact against the MuhRec-faithful float oracle
   (MultiProjBPparallel.cpp:115-167 operation order);
-* fused mode: every variant gives the same bits as the 16x16 fused kernel
-  (the 2x2-group and pair kernels only add exact zero terms), and relRMSE <= 1e-6 of the
-  range against the oracle.
+* fused mode: relRMSE <= 1e-6 of the range against the oracle.
 
-Sizes are chosen to leave partial tiles on every edge (width 100 is not a
-multiple of 16 or 32; 30 and 17 slices are not multiples of 4 or 16).
+Sizes leave partial tiles on every edge (widths 100 and 520 are not
+multiples of 16; 30, 17 and 13 slices are not multiples of 4 or 16).
 """
 import numpy as np
 import pytest
@@ -24,22 +22,13 @@ from conftest import rel_err
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = [
-    # impl, order, zt, kc, pipe
-    (1, 0, 16, 16, 0), (1, 1, 16, 16, 0), (1, 1, 16, 8, 0), (1, 1, 16, 4, 0), (1, 1, 8, 32, 0), (1, 0, 32, 8, 0),
-    (1, 1, 32, 4, 0), (1, 1, 32, 8, 0), (1, 1, 64, 2, 0), (1, 1, 64, 4, 0),
-    (1, 1, 32, 4, 1), (1, 0, 16, 8, 1), (1, 1, 16, 8, 0), (1, 1, 64, 2, 1),
-    (2, 0, 16, 8, 0), (2, 1, 16, 8, 0), (2, 1, 16, 16, 0), (2, 1, 8, 8, 0), (2, 0, 8, 16, 0),
-    (3, 1, 16, 8, 0), (3, 0, 16, 4, 0), (3, 1, 8, 8, 0), (3, 1, 32, 4, 0),
-]
-
-
-def _set_variant(monkeypatch, impl, order, zt, kc, pipe=0):
-    monkeypatch.setenv("MR_BP_IMPL", str(impl))
-    monkeypatch.setenv("MR_BP_ORDER", str(order))
-    monkeypatch.setenv("MR_BP_ZT", str(zt))
-    monkeypatch.setenv("MR_BP_KC", str(kc))
-    monkeypatch.setenv("MR_BP_PIPE", str(pipe))
+# width, slices, projections, centre, direction, kernel the plan must launch
+SHAPES = [(520, 36, 21, 259.5, 0, "bp_lds_kernel<32,0,{},4,1> order=1 sw=4"),
+          (512, 64, 17, 255.0, 1, "bp_lds_kernel<32,0,{},4,1> order=1 sw=4"),
+          (100, 32, 45, 49.5, 0, "bp_lds_kernel<16,0,{},8,1> order=1 sw=4"),
+          (96, 30, 37, 47.0, 1, "bp_lds_kernel<16,0,{},8,1> order=1 sw=4"),
+          (64, 13, 20, 31.25, 0, "bp_lds_kernel<8,0,{},16,0> order=1 sw=4"),
+          (128, 4, 61, 66.0, 0, "bp_lds_kernel<8,0,{},16,0> order=1 sw=4")]
 
 
 def _case(seed, nu, nv, n_p):
@@ -50,55 +39,63 @@ def _case(seed, nu, nv, n_p):
     return proj, ang, w
 
 
-def _run(dev, proj, ang, w, nu, nv, center, fused, bufsize=16, direction=0):
+def _bp(nu, nv, center, fused, bufsize=16, direction=0, name="MultiProjBPparallel"):
     from imagingsuite_amd.recon import BackProjector
-    bp = BackProjector("MultiProjBPparallel", nu, nv, center, direction=direction,
-                       buffer_size=bufsize, fused=fused)
-    return bp.process(torch.from_numpy(proj).to(dev), ang, w).cpu().numpy()
+    return BackProjector(name, nu, nv, center, direction=direction, buffer_size=bufsize, fused=fused)
 
 
-@pytest.mark.parametrize("impl,order,zt,kc,pipe", VARIANTS)
-@pytest.mark.parametrize("nu,nv,n_p,center,direction", [(100, 32, 45, 49.5, 0), (96, 30, 37, 47.0, 1),
-                                                        (64, 17, 20, 31.25, 0), (128, 64, 61, 66.0, 0)])
-def test_variant_exact_bitexact(orc, dev, monkeypatch, impl, order, zt, kc, pipe, nu, nv, n_p, center,
-                                direction):
-    _set_variant(monkeypatch, impl, order, zt, kc, pipe)
+@pytest.mark.parametrize("nu,nv,n_p,center,direction,kernel", SHAPES)
+def test_shape_exact_bitexact(orc, dev, nu, nv, n_p, center, direction, kernel):
     proj, ang, w = _case(nu + nv + n_p, nu, nv, n_p)
-    vol = _run(dev, proj, ang, w, nu, nv, center, fused=False, direction=direction)
+    bp = _bp(nu, nv, center, False, direction=direction)
+    assert bp.kernel == kernel.format("true")
+    vol = bp.process(torch.from_numpy(proj).to(dev), ang, w).cpu().numpy()
     ref = orc.bp_parallel_f32(proj, ang, w, center, direction, "linear", 16, False)
     nbad = int(np.sum(vol != ref))
-    print(f"impl={impl} order={order} zt={zt} kc={kc} pipe={pipe} {nu}x{nv}x{n_p}: {nbad} voxels differ")
+    print(f"{bp.kernel} {nu}x{nv}x{n_p}: {nbad} voxels differ")
     assert nbad == 0
 
 
-@pytest.mark.parametrize("impl,order,zt,kc,pipe", [v for v in VARIANTS if v[0] in (2, 3) or v[4] == 1]
-                         + [(1, 1, 16, 16, 0)])
-def test_variant_fused_matches_lds16(orc, dev, monkeypatch, impl, order, zt, kc, pipe):
-    nu, nv, n_p, center = 300, 32, 120, 151.5
-    proj, ang, w = _case(7, nu, nv, n_p)
-    _set_variant(monkeypatch, 1, 0, 16, 16)
-    base = _run(dev, proj, ang, w, nu, nv, center, fused=True, bufsize=0)
-    _set_variant(monkeypatch, impl, order, zt, kc, pipe)
-    vol = _run(dev, proj, ang, w, nu, nv, center, fused=True, bufsize=0)
-    ref = orc.bp_parallel_f32(proj, ang, w, center, 0, "linear", 0, False)
+@pytest.mark.parametrize("nu,nv,n_p,center,direction,kernel", SHAPES)
+def test_shape_fused_close(orc, dev, nu, nv, n_p, center, direction, kernel):
+    proj, ang, w = _case(7 + nu + nv, nu, nv, n_p)
+    bp = _bp(nu, nv, center, True, bufsize=0, direction=direction)
+    assert bp.kernel == kernel.format("false")
+    vol = bp.process(torch.from_numpy(proj).to(dev), ang, w).cpu().numpy()
+    ref = orc.bp_parallel_f32(proj, ang, w, center, direction, "linear", 0, False)
     err = rel_err(vol, ref)[0]
-    print(f"fused impl={impl} order={order} zt={zt} kc={kc}: relRMSE vs oracle {err:.2e}, "
-          f"{int(np.sum(vol != base))} voxels differ from the 16x16 kernel")
+    print(f"{bp.kernel} {nu}x{nv}x{n_p}: relRMSE vs oracle {err:.2e}")
     assert err <= 1e-6
-    assert np.array_equal(vol, base)
 
 
-@pytest.mark.parametrize("zt,kc,pipe", [(32, 4, 1), (16, 8, 1), (16, 16, 0)])
-@pytest.mark.parametrize("nu,nv,n_p,center", [(100, 30, 45, 49.5), (64, 17, 20, 31.25)])
-def test_stage64_path_bitexact(orc, dev, monkeypatch, zt, kc, pipe, nu, nv, n_p, center):
-    """MR_BP_STAGE64=1 forces the staging with 64-bit addresses that chunks
-    too large for 32-bit offsets take (never needed at sizes that fit one
-    GPU's memory): bit-exact against the oracle like the default path."""
-    _set_variant(monkeypatch, 1, 1, zt, kc, pipe)
+@pytest.mark.parametrize("nu,nv,interp", [(100, 17, "linear"), (64, 6, "linear"), (90, 21, "nearest")])
+def test_direct_kernel_multiprojbp_offset(orc, dev, nu, nv, interp):
+    """MultiProjBP with slices % 4 != 0 (its second column starts 4*(nv/4)
+    after the first, MultiProjBP.cpp:41) runs the direct-gather kernel:
+    bit-exact against the oracle's MultiProjBP flavour; NearestNeighborBP
+    with such slice counts stays on the LDS kernel."""
+    from imagingsuite_amd.recon import BackProjector
+    n_p = 33
+    proj, ang, w = _case(nu * nv, nu, nv, n_p)
+    name = "NearestNeighborBP" if interp == "nearest" else "MultiProjBP"
+    bp = BackProjector(name, nu, nv, nu / 2 + 0.5, buffer_size=16)
+    vol = bp.process(torch.from_numpy(proj).to(dev), ang, w).cpu().numpy()
+    ref = orc.bp_parallel_f32(proj, ang, w, nu / 2 + 0.5, 0, interp, 16, name == "MultiProjBP")
+    print(f"{name} {bp.kernel} {nu}x{nv}: {int(np.sum(vol != ref))} voxels differ")
+    assert np.array_equal(vol, ref)
+
+
+@pytest.mark.parametrize("nu,nv,n_p,center", [(520, 36, 21, 259.5), (100, 30, 45, 49.5), (64, 13, 20, 31.25)])
+def test_stage64_path_bitexact(orc, dev, monkeypatch, nu, nv, n_p, center):
+    """MR_BP_STAGE64=1 (the library's one debug override, read when a plan
+    is created) forces the staging with 64-bit addresses that chunks too
+    large for 32-bit offsets take (never needed at sizes that fit one GPU's
+    memory): bit-exact against the oracle like the default path."""
     monkeypatch.setenv("MR_BP_STAGE64", "1")
     proj, ang, w = _case(nu + nv + n_p + 1, nu, nv, n_p)
-    vol = _run(dev, proj, ang, w, nu, nv, center, fused=False)
+    bp = _bp(nu, nv, center, False)
+    vol = bp.process(torch.from_numpy(proj).to(dev), ang, w).cpu().numpy()
     ref = orc.bp_parallel_f32(proj, ang, w, center, 0, "linear", 16, False)
     nbad = int(np.sum(vol != ref))
-    print(f"stage64 zt={zt} kc={kc} pipe={pipe} {nu}x{nv}x{n_p}: {nbad} voxels differ")
+    print(f"stage64 {bp.kernel} {nu}x{nv}x{n_p}: {nbad} voxels differ")
     assert nbad == 0

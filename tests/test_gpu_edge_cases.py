@@ -81,3 +81,36 @@ def test_bp_calls_on_two_streams_are_ordered(dev):
         nbad = [int((o != r).sum()) for o, r in zip(outs, ref)]
         print(f"two streams, repeat {rep}: voxels differing {nbad}", flush=True)
         assert sum(nbad) == 0
+
+
+def test_side_stream_result_ordered_before_current_stream(dev):
+    """A back-projection enqueued on another stream returns before its
+    kernels end; the volume it allocates must still be complete for work the
+    caller then queues on the current stream (no explicit synchronisation),
+    and the inputs may be dropped right after the call."""
+    from imagingsuite_amd.recon import BackProjector, ProjectionFilterSingle, fbp_parallel
+    rng = np.random.default_rng(22)
+    nu, nv, n_p = 512, 64, 720
+    proj = rng.normal(size=(n_p, nv, nu)).astype(np.float32)
+    a = np.linspace(0, 180, n_p, endpoint=False).astype(np.float32)
+    w = np.full(n_p, 1.0 / n_p, np.float32)
+    bp = BackProjector("MultiProjBPparallel", nu, nv, nu / 2 + 0.5, buffer_size=0)
+    ref = bp.process(_gpu(proj, dev), a, w).cpu()
+    st = torch.cuda.Stream(device=dev)
+    for rep in range(3):
+        vol = bp.process(_gpu(proj, dev), a, w, stream=st)   # input dropped at once
+        got = vol.cpu()                                       # queued on the current stream
+        nbad = int((got != ref).sum())
+        print(f"side stream, repeat {rep}: voxels differing {nbad}", flush=True)
+        assert nbad == 0
+    filt = ProjectionFilterSingle(nu, "Hanning")
+    t = _gpu(proj, dev)
+    out = torch.empty((nu, nu, nv), dtype=torch.float32, device=dev)
+    fbp_parallel(filt, bp, t, a, w, out)
+    ref2 = out.cpu()
+    for rep in range(2):
+        t = _gpu(proj, dev)
+        out = torch.empty((nu, nu, nv), dtype=torch.float32, device=dev)
+        fbp_parallel(filt, bp, t, a, w, out, stream=st)
+        del t
+        assert torch.equal(out.cpu(), ref2)

@@ -25,15 +25,14 @@ def _case(seed, nu=64, nv=8, n_p=45):
     return rng, ang, w
 
 
-@pytest.mark.parametrize("impl", [1, 2, 0])
 @pytest.mark.parametrize("nu,nv,bufsize,interp", [(64, 8, 16, "linear"), (97, 12, 0, "linear"),
                                                   (64, 4, 16, "nearest"), (150, 6, 16, "linear"),
-                                                  (200, 32, 0, "linear"), (120, 19, 16, "nearest")])
-def test_bp_adjoint_matches_oracle(orc, dev, monkeypatch, impl, nu, nv, bufsize, interp):
-    """impl 1: the chunked LDS kernel with one barrier per line (default;
-    32-, 16- and 4-slice groups); impl 2: the chunked kernel with per-line
-    barriers; impl 0: the direct kernel."""
-    monkeypatch.setenv("MR_BP_ADJ", str(impl))
+                                                  (200, 32, 0, "linear"), (120, 19, 16, "nearest"),
+                                                  (256, 64, 0, "linear"), (130, 64, 16, "nearest"),
+                                                  (333, 40, 0, "linear"), (64, 16, 0, "linear")])
+def test_bp_adjoint_matches_oracle(orc, dev, nu, nv, bufsize, interp):
+    """Slice tiles of 64, 32, 16, 8 and 4 (the largest dividing 4*(nv/4)),
+    odd widths, both interpolations, with and without the buffer drop."""
     from imagingsuite_amd.recon import BackProjector
     rng, ang, w = _case(nu + nv, nu, nv)
     n_p = ang.size
@@ -48,27 +47,41 @@ def test_bp_adjoint_matches_oracle(orc, dev, monkeypatch, impl, nu, nv, bufsize,
     bps = bp.process(torch.from_numpy(s).to(dev), ang, w).cpu().numpy().astype(np.float64)
     lhs = float((bps * G).sum())
     rhs = float((s.astype(np.float64) * got.astype(np.float64)).sum())
-    print(f"BP^T impl={impl} {name} nu={nu} nv={nv}: vs FP64 {e[0]:.2e}/{e[1]:.2e}, dot {abs(lhs - rhs) / abs(lhs):.2e}")
+    print(f"BP^T {name} nu={nu} nv={nv}: vs FP64 {e[0]:.2e}/{e[1]:.2e}, dot {abs(lhs - rhs) / abs(lhs):.2e}")
     assert e[0] <= 1e-6
     assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
 
 
-@pytest.mark.parametrize("nu,nv,bufsize,interp", [(97, 12, 0, "linear"), (200, 32, 16, "linear"),
-                                                  (120, 19, 16, "nearest"), (256, 64, 0, "linear")])
-def test_bp_adjoint_one_barrier_matches_chunked(dev, monkeypatch, nu, nv, bufsize, interp):
-    """The one-barrier-per-line adjoint (precomputed spans, double-buffered
-    staging) stages the same voxels at the same offsets and sums in the same
-    order as the per-line-group form (MR_BP_ADJ=2): identical bits."""
+@pytest.mark.parametrize("interp", ["linear", "nearest"])
+def test_bp_adjoint_every_angle_class(orc, dev, interp):
+    """Angles across the full circle (both stepping classes, both
+    directions, the axis-aligned views) and a centre off the middle, so that
+    bins at both detector edges, including bins 0 and 1, receive voxels."""
     from imagingsuite_amd.recon import BackProjector
-    rng, ang, w = _case(nu + 3 * nv, nu, nv, 61)
-    name = "NearestNeighborBP" if interp == "nearest" else "MultiProjBPparallel"
-    bp = BackProjector(name, nu, nv, nu / 2 + 0.25, buffer_size=bufsize)
-    G = torch.from_numpy(rng.normal(size=(nu, nu, nv)).astype(np.float32)).to(dev)
-    fast = bp.adjoint(G, ang, w).cpu().numpy()
-    monkeypatch.setenv("MR_BP_ADJ", "2")
-    chunked = bp.adjoint(G, ang, w).cpu().numpy()
-    print(f"BP^T one-barrier vs chunked {name} nu={nu} nv={nv}: {int(np.sum(fast != chunked))} values differ")
-    assert np.array_equal(fast, chunked)
+    rng = np.random.default_rng(5)
+    nu, nv, n_p = 90, 8, 73
+    ang = np.concatenate([np.arange(0, 360, 10.0), rng.uniform(0, 360, n_p - 36)]).astype(np.float32)
+    w = rng.uniform(0.5, 1.5, n_p).astype(np.float32)
+    for center in (nu / 2 + 0.5, 12.25, nu - 9.5):
+        name = "NearestNeighborBP" if interp == "nearest" else "MultiProjBPparallel"
+        bp = BackProjector(name, nu, nv, center, buffer_size=0)
+        G = rng.normal(size=(nu, nu, nv)).astype(np.float32)
+        got = bp.adjoint(torch.from_numpy(G).to(dev), ang, w).cpu().numpy()
+        ref = orc.bp_parallel_adjoint_f64(G, n_p, ang, w, center, 0, interp, 0)
+        e = rel_err(got, ref)
+        print(f"BP^T {interp} centre {center}: vs FP64 {e[0]:.2e}/{e[1]:.2e}")
+        assert e[0] <= 1e-6 and e[1] <= 1e-5
+
+
+def test_bp_adjoint_rejects_too_wide(dev):
+    """The adjoint keeps every line's span in LDS: widths beyond 12288 are
+    refused with an error, before any kernel runs."""
+    from imagingsuite_amd._lib import MrError
+    from imagingsuite_amd.recon import BackProjector
+    nu = 12289
+    bp = BackProjector("MultiProjBPparallel", nu, 4, nu / 2, buffer_size=0)
+    with pytest.raises(MrError, match="12288"):
+        bp.adjoint(torch.zeros((nu, nu, 4), device=dev), np.zeros(1, np.float32), np.ones(1, np.float32))
 
 
 @pytest.mark.parametrize("nu,ftype", [(64, "hamming"), (577, "hanning"), (2048, "ram-lak"), (30, "shepp-logan")])

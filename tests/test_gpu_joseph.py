@@ -27,8 +27,11 @@ def _setup(nx=64, nz=4, n_ang=45, seed=0):
     return rng, geom
 
 
-@pytest.mark.parametrize("nx,nz", [(64, 4), (96, 16), (50, 8), (70, 32)])
+@pytest.mark.parametrize("nx,nz", [(64, 4), (96, 16), (50, 8), (70, 32), (128, 64), (40, 12)])
 def test_pair_matches_fp64_oracle(orc, dev, nx, nz):
+    """Every kernel shape: the LDS-staged forward (one barrier per line) and
+    adjoint for slice counts that are multiples of 16 (16- and 32-slice
+    tiles), the direct gathers otherwise."""
     rng, geom = _setup(nx, nz, 37, seed=nx)
     x = rng.random((nx, nx, nz)).astype(np.float32)
     y = rng.random((geom.angles.size, geom.n_det, nz)).astype(np.float32)
@@ -41,35 +44,20 @@ def test_pair_matches_fp64_oracle(orc, dev, nx, nz):
     assert ea[0] <= 1e-6 and et[0] <= 1e-6
 
 
-@pytest.mark.parametrize("nx,nz", [(96, 16), (70, 32), (128, 64)])
-def test_adjoint_lds_matches_direct(dev, monkeypatch, nx, nz):
-    """The LDS-staged adjoint (16- and 32-slice tiles) performs the same float
-    operations per voxel as the direct kernel (bins outside the detector are
-    staged zeros where the direct kernel has zero weights): identical bits."""
-    rng, geom = _setup(nx, nz, 61, seed=nx + nz)
-    y = torch.from_numpy(rng.random((geom.angles.size, geom.n_det, nz)).astype(np.float32)).to(dev)
-    lds = geom.adjoint(y).cpu().numpy()
-    monkeypatch.setenv("MR_JOSEPH_IMPL", "0")
-    direct = geom.adjoint(y).cpu().numpy()
-    print(f"adjoint LDS vs direct nx={nx} nz={nz}: {int(np.sum(lds != direct))} voxels differ")
-    assert np.array_equal(lds, direct)
-
-
-@pytest.mark.parametrize("nx,nz,n_ang", [(96, 16, 61), (70, 32, 37), (128, 64, 90), (40, 32, 7)])
-def test_forward_one_barrier_matches_staged(dev, monkeypatch, nx, nz, n_ang):
-    """The forward with precomputed line spans and double-buffered staging
-    (default) stages the same voxels at the same offsets and sums the same
-    terms in the same line order as the per-line-barrier form
-    (MR_JOSEPH_FWD=1): identical bits, also for partial bin blocks (n_det not
-    a multiple of 16) and chunks shorter than 16 angles."""
-    rng, geom = _setup(nx, nz, n_ang, seed=nx + nz + n_ang)
-    x = torch.from_numpy(rng.random((nx, nx, nz)).astype(np.float32)).to(dev)
-    fast = geom.forward(x).cpu().numpy()
-    monkeypatch.setenv("MR_JOSEPH_FWD", "1")
-    staged = geom.forward(x).cpu().numpy()
-    print(f"forward one-barrier vs per-line nx={nx} nz={nz} n_ang={n_ang}: "
-          f"{int(np.sum(fast != staged))} bins differ")
-    assert np.array_equal(fast, staged)
+def test_forward_wide_volume_vs_fp64(orc, dev):
+    """Volumes wider than 4096 take the per-line-barrier forward (the span
+    table would not fit in LDS beside the staging buffers): checked against
+    the FP64 oracle on a thin problem.  Each bin sums about 4160 float
+    products along its ray, so the float rounding grows with the width: the
+    bound is 1e-5 here instead of 1e-6 (measured 2.7e-6)."""
+    nx, nz = 4160, 16
+    rng, geom = _setup(nx, nz, 5, seed=3)
+    x = rng.random((nx, nx, nz)).astype(np.float32)
+    Ax = geom.forward(torch.from_numpy(x).to(dev)).cpu().numpy()
+    ref = orc.joseph_forward_f64(x, geom.angles.astype(np.float64), geom.n_det, float(geom.g.center))
+    e = rel_err(Ax, ref)
+    print(f"A nx={nx} vs FP64 {e[0]:.2e}/{e[1]:.2e}")
+    assert e[0] <= 1e-5
 
 
 def test_dot_product_gpu(dev):

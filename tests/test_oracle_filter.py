@@ -108,3 +108,17 @@ def test_taper_is_sigmoid_padding(orc):
     assert tap[0] < 1e-3 and tap[-1] > 0.999 and np.all(np.diff(tap) > 0)
     # Sigmoid(x, 0, 0.07) at x = 0 (i = nInsert/2) is exactly one half
     assert abs(tap[192] - 0.5) < 1e-7
+
+
+@pytest.mark.parametrize("n,ftype", [(256, "ram-lak"), (2048, "hanning"), (100, "hamming"), (37, "shepp-logan")])
+def test_single_thread_float_fft_port_matches_oracle(orc, n, ftype):
+    """The CPU baseline's timing port of MuhRec's single-threaded float FFT
+    loop (filter_rows_fft32_serial) computes the same filter as the oracle:
+    padding with the 16-sample gap, H, the Nyquist pass-through and the
+    pi/(2P) scale.  Only float FFT rounding separates them; a dropped scale,
+    a shifted crop or a filtered Nyquist bin would be far above 5e-6."""
+    rng = np.random.default_rng(n)
+    x = (rng.random((48, n)) + 0.5).astype(np.float32)
+    a = orc.filter_rows_f32(x, ftype)
+    b = orc.filter_rows_fft32_serial(x, ftype, batch=17)
+    assert np.abs(a - b).max() <= 5e-6 * np.abs(a).max()
