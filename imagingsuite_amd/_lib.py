@@ -35,6 +35,7 @@ EXPORTED_SYMBOLS = (
     "mr_joseph_forward", "mr_joseph_adjoint", "mr_linear_forward",
     "mr_iter_plan_create", "mr_iter_run", "mr_iter_plan_destroy",
     "mr_fbp_stream_create", "mr_fbp_stream_run", "mr_fbp_stream_destroy", "mr_host_alloc", "mr_host_free",
+    "mr_copy_h2d", "mr_copy_d2h", "mr_volume_zxy_to_xyz",
     "mr_lognorm_reference", "mr_lognorm", "mr_lognorm_logdose",
     "mr_batch_plan_create", "mr_batch_plan_info", "mr_batch_run", "mr_batch_run_graph",
     "mr_batch_run_sequential", "mr_batch_plan_destroy",
@@ -118,6 +119,9 @@ def load():
     L.mr_bp_plan_mask.argtypes = [vp, szp, szp]
     L.mr_bp_plan_kernel.argtypes = [vp, ctypes.c_char_p, sz]
     L.mr_bp_parallel.argtypes = [vp, vp, sz, fp, fp, vp, i32, vp]
+    L.mr_copy_h2d.argtypes = [vp, vp, sz, vp]
+    L.mr_copy_d2h.argtypes = [vp, vp, sz, vp]
+    L.mr_volume_zxy_to_xyz.argtypes = [vp, vp, sz, sz, vp]
     L.mr_bp_parallel_noalloc.argtypes = [ctypes.POINTER(BpParams), vp, sz, fp, fp, vp, i32, vp]
     L.mr_fbp_parallel.argtypes = [vp, vp, vp, sz, fp, fp, vp, vp]
     L.mr_fdk_window.argtypes = [ctypes.POINTER(FdkParams), szp]

@@ -87,6 +87,17 @@ std::string list_text(const float *v, size_t n)
 
 }  // namespace
 
+// Interactor that asks for an abort from its `limit`-th progress report on
+// (the GUI's cancel button, pressed at a known point).
+class AbortAfter : public kipl::interactors::InteractionBase {
+public:
+    explicit AbortAfter(int limit) : limit_(limit) {}
+    bool SetProgress(float, std::string) override { return ++calls_ >= limit_; }
+
+private:
+    int limit_, calls_ = 0;
+};
+
 struct mre_engine {
     void *so = nullptr;
     GetModule_t get = nullptr;
@@ -101,6 +112,7 @@ struct mre_engine {
     // (never set, ReconEngine.cpp:505); 2: the same with the last flag on the
     // final projection
     int projection_at_a_time = 0;
+    AbortAfter *abort_bp = nullptr;   // bp_abort_after: the back-projector's interactor
 };
 
 extern "C" const char *mre_last_error(void) { return g_err.c_str(); }
@@ -113,6 +125,7 @@ extern "C" int mre_engine_destroy(mre_engine *e)
         if (e->bp) e->destroy("muhrecbp", e->bp);
     }
     if (e->so) dlclose(e->so);
+    delete e->abort_bp;
     delete e;
     return 0;
 }
@@ -145,7 +158,10 @@ extern "C" int mre_engine_create(mre_engine **engine, const char *module_so, con
             if (!e->pre) throw ReconException(std::string("Failed to create module ") + preproc_module, __FILE__, __LINE__);
             e->pre->Configure(e->cfg, e->pre_params);
         }
-        e->bp = static_cast<BackProjectorModuleBase *>(e->get("muhrecbp", bp_module, nullptr));
+        const auto kvc = parse_kv(config);
+        const auto ab = kvc.find("bp_abort_after");
+        if (ab != kvc.end()) e->abort_bp = new AbortAfter(std::stoi(ab->second));
+        e->bp = static_cast<BackProjectorModuleBase *>(e->get("muhrecbp", bp_module, e->abort_bp));
         if (!e->bp) throw ReconException(std::string("Failed to create module ") + bp_module, __FILE__, __LINE__);
         e->bp->Configure(e->cfg, e->bp_params);
         e->slice_block = (size_t)GetIntParameter(e->bp_params, "SliceBlock");

@@ -172,9 +172,9 @@ private:
                                             m_bUseBias ? 1 : 0, m_fBiasWeight, m_nPaddingDoubler));
         const size_t n = n_rows * width;
         buf_.reserve(n);
-        MRM_HIP(hipMemcpy(buf_.p, rows, n * sizeof(float), hipMemcpyHostToDevice));
+        MRM_CHECK(mr_copy_h2d(buf_.p, rows, n * sizeof(float), nullptr));
         MRM_CHECK(mr_filter_rows(plan_, buf_.p, n_rows, nullptr));
-        MRM_HIP(hipMemcpy(rows, buf_.p, n * sizeof(float), hipMemcpyDeviceToHost));
+        MRM_CHECK(mr_copy_d2h(rows, buf_.p, n * sizeof(float), nullptr));
     }
 
     int m_FilterType = 3;  // Hamming
@@ -273,10 +273,10 @@ protected:
         if ((nOBCount != 0 || nDCCount != 0) && npix_ != N)
             throw ReconException("Flat field does not match projection", __FILE__, __LINE__);
         buf_.reserve(N);
-        MRM_HIP(hipMemcpy(buf_.p, img.GetDataPtr(), N * sizeof(float), hipMemcpyHostToDevice));
+        MRM_CHECK(mr_copy_h2d(buf_.p, img.GetDataPtr(), N * sizeof(float), nullptr));
         MRM_CHECK(mr_lognorm_logdose(buf_.p, 1, N, nOBCount != 0 ? dflat_.p : nullptr,
                                      nDCCount != 0 ? ddark_.p : nullptr, &cLogDose, nullptr));
-        MRM_HIP(hipMemcpy(img.GetDataPtr(), buf_.p, N * sizeof(float), hipMemcpyDeviceToHost));
+        MRM_CHECK(mr_copy_d2h(img.GetDataPtr(), buf_.p, N * sizeof(float), nullptr));
         return 0;
     }
 
@@ -294,10 +294,10 @@ protected:
         if (bUseNormROI) GetFloatParameterVector(coeff, "dose", dose.data(), (int)nProj);
         if (UpdateStatus(0.0f, m_sModuleName)) return 0;
         buf_.reserve(img.Size());
-        MRM_HIP(hipMemcpy(buf_.p, img.GetDataPtr(), img.Size() * sizeof(float), hipMemcpyHostToDevice));
+        MRM_CHECK(mr_copy_h2d(buf_.p, img.GetDataPtr(), img.Size() * sizeof(float), nullptr));
         MRM_CHECK(mr_lognorm(buf_.p, nProj, N, nOBCount != 0 ? dflat_.p : nullptr, nDCCount != 0 ? ddark_.p : nullptr,
                              bUseNormROI ? dose.data() : nullptr, fDarkDose, nullptr));
-        MRM_HIP(hipMemcpy(img.GetDataPtr(), buf_.p, img.Size() * sizeof(float), hipMemcpyDeviceToHost));
+        MRM_CHECK(mr_copy_d2h(img.GetDataPtr(), buf_.p, img.Size() * sizeof(float), nullptr));
         UpdateStatus(1.0f, m_sModuleName);
         return 0;
     }
@@ -313,12 +313,19 @@ private:
 
 // -------------------------------------------------------------------------
 // MultiProjBPparallel / MultiProjBP / NearestNeighborBP
+//
+// The volume is kept on the device in the kernels' ZXY layout and handed to
+// MuhRec as a MatrixXYZ TImage (transposed on the GPU before the copy back):
+// MuhRec reads slices through the non-virtual GetSlice, which copies an XYZ
+// slice row by row instead of gathering it with a stride
+// (BackProjectorModuleBase.cpp:62-124); MatrixAlignment tells every reader
+// which layout the volume has (the FDK modules use XYZ as well).
 // -------------------------------------------------------------------------
 class HipStdBackProjector : public BackProjectorModuleBase {
 public:
     HipStdBackProjector(const std::string &name, int interp, int multiprojbp,
                         kipl::interactors::InteractionBase *interactor)
-        : BackProjectorModuleBase("muhrec", name, MatrixZXY, interactor), interp_(interp),
+        : BackProjectorModuleBase("muhrec", name, MatrixXYZ, interactor), interp_(interp),
           multiprojbp_(multiprojbp)
     {
         nSubVolume[0] = nSubVolume[1] = 1;
@@ -362,9 +369,9 @@ public:
                     ? roi[3]
                     : roi[3] - roi[1];
         for (int i = 0; i < 4; ++i) mConfig.ProjectionInfo.roi[i] = roi[i];
-        MatrixDims[0] = SizeV;
+        MatrixDims[0] = SizeU;
         MatrixDims[1] = SizeU;
-        MatrixDims[2] = SizeU;
+        MatrixDims[2] = SizeV;
         volume.Resize(MatrixDims);
         volume = 0.0f;
         BuildCircleMask();
@@ -376,13 +383,18 @@ public:
         p.center = mConfig.ProjectionInfo.fCenter;
         p.direction = (int)mConfig.ProjectionInfo.eDirection;
         p.interp = interp_;
-        p.buffer_size = 0;  // the module applies ProjectionBufferSize itself (see flush)
+        // a stack handed to Process(3D) from an empty buffer is back-projected
+        // in one go with the drop done by the kernels' used-projection list:
+        // every ProjectionBufferSize-th projection (sizes 0 and 1 drop every
+        // projection, as the reference's counter does); see Process
+        p.buffer_size = bufferDrop_ ? (int)std::max<size_t>(nProjectionBufferSize, 1) : 0;
         p.multiprojbp = multiprojbp_;
         p.use_roi = mConfig.MatrixInfo.bUseROI ? 1 : 0;
         for (int i = 0; i < 4; ++i) p.roi[i] = mConfig.MatrixInfo.roi[i];
         p.fused = fused_ ? 1 : 0;
         MRM_CHECK(mr_bp_plan_create(&plan_, &p));
         dvol_.reserve(volume.Size());
+        dxyz_.reserve(volume.Size());
         MRM_HIP(hipMemset(dvol_.p, 0, volume.Size() * sizeof(float)));
         pending_.clear();
         pending_angle_.clear();
@@ -390,9 +402,10 @@ public:
         nProjCounter = 0;
     }
 
-    // StdBackProjectorBase::Process(2D): buffer the projection; when the
-    // buffer is full the newest one is dropped (reference :104-114) unless
-    // BufferDrop is false; the buffered ones are back-projected.
+    // StdBackProjectorBase::Process(2D) (reference :64-118): buffer the
+    // projection; when the last flag is set or the buffer is full, the
+    // buffered projections are back-projected, without the newest one when
+    // the buffer was full (unless BufferDrop is false).
     size_t Process(kipl::base::TImage<float, 2> proj, float angle, float weight, bool bLast) override
     {
         if (volume.Size() == 0) throw ReconException("The target matrix is not allocated.", __FILE__, __LINE__);
@@ -413,7 +426,18 @@ public:
         return nProjCounter;
     }
 
-    // StdBackProjectorBase::Process(3D) (reference :120-149).
+    // StdBackProjectorBase::Process(3D) (reference :120-149): the reference
+    // feeds the projections to Process(2D) one by one with the last flag on
+    // the final one.  From an empty buffer that back-projects every
+    // projection except each ProjectionBufferSize-th (all of them with
+    // BufferDrop false), in order, which is exactly the plan's used list: the
+    // stack is copied to the device once and back-projected in launches of
+    // whole buffers, with the progress/abort poll between launches (the
+    // reference polls per projection; here every 256 projections or so,
+    // about 20 ms of GPU work at 2048^2 x 256).  Per voxel the additions
+    // happen in the same order as one launch, so the result is bit-identical
+    // to it.  A stack arriving while Process(2D) has projections buffered
+    // continues that buffer projection by projection.
     size_t Process(kipl::base::TImage<float, 3> projections, std::map<std::string, std::string> parameters) override
     {
         if (volume.Size() == 0) throw ReconException("The target matrix is not allocated.", __FILE__, __LINE__);
@@ -424,52 +448,63 @@ public:
         GetFloatParameterVector(parameters, "weights", weights.data(), (int)nProj);
         GetFloatParameterVector(parameters, "angles", angles.data(), (int)nProj);
         const size_t psz = SizeU * SizeV;
-        // The reference back-projects each full buffer in turn; the order of
-        // additions per voxel is the projection order either way, so the used
-        // projections are collected and back-projected in one launch.
-        for (size_t i = 0; i < nProj; ++i) {
-            if (UpdateStatus((float)i / nProj, "Back-projecting")) {
-                discard();
+        if (nProjCounter != 0 || !pending_angle_.empty()) {
+            for (size_t i = 0; i < nProj; ++i) {
+                if (UpdateStatus((float)i / nProj, "Back-projecting")) {
+                    discard();
+                    return 0;
+                }
+                kipl::base::TImage<float, 2> img(projections.Dims());
+                std::memcpy(img.GetDataPtr(), projections.GetLinePtr(0, i), sizeof(float) * psz);
+                Process(img, angles[i], weights[i], i == nProj - 1);
+            }
+            return 0;
+        }
+        if (UpdateStatus(0.0f, "Back-projecting")) return 0;
+        dproj_.reserve(nProj * psz);
+        MRM_CHECK(mr_copy_h2d(dproj_.p, projections.GetDataPtr(), nProj * psz * sizeof(float), nullptr));
+        // launches of whole buffers, so that the drop pattern restarts where
+        // the reference's buffer does
+        const size_t B = std::max<size_t>(nProjectionBufferSize, 1);
+        const size_t chunk = std::max<size_t>(B, (256 + B - 1) / B * B);
+        for (size_t p0 = 0; p0 < nProj; p0 += chunk) {
+            if (p0 > 0 && UpdateStatus((float)p0 / nProj, "Back-projecting")) {
+                MRM_HIP(hipDeviceSynchronize());
+                download();   // what was back-projected so far, as the reference's volume holds it
                 return 0;
             }
-            pending_.insert(pending_.end(), projections.GetLinePtr(0, i), projections.GetLinePtr(0, i) + psz);
-            pending_angle_.push_back(angles[i]);
-            pending_weight_.push_back(weights[i]);
-            ++nProjCounter;
-            if (i == nProj - 1 || nProjectionBufferSize <= nProjCounter) {
-                if (nProjectionBufferSize <= nProjCounter && bufferDrop_) {
-                    pending_.resize(pending_.size() - psz);
-                    pending_angle_.pop_back();
-                    pending_weight_.pop_back();
-                }
-                nProjCounter = 0;
-            }
+            const size_t n = std::min(chunk, nProj - p0);
+            MRM_CHECK(mr_bp_parallel(plan_, dproj_.p + p0 * psz, n, angles.data() + p0, weights.data() + p0,
+                                     dvol_.p, 1, nullptr));
         }
-        flush();
+        download();
+        UpdateStatus(1.0f, "Back-projecting");
         return 0;
     }
 
     void GetHistogram(float *axis, size_t *hist, size_t nBins) override
     {
         // Min()/Max() of the reference read the first slice of each masked
-        // column (StdBackProjectorBase.cpp:282-320).
+        // column (StdBackProjectorBase.cpp:282-320); the volume is XYZ here:
+        // voxel (x, y, z) at volume.GetLinePtr(y, z)[x]
         float mn = std::numeric_limits<float>::max(), mx = -std::numeric_limits<float>::max();
-        for (size_t y = 0; y < mask.size(); ++y)
+        for (size_t y = 0; y < mask.size(); ++y) {
+            const float *row = volume.GetLinePtr(y, 0);
             for (size_t x = mask[y].first; x < mask[y].second; ++x) {
-                const float v = volume.GetLinePtr(x, y)[0];
-                mn = std::min(mn, v);
-                mx = std::max(mx, v);
+                mn = std::min(mn, row[x]);
+                mx = std::max(mx, row[x]);
             }
+        }
         std::memset(axis, 0, sizeof(float) * nBins);
         std::memset(hist, 0, sizeof(size_t) * nBins);
         const float scale = (mx - mn) / (nBins + 1);
         const float inv = 1.0f / scale;
-        for (size_t y = 0; y < mask.size(); ++y)
-            for (size_t x = mask[y].first; x < mask[y].second; ++x) {
-                const float *p = volume.GetLinePtr(x, y);
-                for (size_t z = 0; z < volume.Size(0); ++z) {
-                    size_t idx = static_cast<size_t>(inv * (p[z] - mn));
-                    if (p[z] < mn) idx = 0;
+        for (size_t z = 0; z < volume.Size(2); ++z)
+            for (size_t y = 0; y < mask.size(); ++y) {
+                const float *row = volume.GetLinePtr(y, z);
+                for (size_t x = mask[y].first; x < mask[y].second; ++x) {
+                    size_t idx = static_cast<size_t>(inv * (row[x] - mn));
+                    if (row[x] < mn) idx = 0;
                     else if (nBins <= idx) idx = nBins - 1;
                     hist[idx]++;
                 }
@@ -478,27 +513,28 @@ public:
         for (size_t i = 1; i < nBins; ++i) axis[i] = axis[i - 1] + scale;
     }
 
-    void GetMatrixDims(size_t *dims) override
-    {
-        dims[0] = volume.Size(1);
-        dims[1] = volume.Size(2);
-        dims[2] = volume.Size(0);
-    }
-
 private:
-    // Back-project every buffered projection (one kernel launch) into the
-    // device volume and refresh the host volume.
+    // Back-project every buffered projection (the buffer holds at most
+    // ProjectionBufferSize - 1 after the drop, so the plan's drop rule never
+    // fires) into the device volume and refresh the host volume.
     void flush()
     {
         const size_t n = pending_angle_.size();
         if (n > 0) {
             dproj_.reserve(pending_.size());
-            MRM_HIP(hipMemcpy(dproj_.p, pending_.data(), pending_.size() * sizeof(float), hipMemcpyHostToDevice));
+            MRM_CHECK(mr_copy_h2d(dproj_.p, pending_.data(), pending_.size() * sizeof(float), nullptr));
             MRM_CHECK(mr_bp_parallel(plan_, dproj_.p, n, pending_angle_.data(), pending_weight_.data(), dvol_.p,
                                      1, nullptr));
-            MRM_HIP(hipMemcpy(volume.GetDataPtr(), dvol_.p, volume.Size() * sizeof(float), hipMemcpyDeviceToHost));
+            download();
         }
         discard();
+    }
+
+    // host volume (XYZ) from the device volume (ZXY)
+    void download()
+    {
+        MRM_CHECK(mr_volume_zxy_to_xyz(dvol_.p, dxyz_.p, SizeU, SizeV, nullptr));
+        MRM_CHECK(mr_copy_d2h(volume.GetDataPtr(), dxyz_.p, volume.Size() * sizeof(float), nullptr));
     }
 
     void discard()
@@ -515,7 +551,7 @@ private:
     bool bufferDrop_ = true, fused_ = false;
     std::vector<float> pending_, pending_angle_, pending_weight_;
     mr_bp_plan *plan_ = nullptr;
-    DevBuf dproj_, dvol_;
+    DevBuf dproj_, dvol_, dxyz_;
 };
 
 // -------------------------------------------------------------------------
@@ -620,11 +656,10 @@ public:
         if (UpdateStatus(0.0f, "FDK back-projection")) return 0;
         dproj_.reserve(projections.Size());
         dvol_.reserve(volume.Size());
-        MRM_HIP(hipMemcpy(dproj_.p, projections.GetDataPtr(), projections.Size() * sizeof(float),
-                          hipMemcpyHostToDevice));
+        MRM_CHECK(mr_copy_h2d(dproj_.p, projections.GetDataPtr(), projections.Size() * sizeof(float), nullptr));
         MRM_CHECK(mr_fdk_bp(plan_, dproj_.p, nProj, projections.Size(1), angles.data(), weights.data(), dvol_.p,
                             nullptr));
-        MRM_HIP(hipMemcpy(volume.GetDataPtr(), dvol_.p, volume.Size() * sizeof(float), hipMemcpyDeviceToHost));
+        MRM_CHECK(mr_copy_d2h(volume.GetDataPtr(), dvol_.p, volume.Size() * sizeof(float), nullptr));
         UpdateStatus(1.0f, "FDK back-projection");
         return 0;
     }

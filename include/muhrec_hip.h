@@ -211,6 +211,23 @@ int mr_fbp_stream_destroy(mr_fbp_stream *stream);
 int mr_host_alloc(void **ptr, size_t bytes);
 int mr_host_free(void *ptr);
 
+/* Copies between host memory and the device for callers that hold pageable
+ * host buffers, as MuhRec's modules do (kipl TImage; no reference
+ * counterpart: the reference has no device).  Pageable memory goes through
+ * two pinned staging buffers of the library (64 MiB each per device, kept
+ * until the process ends), filled or drained by host threads while the DMA
+ * engine moves the other one; mr_host_alloc memory is copied by DMA
+ * directly.  Both return when the copy is complete; calls from several
+ * threads on one device take turns. */
+int mr_copy_h2d(void *d_dst, const void *h_src, size_t bytes, void *stream);
+int mr_copy_d2h(void *h_dst, const void *d_src, size_t bytes, void *stream);
+/* d_xyz[z][y][x] = d_zxy[y][x][z] for a width x width x slices volume: the
+ * back-projector's MatrixZXY volume to MatrixXYZ, whose slices
+ * BackProjectorModuleBase::GetSlice copies row by row
+ * (frameworks/tomography/Framework/ReconFramework/src/BackProjectorModuleBase.cpp:62-124).
+ * Out of place; enqueued on `stream` without waiting. */
+int mr_volume_zxy_to_xyz(const float *d_zxy, float *d_xyz, size_t width, size_t slices, void *stream);
+
 /* ------------------------------------------------------------------------
  * FDK cone-beam back-projection (FDKbp_single and FDKbp).
  * Replaces FdkReconBase::SetROI/Process(3D)

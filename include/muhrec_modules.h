@@ -42,7 +42,9 @@ int GetModuleList(const char *application, void *listptr);
  *   the back-projector's Process(2D, angle, weight, last).  "reference" passes
  *   Run's last flag (n_proj < i+1, never set, so the projections after the
  *   last full ProjectionBufferSize buffer are not back-projected); "final"
- *   sets it on the final projection, as StdBackProjectorBase::Process(3D) does.
+ *   sets it on the final projection, as StdBackProjectorBase::Process(3D) does;
+ *   bp_abort_after=k: the back-projector gets an interactor whose k-th
+ *   progress report asks for an abort (the GUI's cancel, at a known point).
  * Errors return a non-zero code; mre_last_error() describes them.
  * ------------------------------------------------------------------------ */
 typedef struct mre_engine mre_engine;

@@ -142,3 +142,35 @@ def test_engine_projection_at_a_time(orc, dev, n_p, bufsize, drop, last):
     ref = orc.bp_parallel_f32(np.ascontiguousarray(f32[idx]), angles[idx], weights[idx], center, 0, "linear",
                               0).transpose(2, 0, 1)
     assert rel_err(vol, ref)[0] <= 1e-5
+
+
+@pytest.mark.parametrize("abort_after,n_done", [(1, 0), (2, 256), (3, 512)])
+def test_backprojector_cancel_between_launches(orc, dev, abort_after, n_done):
+    """Cancellation through the interactor (the GUI's cancel): the
+    back-projector polls before the stack and between its launches of whole
+    ProjectionBufferSize buffers (every 256 projections); an abort at the
+    k-th poll leaves the volume with exactly the projections back-projected
+    before it, as the reference's buffer-by-buffer volume would hold them."""
+    from imagingsuite_amd import engine, scan
+    from imagingsuite_amd.recon import BackProjector, ProjectionFilterSingle
+    nu, nv, n_p = 64, 8, 600
+    center = 32.5
+    rng = np.random.default_rng(abort_after)
+    proj = (rng.random((n_p, nv, nu)) + 0.5).astype(np.float32)
+    # values that survive the engine's 6-digit text transport unchanged
+    angles, weights = scan.engine_angles_weights(n_p, 0.0, 179.7, 0.1)
+    cfg = {"geometry": "parallel", "center": center, "roi": f"0 0 {nu} {nv}", "resolution": "0.1 0.1",
+           "bp_abort_after": abort_after}
+    eng = engine.Engine("MultiProjBPparallel", {"ProjectionBufferSize": 16, "SliceBlock": nv, "SubVolume": "1 1"},
+                        cfg, "ProjectionFilterSingle", FILTER)
+    vol = eng.run(proj, angles, weights, nv)
+    eng.close()
+    print(f"abort at poll {abort_after}: {n_done} projections expected", flush=True)
+    if n_done == 0:
+        assert float(np.abs(vol).max()) == 0.0
+        return
+    t = torch.from_numpy(proj).to(dev)
+    ProjectionFilterSingle(nu, "Hanning").process(t)
+    bp = BackProjector("MultiProjBPparallel", nu, nv, center, buffer_size=16)
+    direct = bp.process(t[:n_done].contiguous(), angles[:n_done], weights[:n_done]).cpu().numpy().transpose(2, 0, 1)
+    assert np.array_equal(vol, direct)
