@@ -16,6 +16,10 @@ MR_ERR_INVALID = 1
 MR_ERR_HIP = 2
 MR_ERR_UNSUPPORTED = 3
 MR_ERR_INTERNAL = 4
+MR_ERR_ABORTED = 5
+# int (*mr_slab_fn)(void *user, size_t first_slice, size_t n_slices, const float *slices)
+SLAB_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t,
+                           ctypes.POINTER(ctypes.c_float))
 
 FILTER_TYPES = {"ram-lak": 0, "shepp-logan": 1, "hanning": 2, "hamming": 3, "butterworth": 4}
 ITER_METHODS = {"sirt": 0, "landweber": 1, "cgls": 2}
@@ -34,7 +38,7 @@ EXPORTED_SYMBOLS = (
     "mr_fdk_window", "mr_fdk_plan_create", "mr_fdk_plan_destroy", "mr_fdk_plan_window", "mr_fdk_bp",
     "mr_joseph_forward", "mr_joseph_adjoint", "mr_linear_forward",
     "mr_iter_plan_create", "mr_iter_run", "mr_iter_plan_destroy",
-    "mr_fbp_stream_create", "mr_fbp_stream_run", "mr_fbp_stream_destroy", "mr_host_alloc", "mr_host_free",
+    "mr_fbp_stream_create", "mr_fbp_stream_run", "mr_fbp_stream_run_slabs", "mr_fbp_stream_destroy", "mr_host_alloc", "mr_host_free",
     "mr_copy_h2d", "mr_copy_d2h", "mr_volume_zxy_to_xyz",
     "mr_lognorm_reference", "mr_lognorm", "mr_lognorm_logdose",
     "mr_batch_plan_create", "mr_batch_plan_info", "mr_batch_run", "mr_batch_run_graph",
@@ -138,6 +142,7 @@ def load():
     L.mr_fbp_stream_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(BpParams), sz, sz, sz, i32, f32, f32,
                                        i32, f32, sz]
     L.mr_fbp_stream_run.argtypes = [vp, vp, fp, fp, vp, i32]
+    L.mr_fbp_stream_run_slabs.argtypes = [vp, vp, fp, fp, vp, SLAB_FN, vp]
     L.mr_fbp_stream_destroy.argtypes = [vp]
     L.mr_host_alloc.argtypes = [ctypes.POINTER(vp), sz]
     L.mr_host_free.argtypes = [vp]

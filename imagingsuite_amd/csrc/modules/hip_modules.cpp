@@ -147,11 +147,21 @@ protected:
         return 0;
     }
 
+    // ProjectionFilterBase::ProcessCore(3D) (reference :103-118) filters the
+    // projections one by one and polls the interactor before each; here the
+    // stack goes through the GPU in chunks of 64 projections with the poll
+    // between chunks (an abort leaves the later projections unfiltered, as
+    // the reference's loop does).
     int ProcessCore(kipl::base::TImage<float, 3> &img, std::map<std::string, std::string> &) override
     {
         if (img.Size() == 0) throw ReconException("Empty projection image", __FILE__, __LINE__);
-        if (UpdateStatus(0.0f, m_sModuleName)) return 0;
-        filter_rows(img.GetDataPtr(), img.Size(1) * img.Size(2), img.Size(0));
+        const size_t nProj = img.Size(2), per = img.Size(0) * img.Size(1);
+        constexpr size_t kChunk = 64;
+        for (size_t p0 = 0; p0 < nProj; p0 += kChunk) {
+            if (UpdateStatus((float)p0 / nProj, m_sModuleName)) return 0;
+            const size_t n = std::min(kChunk, nProj - p0);
+            filter_rows(img.GetDataPtr() + p0 * per, img.Size(1) * n, img.Size(0));
+        }
         UpdateStatus(1.0f, m_sModuleName);
         return 0;
     }

@@ -226,7 +226,7 @@ int staging(mr_fbp_stream *s, bool in)
 }
 
 int run_stream(mr_fbp_stream *s, const float *h_proj, const float *angles, const float *weights, float *h_vol,
-               bool serial)
+               bool serial, mr_slab_fn on_slab = nullptr, void *user = nullptr)
 {
     const size_t w = s->width, rows = s->rows, np = s->n_proj, sb = s->sb;
     const bool pin_in = mr::is_pinned(h_proj, sizeof(float) * np * rows * w);
@@ -248,6 +248,18 @@ int run_stream(mr_fbp_stream *s, const float *h_proj, const float *angles, const
             const size_t a = n * (size_t)t / (size_t)nt, e = n * (size_t)(t + 1) / (size_t)nt;
             std::memcpy(h_vol + z0 * w * w + a, s->h_out[i] + a, sizeof(float) * (e - a));
         });
+        return MR_OK;
+    };
+    // slab b is in the caller's volume: hand it over (the GPU works on the
+    // later slabs meanwhile); a nonzero return stops the run
+    auto deliver = [&](size_t b) -> int {
+        if (!on_slab) return MR_OK;
+        const size_t z0 = b * sb, nr = std::min(sb, rows - z0);
+        if (pin_out) MR_HIP(hipEventSynchronize(s->ev_out[b & 1]));
+        if (on_slab(user, z0, nr, h_vol + z0 * w * w) != 0) {
+            mr::set_error("the slab callback stopped the reconstruction");
+            return MR_ERR_ABORTED;
+        }
         return MR_OK;
     };
     for (size_t b = 0; b < nb; ++b) {
@@ -293,12 +305,15 @@ int run_stream(mr_fbp_stream *s, const float *h_proj, const float *angles, const
         MR_HIP(hipMemcpyAsync(dst, s->d_out[i], sizeof(float) * nr * w * w, hipMemcpyDeviceToHost, s->d2h));
         MR_HIP(hipEventRecord(s->ev_out[i], s->d2h));
         if (!serial && !pin_out && b >= 1 && (rc = scatter(b - 1)) != MR_OK) return rc;
+        if (!serial && b >= 1 && (rc = deliver(b - 1)) != MR_OK) return rc;
         if (serial) {
             MR_HIP(hipStreamSynchronize(s->d2h));
             if (!pin_out && (rc = scatter(b)) != MR_OK) return rc;
+            if ((rc = deliver(b)) != MR_OK) return rc;
         }
     }
     if (!serial && !pin_out && (rc = scatter(nb - 1)) != MR_OK) return rc;
+    if (!serial && (rc = deliver(nb - 1)) != MR_OK) return rc;
     MR_HIP(hipStreamSynchronize(s->d2h));
     MR_HIP(hipStreamSynchronize(s->cs));
     MR_HIP(hipStreamSynchronize(s->h2d));
@@ -306,6 +321,24 @@ int run_stream(mr_fbp_stream *s, const float *h_proj, const float *angles, const
 }
 
 }  // namespace
+
+extern "C" int mr_fbp_stream_run_slabs(mr_fbp_stream *stream, const float *h_proj, const float *angles,
+                                       const float *weights, float *h_volume, mr_slab_fn on_slab, void *user)
+{
+    MR_GUARD_BEGIN
+    mr::clear_error();
+    MR_REQUIRE(stream != nullptr && h_proj != nullptr && angles != nullptr && weights != nullptr &&
+                   h_volume != nullptr && on_slab != nullptr,
+               "NULL argument");
+    int rc = run_stream(stream, h_proj, angles, weights, h_volume, false, on_slab, user);
+    if (rc != MR_OK) {
+        (void)hipStreamSynchronize(stream->h2d);
+        (void)hipStreamSynchronize(stream->cs);
+        (void)hipStreamSynchronize(stream->d2h);
+    }
+    return rc;
+    MR_GUARD_END
+}
 
 extern "C" int mr_fbp_stream_run(mr_fbp_stream *stream, const float *h_proj, const float *angles,
                                  const float *weights, float *h_volume, int serial)

@@ -592,6 +592,41 @@ class FBPStream:
                                             int(bool(serial))))
         return out
 
+    def run_slabs(self, proj: np.ndarray, angles, weights, on_slab, out: Optional[np.ndarray] = None):
+        """As ``run``, handing each slab to ``on_slab(first_slice, slices)``
+        (``slices`` a [n, width, width] view of ``out``) as soon as it is
+        complete, while the GPU works on the later slabs
+        (mr_fbp_stream_run_slabs).  ``on_slab`` returning True stops the run
+        (MrError with MR_ERR_ABORTED)."""
+        shape = (self.n_proj, self.rows, self.width)
+        if not isinstance(proj, np.ndarray) or proj.dtype != np.float32 or proj.shape != shape \
+                or not proj.flags.c_contiguous:
+            raise MrError(_lib.MR_ERR_INVALID, f"proj must be a contiguous float32 array of shape {shape}")
+        oshape = (self.rows, self.width, self.width)
+        if out is None:
+            out = np.empty(oshape, np.float32)
+        if out.dtype != np.float32 or out.shape != oshape or not out.flags.c_contiguous:
+            raise MrError(_lib.MR_ERR_INVALID, f"out must be a contiguous float32 array of shape {oshape}")
+        a = _f32arr(angles, self.n_proj)
+        w = _f32arr(weights, self.n_proj)
+        fp = ctypes.POINTER(ctypes.c_float)
+        errors = []
+
+        def cb(_user, first, n, _ptr):
+            try:
+                return 1 if on_slab(int(first), out[first:first + n]) else 0
+            except Exception as ex:   # an exception must not cross the C ABI
+                errors.append(ex)
+                return 1
+
+        fn = _lib.SLAB_FN(cb)
+        rc = _lib.load().mr_fbp_stream_run_slabs(self._s, proj.ctypes.data_as(ctypes.c_void_p), a.ctypes.data_as(fp),
+                                                 w.ctypes.data_as(fp), out.ctypes.data_as(ctypes.c_void_p), fn, None)
+        if errors:
+            raise errors[0]
+        check(rc)
+        return out
+
     def close(self):
         if getattr(self, "_s", None) and self._s.value:
             try:

@@ -26,7 +26,8 @@ enum mr_status {
     MR_ERR_INVALID = 1,     /* bad argument (the reference throws ReconException) */
     MR_ERR_HIP = 2,         /* a HIP runtime call failed */
     MR_ERR_UNSUPPORTED = 3, /* configuration outside what the kernels implement */
-    MR_ERR_INTERNAL = 4     /* unexpected host-side exception */
+    MR_ERR_INTERNAL = 4,    /* unexpected host-side exception */
+    MR_ERR_ABORTED = 5      /* the caller asked to stop (a callback returned nonzero) */
 };
 
 /* ProjectionFilterBase::FilterType,
@@ -206,6 +207,18 @@ int mr_fbp_stream_create(mr_fbp_stream **stream, const mr_bp_params *bp, size_t 
                          float bias_weight, size_t padding_doubler);
 int mr_fbp_stream_run(mr_fbp_stream *stream, const float *h_proj, const float *angles, const float *weights,
                       float *h_volume, int serial);
+/* The same run with each slab handed over as soon as its slices are in
+ * h_volume, while the GPU works on the later slabs: the engine's per-block
+ * output (ReconEngine::Process3D's TransferMatrix and, with
+ * bAutomaticSerialize, the write of the block's slices,
+ * frameworks/tomography/Framework/ReconFramework/src/ReconEngine.cpp:1703-1710)
+ * runs beside the reconstruction instead of after it.  on_slab(user,
+ * first_slice, n_slices, slices) is called on the calling thread, slabs in
+ * order; `slices` points into h_volume ([n_slices][width][width]).  A nonzero
+ * return stops the run (MR_ERR_ABORTED; the later slabs are not delivered). */
+typedef int (*mr_slab_fn)(void *user, size_t first_slice, size_t n_slices, const float *slices);
+int mr_fbp_stream_run_slabs(mr_fbp_stream *stream, const float *h_proj, const float *angles, const float *weights,
+                            float *h_volume, mr_slab_fn on_slab, void *user);
 int mr_fbp_stream_destroy(mr_fbp_stream *stream);
 /* Pinned (page-locked) host memory that the streaming path copies by DMA. */
 int mr_host_alloc(void **ptr, size_t bytes);

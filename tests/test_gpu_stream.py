@@ -64,3 +64,39 @@ def test_stream_rejects_bad_shapes(dev):
         st.run(np.zeros((10, 8, 31), np.float32), np.zeros(10), np.ones(10))
     with pytest.raises(MrError, match="shape"):
         st.run(np.zeros((10, 8, 32), np.float32), np.zeros(10), np.ones(10), out=np.zeros((8, 32, 31), np.float32))
+
+
+@pytest.mark.parametrize("pinned", [False, True])
+def test_stream_slabs_handed_over_complete_and_in_order(dev, pinned):
+    """mr_fbp_stream_run_slabs hands every slab over in order, each already
+    complete when its callback runs (a copy taken inside the callback equals
+    the finished volume), and a callback asking to stop ends the run with
+    MR_ERR_ABORTED and no later slab."""
+    from imagingsuite_amd._lib import MR_ERR_ABORTED, MrError
+    from imagingsuite_amd.recon import FBPStream, host_empty
+    rng = np.random.default_rng(9)
+    n_p, rows, w, sb = 60, 45, 80, 8
+    proj = rng.random((n_p, rows, w), dtype=np.float32)
+    angles = (np.arange(n_p) * (180.0 / n_p)).astype(np.float32)
+    weights = np.full(n_p, np.float32(1.0 / n_p))
+    st = FBPStream(w, rows, n_p, w / 2, slice_block=sb, buffer_size=16)
+    ref = st.run(proj, angles, weights)
+    seen, copies = [], {}
+    out = host_empty((rows, w, w)) if pinned else np.zeros((rows, w, w), np.float32)
+
+    def on_slab(first, slices):
+        seen.append((first, slices.shape[0]))
+        copies[first] = slices.copy()
+        return False
+
+    st.run_slabs(proj, angles, weights, on_slab, out=out)
+    assert seen == [(z, min(sb, rows - z)) for z in range(0, rows, sb)]
+    for z, c in copies.items():
+        assert np.array_equal(c, ref[z:z + c.shape[0]])
+    assert np.array_equal(out, ref)
+    seen.clear()
+    with pytest.raises(MrError) as ei:
+        st.run_slabs(proj, angles, weights, lambda first, s: (seen.append(first) or first >= 2 * sb))
+    assert ei.value.code == MR_ERR_ABORTED
+    assert seen == [0, sb, 2 * sb]
+    st.close()
