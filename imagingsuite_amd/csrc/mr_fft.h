@@ -137,4 +137,82 @@ __device__ __forceinline__ void fft_lds2(float2 *a, const float2 *__restrict__ t
     for (; ns < P; ns <<= 3, lns += 3) fft_pass<P, NT, 8, INV>(a, twl, ns, lns);
 }
 
+// The same passes with the first pass's inputs and the last pass's outputs
+// taken from / handed to functors instead of the LDS line, so that a filter
+// can build its padded input in the first pass, multiply by its spectrum in
+// the last forward pass and crop/scale in the last inverse pass without an
+// LDS round trip (and its barriers) of its own.  load(i) returns input i;
+// store(i, v) receives output i.  LOAD_F / STORE_F select the functors for
+// the first / last pass; STORE_LDS says that the store functor writes the
+// LDS line (then the pass ends with a barrier, as an LDS pass does).
+template <int P, int NT, int R, bool INV, bool LOAD_F, bool STORE_F, bool STORE_LDS, class Load, class Store>
+__device__ __forceinline__ void fft_pass_io(float2 *a, const float2 *__restrict__ twl, int ns, int lns, Load load,
+                                            Store store)
+{
+    constexpr int B = P / R;
+    constexpr int PER = (B + NT - 1) / NT;
+    constexpr int LR = ilog2c(R);
+    constexpr int L = ilog2c(P);
+    const int t = threadIdx.x;
+    float2 v[PER][R];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int j = t + q * NT;
+        if (j < B) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[q][r] = LOAD_F ? load(j + r * B) : a[lpad(j + r * B)];
+            if (R == 8 && ns > 1) {
+                const int k = j & (ns - 1);
+                float2 w1 = twl[k << (L - 3 - lns)];
+                if (INV) w1.y = -w1.y;
+                const float2 w2 = cmul(w1, w1), w3 = cmul(w1, w2), w4 = cmul(w2, w2);
+                const float2 w5 = cmul(w1, w4), w6 = cmul(w3, w3), w7 = cmul(w3, w4);
+                const float2 w[8] = {make_float2(1.f, 0.f), w1, w2, w3, w4, w5, w6, w7};
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], w[r]);
+            }
+            dft_r<R, INV>(v[q]);
+        }
+    }
+    // in place: every read of the line must be done before it is overwritten
+    if (!LOAD_F && !(STORE_F && !STORE_LDS)) __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int j = t + q * NT;
+        if (j < B) {
+            const int k = j & (ns - 1);
+            const int d0 = ((j >> lns) << (lns + LR)) + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (STORE_F) store(d0 + r * ns, v[q][r]);
+                else a[lpad(d0 + r * ns)] = v[q][r];
+            }
+        }
+    }
+    if (!STORE_F || STORE_LDS) __syncthreads();
+}
+
+// FFT of the line whose first pass reads load(i) (the line in LDS is
+// written by that pass) and whose last pass hands every output to store(i, v)
+// (STORE_LDS: store writes the LDS line, which the caller reads afterwards).
+template <int P, int NT, bool INV, bool LOAD_F, bool STORE_F, bool STORE_LDS, class Load, class Store>
+__device__ __forceinline__ void fft_lds2_io(float2 *a, const float2 *__restrict__ twl, Load load, Store store)
+{
+    constexpr int L = ilog2c(P);
+    constexpr int FIRST = L % 3 == 1 ? 2 : (L % 3 == 2 ? 4 : 8);   // radix of the first pass
+    constexpr int NPASS = (L % 3 == 0 ? 0 : 1) + L / 3;
+    auto nothing_load = [](int) { return make_float2(0.f, 0.f); };
+    auto nothing_store = [](int, float2) {};
+    if constexpr (NPASS == 1) {
+        fft_pass_io<P, NT, FIRST, INV, LOAD_F, STORE_F, STORE_LDS>(a, twl, 1, 0, load, store);
+        return;
+    }
+    fft_pass_io<P, NT, FIRST, INV, LOAD_F, false, false>(a, twl, 1, 0, load, nothing_store);
+    int ns = FIRST, lns = ilog2c(FIRST);
+#pragma unroll 1
+    for (int pass = 1; pass < NPASS - 1; ++pass, ns <<= 3, lns += 3)
+        fft_pass_io<P, NT, 8, INV, false, false, false>(a, twl, ns, lns, nothing_load, nothing_store);
+    fft_pass_io<P, NT, 8, INV, false, STORE_F, STORE_LDS>(a, twl, ns, lns, nothing_load, store);
+}
+
 }  // namespace mr

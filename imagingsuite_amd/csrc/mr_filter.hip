@@ -72,6 +72,51 @@ __global__ __launch_bounds__(NT) void row_filter_kernel2(RowFilterArgs a)
     float *row1 = has1 ? row0 + a.n : row0;
     const int t = threadIdx.x;
     for (int i = t; i < P / 8; i += NT) twl[i] = a.tw[i];
+    float *o0 = a.out ? a.out + r0 * a.n : row0;
+    float *o1 = o0 + a.n;
+    if (a.mode != 2) {
+        // The padded line is built in the first pass's loads, the spectrum
+        // multiplied in the last forward pass and the kept samples cropped
+        // and scaled in the last inverse pass: no LDS round trip of their
+        // own.  Same values and operations as staging the line first.
+        float f0 = 0.f, l0 = 0.f, f1 = 0.f, l1 = 0.f, w0 = 1.f, w1 = 1.f, v0 = 0.f, v1 = 0.f;
+        if (a.mode == 0) {
+            f0 = row0[0]; l0 = row0[a.n - 1];
+            f1 = row1[0]; l1 = row1[a.n - 1];
+        } else {
+            w0 = a.pre_w ? a.pre_w[r0 / a.rows_per_proj] : 1.0f;
+            w1 = (a.pre_w && has1) ? a.pre_w[(r0 + 1) / a.rows_per_proj] : 1.0f;
+            v0 = fdk_row_vv2(a, r0);
+            v1 = fdk_row_vv2(a, r0 + 1);
+        }
+        auto load = [&](int i) -> float2 {
+            if (a.mode == 0)
+                return make_float2(pb_pad(row0, a.n, i, a.insert, P, a.taper, f0, l0),
+                                   has1 ? pb_pad(row1, a.n, i, a.insert, P, a.taper, f1, l1) : 0.0f);
+            return make_float2(fdk_pad(row0, a.n, i, a.insert, P, w0, a, v0),
+                               has1 ? fdk_pad(row1, a.n, i, a.insert, P, w1, a, v1) : 0.0f);
+        };
+        auto spectrum = [&](int i, float2 v) {
+            const float h = a.hs[i];
+            line[lpad(i)] = make_float2(v.x * h, v.y * h);
+        };
+        __syncthreads();   // the twiddles
+        fft_lds2_io<P, NT, false, true, true, true>(line, twl, load, spectrum);
+        auto none = [](int) { return make_float2(0.f, 0.f); };
+        auto crop = [&](int i, float2 v) {
+            const int d = i - a.insert;
+            if (d < 0 || d >= a.n) return;
+            if (a.mode == 0) {
+                o0[d] = v.x * a.post0;
+                if (has1) o1[d] = v.y * a.post0;
+            } else {
+                o0[d] = ((v.x * a.post0) * a.post1) * a.post2;
+                if (has1) o1[d] = ((v.y * a.post0) * a.post1) * a.post2;
+            }
+        };
+        fft_lds2_io<P, NT, true, false, true, false>(line, twl, none, crop);
+        return;
+    }
     if (a.mode == 0) {
         const float f0 = row0[0], l0 = row0[a.n - 1];
         const float f1 = row1[0], l1 = row1[a.n - 1];
@@ -108,8 +153,6 @@ __global__ __launch_bounds__(NT) void row_filter_kernel2(RowFilterArgs a)
     }
     __syncthreads();
     fft_lds2<P, NT, true>(line, twl);
-    float *o0 = a.out ? a.out + r0 * a.n : row0;
-    float *o1 = o0 + a.n;
     if (a.mode == 0) {
         for (int i = t; i < a.n; i += NT) {
             const float2 v = line[lpad(i + a.insert)];
@@ -189,8 +232,12 @@ template <int P>
 static int launch_row_filter(const RowFilterArgs &a, hipStream_t s)
 {
     // one radix-8 butterfly per thread and pass: P / 8 threads (at most
-    // 1024); measured 7.8-8.1 ms for the 384 000 rows of 2048 at P = 8192,
-    // against 9.3-9.8 ms with P / 16 threads and 19.75 ms for a radix-4 FFT
+    // 1024); measured for the 384 000 rows of 2048 at P = 8192: 6.75 ms with
+    // the padding, spectrum and crop fused into the first and last passes,
+    // 7.8-8.1 ms without, 9.3-9.8 ms with P / 16 threads, 19.75 ms with a
+    // radix-4 FFT; the line split into 4 or 8 wave-local FFTs of length P / 4
+    // or P / 8 (two block barriers per row pair instead of two per pass)
+    // took 11.0 / 8.4 ms (2 / 4 waves per SIMD)
     constexpr int NT2 = (P / 8) < 64 ? 64 : ((P / 8) > 1024 ? 1024 : (P / 8));
     const long pairs = (a.n_rows + 1) / 2;
     if (pairs == 0) return MR_OK;
