@@ -224,6 +224,8 @@ def profile_counters(name, kernel, **match):
         return None, None
     with open(path) as f:
         tj = json.load(f)
+    if "kernels" in tj:   # one file for the kernels of one workload
+        tj = tj["kernels"].get(kernel, {})
     if tj.get("kernel") != kernel or any(tj.get(k) != v for k, v in match.items()):
         return None, None
     return tj.get("hbm_bytes_per_launch"), tj.get("l2_bytes_per_launch")
@@ -324,8 +326,10 @@ def bench_parallel(args):
     # the ramp filter beside it: FFT FLOPs (5 P log2 P + P per row, two rows per complex FFT)
     P = 1 << (int(math.ceil(math.log2(nu))) + 2)
     f_rows = n_p * nv
-    r["filter_roofline"] = roofline("row_filter_kernel2", head["filt"], f_rows, 5 * P * math.log2(P) + P,
-                                    alg_bytes=8.0 * nu * f_rows)
+    fname = f"row_filter_kernel2<{P},{min(max(P // 8, 64), 1024)}>"
+    f_traffic, f_l2 = profile_counters(f"filter_traffic_c{args.config}.json", fname)
+    r["filter_roofline"] = roofline(fname, head["filt"], f_rows, 5 * P * math.log2(P) + P,
+                                    alg_bytes=8.0 * nu * f_rows, traffic=f_traffic, l2_bytes=f_l2)
     r["kernels_ms"] = {"row_filter": round(head["filt"], 3), "bp_prep": round(head["prep"], 3),
                        "bp_parallel": round(head["bp"], 3)}
     if False in res and True in res:
@@ -403,19 +407,22 @@ def plugin_leg(host, angles, weights, cfg, center, buf, updates):
     (ReconEngine's SliceBlock loop) loads libMuhRecHIPModules.so with
     dlopen/GetModule and runs ProjectionFilterSingle + MultiProjBPparallel
     module objects on host TImages (a host stack in, host slices out, angles
-    and weights as 6-digit text), at SliceBlock 256 and 32."""
+    and weights as 6-digit text), at SliceBlock 256 and 32 with the filter's
+    device copy of each slab handed to the back-projector, and at SliceBlock
+    256 with the back-projector copying the host slab again."""
     from imagingsuite_amd import engine
     nu, nv = cfg["width"], cfg["slices"]
     flt = {"filtertype": cfg["filtertype"], "cutoff": 0.5, "order": 1, "usebias": "true", "biasweight": 0.1,
            "paddingdoubler": 2}
     out = {}
-    for sb in (256, 32):
-        log(f"[bench] plugin path, SliceBlock {sb} ...")
+    for sb, handoff in ((256, True), (32, True), (256, False)):
+        log(f"[bench] plugin path, SliceBlock {sb}, device hand-off {handoff} ...")
         bpp = {"ProjectionBufferSize": 16, "SliceBlock": sb, "SubVolume": "1 1",
                "BufferDrop": "true" if buf else "false", "fused": "true"}
         e = engine.Engine("MultiProjBPparallel", bpp,
                           {"geometry": "parallel", "center": center, "roi": f"0 0 {nu} {nv}",
-                           "resolution": f"{cfg['pixel_mm']} {cfg['pixel_mm']}"},
+                           "resolution": f"{cfg['pixel_mm']} {cfg['pixel_mm']}",
+                           "device_handoff": "true" if handoff else "false"},
                           "ProjectionFilterSingle", flt)
         vol = np.empty((nv, nu, nu), np.float32)
         e.run(host, angles, weights, nv, out=vol)
@@ -426,10 +433,12 @@ def plugin_leg(host, angles, weights, cfg, center, buf, updates):
             times.append((time.perf_counter() - t0) * 1e3)
         e.close()
         ms = min(times)
-        out[f"slice_block_{sb}"] = {"ms": round(ms, 1), "GUPS": round(updates / (ms / 1e3) / 1e9, 1)}
-        log(f"[bench] plugin path SliceBlock {sb}: {ms:.1f} ms")
+        key = f"slice_block_{sb}" + ("" if handoff else "_host_copy")
+        out[key] = {"ms": round(ms, 1), "GUPS": round(updates / (ms / 1e3) / 1e9, 1)}
+        log(f"[bench] plugin path SliceBlock {sb}, hand-off {handoff}: {ms:.1f} ms")
     out["note"] = ("host stack in, host slices out through GetModule/Configure/SetROI/Process/GetSlice; "
-                   "best of 2 runs after one warm-up")
+                   "best of 2 runs after one warm-up; _host_copy: the back-projector copies the filtered "
+                   "host slab to the device again instead of taking the filter's device copy")
     return out
 
 
@@ -615,11 +624,19 @@ def joseph_leg(steps, full, fp64_gd=False):
     upd = float(nx) * nx * nz * n_a
     ms_a, ka = profiled(lambda: timed(lambda: geom.forward(x), steps, 1, "A"), ["joseph_forward"])
     ms_t, kt = profiled(lambda: timed(lambda: geom.adjoint(y), steps, 1, "AT"), ["joseph_adjoint"])
+    alg = 4.0 * (nx * nx * nz + n_a * nz * nx)   # volume + sinogram
+
+    def rl(kname, kms):
+        # counters of profiles/adjoint_traffic_c5.json, taken at this size only
+        tr, l2 = profile_counters("adjoint_traffic_c5.json", kname) if (nx, nz, n_a) == (1024, 64, 900) \
+            else (None, None)
+        return roofline(kname, kms, upd, 4, alg_bytes=alg, traffic=tr, l2_bytes=l2)
+
     out = {"workload": "matched Joseph pair and FBP pair, 1024^2 x 64 slices x 900 angles",
            "A": {"ms": round(ms_a, 3), "GUPS": round(upd / (ms_a / 1e3) / 1e9, 2),
-                 "roofline": roofline("joseph_forward_lds2_kernel", ka["joseph_forward"][0], upd, 4)},
+                 "roofline": rl("joseph_forward_lds2_kernel<32>", ka["joseph_forward"][0])},
            "AT": {"ms": round(ms_t, 3), "GUPS": round(upd / (ms_t / 1e3) / 1e9, 2),
-                  "roofline": roofline("joseph_adjoint_lds_kernel", kt["joseph_adjoint"][0], upd, 4)}}
+                  "roofline": rl("joseph_adjoint_lds_kernel<32,4>", kt["joseph_adjoint"][0])}}
     # gradient through filtered back-projection: F^T BP^T
     af = (np.arange(n_a) * (180.0 / n_a)).astype(np.float32)
     wf = np.full(n_a, np.float32(1.0 / n_a))
@@ -629,9 +646,9 @@ def joseph_leg(steps, full, fp64_gd=False):
     ms_f, kf = profiled(lambda: timed(lambda: op.forward(p), steps, 1, "FBP"), ["bp_parallel"])
     ms_b, kb = profiled(lambda: timed(lambda: op.adjoint(g), steps, 1, "FBP^T"), ["bp_adjoint"])
     out["FBP"] = {"ms": round(ms_f, 3), "GUPS": round(upd / (ms_f / 1e3) / 1e9, 2),
-                  "roofline": roofline("bp_lds_kernel", kf["bp_parallel"][0], upd, 4)}
+                  "roofline": rl(op.bp.kernel, kf["bp_parallel"][0])}
     out["FBP_T"] = {"ms": round(ms_b, 3), "GUPS": round(upd / (ms_b / 1e3) / 1e9, 2),
-                    "roofline": roofline("bp_adjoint_lds2_kernel", kb["bp_adjoint"][0], upd, 4)}
+                    "roofline": rl("bp_adjoint_kernel<0,32>", kb["bp_adjoint"][0])}
     t0 = time.perf_counter()
     xr, losses, _ = gradient_descent(y, geom, iters=50, lam=1e-3)
     torch.cuda.synchronize()
