@@ -113,6 +113,7 @@ struct mre_engine {
     // final projection
     int projection_at_a_time = 0;
     AbortAfter *abort_bp = nullptr;   // bp_abort_after: the back-projector's interactor
+    float rescale_between = 1.0f;     // stack multiplied by this between the two modules
 };
 
 extern "C" const char *mre_last_error(void) { return g_err.c_str(); }
@@ -153,12 +154,24 @@ extern "C" int mre_engine_create(mre_engine **engine, const char *module_so, con
         e->bp_params = parse_kv(bp_params);
         e->cfg.backprojector.m_sModule = bp_module;
         e->cfg.backprojector.parameters = e->bp_params;
+        const auto kvc = parse_kv(config);
+        // the filter hands its device copy of the stack to a parallel-beam
+        // back-projector of the same library unless the caller says otherwise
+        const std::string bpn = bp_module;
+        const auto dh = kvc.find("device_handoff");
+        if (dh != kvc.end() && dh->second != "true" && dh->second != "false")
+            throw ReconException("device_handoff must be true or false", __FILE__, __LINE__);
+        if (preproc_module && std::string(preproc_module) == "ProjectionFilterSingle" &&
+            (bpn == "MultiProjBPparallel" || bpn == "MultiProjBP" || bpn == "NearestNeighborBP") &&
+            e->pre_params.find("keepondevice") == e->pre_params.end())
+            e->pre_params["keepondevice"] = (dh == kvc.end() || dh->second == "true") ? "true" : "false";
+        const auto rb = kvc.find("rescale_between");
+        if (rb != kvc.end()) e->rescale_between = std::stof(rb->second);
         if (preproc_module && *preproc_module) {
             e->pre = static_cast<PreprocModuleBase *>(e->get("muhrec", preproc_module, nullptr));
             if (!e->pre) throw ReconException(std::string("Failed to create module ") + preproc_module, __FILE__, __LINE__);
             e->pre->Configure(e->cfg, e->pre_params);
         }
-        const auto kvc = parse_kv(config);
         const auto ab = kvc.find("bp_abort_after");
         if (ab != kvc.end()) e->abort_bp = new AbortAfter(std::stoi(ab->second));
         e->bp = static_cast<BackProjectorModuleBase *>(e->get("muhrecbp", bp_module, e->abort_bp));
@@ -255,6 +268,10 @@ extern "C" int mre_engine_run(mre_engine *e, const float *projections, size_t n_
             for (size_t p = 0; p < n_proj; ++p)
                 std::memcpy(img.GetLinePtr(0, p), projections + (p * rows + r0) * width, sizeof(float) * nr * width);
             if (e->pre) e->pre->Process(img, params);
+            if (e->rescale_between != 1.0f) {
+                float *d = img.GetDataPtr();
+                for (size_t i = 0; i < img.Size(); ++i) d[i] *= e->rescale_between;
+            }
             e->bp->SetROI(block);
             e->bp->Process(img, params);
             const size_t ns = block[3] - block[1];

@@ -18,9 +18,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <sstream>
 #include <vector>
 
@@ -89,6 +91,82 @@ struct DevBuf {
     ~DevBuf() { release(); }
 };
 
+// Device hand-off of a filtered stack.  MuhRec's engine passes the
+// projection TImage its pre-processing modules changed in place straight on
+// to the back-projector's Process(3D)
+// (frameworks/tomography/Framework/ReconFramework/src/ReconEngine.cpp:1499-1511,1663-1714).
+// With keepondevice=true this library's ProjectionFilterSingle keeps the
+// filtered stack in a device buffer of its own after writing it back to the
+// host image, and records here which host image that buffer mirrors; the
+// parallel-beam back-projectors of this library then read it from there
+// instead of copying the host stack to the device again.  A record is taken
+// at most once, and only for the same host buffer (address, length, device)
+// whose sampled values still equal what the filter wrote, so a stack that
+// another module changed in between is copied as usual.  keepondevice is for
+// chains whose last pre-processing module is this filter.
+class DeviceStack {
+public:
+    static DeviceStack &get()
+    {
+        static DeviceStack *s = new DeviceStack();   // never destroyed: no HIP call at exit
+        return *s;
+    }
+    void publish(const void *owner, const float *dev, const float *host, size_t n)
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        int d = 0;
+        MRM_HIP(hipGetDevice(&d));
+        owner_ = owner;
+        dev_ = dev;
+        host_ = host;
+        n_ = n;
+        device_ = d;
+        probe_.resize(std::min(n, kProbes));
+        for (size_t k = 0; k < probe_.size(); ++k) probe_[k] = host[index(k)];
+    }
+    // the owner's buffer is about to change or go away
+    void retract(const void *owner)
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (owner_ == owner) clear();
+    }
+    void count_taken() { taken_.fetch_add(1); }
+    size_t taken() const { return taken_.load(); }
+    // device copy of host[0, n) or nullptr
+    const float *take(const float *host, size_t n)
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (host_ == nullptr || host != host_ || n != n_) return nullptr;
+        int d = 0;
+        MRM_HIP(hipGetDevice(&d));
+        const float *dev = dev_;
+        bool same = d == device_;
+        for (size_t k = 0; same && k < probe_.size(); ++k)
+            same = std::memcmp(&probe_[k], &host[index(k)], sizeof(float)) == 0;
+        clear();
+        return same ? dev : nullptr;
+    }
+
+private:
+    static constexpr size_t kProbes = 4096;
+    size_t index(size_t k) const { return probe_.size() < 2 ? 0 : k * (n_ - 1) / (probe_.size() - 1); }
+    void clear()
+    {
+        owner_ = nullptr;
+        dev_ = host_ = nullptr;
+        n_ = 0;
+        device_ = -1;
+        probe_.clear();
+    }
+    std::mutex mu_;
+    const void *owner_ = nullptr;
+    const float *dev_ = nullptr, *host_ = nullptr;
+    size_t n_ = 0;
+    int device_ = -1;
+    std::vector<float> probe_;
+    std::atomic<size_t> taken_{0};
+};
+
 const char *kFilterNames[] = {"Ram-Lak", "Shepp-Logan", "Hanning", "Hamming", "Butterworth"};
 
 int filter_enum(const std::string &str)
@@ -109,7 +187,11 @@ public:
         : PreprocModuleBase("ProjectionFilterSingle", interactor)
     {
     }
-    ~HipProjectionFilterSingle() override { drop_plan(); }
+    ~HipProjectionFilterSingle() override
+    {
+        DeviceStack::get().retract(this);
+        drop_plan();
+    }
 
     int Configure(ReconConfig config, std::map<std::string, std::string> parameters) override
     {
@@ -120,9 +202,13 @@ public:
         m_bUseBias = string2bool(GetStringParameter(parameters, "usebias"));
         m_fBiasWeight = GetFloatParameter(parameters, "biasweight");
         m_nPaddingDoubler = (size_t)GetIntParameter(parameters, "paddingdoubler");
+        // optional, HIP-only: keep the filtered stack on the device for this
+        // library's back-projector (see DeviceStack)
+        keep_ = optional_bool(parameters, "keepondevice", false);
         nImageSize = N;
         if (!((0 < m_fCutOff) && (m_fCutOff <= 0.5f)))
             throw ReconException("Cut off frequency is not in valid range", __FILE__, __LINE__);
+        DeviceStack::get().retract(this);
         drop_plan();
         return 0;
     }
@@ -143,7 +229,8 @@ protected:
     int ProcessCore(kipl::base::TImage<float, 2> &img, std::map<std::string, std::string> &) override
     {
         if (img.Size() == 0) throw ReconException("Empty projection image", __FILE__, __LINE__);
-        filter_rows(img.GetDataPtr(), img.Size(1), img.Size(0));
+        DeviceStack::get().retract(this);
+        filter_rows(img.GetDataPtr(), img.Size(1), img.Size(0), nullptr);
         return 0;
     }
 
@@ -155,13 +242,17 @@ protected:
     int ProcessCore(kipl::base::TImage<float, 3> &img, std::map<std::string, std::string> &) override
     {
         if (img.Size() == 0) throw ReconException("Empty projection image", __FILE__, __LINE__);
+        DeviceStack::get().retract(this);
         const size_t nProj = img.Size(2), per = img.Size(0) * img.Size(1);
+        if (keep_) stack_.reserve(img.Size());
         constexpr size_t kChunk = 64;
         for (size_t p0 = 0; p0 < nProj; p0 += kChunk) {
             if (UpdateStatus((float)p0 / nProj, m_sModuleName)) return 0;
             const size_t n = std::min(kChunk, nProj - p0);
-            filter_rows(img.GetDataPtr() + p0 * per, img.Size(1) * n, img.Size(0));
+            filter_rows(img.GetDataPtr() + p0 * per, img.Size(1) * n, img.Size(0),
+                        keep_ ? stack_.p + p0 * per : nullptr);
         }
+        if (keep_) DeviceStack::get().publish(this, stack_.p, img.GetDataPtr(), img.Size());
         UpdateStatus(1.0f, m_sModuleName);
         return 0;
     }
@@ -173,7 +264,8 @@ private:
         plan_ = nullptr;
     }
 
-    void filter_rows(float *rows, size_t n_rows, size_t width)
+    // filters the host rows through the device buffer dev (buf_ when null)
+    void filter_rows(float *rows, size_t n_rows, size_t width, float *dev)
     {
         if (width != nImageSize)
             throw ReconException("Projection width does not match the configured ROI", __FILE__, __LINE__);
@@ -181,10 +273,13 @@ private:
             MRM_CHECK(mr_filter_plan_create(&plan_, nImageSize, m_FilterType, m_fCutOff, m_fOrder,
                                             m_bUseBias ? 1 : 0, m_fBiasWeight, m_nPaddingDoubler));
         const size_t n = n_rows * width;
-        buf_.reserve(n);
-        MRM_CHECK(mr_copy_h2d(buf_.p, rows, n * sizeof(float), nullptr));
-        MRM_CHECK(mr_filter_rows(plan_, buf_.p, n_rows, nullptr));
-        MRM_CHECK(mr_copy_d2h(rows, buf_.p, n * sizeof(float), nullptr));
+        if (!dev) {
+            buf_.reserve(n);
+            dev = buf_.p;
+        }
+        MRM_CHECK(mr_copy_h2d(dev, rows, n * sizeof(float), nullptr));
+        MRM_CHECK(mr_filter_rows(plan_, dev, n_rows, nullptr));
+        MRM_CHECK(mr_copy_d2h(rows, dev, n * sizeof(float), nullptr));
     }
 
     int m_FilterType = 3;  // Hamming
@@ -192,8 +287,9 @@ private:
     bool m_bUseBias = true;
     float m_fBiasWeight = 0.1f;
     size_t m_nPaddingDoubler = 2, nImageSize = 0;
+    bool keep_ = false;
     mr_filter_plan *plan_ = nullptr;
-    DevBuf buf_;
+    DevBuf buf_, stack_;
 };
 
 // -------------------------------------------------------------------------
@@ -471,8 +567,14 @@ public:
             return 0;
         }
         if (UpdateStatus(0.0f, "Back-projecting")) return 0;
-        dproj_.reserve(nProj * psz);
-        MRM_CHECK(mr_copy_h2d(dproj_.p, projections.GetDataPtr(), nProj * psz * sizeof(float), nullptr));
+        // the filter's device copy of this very stack, else a copy of the host stack
+        const float *src = DeviceStack::get().take(projections.GetDataPtr(), nProj * psz);
+        if (src) DeviceStack::get().count_taken();
+        if (!src) {
+            dproj_.reserve(nProj * psz);
+            MRM_CHECK(mr_copy_h2d(dproj_.p, projections.GetDataPtr(), nProj * psz * sizeof(float), nullptr));
+            src = dproj_.p;
+        }
         // launches of whole buffers, so that the drop pattern restarts where
         // the reference's buffer does
         const size_t B = std::max<size_t>(nProjectionBufferSize, 1);
@@ -484,7 +586,7 @@ public:
                 return 0;
             }
             const size_t n = std::min(chunk, nProj - p0);
-            MRM_CHECK(mr_bp_parallel(plan_, dproj_.p + p0 * psz, n, angles.data() + p0, weights.data() + p0,
+            MRM_CHECK(mr_bp_parallel(plan_, src + p0 * psz, n, angles.data() + p0, weights.data() + p0,
                                      dvol_.p, 1, nullptr));
         }
         download();
@@ -778,3 +880,5 @@ extern "C" int GetModuleList(const char *application, void *listptr)
     }
     return -1;
 }
+
+extern "C" size_t mr_modules_device_handoffs(void) { return DeviceStack::get().taken(); }

@@ -61,6 +61,14 @@ def module_list(application: str, module_so: str = MODULES_PATH) -> Dict[str, Di
     return out
 
 
+def device_handoffs(module_so: str = MODULES_PATH) -> int:
+    """Stacks the module library's parallel-beam back-projectors took from
+    ProjectionFilterSingle's device copy (keepondevice) since it was loaded."""
+    L = ctypes.CDLL(module_so)   # the same loaded instance the engine opened
+    L.mr_modules_device_handoffs.restype = ctypes.c_size_t
+    return int(L.mr_modules_device_handoffs())
+
+
 class Engine:
     """One pre-processing module + one back-projector, loaded by name."""
 

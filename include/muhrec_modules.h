@@ -44,7 +44,14 @@ int GetModuleList(const char *application, void *listptr);
  *   last full ProjectionBufferSize buffer are not back-projected); "final"
  *   sets it on the final projection, as StdBackProjectorBase::Process(3D) does;
  *   bp_abort_after=k: the back-projector gets an interactor whose k-th
- *   progress report asks for an abort (the GUI's cancel, at a known point).
+ *   progress report asks for an abort (the GUI's cancel, at a known point);
+ *   device_handoff=true|false (default true): with ProjectionFilterSingle
+ *   before a parallel-beam back-projector of the same library, the filter
+ *   gets keepondevice=true (unless its parameters set it) and the
+ *   back-projector reads the filtered stack from the filter's device buffer
+ *   instead of copying the host stack again;
+ *   rescale_between=f: the stack is multiplied by f between the two modules
+ *   (a stand-in for another module that changes it in between).
  * Errors return a non-zero code; mre_last_error() describes them.
  * ------------------------------------------------------------------------ */
 typedef struct mre_engine mre_engine;
@@ -61,6 +68,11 @@ int mre_engine_run(mre_engine *engine, const float *projections, size_t n_proj, 
                    size_t width, const float *angles, const float *weights, float *volume);
 int mre_engine_destroy(mre_engine *engine);
 const char *mre_last_error(void);
+/* Module library (libMuhRecHIPModules.so), no reference counterpart:
+ * number of stacks the parallel-beam back-projectors took from
+ * ProjectionFilterSingle's device copy (keepondevice=true) instead of
+ * copying the host stack. */
+size_t mr_modules_device_handoffs(void);
 /* GetModuleList of a module library, serialised as "name:key=value,...;" */
 int mre_module_list(const char *module_so, const char *application, char *buf, size_t len);
 

@@ -52,6 +52,40 @@ def test_engine_parallel_matches_direct_and_oracle(orc, dev, bp_name, slices, sb
     assert e[0] <= 1e-4
 
 
+def test_engine_device_handoff(orc, dev):
+    """The filter's device copy of the stack goes to the back-projector
+    (keepondevice): bit-identical to copying the host stack again, taken once
+    per slab, and not taken when the host stack changed in between."""
+    from imagingsuite_amd import engine, scan
+    nu, n_p, slices, sb = 96, 90, 48, 16
+    angles, weights = scan.engine_angles_weights(n_p, 0.0, 178.0, 0.1)
+    rng = np.random.default_rng(11)
+    proj = (rng.random((n_p, slices, nu)) + 0.5).astype(np.float32)
+    bpp = {"ProjectionBufferSize": 16, "SliceBlock": sb, "SubVolume": "1 1"}
+    cfg = {"geometry": "parallel", "center": 48.5, "roi": f"0 0 {nu} {slices}", "resolution": "0.1 0.1"}
+
+    def run(handoff, rescale=1.0):
+        c = dict(cfg, device_handoff="true" if handoff else "false", rescale_between=rescale)
+        eng = engine.Engine("MultiProjBPparallel", bpp, c, "ProjectionFilterSingle", FILTER)
+        n0 = engine.device_handoffs()
+        vol = eng.run(proj, angles, weights, slices)
+        return vol, engine.device_handoffs() - n0
+
+    v_on, taken_on = run(True)
+    v_off, taken_off = run(False)
+    print(f"hand-offs taken: on {taken_on}, off {taken_off}")
+    assert taken_on == slices // sb and taken_off == 0
+    assert np.array_equal(v_on, v_off)
+    # a stack changed between the modules is copied again: the result is that of the changed stack
+    v2_on, taken2 = run(True, 2.0)
+    v2_off, _ = run(False, 2.0)
+    print(f"rescaled in between: hand-offs taken {taken2}")
+    assert taken2 == 0
+    assert np.array_equal(v2_on, v2_off)
+    e = rel_err(v2_on, 2.0 * v_on)
+    assert e[0] <= 1e-6
+
+
 @pytest.mark.parametrize("module", ["FDKbp_single", "FDKbp"])
 def test_engine_fdk_slabs(orc, dev, module):
     """FDKbp_single / FDKbp through the driver: per slab the engine computes
