@@ -431,10 +431,11 @@ def plugin_leg(host, angles, weights, cfg, center, buf, updates):
             t0 = time.perf_counter()
             e.run(host, angles, weights, nv, out=vol)
             times.append((time.perf_counter() - t0) * 1e3)
-        e.close()
         ms = min(times)
         key = f"slice_block_{sb}" + ("" if handoff else "_host_copy")
-        out[key] = {"ms": round(ms, 1), "GUPS": round(updates / (ms / 1e3) / 1e9, 1)}
+        out[key] = {"ms": round(ms, 1), "GUPS": round(updates / (ms / 1e3) / 1e9, 1),
+                    "phases_ms": e.phase_times()}
+        e.close()
         log(f"[bench] plugin path SliceBlock {sb}, hand-off {handoff}: {ms:.1f} ms")
     out["note"] = ("host stack in, host slices out through GetModule/Configure/SetROI/Process/GetSlice; "
                    "best of 2 runs after one warm-up; _host_copy: the back-projector copies the filtered "

@@ -6,6 +6,7 @@
 // (frameworks/tomography/Framework/ReconFramework/src/ReconEngine.cpp:531-574, 838-1185, 1314-1714).
 #include <dlfcn.h>
 
+#include <chrono>
 #include <cstring>
 #include <sstream>
 #include <string>
@@ -114,7 +115,30 @@ struct mre_engine {
     int projection_at_a_time = 0;
     AbortAfter *abort_bp = nullptr;   // bp_abort_after: the back-projector's interactor
     float rescale_between = 1.0f;     // stack multiplied by this between the two modules
+    // wall time of the last run's phases, ms: slab copy-in, pre-processing
+    // Process, back-projector SetROI, back-projector Process, GetSlice copy-out
+    double phase_ms[5] = {0, 0, 0, 0, 0};
 };
+
+namespace {
+
+class PhaseClock {
+public:
+    explicit PhaseClock(double *acc) : acc_(acc), t_(std::chrono::steady_clock::now()) {}
+    // adds the time since the previous lap to phase i
+    void lap(int i)
+    {
+        const auto now = std::chrono::steady_clock::now();
+        acc_[i] += std::chrono::duration<double, std::milli>(now - t_).count();
+        t_ = now;
+    }
+
+private:
+    double *acc_;
+    std::chrono::steady_clock::time_point t_;
+};
+
+}  // namespace
 
 extern "C" const char *mre_last_error(void) { return g_err.c_str(); }
 
@@ -205,6 +229,8 @@ extern "C" int mre_engine_run(mre_engine *e, const float *projections, size_t n_
         return 1;
     }
     try {
+        for (double &t : e->phase_ms) t = 0.0;
+        PhaseClock clock(e->phase_ms);
         const auto &P = e->cfg.ProjectionInfo;
         const size_t *roi = P.roi;
         const bool cone = P.beamgeometry == ReconConfig::cProjections::BeamGeometry_Cone;
@@ -267,24 +293,36 @@ extern "C" int mre_engine_run(mre_engine *e, const float *projections, size_t n_
             kipl::base::TImage<float, 3> img(dims);
             for (size_t p = 0; p < n_proj; ++p)
                 std::memcpy(img.GetLinePtr(0, p), projections + (p * rows + r0) * width, sizeof(float) * nr * width);
+            clock.lap(0);
             if (e->pre) e->pre->Process(img, params);
             if (e->rescale_between != 1.0f) {
                 float *d = img.GetDataPtr();
                 for (size_t i = 0; i < img.Size(); ++i) d[i] *= e->rescale_between;
             }
+            clock.lap(1);
             e->bp->SetROI(block);
+            clock.lap(2);
             e->bp->Process(img, params);
+            clock.lap(3);
             const size_t ns = block[3] - block[1];
             for (size_t i = 0; i < ns; ++i) {
                 kipl::base::TImage<float, 2> slice = e->bp->GetSlice(i);
                 std::memcpy(volume + (block[1] - roi[1] + i) * width * width, slice.GetDataPtr(),
                             sizeof(float) * std::min(slice.Size(), width * width));
             }
+            clock.lap(4);
         }
     } catch (const std::exception &ex) {
         g_err = ex.what();
         return 2;
     }
+    return 0;
+}
+
+extern "C" int mre_engine_phase_times(const mre_engine *e, double *ms, size_t n)
+{
+    if (!e || !ms) return 1;
+    for (size_t i = 0; i < n && i < 5; ++i) ms[i] = e->phase_ms[i];
     return 0;
 }
 

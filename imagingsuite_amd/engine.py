@@ -35,6 +35,8 @@ def load():
         L.mre_last_error.restype = ctypes.c_char_p
         L.mre_module_list.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, sz]
         L.mre_module_list.restype = ctypes.c_int
+        L.mre_engine_phase_times.argtypes = [vp, ctypes.POINTER(ctypes.c_double), sz]
+        L.mre_engine_phase_times.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -102,6 +104,14 @@ class Engine:
         if rc != 0:
             raise EngineError(load().mre_last_error().decode())
         return vol
+
+    PHASES = ("copy_in", "preprocess", "set_roi", "backproject", "get_slice")
+
+    def phase_times(self) -> Dict[str, float]:
+        """Wall time (ms) of the last run's phases, summed over slabs."""
+        a = (ctypes.c_double * 5)()
+        load().mre_engine_phase_times(self._h, a, 5)
+        return dict(zip(self.PHASES, (round(v, 1) for v in a)))
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:

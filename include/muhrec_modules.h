@@ -67,6 +67,11 @@ int mre_engine_create(mre_engine **engine, const char *module_so, const char *pr
 int mre_engine_run(mre_engine *engine, const float *projections, size_t n_proj, size_t rows,
                    size_t width, const float *angles, const float *weights, float *volume);
 int mre_engine_destroy(mre_engine *engine);
+/* Wall time in ms of the last mre_engine_run's phases, summed over slabs:
+ * [0] slab copy-in, [1] pre-processing Process, [2] back-projector SetROI,
+ * [3] back-projector Process, [4] GetSlice copy-out (SliceBlock loop only;
+ * n <= 5 entries are written). */
+int mre_engine_phase_times(const mre_engine *engine, double *ms, size_t n);
 const char *mre_last_error(void);
 /* Module library (libMuhRecHIPModules.so), no reference counterpart:
  * number of stacks the parallel-beam back-projectors took from

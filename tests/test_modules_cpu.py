@@ -74,3 +74,20 @@ def test_missing_library():
 def test_configure_valid_without_device():
     e = engine.Engine("MultiProjBPparallel", BP_PARAMS, CFG, "ProjectionFilterSingle", FILTER_PARAMS)
     e.close()
+
+
+def test_modules_header_symbols_exported():
+    """Every function include/muhrec_modules.h declares is exported: the plugin
+    quartet and mr_modules_* by the module library, mre_* by the driver."""
+    import os
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    txt = open(os.path.join(root, "include", "muhrec_modules.h")).read()
+    names = set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(\w+)\s*\(", txt, re.M))
+    assert {"GetModule", "Destroy", "LibVersion", "GetModuleList", "mr_modules_device_handoffs",
+            "mre_engine_create", "mre_engine_run", "mre_engine_phase_times"} <= names
+    mods, eng = ctypes.CDLL(engine.MODULES_PATH), ctypes.CDLL(engine.ENGINE_PATH)
+    for n in sorted(names):
+        lib = eng if n.startswith("mre_") else mods
+        assert hasattr(lib, n), n
+    assert engine.device_handoffs() >= 0
