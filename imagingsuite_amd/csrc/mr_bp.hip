@@ -540,16 +540,35 @@ static int launch_bp(const BpArgs &a, hipStream_t s)
 // On one line the detector position u is monotone along the line with slope
 // |sin| (or |cos|) >= 1/sqrt 2, so the voxels a bin receives from form one
 // run of at most 3 (the interval of u that maps to it is 2 wide; 1 for
-// nearest neighbour; bin 1 of the linear form also takes u in (-1, 0), which
-// (int) truncates to bin 0: 3 wide, up to 5 voxels).  The thread finds the
-// run's first voxel from the affine form (one division, then a one-step
-// fix-up with the forward's own float operations), and accumulates exactly
-// that window: the weights are recomputed with the forward's float
-// operations, and zero for window positions outside the run.  A line whose
-// staged span holds no voxel of the mask is skipped.  The volume is read once
-// per chunk; slices at or beyond 4*(nv/4) and dropped projections get no
-// gradient (the caller zero-fills the output first).
+// nearest neighbour).  The thread finds the run's first voxel from the affine
+// form (one division, then a one-step fix-up), and accumulates that window of
+// 3: the weights are the forward's float values (the hat below for the
+// linear form, the forward's own operations for nearest neighbour), zero for
+// window positions outside the run.  Bin 1 of the linear form also takes u in
+// (-1, 0), which (int) truncates to bin 0 (a run of up to 5), and bin nu - 1
+// takes only n = nu - 2: bp_adjoint_edge_kernel computes those two bins.  A
+// line whose staged span holds no voxel of the mask is skipped.  The volume is
+// read once per chunk; slices at or beyond 4*(nv/4) and dropped projections
+// get no gradient (the caller zero-fills the output first).  1024^2 x 64 x
+// 900 (all projections): 9.6 ms with the hat against 11.5 ms with the
+// forward's operations for every weight.
 constexpr int kAdjNB = 16, kAdjKA = 16, kAdjW = 80;
+
+// The forward's weight of the voxel at position p (1-based) of line L for
+// bin u, with its float operations (MultiProjBPparallel.cpp:130-161): 0 when
+// the voxel does not contribute to u.
+template <int INTERP>
+__device__ __forceinline__ float adj_weight(float su, float c, float s, bool rows, int L, int p, int u, int num2)
+{
+    const int x = rows ? p : L, y = rows ? L : p;
+    const float lsu = su + c * (float)y;
+    const float uu = lsu - s * (float)x;
+    const int n = (int)uu;
+    const float wb = fabsf(uu - (float)n);
+    const bool in = n >= 0 && n <= num2;
+    const float w_n = INTERP == 1 ? 1.0f : 1.0f - wb;
+    return (in && n == u) ? w_n : ((INTERP == 0 && in && n == u - 1) ? wb : 0.0f);
+}
 
 template <int INTERP, int ZT>
 __global__ __launch_bounds__(256) void bp_adjoint_kernel(const float *__restrict__ g, float *__restrict__ out,
@@ -678,19 +697,22 @@ __global__ __launch_bounds__(256) void bp_adjoint_kernel(const float *__restrict
     };
     // weight of the voxel at position p of line L for this thread's bin (0
     // when it does not contribute), with the forward's float operations
-    auto weight = [&](int L, int p) -> float {
+    auto weight = [&](int L, int p) -> float { return adj_weight<INTERP>(su, c, s, rows, L, p, u, num2); };
+    // The same weight for bins 0 and 2 .. nu - 2 of the linear form: the hat
+    // 1 - |u' - u| at the forward's position u'.  For those bins it is the
+    // forward's float value: u' - u and 1 - |u' - u| are exact (Sterbenz,
+    // u' in (u - 1, u + 1) and u >= 2; 1 - |u'| itself for bin 0), and the
+    // range tests on n cannot fail.  Bin 1 (which also takes u' in (-1, 0),
+    // weight |u'|) and bin nu - 1 (n <= nu - 2 only) differ: this kernel does
+    // not write them, bp_adjoint_edge_kernel does.
+    auto hat = [&](int L, int p) -> float {
         const int x = rows ? p : L, y = rows ? L : p;
         const float lsu = su + c * (float)y;
         const float uu = lsu - s * (float)x;
-        const int n = (int)uu;
-        const float wb = fabsf(uu - (float)n);
-        const bool in = n >= 0 && n <= num2;
-        const float w_n = INTERP == 1 ? 1.0f : 1.0f - wb;
-        return (in && n == u) ? w_n : ((INTERP == 0 && in && n == u - 1) ? wb : 0.0f);
+        return fmaxf(0.0f, 1.0f - fabsf(uu - fu));
     };
-    // adds the voxel at position p of line L (staged in seg[buf]) with its weight
-    auto add = [&](int L, int p, int buf, const int2 sp) {
-        float w = weight(L, p);
+    // adds the voxel at position p of line L (staged in seg[buf]) with weight w
+    auto add = [&](int L, int p, int buf, const int2 sp, float w) {
         // staged positions only (a position outside [lo, hi] is outside the
         // mask or past the window's run: weight 0, and a staged value is read)
         int j = p - sp.x;
@@ -707,8 +729,27 @@ __global__ __launch_bounds__(256) void bp_adjoint_kernel(const float *__restrict
             acc[4 * gq + 3] = __builtin_fmaf(w, x4.w, acc[4 * gq + 3]);
         }
     };
-    // bin 1 of the linear form also takes u in (-1, 0): a run of up to 5
-    const bool wide = INTERP == 0 && __any(active && u == 1);
+    // the run's window on line L (first voxel: the estimate, one step down
+    // if the voxel before it contributes, one step up if it does not)
+    auto consume = [&](int L, int buf, auto wf) {
+        float pa, pb;
+        if (rows) {
+            const float lsu = su + c * (float)L;
+            pa = (lsu - uhi) * inv;
+            pb = (lsu - ulo) * inv;
+        } else {
+            const float bb = s * (float)L - su;
+            pa = (ulo + bb) * inv;
+            pb = (uhi + bb) * inv;
+        }
+        const int est = (int)ceilf(fminf(pa, pb));
+        const int p0 = wf(L, est - 1) != 0.0f ? est - 1 : (wf(L, est) != 0.0f ? est : est + 1);
+        const int2 sp = spans[L - 1];
+        // one window position at a time: 112 instead of 146 registers, four
+        // waves per SIMD (1024^2 x 64 x 900: 11.5 against 12.1 ms)
+#pragma unroll 1
+        for (int i = 0; i < 3; ++i) add(L, p0 + i, buf, sp, wf(L, p0 + i));
+    };
     if (live(1)) {
         fetch(1);
         put(1, 0);
@@ -721,37 +762,99 @@ __global__ __launch_bounds__(256) void bp_adjoint_kernel(const float *__restrict
         if (L < nu && live(L + 1)) put(L + 1, buf ^ 1);
         if (L + 2 <= nu && live(L + 2)) fetch(L + 2);
         if (active && live(L)) {
-            float pa, pb;
-            if (rows) {
-                const float lsu = su + c * (float)L;
-                pa = (lsu - uhi) * inv;
-                pb = (lsu - ulo) * inv;
-            } else {
-                const float bb = s * (float)L - su;
-                pa = (ulo + bb) * inv;
-                pb = (uhi + bb) * inv;
-            }
-            // first voxel of the run: the estimate, one step down if the
-            // voxel before it contributes, one step up if it does not
-            const int est = (int)ceilf(fminf(pa, pb));
-            const int p0 = weight(L, est - 1) != 0.0f ? est - 1 : (weight(L, est) != 0.0f ? est : est + 1);
-            const int2 sp = spans[L - 1];
-            // one window position at a time: 112 instead of 146 registers, four
-            // waves per SIMD (1024^2 x 64 x 900: 11.5 against 12.1 ms)
-#pragma unroll 1
-            for (int i = 0; i < 3; ++i) add(L, p0 + i, buf, sp);
-            if (wide && u == 1) {
-                add(L, p0 + 3, buf, sp);
-                add(L, p0 + 4, buf, sp);
-            }
+            if constexpr (INTERP == 1) consume(L, buf, weight);
+            else consume(L, buf, hat);
         }
         __syncthreads();
     }
-    if (!active) return;
+    // bins 1 and nu - 1 of the linear form are bp_adjoint_edge_kernel's
+    if (!active || (INTERP == 0 && (u == 1 || u == nu - 1))) return;
     const float w = wts[idx[k]];
     float *o = out + (size_t)idx[k] * nv * nu + u;
 #pragma unroll
     for (int z = 0; z < ZT; ++z) o[(size_t)(z0 + z) * nu] = w * acc[z];
+}
+
+// Bins 1 and nu - 1 of the linear form, whose weights the hat of
+// bp_adjoint_kernel does not reproduce.  One wave per (used projection, edge
+// bin), a lane per slice: the wave walks the lines in bp_adjoint_kernel's
+// order (the projection's stepping class, lines ascending, positions
+// ascending within a line, voxels of the mask only) and adds each
+// contributing voxel with the forward's weight by one FMA, the same float
+// operations in the same order as the other bins get there (the additions of
+// zero weights that kernel makes and this one skips leave a sum unchanged).
+// The line walk is the same for every lane, so the mask and the weights are
+// wave-uniform; 4 lines are gathered before their FMAs so that their loads
+// are in flight together.  Writes only bins 1 and nu - 1 and may run
+// concurrently with bp_adjoint_kernel.
+__global__ __launch_bounds__(64) void bp_adjoint_edge_kernel(const float *__restrict__ g, float *__restrict__ out,
+                                                             const float4 *__restrict__ tab,
+                                                             const int2 *__restrict__ mask,
+                                                             const int2 *__restrict__ cmask,
+                                                             const int *__restrict__ idx,
+                                                             const float *__restrict__ wts, int nu, int nv, int nz4)
+{
+    constexpr int kLines = 4, kPos = 8;   // lines per gather; window positions est - 2 .. est + 5
+    const int k = (int)blockIdx.x >> 1, e = (int)blockIdx.x & 1;
+    const int u = e == 0 ? 1 : nu - 1;
+    if (u >= nu || (e == 1 && u == 1)) return;
+    const float4 tb = tab[k];
+    const float su = tb.x, c = tb.y, s = tb.z;
+    const bool rows = fabsf(s) >= fabsf(c);
+    const float fu = (float)u;
+    const float ulo = u <= 1 ? -1.0f : fu - 1.0f, uhi = fu + 1.0f;
+    const float inv = rows ? 1.0f / s : 1.0f / c;
+    const int2 *lmask = rows ? mask : cmask;
+    const float wk = wts[idx[k]];
+    float *o = out + (size_t)idx[k] * nv * nu + u;
+    for (int z = (int)threadIdx.x; z - (int)threadIdx.x < nz4; z += 64) {
+        const bool zin = z < nz4;
+        float acc = 0.0f;
+        for (int L0 = 1; L0 <= nu; L0 += kLines) {
+            float wv[kLines][kPos], xv[kLines][kPos];
+#pragma unroll
+            for (int l = 0; l < kLines; ++l) {
+                const int L = L0 + l;
+                int lo = 1, hi = 0;
+                if (L <= nu) {
+                    const int2 m = lmask[L - 1];   // positions m.x + 1 .. m.y (1-based)
+                    float pa, pb;
+                    if (rows) {
+                        const float lsu = su + c * (float)L;
+                        pa = (lsu - uhi) * inv;
+                        pb = (lsu - ulo) * inv;
+                    } else {
+                        const float bb = s * (float)L - su;
+                        pa = (ulo + bb) * inv;
+                        pb = (uhi + bb) * inv;
+                    }
+                    // the run (up to 5 voxels for bin 1, 3 otherwise) lies in est - 1 .. est + 4
+                    const int est = (int)ceilf(fminf(pa, pb));
+                    lo = max(est - 2, m.x + 1);
+                    hi = min(est - 2 + kPos - 1, m.y);
+                }
+#pragma unroll
+                for (int i = 0; i < kPos; ++i) {
+                    const int p = lo + i;
+                    const bool inr = p <= hi;
+                    const float w = inr ? adj_weight<0>(su, c, s, rows, L, p, u, nu - 2) : 0.0f;
+                    wv[l][i] = w;
+                    xv[l][i] = 0.0f;
+                    if (w != 0.0f && zin) {
+                        const size_t off = rows ? ((size_t)(L - 1) * nu + (p - 1)) * nv
+                                                : ((size_t)(p - 1) * nu + (L - 1)) * nv;
+                        xv[l][i] = g[off + z];
+                    }
+                }
+            }
+#pragma unroll
+            for (int l = 0; l < kLines; ++l)
+#pragma unroll
+                for (int i = 0; i < kPos; ++i)
+                    if (wv[l][i] != 0.0f) acc = __builtin_fmaf(wv[l][i], xv[l][i], acc);
+        }
+        if (zin) o[(size_t)z * nu] = wk * acc;
+    }
 }
 
 // Grid of the tiled kernel (16 x 16 voxel tiles): the XCD-striped 1-D grid
@@ -863,6 +966,8 @@ struct mr_bp_plan {
     size_t h_stage_cap = 0;               // bytes
     hipEvent_t stage_ev = nullptr;        // recorded after the last upload from h_stage
     hipEvent_t done_ev = nullptr;         // recorded after the last mr_bp_parallel's kernels
+    hipStream_t side = nullptr;           // the adjoint's edge-bin kernel (default priority)
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace {
@@ -969,6 +1074,10 @@ extern "C" int mr_bp_plan_destroy(mr_bp_plan *plan)
         (void)hipEventDestroy(plan->stage_ev);
     }
     if (plan->done_ev) (void)hipEventDestroy(plan->done_ev);
+    if (plan->side) (void)hipStreamSynchronize(plan->side);
+    if (plan->ev_fork) (void)hipEventDestroy(plan->ev_fork);
+    if (plan->ev_join) (void)hipEventDestroy(plan->ev_join);
+    if (plan->side) (void)hipStreamDestroy(plan->side);
     if (plan->h_stage) (void)hipHostFree(plan->h_stage);
     delete plan;
     return MR_OK;
@@ -1244,6 +1353,21 @@ extern "C" int mr_bp_parallel_adjoint(mr_bp_plan *plan, const float *d_grad_volu
     const bool nn = q.interp == MR_INTERP_NEAREST;
     const size_t dyn = sizeof(int2) * q.width;
     mr::ProfScope prof("bp_adjoint", s);
+    if (!nn) {
+        // bins 1 and nu - 1 on the plan's side stream, beside the main kernel
+        if (!plan->side) {
+            MR_HIP(hipStreamCreateWithFlags(&plan->side, hipStreamNonBlocking));
+            MR_HIP(hipEventCreateWithFlags(&plan->ev_fork, hipEventDisableTiming));
+            MR_HIP(hipEventCreateWithFlags(&plan->ev_join, hipEventDisableTiming));
+        }
+        MR_HIP(hipEventRecord(plan->ev_fork, s));
+        MR_HIP(hipStreamWaitEvent(plan->side, plan->ev_fork, 0));
+        hipLaunchKernelGGL(mr::bp_adjoint_edge_kernel, dim3((unsigned)(2 * nk)), dim3(64), 0, plan->side,
+                           d_grad_volume, d_grad_proj, plan->d_tab, plan->d_mask, plan->d_cmask, plan->d_idx,
+                           plan->d_w, (int)q.width, (int)q.slices, (int)nz4);
+        MR_HIP(hipGetLastError());
+        MR_HIP(hipEventRecord(plan->ev_join, plan->side));
+    }
 #define MR_ADJ_LAUNCH(I, Z)                                                                                        \
     hipLaunchKernelGGL((mr::bp_adjoint_kernel<I, Z>), grid, dim3(256), dyn, s, d_grad_volume, d_grad_proj,        \
                        plan->d_tab, plan->d_mask, plan->d_cmask, plan->d_idx, plan->d_w, d_chunks, d_order,      \
@@ -1256,6 +1380,7 @@ extern "C" int mr_bp_parallel_adjoint(mr_bp_plan *plan, const float *d_grad_volu
     }
 #undef MR_ADJ_LAUNCH
     MR_HIP(hipGetLastError());
+    if (!nn) MR_HIP(hipStreamWaitEvent(s, plan->ev_join, 0));
     MR_HIP(hipStreamSynchronize(s));
     return MR_OK;
     MR_GUARD_END
