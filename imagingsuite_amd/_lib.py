@@ -31,7 +31,8 @@ BATCH_FUSED = 2
 EXPORTED_SYMBOLS = (
     "mr_last_error", "mr_version", "mr_device_count",
     "mr_profile_enable", "mr_profile_reset", "mr_profile_query",
-    "mr_filter_plan_create", "mr_filter_plan_destroy", "mr_filter_plan_info", "mr_filter_rows",
+    "mr_filter_plan_create", "mr_filter_plan_destroy", "mr_filter_plan_info", "mr_filter_plan_kernel",
+    "mr_filter_rows",
     "mr_filter_rows_adjoint", "mr_bp_parallel_adjoint",
     "mr_bp_plan_create", "mr_bp_plan_destroy", "mr_bp_plan_mask", "mr_bp_plan_kernel", "mr_bp_parallel",
     "mr_bp_parallel_noalloc", "mr_fbp_parallel",
@@ -115,6 +116,7 @@ def load():
     L.mr_filter_plan_create.argtypes =[ctypes.POINTER(vp), sz, i32, f32, f32, i32, f32, sz]
     L.mr_filter_plan_destroy.argtypes = [vp]
     L.mr_filter_plan_info.argtypes = [vp, szp, szp, fp]
+    L.mr_filter_plan_kernel.argtypes = [vp, ctypes.c_char_p, sz]
     L.mr_filter_rows.argtypes = [vp, vp, sz, vp]
     L.mr_filter_rows_adjoint.argtypes = [vp, vp, sz, vp]
     L.mr_bp_parallel_adjoint.argtypes = [vp, vp, sz, fp, fp, vp, vp]

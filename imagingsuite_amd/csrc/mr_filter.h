@@ -13,7 +13,9 @@ struct RowFilterArgs {
     long n_rows;
     int n;                      // row width
     int mode;                   // 0 ProjectionFilterSingle, 1 FDK ramp_filter_tuned,
-                                // 2 adjoint of mode 0
+                                // 2 adjoint of mode 0, 3 ProjectionFilterSingle as a
+                                // short convolution (see resp_first), 4 adjoint of
+                                // mode 3
     int insert;                 // mode 0: nInsert; mode 1: (P - n) / 2
     const float *taper;         // mode 0: nInsert sigmoid weights
     const float *hs;            // P real spectral multipliers
@@ -25,6 +27,15 @@ struct RowFilterArgs {
     float cw_u, cw_v;           // pix_u / SDD, pix_v / SDD
     float cw_c0, cw_r0;         // piercing point (column, row) in the window
     float *out;                 // destination rows (nullptr: in place)
+    // mode 3: the row less the straight line from its first to its last
+    // value (s = i * ramp_inv, line = first (1 - s) + last s) is convolved
+    // through the short spectrum in hs; resp_first / resp_last are the
+    // filter's responses at the n output samples to the left taper plus
+    // 1 - s on the row, and to the right taper plus s on the row.  Mode 4:
+    // the responses to the two tapers alone, whose dot products with the
+    // row fold back onto its first and last sample
+    const float *resp_first, *resp_last;
+    float ramp_inv;             // 1 / (n - 1) in float
 };
 
 int run_row_filter(const RowFilterArgs &a, int P, hipStream_t s);

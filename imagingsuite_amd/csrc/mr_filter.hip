@@ -12,9 +12,13 @@
 #include "mr_host.h"
 #include "mr_filter.h"
 
+#include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <complex>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 namespace mr {
@@ -80,7 +84,8 @@ __global__ __launch_bounds__(NT) void row_filter_kernel2(RowFilterArgs a)
         // and scaled in the last inverse pass: no LDS round trip of their
         // own.  Same values and operations as staging the line first.
         float f0 = 0.f, l0 = 0.f, f1 = 0.f, l1 = 0.f, w0 = 1.f, w1 = 1.f, v0 = 0.f, v1 = 0.f;
-        if (a.mode == 0) {
+        float sa0 = 0.f, sa1 = 0.f, sb0 = 0.f, sb1 = 0.f;   // mode 4: the rows' dot products with the responses
+        if (a.mode == 0 || a.mode == 3) {
             f0 = row0[0]; l0 = row0[a.n - 1];
             f1 = row1[0]; l1 = row1[a.n - 1];
         } else {
@@ -90,6 +95,21 @@ __global__ __launch_bounds__(NT) void row_filter_kernel2(RowFilterArgs a)
             v1 = fdk_row_vv2(a, r0 + 1);
         }
         auto load = [&](int i) -> float2 {
+            if (a.mode == 4) {
+                if (i >= a.n) return make_float2(0.0f, 0.0f);
+                const float x0 = row0[i], x1 = has1 ? row1[i] : 0.0f;
+                const float ra = a.resp_first[i], rb = a.resp_last[i];
+                sa0 += ra * x0;
+                sa1 += ra * x1;
+                sb0 += rb * x0;
+                sb1 += rb * x1;
+                return make_float2(x0, x1);
+            }
+            if (a.mode == 3) {
+                if (i >= a.n) return make_float2(0.0f, 0.0f);
+                const float s = (float)i * a.ramp_inv, t1 = 1.0f - s;
+                return make_float2(row0[i] - (f0 * t1 + l0 * s), has1 ? row1[i] - (f1 * t1 + l1 * s) : 0.0f);
+            }
             if (a.mode == 0)
                 return make_float2(pb_pad(row0, a.n, i, a.insert, P, a.taper, f0, l0),
                                    has1 ? pb_pad(row1, a.n, i, a.insert, P, a.taper, f1, l1) : 0.0f);
@@ -102,8 +122,48 @@ __global__ __launch_bounds__(NT) void row_filter_kernel2(RowFilterArgs a)
         };
         __syncthreads();   // the twiddles
         fft_lds2_io<P, NT, false, true, true, true>(line, twl, load, spectrum);
+        __shared__ float red4[NT / 64 + 1][4];
+        float A0 = 0.f, A1 = 0.f, B0 = 0.f, B1 = 0.f;
+        if (a.mode == 4) {
+            // block sums of the dot products, read after the barrier below
+            for (int off = 32; off > 0; off >>= 1) {
+                sa0 += __shfl_xor(sa0, off);
+                sa1 += __shfl_xor(sa1, off);
+                sb0 += __shfl_xor(sb0, off);
+                sb1 += __shfl_xor(sb1, off);
+            }
+            if ((t & 63) == 0) {
+                red4[t >> 6][0] = sa0;
+                red4[t >> 6][1] = sa1;
+                red4[t >> 6][2] = sb0;
+                red4[t >> 6][3] = sb1;
+            }
+            __syncthreads();
+            for (int wv = 0; wv < (NT + 63) / 64; ++wv) {
+                A0 += red4[wv][0];
+                A1 += red4[wv][1];
+                B0 += red4[wv][2];
+                B1 += red4[wv][3];
+            }
+        }
         auto none = [](int) { return make_float2(0.f, 0.f); };
         auto crop = [&](int i, float2 v) {
+            if (a.mode == 4) {
+                if (i >= a.n) return;
+                float y0 = v.x, y1 = v.y;
+                if (i == 0) { y0 += A0; y1 += A1; }
+                if (i == a.n - 1) { y0 += B0; y1 += B1; }
+                o0[i] = y0;
+                if (has1) o1[i] = y1;
+                return;
+            }
+            if (a.mode == 3) {
+                if (i >= a.n) return;
+                const float rf = a.resp_first[i], rl = a.resp_last[i];
+                o0[i] = (v.x + f0 * rf) + l0 * rl;
+                if (has1) o1[i] = (v.y + f1 * rf) + l1 * rl;
+                return;
+            }
             const int d = i - a.insert;
             if (d < 0 || d >= a.n) return;
             if (a.mode == 0) {
@@ -246,7 +306,10 @@ static int launch_row_filter(const RowFilterArgs &a, hipStream_t s)
         set_error("too many rows for one launch");
         return MR_ERR_INVALID;
     }
-    ProfScope prof(a.mode == 0 ? "row_filter_pb" : (a.mode == 1 ? "row_filter_fdk" : "row_filter_adjoint"), s);
+    ProfScope prof((a.mode == 0 || a.mode == 3) ? "row_filter_pb"
+                                                : (a.mode == 1 ? "row_filter_fdk" : "row_filter_adjoint"),
+                   s);
+    MR_REQUIRE((a.mode != 3 && a.mode != 4) || (a.resp_first && a.resp_last), "short filter without responses");
     const size_t lds = sizeof(float2) * (lds_len(P) + P / 8);
     if (lds > 65536)
         MR_HIP(hipFuncSetAttribute((const void *)row_filter_kernel2<P, NT2>,
@@ -299,6 +362,100 @@ std::vector<float> make_twiddles(size_t P)
 
 }  // namespace mr
 
+namespace mr {
+namespace {
+
+// In-place radix-2 FFT of n = 2^k doubles: sign -1 forward, +1 inverse
+// (unnormalised).
+void fft_double(std::vector<std::complex<double>> &x, int sign)
+{
+    const size_t n = x.size();
+    for (size_t i = 1, j = 0; i < n; ++i) {
+        size_t bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(x[i], x[j]);
+    }
+    const double pi = 3.1415926535897932384626433832795;
+    for (size_t len = 2; len <= n; len <<= 1) {
+        const size_t h = len / 2;
+        for (size_t k = 0; k < h; ++k) {
+            const double ang = sign * 2.0 * pi * (double)k / (double)len;
+            const std::complex<double> w(std::cos(ang), std::sin(ang));
+            for (size_t i = k; i < n; i += len) {
+                const std::complex<double> u = x[i], v = x[i + h] * w;
+                x[i] = u + v;
+                x[i + h] = u - v;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+// The ProjectionFilterSingle filter of a row of width n as a short
+// convolution.  The filter is linear in the padded line, which is the row at
+// [ins, ins + n), its first value times the left taper and its last value
+// times the right taper (ProjectionFilter.cpp:346-361).  Write the row as
+// d + first (1 - s) + last s, s = i / (n - 1) in float (d is 0 at both
+// ends, so its convolution has no edge jumps to cancel); then
+//   y = conv(d, g) + first * resp_first + last * resp_last,
+// with g = post0 * IFFT_P(hs) the filter's (real, even) impulse response,
+// resp_first its response to the left taper plus 1 - s on the row and
+// resp_last to the right taper plus s.  d reaches only lags |m| < n, so
+// conv(d, g) is a Q-point circular convolution for any Q >= 2n - 1: the
+// kernel runs a Q-point FFT (Q = P/2 for paddingdoubler 2) instead of the
+// P-point one.  g, its Q-point spectrum (divided by Q) and the responses are
+// computed here in double from the float s the kernel uses.  Returns Q, or 0
+// when no power of two in [64, P) holds 2n - 1.
+size_t short_filter(size_t P, size_t n, size_t ins, const std::vector<float> &hs, const std::vector<float> &taper,
+                    float post0, std::vector<float> &hq, std::vector<float> &resp, std::vector<float> &resp_adj)
+{
+    size_t Q = 64;
+    while (Q < 2 * n - 1) Q <<= 1;
+    if (Q >= P) return 0;
+    // g = post0 * IFFT_P(hs), unnormalised
+    std::vector<std::complex<double>> g(P);
+    for (size_t k = 0; k < P; ++k) g[k] = std::complex<double>((double)hs[k], 0.0);
+    fft_double(g, +1);
+    for (auto &v : g) v *= (double)post0;
+    // the Q-point kernel: lags 0 .. n-1 and -(n-1) .. -1
+    std::vector<std::complex<double>> gq(Q, std::complex<double>(0.0, 0.0));
+    for (size_t m = 0; m < n; ++m) gq[m] = g[m];
+    for (size_t m = 1; m < n; ++m) gq[Q - m] = g[P - m];
+    fft_double(gq, -1);
+    hq.resize(Q);
+    for (size_t k = 0; k < Q; ++k) hq[k] = (float)(gq[k].real() / (double)Q);
+    // responses at the output samples to the two tapers, exactly as the
+    // padded line holds them (left: i < ins; right: P + 15 - i < ins,
+    // i >= ins + n), with 1 - s (left) or s (right) on the row
+    // (the adjoint's: the tapers alone)
+    const float inv = 1.0f / (float)(n - 1);
+    resp.assign(2 * n, 0.0f);
+    resp_adj.assign(2 * n, 0.0f);
+    for (int form = 0; form < 4; ++form) {
+        const int side = form & 1;
+        const bool ramp = form < 2;
+        std::vector<std::complex<double>> t(P, std::complex<double>(0.0, 0.0));
+        for (size_t i = 0; i < P; ++i) {
+            if (side == 0 && i < ins) t[i] = (double)taper[i];
+            if (side == 1 && i >= ins + n && P + 15 - i < ins) t[i] = (double)taper[P + 15 - i];
+        }
+        for (size_t i = 0; ramp && i < n; ++i) {
+            const float s = (float)i * inv, t1 = 1.0f - s;
+            t[ins + i] = (double)(side == 0 ? t1 : s);
+        }
+        fft_double(t, -1);
+        for (size_t k = 0; k < P; ++k) t[k] *= (double)hs[k];
+        fft_double(t, +1);
+        std::vector<float> &dst = ramp ? resp : resp_adj;
+        for (size_t i = 0; i < n; ++i) dst[side * n + i] = (float)(t[ins + i].real() * (double)post0);
+    }
+    return Q;
+}
+
+}  // namespace mr
+
 // --------------------------------------------------------------------------
 // ProjectionFilterSingle plan (C ABI)
 // --------------------------------------------------------------------------
@@ -310,6 +467,12 @@ struct mr_filter_plan {
     float *d_hs = nullptr;      // P
     float *d_taper = nullptr;   // n_insert (>= 1)
     float2 *d_tw = nullptr;     // P
+    // the short form (Q > 0): the spectrum of the Q-point convolution, its
+    // twiddles, and the responses to the two tapers (2 x width)
+    size_t Q = 0;
+    float *d_hq = nullptr;
+    float2 *d_twq = nullptr;
+    float *d_resp = nullptr;    // 4 x width: forward (ramp) responses, then the adjoint's
 };
 
 extern "C" int mr_filter_plan_create(mr_filter_plan **plan, size_t width, int filter_type,
@@ -348,6 +511,22 @@ extern "C" int mr_filter_plan_create(mr_filter_plan **plan, size_t width, int fi
     if (e == hipSuccess)
         e = hipMemcpy(p->d_taper, taper.data(), sizeof(float) * taper.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_tw, tw.data(), sizeof(float) * 2 * P, hipMemcpyHostToDevice);
+    // the short form of the same filter (a Q-point FFT), when Q < P
+    std::vector<float> hq, resp, resp_adj;
+    const size_t Q =
+        e == hipSuccess ? mr::short_filter(P, width, n_insert, hs, taper, p->scale, hq, resp, resp_adj) : 0;
+    if (Q > 0) {
+        const std::vector<float> twq = mr::make_twiddles(Q);
+        resp.insert(resp.end(), resp_adj.begin(), resp_adj.end());
+        e = hipMalloc(&p->d_hq, sizeof(float) * Q);
+        if (e == hipSuccess) e = hipMalloc(&p->d_twq, sizeof(float2) * Q);
+        if (e == hipSuccess) e = hipMalloc(&p->d_resp, sizeof(float) * 4 * width);
+        if (e == hipSuccess) e = hipMemcpy(p->d_hq, hq.data(), sizeof(float) * Q, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(p->d_twq, twq.data(), sizeof(float) * 2 * Q, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(p->d_resp, resp.data(), sizeof(float) * 4 * width, hipMemcpyHostToDevice);
+        p->Q = Q;
+    }
     if (e != hipSuccess) {
         mr_filter_plan_destroy(p);
         return mr::hip_fail(e, "mr_filter_plan_create upload", __FILE__, __LINE__);
@@ -364,6 +543,9 @@ extern "C" int mr_filter_plan_destroy(mr_filter_plan *plan)
     if (plan->d_hs) (void)hipFree(plan->d_hs);
     if (plan->d_taper) (void)hipFree(plan->d_taper);
     if (plan->d_tw) (void)hipFree(plan->d_tw);
+    if (plan->d_hq) (void)hipFree(plan->d_hq);
+    if (plan->d_twq) (void)hipFree(plan->d_twq);
+    if (plan->d_resp) (void)hipFree(plan->d_resp);
     delete plan;
     return MR_OK;
     MR_GUARD_END
@@ -382,6 +564,20 @@ extern "C" int mr_filter_plan_info(const mr_filter_plan *plan, size_t *fft_size,
     MR_GUARD_END
 }
 
+extern "C" int mr_filter_plan_kernel(const mr_filter_plan *plan, char *name, size_t len)
+{
+    MR_GUARD_BEGIN
+    mr::clear_error();
+    MR_REQUIRE(plan != nullptr && name != nullptr && len > 0, "NULL argument");
+    const size_t P = plan->Q > 0 ? plan->Q : plan->P;
+    const size_t nt = std::min<size_t>(std::max<size_t>(P / 8, 64), 1024);
+    const std::string d = "row_filter_kernel2<" + std::to_string(P) + "," + std::to_string(nt) + ">" +
+                          (plan->Q > 0 ? " short" : "");
+    std::snprintf(name, len, "%s", d.c_str());
+    return MR_OK;
+    MR_GUARD_END
+}
+
 int mr::filter_rows_to(mr_filter_plan *plan, const float *in, float *out, size_t n_rows, hipStream_t s)
 {
     if (n_rows == 0) return MR_OK;
@@ -390,6 +586,16 @@ int mr::filter_rows_to(mr_filter_plan *plan, const float *in, float *out, size_t
     a.insert = (int)plan->n_insert; a.taper = plan->d_taper; a.hs = plan->d_hs; a.tw = plan->d_tw;
     a.post0 = plan->scale; a.post1 = 1.0f; a.post2 = 1.0f; a.pre_w = nullptr; a.rows_per_proj = 1;
     a.out = out;
+    if (plan->Q > 0) {
+        // the same filter as a Q-point convolution plus the taper responses
+        a.mode = 3;
+        a.hs = plan->d_hq;
+        a.tw = plan->d_twq;
+        a.resp_first = plan->d_resp;
+        a.resp_last = plan->d_resp + plan->width;
+        a.ramp_inv = 1.0f / (float)(plan->width - 1);
+        return mr::run_row_filter(a, (int)plan->Q, s);
+    }
     return mr::run_row_filter(a, (int)plan->P, s);
 }
 
@@ -407,6 +613,17 @@ extern "C" int mr_filter_rows_adjoint(mr_filter_plan *plan, float *d_rows, size_
     a.insert = (int)plan->n_insert; a.taper = plan->d_taper; a.hs = plan->d_hs; a.tw = plan->d_tw;
     a.post0 = plan->scale; a.post1 = 1.0f; a.post2 = 1.0f; a.pre_w = nullptr; a.rows_per_proj = 1;
     a.out = nullptr;
+    if (plan->Q > 0) {
+        // adjoint of the short form: the same convolution (the impulse
+        // response is even) plus the taper responses' dot products folded
+        // onto the row ends
+        a.mode = 4;
+        a.hs = plan->d_hq;
+        a.tw = plan->d_twq;
+        a.resp_first = plan->d_resp + 2 * plan->width;
+        a.resp_last = plan->d_resp + 3 * plan->width;
+        return mr::run_row_filter(a, (int)plan->Q, (hipStream_t)stream);
+    }
     return mr::run_row_filter(a, (int)plan->P, (hipStream_t)stream);
     MR_GUARD_END
 }

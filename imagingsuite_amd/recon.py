@@ -98,6 +98,13 @@ class ProjectionFilterSingle:
                                     H.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
         self.filter = H
 
+    @property
+    def kernel(self) -> str:
+        """The kernel form this plan launches (mr_filter_plan_kernel)."""
+        buf = ctypes.create_string_buffer(128)
+        check(_lib.load().mr_filter_plan_kernel(self._plan, buf, 128))
+        return buf.value.decode()
+
     def process(self, proj, stream=None):
         """Filters ``proj`` ([..., width] CUDA float32) in place and returns it."""
         import torch

@@ -77,6 +77,12 @@ int mr_filter_plan_destroy(mr_filter_plan *plan);
 /* fft_size, n_insert and (optionally) the fft_size/2 filter coefficients H. */
 int mr_filter_plan_info(const mr_filter_plan *plan, size_t *fft_size, size_t *n_insert,
                         float *h_filter);
+/* The kernel form this plan launches, e.g. "row_filter_kernel2<4096,512> short"
+ * (the filter as a 4096-point convolution plus the taper responses, when
+ * 2 width - 1 fits a power of two below the FFT length) or
+ * "row_filter_kernel2<8192,1024>".  Diagnostics only; no counterpart in the
+ * reference. */
+int mr_filter_plan_kernel(const mr_filter_plan *plan, char *name, size_t len);
 /* Filters n_rows rows of `width` samples in place (device memory).  A projection
  * stack [n_proj][rows][width] is n_proj*rows rows. */
 int mr_filter_rows(mr_filter_plan *plan, float *d_rows, size_t n_rows, void *stream);
