@@ -323,13 +323,16 @@ def bench_parallel(args):
                      "precision": "fused" if True in res else "exact"})
     r["roofline"] = roofline(head["kernel"], head["bp"], nu * nu * nv * n_used, BP_FLOP_PER_UPDATE,
                              alg_bytes=bp_bytes, traffic=traffic, l2_bytes=l2)
-    # the ramp filter beside it: FFT FLOPs (5 P log2 P + P per row, two rows per complex FFT)
-    P = 1 << (int(math.ceil(math.log2(nu))) + 2)
+    # the ramp filter beside it: FFT FLOPs of the transform length the kernel
+    # runs (5 L log2 L + L per row, two rows per complex FFT; L = the short
+    # convolution's length, half the reference's padded length, when it applies)
+    fname = filt.kernel
+    L = int(fname.split("<")[1].split(",")[0])
     f_rows = n_p * nv
-    fname = f"row_filter_kernel2<{P},{min(max(P // 8, 64), 1024)}>"
     f_traffic, f_l2 = profile_counters(f"filter_traffic_c{args.config}.json", fname)
-    r["filter_roofline"] = roofline(fname, head["filt"], f_rows, 5 * P * math.log2(P) + P,
+    r["filter_roofline"] = roofline(fname, head["filt"], f_rows, 5 * L * math.log2(L) + L,
                                     alg_bytes=8.0 * nu * f_rows, traffic=f_traffic, l2_bytes=f_l2)
+    r["filter_roofline"]["reference_fft_size"] = filt.fft_size
     r["kernels_ms"] = {"row_filter": round(head["filt"], 3), "bp_prep": round(head["prep"], 3),
                        "bp_parallel": round(head["bp"], 3)}
     if False in res and True in res:
