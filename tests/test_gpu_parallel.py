@@ -75,6 +75,29 @@ def test_filter_short_form(orc, dev, width, ftype):
         assert err <= 2.5e-6
 
 
+@pytest.mark.parametrize("width,pd", [(577, 1), (512, 1), (300, 3), (1024, 0)])
+def test_filter_padding_doubler(orc, dev, width, pd):
+    """Other paddingdoubler values: the FFT length 2^(ceil(log2 width) + pd)
+    follows the reference, the short form is taken only where a power of two
+    below it holds 2 width - 1 (pd 3), and the result matches the FP64
+    filter of the same padding."""
+    from imagingsuite_amd.recon import ProjectionFilterSingle
+    f = ProjectionFilterSingle(width, filtertype="hamming", cutoff=0.5, paddingdoubler=pd)
+    assert f.fft_size == orc.filter_size(width, pd)
+    Q = 64
+    while Q < 2 * width - 1:
+        Q *= 2
+    assert f.kernel.endswith(" short") == (Q < f.fft_size)
+    rng = np.random.default_rng(width + pd)
+    rows = (rng.normal(size=(5, width)) + 3.0).astype(np.float32)
+    t = _gpu(rows, dev)
+    f.process(t)
+    ref = orc.filter_rows_f64(rows, "hamming", 0.5, 1.0, True, 0.1, pd)
+    err = np.abs(t.cpu().numpy() - ref).max() / np.abs(ref).max()
+    print(f"filter width={width} paddingdoubler={pd} ({f.kernel}): max err vs FP64 {err:.2e}")
+    assert err <= 2.5e-6
+
+
 def _random_stack(rng, n_p, nv, nu):
     return rng.normal(size=(n_p, nv, nu)).astype(np.float32)
 
