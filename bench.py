@@ -342,6 +342,8 @@ def bench_parallel(args):
                            "note": "reference float order per voxel, bit-identical to MultiProjBPparallel"}
     if not args.no_stream:
         r["pcie_inclusive"] = stream_leg(pristine, angles, weights, cfg, center, buf, nu, nv, n_p, updates)
+    if args.config == 2 and not args.no_legs:
+        r["lognorm_filter"] = lognorm_leg(pristine, filt, max(2, min(args.steps, 5)))
     host = None
     if not args.no_cpu_baseline or (args.config == 2 and not args.no_legs):
         host = pristine.cpu().numpy()
@@ -362,6 +364,44 @@ def bench_parallel(args):
         r["cpu_baseline"] = cpu_baseline(host[:args.cpu_sample_proj, :args.cpu_sample_slices], angles, weights, cfg,
                                          min(args.cpu_sample_proj, n_p), min(args.cpu_sample_slices, nv))
     emit(r)
+
+
+def lognorm_leg(pristine, filt, steps):
+    """FullLogNorm in front of the filter on the config-2 stack as raw counts
+    (flat and dark references, per-projection doses): the two kernels one
+    after the other against the normalisation fused into the filter's row
+    load (mr_filter_rows_lognorm, the same values)."""
+    import torch
+    from imagingsuite_amd.recon import FullLogNorm
+    log("[bench] lognorm + filter leg ...")
+    n_p, nv, nu = pristine.shape
+    g = torch.Generator(device=pristine.device)
+    g.manual_seed(7)
+    flat = 3000.0 + 500.0 * torch.rand((nv, nu), device=pristine.device, generator=g)
+    dark = 80.0 + 20.0 * torch.rand((nv, nu), device=pristine.device, generator=g)
+    dose = np.linspace(950.0, 1050.0, n_p).astype(np.float32)
+    raw = ((flat - dark)[None] * torch.exp(-torch.clamp(pristine, max=20.0)) + dark[None]).contiguous()
+    norm = FullLogNorm(flat, dark, flat_dose=1000.0, dark_dose=10.0, usenormregion=True)
+    work = torch.empty_like(raw)
+    out = {}
+    for name, fn in (("separate", lambda w: (norm.process(w, dose), filt.process(w))),
+                     ("fused", lambda w: norm.process_and_filter(w, filt, dose))):
+        times = []
+        for _ in range(steps + 1):
+            work.copy_(raw)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn(work)
+            e1.record()
+            torch.cuda.synchronize()
+            times.append(e0.elapsed_time(e1))
+        out[name + "_ms"] = round(float(np.median(times[1:])), 3)
+    out["note"] = ("raw counts of the config-2 stack (flat and dark references, doses) to filtered -log "
+                   "projections; separate: mr_lognorm then mr_filter_rows; fused: mr_filter_rows_lognorm, "
+                   "same values; median of the timed calls (host dose-table upload and sync included)")
+    del raw, work
+    return out
 
 
 def stream_leg(pristine, angles, weights, cfg, center, buf, nu, nv, n_p, updates):

@@ -8,6 +8,17 @@
 
 namespace mr {
 
+// FullLogNorm of one pixel (NormPlugins.cpp:411-558, non-LUT path):
+// p = img - dark (or img); p <= 0 ? 0 : flat_log - log(p) + dl (no flat:
+// -log(p) + dl).  Shared by the normalisation kernel and the filter's fused
+// load, so that both give the same values.
+__device__ __forceinline__ float lognorm_px(float v, const float *flat_log, const float *dark, size_t i, float dl)
+{
+    const float p = dark ? v - dark[i] : v;
+    if (p <= 0.0f) return 0.0f;
+    return flat_log ? flat_log[i] - logf(p) + dl : -logf(p) + dl;
+}
+
 struct RowFilterArgs {
     float *rows;                // n_rows x n, filtered in place
     long n_rows;
@@ -36,6 +47,11 @@ struct RowFilterArgs {
     // row fold back onto its first and last sample
     const float *resp_first, *resp_last;
     float ramp_inv;             // 1 / (n - 1) in float
+    // mode 3 with norm set: each row is FullLogNorm-normalised on load (row
+    // q of projection j: pixels q n .. q n + n - 1 of flat_log / dark, dose
+    // term dl[j]; rows_per_proj rows per projection)
+    int norm;
+    const float *norm_flat_log, *norm_dark, *norm_dl;
 };
 
 int run_row_filter(const RowFilterArgs &a, int P, hipStream_t s);

@@ -85,7 +85,23 @@ __global__ __launch_bounds__(NT) void row_filter_kernel2(RowFilterArgs a)
         // own.  Same values and operations as staging the line first.
         float f0 = 0.f, l0 = 0.f, f1 = 0.f, l1 = 0.f, w0 = 1.f, w1 = 1.f, v0 = 0.f, v1 = 0.f;
         float sa0 = 0.f, sa1 = 0.f, sb0 = 0.f, sb1 = 0.f;   // mode 4: the rows' dot products with the responses
-        if (a.mode == 0 || a.mode == 3) {
+        // mode 3 with norm: row r is row r % rows_per_proj of projection r / rows_per_proj
+        const float *nf = a.norm_flat_log, *nd = a.norm_dark;
+        size_t pix0 = 0, pix1 = 0;
+        float dl0 = 0.f, dl1 = 0.f;
+        if (a.mode == 3 && a.norm) {
+            const long rr1 = has1 ? r0 + 1 : r0;
+            pix0 = (size_t)(r0 % a.rows_per_proj) * a.n;
+            pix1 = (size_t)(rr1 % a.rows_per_proj) * a.n;
+            dl0 = a.norm_dl ? a.norm_dl[r0 / a.rows_per_proj] : 0.0f;
+            dl1 = a.norm_dl ? a.norm_dl[rr1 / a.rows_per_proj] : 0.0f;
+        }
+        auto raw0 = [&](int i) { return a.norm ? lognorm_px(row0[i], nf, nd, pix0 + i, dl0) : row0[i]; };
+        auto raw1 = [&](int i) { return a.norm ? lognorm_px(row1[i], nf, nd, pix1 + i, dl1) : row1[i]; };
+        if (a.mode == 3) {
+            f0 = raw0(0); l0 = raw0(a.n - 1);
+            f1 = raw1(0); l1 = raw1(a.n - 1);
+        } else if (a.mode == 0) {
             f0 = row0[0]; l0 = row0[a.n - 1];
             f1 = row1[0]; l1 = row1[a.n - 1];
         } else {
@@ -108,7 +124,7 @@ __global__ __launch_bounds__(NT) void row_filter_kernel2(RowFilterArgs a)
             if (a.mode == 3) {
                 if (i >= a.n) return make_float2(0.0f, 0.0f);
                 const float s = (float)i * a.ramp_inv, t1 = 1.0f - s;
-                return make_float2(row0[i] - (f0 * t1 + l0 * s), has1 ? row1[i] - (f1 * t1 + l1 * s) : 0.0f);
+                return make_float2(raw0(i) - (f0 * t1 + l0 * s), has1 ? raw1(i) - (f1 * t1 + l1 * s) : 0.0f);
             }
             if (a.mode == 0)
                 return make_float2(pb_pad(row0, a.n, i, a.insert, P, a.taper, f0, l0),
@@ -625,6 +641,56 @@ extern "C" int mr_filter_rows_adjoint(mr_filter_plan *plan, float *d_rows, size_
         return mr::run_row_filter(a, (int)plan->Q, (hipStream_t)stream);
     }
     return mr::run_row_filter(a, (int)plan->P, (hipStream_t)stream);
+    MR_GUARD_END
+}
+
+extern "C" int mr_filter_rows_lognorm(mr_filter_plan *plan, float *d_proj, size_t n_proj, size_t rows,
+                                      const float *d_flat_log, const float *d_dark, const float *dose,
+                                      float dark_dose, void *stream)
+{
+    MR_GUARD_BEGIN
+    mr::clear_error();
+    MR_REQUIRE(plan != nullptr, "plan is NULL");
+    MR_REQUIRE(n_proj == 0 || (d_proj != nullptr && rows > 0), "NULL projection pointer or empty projection");
+    MR_REQUIRE(n_proj <= 65535, "at most 65535 projections per call");
+    if (n_proj == 0) return MR_OK;
+    hipStream_t s = (hipStream_t)stream;
+    if (plan->Q == 0) {
+        // the P-point filter has no fused load: normalise, then filter
+        int rc = mr_lognorm(d_proj, n_proj, rows * plan->width, d_flat_log, d_dark, dose, dark_dose, stream);
+        if (rc != MR_OK) return rc;
+        return mr::filter_rows_to(plan, d_proj, d_proj, n_proj * rows, s);
+    }
+    // per-projection dose terms as mr_lognorm forms them
+    float *d_dl = nullptr;
+    std::vector<float> dl;
+    if (dose) {
+        dl.resize(n_proj);
+        for (size_t j = 0; j < n_proj; ++j) {
+            const float d = dose[j] - dark_dose;
+            dl[j] = std::log(d < 1 ? 1.0f : d);
+            MR_REQUIRE(dl[j] == dl[j], "The projection dose is a NaN");
+        }
+        MR_HIP(hipMallocAsync((void **)&d_dl, sizeof(float) * n_proj, s));
+        hipError_t e = hipMemcpyAsync(d_dl, dl.data(), sizeof(float) * n_proj, hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) {
+            (void)hipFreeAsync(d_dl, s);
+            return mr::hip_fail(e, "hipMemcpyAsync", __FILE__, __LINE__);
+        }
+    }
+    mr::RowFilterArgs a{};
+    a.rows = d_proj; a.n_rows = (long)(n_proj * rows); a.n = (int)plan->width; a.mode = 3;
+    a.hs = plan->d_hq; a.tw = plan->d_twq; a.out = d_proj;
+    a.resp_first = plan->d_resp; a.resp_last = plan->d_resp + plan->width;
+    a.ramp_inv = 1.0f / (float)(plan->width - 1);
+    a.rows_per_proj = (int)rows;
+    a.norm = 1; a.norm_flat_log = d_flat_log; a.norm_dark = d_dark; a.norm_dl = d_dl;
+    const int rc = mr::run_row_filter(a, (int)plan->Q, s);
+    if (d_dl) (void)hipFreeAsync(d_dl, s);
+    if (rc != MR_OK) return rc;
+    // the host dose table must outlive the copy
+    MR_HIP(hipStreamSynchronize(s));
+    return MR_OK;
     MR_GUARD_END
 }
 

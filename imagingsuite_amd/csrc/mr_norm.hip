@@ -13,19 +13,13 @@
 // handles float4 groups of one projection row range; log is the device
 // logf (within 1 ulp of the host's).
 #include "mr_common.h"
+#include "mr_filter.h"
 
 #include <algorithm>
 #include <cmath>
 #include <vector>
 
 namespace mr {
-
-__device__ __forceinline__ float lognorm_px(float v, const float *flat_log, const float *dark, size_t i, float dl)
-{
-    const float p = dark ? v - dark[i] : v;
-    if (p <= 0.0f) return 0.0f;
-    return flat_log ? flat_log[i] - logf(p) + dl : -logf(p) + dl;
-}
 
 __global__ __launch_bounds__(256) void lognorm_reference_kernel(float *flat, const float *dark, size_t n, float dose)
 {

@@ -303,6 +303,30 @@ class FullLogNorm:
                                      self.dark_dose, _stream(stream)))
         return proj
 
+    def process_and_filter(self, proj, filt: "ProjectionFilterSingle", dose=None, stream=None):
+        """Normalisation and ``filt`` in one pass over ``proj`` (raw counts
+        in, filtered -log projections out, in place): the same values as
+        ``process`` followed by ``filt.process`` (mr_filter_rows_lognorm)."""
+        _require_cuda_f32(proj, 3, "proj")
+        n_proj, rows, width = proj.shape
+        if width != filt.width:
+            raise MrError(_lib.MR_ERR_INVALID, "row width does not match the configured ROI")
+        for ref in (self.flat_log, self.dark):
+            if ref is not None and ref.numel() != rows * width:
+                raise MrError(_lib.MR_ERR_INVALID, "reference image does not match the projection size")
+        d = None
+        if self.usenormregion:
+            if dose is None:
+                raise MrError(_lib.MR_ERR_INVALID, "usenormregion needs the per-projection dose list")
+            d = _f32arr(dose, n_proj)
+        fp = ctypes.POINTER(ctypes.c_float)
+        check(_lib.load().mr_filter_rows_lognorm(filt._plan, _ptr(proj), n_proj, rows,
+                                                 _ptr(self.flat_log) if self.flat_log is not None else None,
+                                                 _ptr(self.dark) if self.dark is not None else None,
+                                                 d.ctypes.data_as(fp) if d is not None else None,
+                                                 self.dark_dose, _stream(stream)))
+        return proj
+
 
 class FDKbp_single:
     """FDK cone-beam back-projector for one slab (SetROI(roi) + Process(3D)),

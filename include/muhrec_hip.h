@@ -189,6 +189,14 @@ int mr_lognorm(float *d_proj, size_t n_proj, size_t n_pix, const float *d_flat_l
  * (ProcessCore(2D), NormPlugins.cpp:343-357). */
 int mr_lognorm_logdose(float *d_proj, size_t n_proj, size_t n_pix, const float *d_flat_log, const float *d_dark,
                        const float *dose_log, void *stream);
+/* FullLogNorm fused into ProjectionFilterSingle's row load: d_proj
+ * [n_proj][rows][width] raw counts become the filtered -log projections in
+ * one pass (arguments as mr_lognorm, n_pix = rows * width); the same values
+ * as mr_lognorm followed by mr_filter_rows.  No reference counterpart: the
+ * reference runs the two modules one after the other. */
+int mr_filter_rows_lognorm(mr_filter_plan *plan, float *d_proj, size_t n_proj, size_t rows,
+                           const float *d_flat_log, const float *d_dark, const float *dose, float dark_dose,
+                           void *stream);
 
 /* ------------------------------------------------------------------------
  * Streaming FBP of a host-resident stack (engine side).  Replaces the
