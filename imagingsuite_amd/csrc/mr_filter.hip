@@ -507,8 +507,15 @@ extern "C" int mr_filter_plan_create(mr_filter_plan **plan, size_t width, int fi
     int rc = mr::build_pb_filter(width, filter_type, cutoff, order, use_bias != 0, bias_weight,
                                  padding_doubler, P, n_insert, H, taper);
     if (rc != MR_OK) return rc;
-    if (P < 64 || P > 16384) {
-        mr::set_error("FFT length " + std::to_string(P) + " outside the supported 64..16384");
+    // the kernels run transforms of 64..16384 points: the P-point line, or
+    // the short form's Q points (Q the power of two >= max(64, 2 width - 1)),
+    // which also serves widths whose P exceeds 16384 (up to width 8192)
+    size_t Qs = 64;
+    while (Qs < 2 * width - 1) Qs <<= 1;
+    const bool short_fits = Qs < P && Qs <= 16384;
+    if (P < 64 || (P > 16384 && !short_fits) || P > ((size_t)1 << 20)) {
+        mr::set_error("FFT length " + std::to_string(P) + " outside the supported 64..16384 (widths up to 8192 "
+                      "through the short form)");
         return MR_ERR_UNSUPPORTED;
     }
     auto *p = new mr_filter_plan();
@@ -529,8 +536,9 @@ extern "C" int mr_filter_plan_create(mr_filter_plan **plan, size_t width, int fi
     if (e == hipSuccess) e = hipMemcpy(p->d_tw, tw.data(), sizeof(float) * 2 * P, hipMemcpyHostToDevice);
     // the short form of the same filter (a Q-point FFT), when Q < P
     std::vector<float> hq, resp, resp_adj;
-    const size_t Q =
-        e == hipSuccess ? mr::short_filter(P, width, n_insert, hs, taper, p->scale, hq, resp, resp_adj) : 0;
+    const size_t Q = (e == hipSuccess && short_fits)
+                         ? mr::short_filter(P, width, n_insert, hs, taper, p->scale, hq, resp, resp_adj)
+                         : 0;
     if (Q > 0) {
         const std::vector<float> twq = mr::make_twiddles(Q);
         resp.insert(resp.end(), resp_adj.begin(), resp_adj.end());

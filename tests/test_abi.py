@@ -41,6 +41,20 @@ def test_invalid_cutoff_rejected_without_device():
     assert not plan.value
 
 
+def test_filter_width_limit_without_device():
+    """Widths up to 8192 run through the short form (a Q <= 16384 point
+    convolution even where the padded length exceeds 16384); wider rows,
+    whose short form would need 32768 points, are refused before any device
+    call."""
+    import imagingsuite_amd._lib as L
+    lib = L.load()
+    plan = ctypes.c_void_p()
+    rc = lib.mr_filter_plan_create(ctypes.byref(plan), 8193, 3, 0.5, 1.0, 1, 0.1, 2)
+    assert rc == L.MR_ERR_UNSUPPORTED
+    assert b"outside the supported" in lib.mr_last_error()
+    assert not plan.value
+
+
 def test_bp_plan_rejects_bad_params():
     import imagingsuite_amd._lib as L
     lib = L.load()

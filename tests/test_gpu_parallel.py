@@ -98,6 +98,24 @@ def test_filter_padding_doubler(orc, dev, width, pd):
     assert err <= 2.5e-6
 
 
+@pytest.mark.parametrize("width", [6000, 8192])
+def test_filter_wide_rows(orc, dev, width):
+    """Rows wider than 4096, whose padded length (32768) exceeds the largest
+    LDS transform: the short form's 16384-point convolution filters them,
+    within 2.5e-6 of the FP64 filter's largest value."""
+    from imagingsuite_amd.recon import ProjectionFilterSingle
+    f = ProjectionFilterSingle(width, filtertype="ram-lak")
+    assert f.fft_size == 32768 and f.kernel == "row_filter_kernel2<16384,1024> short"
+    rng = np.random.default_rng(width)
+    rows = (rng.normal(size=(3, width)) + 2.0).astype(np.float32)
+    t = _gpu(rows, dev)
+    f.process(t)
+    ref = orc.filter_rows_f64(rows, "ram-lak")
+    err = np.abs(t.cpu().numpy() - ref).max() / np.abs(ref).max()
+    print(f"filter width={width} ({f.kernel}): max err vs FP64 {err:.2e}")
+    assert err <= 2.5e-6
+
+
 def _random_stack(rng, n_p, nv, nu):
     return rng.normal(size=(n_p, nv, nu)).astype(np.float32)
 
