@@ -10,6 +10,7 @@
 #include <cstring>
 #include <sstream>
 #include <string>
+#include <vector>
 
 #include "mr/module_api.h"
 #include "muhrec_hip.h"
@@ -22,6 +23,7 @@ thread_local std::string g_err;
 typedef void *(*GetModule_t)(const char *, const char *, void *);
 typedef int (*Destroy_t)(const char *, void *);
 typedef int (*GetModuleList_t)(const char *, void *);
+typedef int (*SetReferences_t)(void *, const float *, const float *, size_t, size_t, float, float);
 
 std::map<std::string, std::string> parse_kv(const char *s)
 {
@@ -74,6 +76,8 @@ ReconConfig make_config(const std::map<std::string, std::string> &kv)
     if (auto v = get("roi")) parse_n(*v, c.ProjectionInfo.roi, 4);
     if (auto v = get("projection_roi3")) c.ProjectionInfo.projection_roi[3] = std::stoul(*v);
     if (auto v = get("usematrixroi")) c.MatrixInfo.bUseROI = (*v == "true");
+    if (auto v = get("ob_count")) c.ProjectionInfo.nOBCount = std::stoul(*v);
+    if (auto v = get("dc_count")) c.ProjectionInfo.nDCCount = std::stoul(*v);
     if (auto v = get("matrixroi")) parse_n(*v, c.MatrixInfo.roi, 4);
     return c;
 }
@@ -103,9 +107,13 @@ struct mre_engine {
     void *so = nullptr;
     GetModule_t get = nullptr;
     Destroy_t destroy = nullptr;
-    PreprocModuleBase *pre = nullptr;
+    // the pre-processing chain, in order
+    std::vector<PreprocModuleBase *> pres;
+    std::vector<std::string> pre_names;
+    std::vector<std::map<std::string, std::string>> pre_params;
     BackProjectorModuleBase *bp = nullptr;
-    std::map<std::string, std::string> pre_params, bp_params;
+    std::map<std::string, std::string> bp_params;
+    std::vector<float> doses;   // per-projection doses ("dose" list); empty: 1
     ReconConfig cfg;
     size_t slice_block = 32;
     // 0: Run3DFull's SliceBlock loop (Process(3D)); 1: ReconEngine::Run, one
@@ -146,7 +154,8 @@ extern "C" int mre_engine_destroy(mre_engine *e)
 {
     if (!e) return 0;
     if (e->destroy) {
-        if (e->pre) e->destroy("muhrec", e->pre);
+        for (auto *m : e->pres)
+            if (m) e->destroy("muhrec", m);
         if (e->bp) e->destroy("muhrecbp", e->bp);
     }
     if (e->so) dlclose(e->so);
@@ -174,27 +183,56 @@ extern "C" int mre_engine_create(mre_engine **engine, const char *module_so, con
         e->destroy = reinterpret_cast<Destroy_t>(dlsym(e->so, "Destroy"));
         if (!e->destroy) throw ReconException("Could not find symbol Destroy", __FILE__, __LINE__);
         e->cfg = make_config(parse_kv(config));
-        e->pre_params = parse_kv(preproc_params);
+        // pre-processing chain: module names and their parameter strings separated by '|'
+        auto split = [](const char *str) {
+            std::vector<std::string> out;
+            std::string v = str ? str : "";
+            size_t pos = 0;
+            for (;;) {
+                const size_t end = v.find('|', pos);
+                out.push_back(v.substr(pos, end == std::string::npos ? std::string::npos : end - pos));
+                if (end == std::string::npos) break;
+                pos = end + 1;
+            }
+            return out;
+        };
+        if (preproc_module && *preproc_module) {
+            e->pre_names = split(preproc_module);
+            const std::vector<std::string> ps = split(preproc_params);
+            for (size_t i = 0; i < e->pre_names.size(); ++i)
+                e->pre_params.push_back(parse_kv(i < ps.size() ? ps[i].c_str() : ""));
+        }
         e->bp_params = parse_kv(bp_params);
         e->cfg.backprojector.m_sModule = bp_module;
         e->cfg.backprojector.parameters = e->bp_params;
         const auto kvc = parse_kv(config);
-        // the filter hands its device copy of the stack to a parallel-beam
-        // back-projector of the same library unless the caller says otherwise
+        // each module of this library hands its device copy of the stack to
+        // the next one when that can take it (FullLogNorm ->
+        // ProjectionFilterSingle -> a parallel-beam back-projector) unless the
+        // caller says otherwise
         const std::string bpn = bp_module;
         const auto dh = kvc.find("device_handoff");
         if (dh != kvc.end() && dh->second != "true" && dh->second != "false")
             throw ReconException("device_handoff must be true or false", __FILE__, __LINE__);
-        if (preproc_module && std::string(preproc_module) == "ProjectionFilterSingle" &&
-            (bpn == "MultiProjBPparallel" || bpn == "MultiProjBP" || bpn == "NearestNeighborBP") &&
-            e->pre_params.find("keepondevice") == e->pre_params.end())
-            e->pre_params["keepondevice"] = (dh == kvc.end() || dh->second == "true") ? "true" : "false";
+        const bool handoff = dh == kvc.end() || dh->second == "true";
+        const bool bp_takes = bpn == "MultiProjBPparallel" || bpn == "MultiProjBP" || bpn == "NearestNeighborBP";
+        for (size_t i = 0; i < e->pre_names.size(); ++i) {
+            const std::string &n = e->pre_names[i];
+            const bool next_takes = i + 1 < e->pre_names.size() ? e->pre_names[i + 1] == "ProjectionFilterSingle"
+                                                                  : bp_takes;
+            const bool gives = (n == "FullLogNorm" && i + 1 < e->pre_names.size() &&
+                                e->pre_names[i + 1] == "ProjectionFilterSingle") ||
+                               (n == "ProjectionFilterSingle" && next_takes && i + 1 == e->pre_names.size());
+            if (gives && e->pre_params[i].find("keepondevice") == e->pre_params[i].end())
+                e->pre_params[i]["keepondevice"] = handoff ? "true" : "false";
+        }
         const auto rb = kvc.find("rescale_between");
         if (rb != kvc.end()) e->rescale_between = std::stof(rb->second);
-        if (preproc_module && *preproc_module) {
-            e->pre = static_cast<PreprocModuleBase *>(e->get("muhrec", preproc_module, nullptr));
-            if (!e->pre) throw ReconException(std::string("Failed to create module ") + preproc_module, __FILE__, __LINE__);
-            e->pre->Configure(e->cfg, e->pre_params);
+        for (size_t i = 0; i < e->pre_names.size(); ++i) {
+            auto *m = static_cast<PreprocModuleBase *>(e->get("muhrec", e->pre_names[i].c_str(), nullptr));
+            if (!m) throw ReconException("Failed to create module " + e->pre_names[i], __FILE__, __LINE__);
+            e->pres.push_back(m);
+            m->Configure(e->cfg, e->pre_params[i]);
         }
         const auto ab = kvc.find("bp_abort_after");
         if (ab != kvc.end()) e->abort_bp = new AbortAfter(std::stoi(ab->second));
@@ -244,6 +282,7 @@ extern "C" int mre_engine_run(mre_engine *e, const float *projections, size_t n_
             // back-projector's Process(2D) with the float angle and weight
             if (cone) throw ReconException("projection-at-a-time runs parallel beam only", __FILE__, __LINE__);
             size_t block[4] = {roi[0], roi[1], roi[2], roi[3]};
+            for (auto *m : e->pres) m->SetROI(block);   // ReconEngine.cpp:420-430
             e->bp->SetROI(block);
             std::map<std::string, std::string> params;
             params["dose"] = "1";
@@ -251,7 +290,8 @@ extern "C" int mre_engine_run(mre_engine *e, const float *projections, size_t n_
             for (size_t p = 0; p < n_proj; ++p) {
                 kipl::base::TImage<float, 2> img(dims);
                 std::memcpy(img.GetDataPtr(), projections + p * rows * width, sizeof(float) * rows * width);
-                if (e->pre) e->pre->Process(img, params);
+                if (!e->doses.empty()) params["dose"] = list_text(&e->doses[p], 1);
+                for (auto *m : e->pres) m->Process(img, params);
                 const bool last = e->projection_at_a_time == 2 ? p + 1 == n_proj : n_proj < p + 1;
                 e->bp->Process(img, angles[p], weights[p], last);
             }
@@ -267,9 +307,14 @@ extern "C" int mre_engine_run(mre_engine *e, const float *projections, size_t n_
         std::map<std::string, std::string> params;
         params["angles"] = list_text(angles, n_proj);
         params["weights"] = list_text(weights, n_proj);
-        std::string dose;
-        for (size_t i = 0; i < n_proj; ++i) dose += "1 ";
-        params["dose"] = dose;
+        if (!e->doses.empty()) {
+            if (e->doses.size() != n_proj) throw ReconException("dose list does not match the projections", __FILE__, __LINE__);
+            params["dose"] = list_text(e->doses.data(), n_proj);
+        } else {
+            std::string dose;
+            for (size_t i = 0; i < n_proj; ++i) dose += "1 ";
+            params["dose"] = dose;
+        }
         for (size_t b = 0; b < nblocks; ++b) {
             size_t block[4] = {roi[0], roi[1] + b * sb, roi[2], std::min(roi[1] + (b + 1) * sb, roi[3])};
             size_t r0 = block[1] - roi[1], r1 = block[3] - roi[1];   // rows of the stack used
@@ -294,7 +339,12 @@ extern "C" int mre_engine_run(mre_engine *e, const float *projections, size_t n_
             for (size_t p = 0; p < n_proj; ++p)
                 std::memcpy(img.GetLinePtr(0, p), projections + (p * rows + r0) * width, sizeof(float) * nr * width);
             clock.lap(0);
-            if (e->pre) e->pre->Process(img, params);
+            // ReconEngine::Process3D (ReconEngine.cpp:1204-1214): SetROI of every
+            // pre-processing module for the block, then its Process
+            // (for cone beam the projections' rows are the detector window)
+            size_t proj_roi[4] = {block[0], r0 + (cone ? 0 : roi[1]), block[2], r1 + (cone ? 0 : roi[1])};
+            for (auto *m : e->pres) m->SetROI(proj_roi);
+            for (auto *m : e->pres) m->Process(img, params);
             if (e->rescale_between != 1.0f) {
                 float *d = img.GetDataPtr();
                 for (size_t i = 0; i < img.Size(); ++i) d[i] *= e->rescale_between;
@@ -316,6 +366,48 @@ extern "C" int mre_engine_run(mre_engine *e, const float *projections, size_t n_
         g_err = ex.what();
         return 2;
     }
+    return 0;
+}
+
+extern "C" int mre_engine_set_references(mre_engine *e, const float *dark, const float *flat, size_t rows,
+                                         size_t width, float dark_dose, float flat_dose)
+{
+    g_err.clear();
+    if (!e) {
+        g_err = "NULL argument";
+        return 1;
+    }
+    auto set = reinterpret_cast<SetReferences_t>(dlsym(e->so, "mr_modules_set_references"));
+    if (!set) {
+        g_err = "the module library has no mr_modules_set_references";
+        return 2;
+    }
+    int found = 0;
+    for (size_t i = 0; i < e->pres.size(); ++i) {
+        if (e->pre_names[i] != "FullLogNorm") continue;
+        const int rc = set(e->pres[i], dark, flat, rows, width, dark_dose, flat_dose);
+        if (rc != 0) {
+            auto err = reinterpret_cast<const char *(*)(void)>(dlsym(e->so, "mr_modules_last_error"));
+            g_err = std::string("SetReferenceImages failed: ") + (err ? err() : "");
+            return 2;
+        }
+        ++found;
+    }
+    if (!found) {
+        g_err = "no FullLogNorm module in the chain";
+        return 2;
+    }
+    return 0;
+}
+
+extern "C" int mre_engine_set_doses(mre_engine *e, const float *dose, size_t n)
+{
+    g_err.clear();
+    if (!e || (n && !dose)) {
+        g_err = "NULL argument";
+        return 1;
+    }
+    e->doses.assign(dose, dose + n);
     return 0;
 }
 

@@ -24,6 +24,7 @@
 #include <limits>
 #include <mutex>
 #include <sstream>
+#include <string>
 #include <vector>
 
 #include "mr/module_api.h"
@@ -230,7 +231,7 @@ protected:
     {
         if (img.Size() == 0) throw ReconException("Empty projection image", __FILE__, __LINE__);
         DeviceStack::get().retract(this);
-        filter_rows(img.GetDataPtr(), img.Size(1), img.Size(0), nullptr);
+        filter_rows(img.GetDataPtr(), img.Size(1), img.Size(0), nullptr, nullptr);
         return 0;
     }
 
@@ -243,6 +244,10 @@ protected:
     {
         if (img.Size() == 0) throw ReconException("Empty projection image", __FILE__, __LINE__);
         DeviceStack::get().retract(this);
+        // the previous module's device copy of this very stack (FullLogNorm
+        // with keepondevice), else the host stack is copied in
+        const float *src = DeviceStack::get().take(img.GetDataPtr(), img.Size());
+        if (src) DeviceStack::get().count_taken();
         const size_t nProj = img.Size(2), per = img.Size(0) * img.Size(1);
         if (keep_) stack_.reserve(img.Size());
         constexpr size_t kChunk = 64;
@@ -250,7 +255,7 @@ protected:
             if (UpdateStatus((float)p0 / nProj, m_sModuleName)) return 0;
             const size_t n = std::min(kChunk, nProj - p0);
             filter_rows(img.GetDataPtr() + p0 * per, img.Size(1) * n, img.Size(0),
-                        keep_ ? stack_.p + p0 * per : nullptr);
+                        keep_ ? stack_.p + p0 * per : nullptr, src ? src + p0 * per : nullptr);
         }
         if (keep_) DeviceStack::get().publish(this, stack_.p, img.GetDataPtr(), img.Size());
         UpdateStatus(1.0f, m_sModuleName);
@@ -264,8 +269,9 @@ private:
         plan_ = nullptr;
     }
 
-    // filters the host rows through the device buffer dev (buf_ when null)
-    void filter_rows(float *rows, size_t n_rows, size_t width, float *dev)
+    // filters the host rows through the device buffer dev (buf_ when null);
+    // the input comes from the device copy src when given, else from rows
+    void filter_rows(float *rows, size_t n_rows, size_t width, float *dev, const float *src)
     {
         if (width != nImageSize)
             throw ReconException("Projection width does not match the configured ROI", __FILE__, __LINE__);
@@ -277,7 +283,8 @@ private:
             buf_.reserve(n);
             dev = buf_.p;
         }
-        MRM_CHECK(mr_copy_h2d(dev, rows, n * sizeof(float), nullptr));
+        if (src) MRM_HIP(hipMemcpy(dev, src, n * sizeof(float), hipMemcpyDeviceToDevice));
+        else MRM_CHECK(mr_copy_h2d(dev, rows, n * sizeof(float), nullptr));
         MRM_CHECK(mr_filter_rows(plan_, dev, n_rows, nullptr));
         MRM_CHECK(mr_copy_d2h(rows, dev, n * sizeof(float), nullptr));
     }
@@ -304,6 +311,7 @@ public:
         : PreprocModuleBase("FullLogNorm", interactor)
     {
     }
+    ~HipFullLogNorm() override { DeviceStack::get().retract(this); }
 
     int Configure(ReconConfig config, std::map<std::string, std::string> parameters) override
     {
@@ -317,6 +325,10 @@ public:
                                  __FILE__, __LINE__);
         auto it = parameters.find("referenceaverage");
         sReferenceAverage = it == parameters.end() ? "ImageWeightedAverage" : it->second;
+        // optional, HIP-only: keep the normalised stack on the device for this
+        // library's ProjectionFilterSingle (see DeviceStack)
+        keep_ = optional_bool(parameters, "keepondevice", false);
+        DeviceStack::get().retract(this);
         return 0;
     }
 
@@ -329,13 +341,30 @@ public:
         return p;
     }
 
+    // NormBase::SetROI (NormPlugins.cpp:41-53) loads the references of the
+    // block's ROI; here the references given to SetReferenceImages (those of
+    // the configured projection ROI, or of one block) are cut to the block's
+    // rows.
     bool SetROI(size_t *roi) override
     {
         if ((nOBCount != 0 || nDCCount != 0) && !refs_set_)
             throw ReconException("FullLogNorm: reference images must be given with SetReferenceImages "
                                  "(file loading is not part of the HIP module)",
                                  __FILE__, __LINE__);
-        (void)roi;
+        row_off_ = 0;
+        block_pix_ = npix_;
+        if (refs_set_ && npix_ != 0) {
+            const size_t w = roi[2] - roi[0], rows = roi[3] - roi[1];
+            if (w != ref_width_) throw ReconException("Reference images do not match the ROI width", __FILE__, __LINE__);
+            const size_t ref_rows = npix_ / ref_width_;
+            if (rows != ref_rows) {
+                const size_t r0 = mConfigRoi1();
+                if (roi[1] < r0 || roi[1] - r0 + rows > ref_rows)
+                    throw ReconException("Reference images do not cover the ROI", __FILE__, __LINE__);
+                row_off_ = (roi[1] - r0) * ref_width_;
+            }
+            block_pix_ = rows * ref_width_;
+        }
         return true;
     }
 
@@ -349,6 +378,7 @@ public:
         npix_ = 0;
         if (nOBCount != 0) {
             npix_ = flat.Size();
+            ref_width_ = flat.Size(0);
             dflat_.reserve(npix_);
             MRM_HIP(hipMemcpy(dflat_.p, flat.GetDataPtr(), npix_ * sizeof(float), hipMemcpyHostToDevice));
         }
@@ -356,12 +386,15 @@ public:
             if (npix_ && dark.Size() != npix_)
                 throw ReconException("Dark and flat images differ in size", __FILE__, __LINE__);
             npix_ = dark.Size();
+            ref_width_ = dark.Size(0);
             ddark_.reserve(npix_);
             MRM_HIP(hipMemcpy(ddark_.p, dark.GetDataPtr(), npix_ * sizeof(float), hipMemcpyHostToDevice));
         }
         if (nOBCount != 0)
             MRM_CHECK(mr_lognorm_reference(dflat_.p, nDCCount != 0 ? ddark_.p : nullptr, npix_, fFlatDose, nullptr));
         refs_set_ = true;
+        row_off_ = 0;
+        block_pix_ = npix_;
     }
 
 protected:
@@ -374,14 +407,15 @@ protected:
             dose = dose <= 0.0f ? 1.0f : dose;
         }
         if (dose != dose) throw ReconException("The projection dose is a NaN", __FILE__, __LINE__);
+        DeviceStack::get().retract(this);
         const float cLogDose = std::log(dose);
         const size_t N = img.Size();
-        if ((nOBCount != 0 || nDCCount != 0) && npix_ != N)
+        if ((nOBCount != 0 || nDCCount != 0) && block_pix_ != N)
             throw ReconException("Flat field does not match projection", __FILE__, __LINE__);
         buf_.reserve(N);
         MRM_CHECK(mr_copy_h2d(buf_.p, img.GetDataPtr(), N * sizeof(float), nullptr));
-        MRM_CHECK(mr_lognorm_logdose(buf_.p, 1, N, nOBCount != 0 ? dflat_.p : nullptr,
-                                     nDCCount != 0 ? ddark_.p : nullptr, &cLogDose, nullptr));
+        MRM_CHECK(mr_lognorm_logdose(buf_.p, 1, N, nOBCount != 0 ? dflat_.p + row_off_ : nullptr,
+                                     nDCCount != 0 ? ddark_.p + row_off_ : nullptr, &cLogDose, nullptr));
         MRM_CHECK(mr_copy_d2h(img.GetDataPtr(), buf_.p, N * sizeof(float), nullptr));
         return 0;
     }
@@ -389,29 +423,35 @@ protected:
     int ProcessCore(kipl::base::TImage<float, 3> &img, std::map<std::string, std::string> &coeff) override
     {
         const size_t N = img.Size(0) * img.Size(1), nProj = img.Size(2);
-        if (nOBCount != 0 && npix_ != N) {
+        if (nOBCount != 0 && block_pix_ != N) {
             std::ostringstream msg;
-            msg << "Flat field (" << npix_ << " pixels) does not match projection (" << img.Size(0) << ", "
+            msg << "Flat field (" << block_pix_ << " pixels) does not match projection (" << img.Size(0) << ", "
                 << img.Size(1) << ")";
             throw ReconException(msg.str(), __FILE__, __LINE__);
         }
-        if (nDCCount != 0 && npix_ != N) throw ReconException("Dark field does not match projection", __FILE__, __LINE__);
+        if (nDCCount != 0 && block_pix_ != N)
+            throw ReconException("Dark field does not match projection", __FILE__, __LINE__);
         std::vector<float> dose(nProj);
         if (bUseNormROI) GetFloatParameterVector(coeff, "dose", dose.data(), (int)nProj);
+        DeviceStack::get().retract(this);
         if (UpdateStatus(0.0f, m_sModuleName)) return 0;
         buf_.reserve(img.Size());
         MRM_CHECK(mr_copy_h2d(buf_.p, img.GetDataPtr(), img.Size() * sizeof(float), nullptr));
-        MRM_CHECK(mr_lognorm(buf_.p, nProj, N, nOBCount != 0 ? dflat_.p : nullptr, nDCCount != 0 ? ddark_.p : nullptr,
-                             bUseNormROI ? dose.data() : nullptr, fDarkDose, nullptr));
+        MRM_CHECK(mr_lognorm(buf_.p, nProj, N, nOBCount != 0 ? dflat_.p + row_off_ : nullptr,
+                             nDCCount != 0 ? ddark_.p + row_off_ : nullptr, bUseNormROI ? dose.data() : nullptr,
+                             fDarkDose, nullptr));
         MRM_CHECK(mr_copy_d2h(img.GetDataPtr(), buf_.p, img.Size() * sizeof(float), nullptr));
+        if (keep_) DeviceStack::get().publish(this, buf_.p, img.GetDataPtr(), img.Size());
         UpdateStatus(1.0f, m_sModuleName);
         return 0;
     }
 
 private:
     ReconConfig m_Config;
+    size_t mConfigRoi1() const { return m_Config.ProjectionInfo.roi[1]; }
     size_t nOBCount = 0, nDCCount = 0, npix_ = 0;
-    bool bUseNormROI = true, bUseLUT = false, refs_set_ = false;
+    size_t ref_width_ = 0, row_off_ = 0, block_pix_ = 0;   // the current block's rows of the references
+    bool bUseNormROI = true, bUseLUT = false, refs_set_ = false, keep_ = false;
     std::string sReferenceAverage = "ImageWeightedAverage";
     float fDarkDose = 0.0f, fFlatDose = 1.0f;
     DevBuf dflat_, ddark_, buf_;
@@ -882,3 +922,32 @@ extern "C" int GetModuleList(const char *application, void *listptr)
 }
 
 extern "C" size_t mr_modules_device_handoffs(void) { return DeviceStack::get().taken(); }
+
+// FullLogNorm's averaged references for a module made by GetModule("muhrec",
+// "FullLogNorm"): the call the reference's module makes itself after loading
+// the files (FullLogNorm::LoadReferenceImages, NormPlugins.cpp:260-297).
+// dark / flat: rows x width host images, NULL when their count is 0.
+// Returns 0, -1 for another module, -2 on an error (mr_modules_last_error).
+namespace {
+thread_local std::string g_mod_err;
+}
+extern "C" int mr_modules_set_references(void *module, const float *dark, const float *flat, size_t rows,
+                                         size_t width, float dark_dose, float flat_dose)
+{
+    g_mod_err.clear();
+    auto *norm = dynamic_cast<HipFullLogNorm *>(reinterpret_cast<PreprocModuleBase *>(module));
+    if (!norm) return -1;
+    try {
+        size_t dims[2] = {width, rows};
+        kipl::base::TImage<float, 2> d(dims), f(dims);
+        if (dark) std::memcpy(d.GetDataPtr(), dark, sizeof(float) * rows * width);
+        if (flat) std::memcpy(f.GetDataPtr(), flat, sizeof(float) * rows * width);
+        norm->SetReferenceImages(d, f, dark_dose, flat_dose);
+    } catch (const std::exception &e) {
+        g_mod_err = e.what();
+        return -2;
+    }
+    return 0;
+}
+
+extern "C" const char *mr_modules_last_error(void) { return g_mod_err.c_str(); }

@@ -37,6 +37,10 @@ def load():
         L.mre_module_list.restype = ctypes.c_int
         L.mre_engine_phase_times.argtypes = [vp, ctypes.POINTER(ctypes.c_double), sz]
         L.mre_engine_phase_times.restype = ctypes.c_int
+        L.mre_engine_set_references.argtypes = [vp, fp, fp, sz, sz, ctypes.c_float, ctypes.c_float]
+        L.mre_engine_set_references.restype = ctypes.c_int
+        L.mre_engine_set_doses.argtypes = [vp, fp, sz]
+        L.mre_engine_set_doses.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -75,12 +79,20 @@ class Engine:
     """One pre-processing module + one back-projector, loaded by name."""
 
     def __init__(self, bp_module: str, bp_params: Dict[str, object], config: Dict[str, object],
-                 preproc_module: str = "", preproc_params: Optional[Dict[str, object]] = None,
-                 module_so: str = MODULES_PATH):
+                 preproc_module="", preproc_params=None, module_so: str = MODULES_PATH):
+        """preproc_module: one module name or a list (the pre-processing
+        chain, in order); preproc_params: its parameter dict, or a list of
+        dicts, one per module."""
         L = load()
         self._h = ctypes.c_void_p()
+        if isinstance(preproc_module, (list, tuple)):
+            preproc_module = "|".join(preproc_module)
+        if isinstance(preproc_params, (list, tuple)):
+            pp = b"|".join(_kv(d) for d in preproc_params)
+        else:
+            pp = _kv(preproc_params)
         rc = L.mre_engine_create(ctypes.byref(self._h), module_so.encode(), preproc_module.encode(),
-                                 _kv(preproc_params), bp_module.encode(), _kv(bp_params), _kv(config))
+                                 pp, bp_module.encode(), _kv(bp_params), _kv(config))
         if rc != 0:
             raise EngineError(L.mre_last_error().decode())
         self.config = config
@@ -104,6 +116,28 @@ class Engine:
         if rc != 0:
             raise EngineError(load().mre_last_error().decode())
         return vol
+
+    def set_references(self, dark: Optional[np.ndarray], flat: Optional[np.ndarray], dark_dose: float,
+                       flat_dose: float):
+        """FullLogNorm's averaged references ([rows, width] float32 or None)
+        and doses, for every FullLogNorm module of the chain."""
+        ref = dark if dark is not None else flat
+        rows, width = ref.shape
+        fp = ctypes.POINTER(ctypes.c_float)
+        d = None if dark is None else np.ascontiguousarray(dark, np.float32)
+        f = None if flat is None else np.ascontiguousarray(flat, np.float32)
+        rc = load().mre_engine_set_references(self._h, None if d is None else d.ctypes.data_as(fp),
+                                              None if f is None else f.ctypes.data_as(fp), rows, width,
+                                              float(dark_dose), float(flat_dose))
+        if rc != 0:
+            raise EngineError(load().mre_last_error().decode())
+
+    def set_doses(self, dose):
+        """Per-projection doses (sent to the modules as 6-digit text)."""
+        d = np.ascontiguousarray(dose, np.float32)
+        rc = load().mre_engine_set_doses(self._h, d.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), d.size)
+        if rc != 0:
+            raise EngineError(load().mre_last_error().decode())
 
     PHASES = ("copy_in", "preprocess", "set_roi", "backproject", "get_slice")
 

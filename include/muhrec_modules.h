@@ -32,10 +32,13 @@ int GetModuleList(const char *application, void *listptr);
  * (frameworks/tomography/Framework/ReconFramework/src/ReconEngine.cpp:838-1185,1314-1714).
  * Projection margins (nMargin) are not read: the filter is row-local.
  * Strings use "key=value" pairs separated by ';'.  Module parameters are the
- * modules' own keys; the configuration keys are
+ * modules' own keys; a pre-processing chain names its modules separated by
+ * '|' (e.g. "FullLogNorm|ProjectionFilterSingle") with their parameter
+ * strings separated the same way.  The configuration keys are
  *   geometry=parallel|cone, center, direction=cw|ccw, resolution="px py",
  *   voxelsize="vx vy vz", sod, sdd, ppoint="px py", roi="x0 y0 x1 y1",
  *   projection_roi3, slice_block, usematrixroi=true|false, matrixroi="x0 y0 x1 y1",
+ *   ob_count, dc_count (nOBCount / nDCCount: FullLogNorm's reference counts),
  *   projection_at_a_time=false|reference|final: instead of the SliceBlock loop,
  *   ReconEngine::Run's projection-at-a-time path (ReconEngine.cpp:455-520):
  *   each projection goes through the pre-processing module's Process(2D) and
@@ -45,11 +48,12 @@ int GetModuleList(const char *application, void *listptr);
  *   sets it on the final projection, as StdBackProjectorBase::Process(3D) does;
  *   bp_abort_after=k: the back-projector gets an interactor whose k-th
  *   progress report asks for an abort (the GUI's cancel, at a known point);
- *   device_handoff=true|false (default true): with ProjectionFilterSingle
- *   before a parallel-beam back-projector of the same library, the filter
- *   gets keepondevice=true (unless its parameters set it) and the
- *   back-projector reads the filtered stack from the filter's device buffer
- *   instead of copying the host stack again;
+ *   device_handoff=true|false (default true): FullLogNorm before
+ *   ProjectionFilterSingle, and ProjectionFilterSingle last before a
+ *   parallel-beam back-projector, all of the same library, get
+ *   keepondevice=true (unless their parameters set it): the next module reads
+ *   the stack from the previous one's device buffer instead of copying the
+ *   host stack again;
  *   rescale_between=f: the stack is multiplied by f between the two modules
  *   (a stand-in for another module that changes it in between).
  * Errors return a non-zero code; mre_last_error() describes them.
@@ -72,12 +76,28 @@ int mre_engine_destroy(mre_engine *engine);
  * [3] back-projector Process, [4] GetSlice copy-out (SliceBlock loop only;
  * n <= 5 entries are written). */
 int mre_engine_phase_times(const mre_engine *engine, double *ms, size_t n);
+/* FullLogNorm's averaged references (rows x width host images, NULL when
+ * ob_count / dc_count is 0) and doses, handed to every FullLogNorm module of
+ * the chain (what the reference's module loads from files itself). */
+int mre_engine_set_references(mre_engine *engine, const float *dark, const float *flat, size_t rows,
+                              size_t width, float dark_dose, float flat_dose);
+/* Per-projection doses for the "dose" parameter list (6-digit text, as
+ * ProjectionReader sends them); without it every dose is 1. */
+int mre_engine_set_doses(mre_engine *engine, const float *dose, size_t n);
 const char *mre_last_error(void);
 /* Module library (libMuhRecHIPModules.so), no reference counterpart:
  * number of stacks the parallel-beam back-projectors took from
  * ProjectionFilterSingle's device copy (keepondevice=true) instead of
  * copying the host stack. */
 size_t mr_modules_device_handoffs(void);
+/* Module library, no reference counterpart: FullLogNorm's
+ * SetReferenceImages for a module from GetModule("muhrec", "FullLogNorm")
+ * (the reference's module loads the files itself,
+ * NormPlugins.cpp:260-297).  0, -1 for another module, -2 on an error
+ * (mr_modules_last_error). */
+int mr_modules_set_references(void *module, const float *dark, const float *flat, size_t rows, size_t width,
+                              float dark_dose, float flat_dose);
+const char *mr_modules_last_error(void);
 /* GetModuleList of a module library, serialised as "name:key=value,...;" */
 int mre_module_list(const char *module_so, const char *application, char *buf, size_t len);
 

@@ -86,6 +86,54 @@ def test_engine_device_handoff(orc, dev):
     assert e[0] <= 1e-6
 
 
+def test_engine_lognorm_filter_chain(orc, dev):
+    """FullLogNorm -> ProjectionFilterSingle -> MultiProjBPparallel through
+    the plugin quartet from raw counts, references and doses set through the
+    driver: the same bits as FullLogNorm, the filter and the back-projector
+    through the C ABI; each module hands its stack to the next on the device
+    (two hand-offs per slab), and the same bits without the hand-offs."""
+    from imagingsuite_amd import engine, scan
+    from imagingsuite_amd.recon import BackProjector, FullLogNorm, ProjectionFilterSingle
+    nu, n_p, slices, sb = 96, 90, 32, 16
+    angles, weights = scan.engine_angles_weights(n_p, 0.0, 178.0, 0.1)
+    rng = np.random.default_rng(21)
+    att = rng.uniform(0.0, 2.0, (n_p, slices, nu)).astype(np.float32)
+    flat = rng.uniform(2000.0, 4000.0, (slices, nu)).astype(np.float32)
+    dark = rng.uniform(50.0, 120.0, (slices, nu)).astype(np.float32)
+    # doses as the driver sends them: 6 significant digits
+    dose = np.array([float(f"{d:g}") for d in rng.uniform(800.0, 1200.0, n_p)], np.float32)
+    raw = ((flat - dark)[None] * np.exp(-att.astype(np.float64)) + dark[None]).astype(np.float32)
+    flat_dose, dark_dose = 1000.0, 10.0
+    cfg = {"geometry": "parallel", "center": 48.5, "roi": f"0 0 {nu} {slices}", "resolution": "0.1 0.1",
+           "ob_count": 5, "dc_count": 5}
+    bpp = {"ProjectionBufferSize": 16, "SliceBlock": sb, "SubVolume": "1 1"}
+
+    def run(handoff):
+        eng = engine.Engine("MultiProjBPparallel", bpp, dict(cfg, device_handoff="true" if handoff else "false"),
+                            ["FullLogNorm", "ProjectionFilterSingle"],
+                            [{"usenormregion": "true", "uselut": "false"}, FILTER])
+        eng.set_references(dark, flat, dark_dose, flat_dose)
+        eng.set_doses(dose)
+        n0 = engine.device_handoffs()
+        vol = eng.run(raw, angles, weights, slices)
+        return vol, engine.device_handoffs() - n0
+
+    v_on, taken_on = run(True)
+    v_off, taken_off = run(False)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    norm = FullLogNorm(t(flat), t(dark), flat_dose=flat_dose, dark_dose=dark_dose, usenormregion=True)
+    filt = ProjectionFilterSingle(nu, "Hanning")
+    bp = BackProjector("MultiProjBPparallel", nu, slices, 48.5, buffer_size=16)
+    g = t(raw)
+    norm.process(g, dose)
+    filt.process(g)
+    direct = bp.process(g, angles, weights).cpu().numpy().transpose(2, 0, 1)
+    print(f"chain: hand-offs {taken_on} / {taken_off}, differing voxels {int(np.sum(v_on != direct))}")
+    assert taken_on == 2 * (slices // sb) and taken_off == 0
+    assert np.array_equal(v_on, v_off)
+    assert np.array_equal(v_on, direct)
+
+
 @pytest.mark.parametrize("module", ["FDKbp_single", "FDKbp"])
 def test_engine_fdk_slabs(orc, dev, module):
     """FDKbp_single / FDKbp through the driver: per slab the engine computes
