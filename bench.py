@@ -480,9 +480,42 @@ def plugin_leg(host, angles, weights, cfg, center, buf, updates):
                     "phases_ms": e.phase_times()}
         e.close()
         log(f"[bench] plugin path SliceBlock {sb}, hand-off {handoff}: {ms:.1f} ms")
+    # MuhRec's usual chain from raw counts: FullLogNorm -> ProjectionFilterSingle -> MultiProjBPparallel
+    log("[bench] plugin path, raw counts through FullLogNorm, SliceBlock 256 ...")
+    rng = np.random.default_rng(11)
+    flat = rng.uniform(3000.0, 3500.0, (nv, nu)).astype(np.float32)
+    dark = rng.uniform(80.0, 100.0, (nv, nu)).astype(np.float32)
+    raw = np.empty_like(host)
+    for i in range(host.shape[0]):   # per projection, to bound the temporaries
+        raw[i] = (flat - dark) * np.exp(-np.minimum(host[i], 20.0)) + dark
+    dose = np.array([float(f"{d:g}") for d in np.linspace(950.0, 1050.0, host.shape[0])], np.float32)
+    bpp = {"ProjectionBufferSize": 16, "SliceBlock": 256, "SubVolume": "1 1",
+           "BufferDrop": "true" if buf else "false", "fused": "true"}
+    e = engine.Engine("MultiProjBPparallel", bpp,
+                      {"geometry": "parallel", "center": center, "roi": f"0 0 {nu} {nv}",
+                       "resolution": f"{cfg['pixel_mm']} {cfg['pixel_mm']}", "ob_count": 5, "dc_count": 5},
+                      ["FullLogNorm", "ProjectionFilterSingle"], [{"usenormregion": "true"}, flt])
+    e.set_references(dark, flat, 10.0, 1000.0)
+    e.set_doses(dose)
+    vol = np.empty((nv, nu, nu), np.float32)
+    e.run(raw, angles, weights, nv, out=vol)
+    times = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        e.run(raw, angles, weights, nv, out=vol)
+        times.append((time.perf_counter() - t0) * 1e3)
+    ms = min(times)
+    out["raw_counts_lognorm_chain_sb256"] = {"ms": round(ms, 1), "GUPS": round(updates / (ms / 1e3) / 1e9, 1),
+                                             "phases_ms": e.phase_times()}
+    e.close()
+    del raw
+    log(f"[bench] plugin path, raw counts chain: {ms:.1f} ms")
     out["note"] = ("host stack in, host slices out through GetModule/Configure/SetROI/Process/GetSlice; "
                    "best of 2 runs after one warm-up; _host_copy: the back-projector copies the filtered "
-                   "host slab to the device again instead of taking the filter's device copy")
+                   "host slab to the device again instead of taking the filter's device copy; "
+                   "raw_counts_lognorm_chain: the stack as raw counts (flat and dark references, doses) through "
+                   "FullLogNorm -> ProjectionFilterSingle -> MultiProjBPparallel, each module handing its stack "
+                   "to the next on the device")
     return out
 
 
