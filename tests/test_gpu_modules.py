@@ -132,6 +132,17 @@ def test_engine_lognorm_filter_chain(orc, dev):
     assert taken_on == 2 * (slices // sb) and taken_off == 0
     assert np.array_equal(v_on, v_off)
     assert np.array_equal(v_on, direct)
+    # ReconEngine::Run's projection-at-a-time path through the same chain
+    # (Process(2D) of each module, the last flag on the final projection):
+    # the same volume
+    eng = engine.Engine("MultiProjBPparallel", bpp, dict(cfg, projection_at_a_time="final"),
+                        ["FullLogNorm", "ProjectionFilterSingle"],
+                        [{"usenormregion": "true", "uselut": "false"}, FILTER])
+    eng.set_references(dark, flat, dark_dose, flat_dose)
+    eng.set_doses(dose)
+    v_2d = eng.run(raw, angles, weights, slices)
+    print(f"chain, projection at a time: differing voxels {int(np.sum(v_2d != direct))}")
+    assert np.array_equal(v_2d, direct)
 
 
 @pytest.mark.parametrize("module", ["FDKbp_single", "FDKbp"])
