@@ -75,7 +75,10 @@ class BpParams(ctypes.Structure):
                 ("center", ctypes.c_float), ("direction", ctypes.c_int),
                 ("interp", ctypes.c_int), ("buffer_size", ctypes.c_int),
                 ("multiprojbp", ctypes.c_int), ("use_roi", ctypes.c_int),
-                ("roi", ctypes.c_size_t * 4), ("fused", ctypes.c_int)]
+                ("roi", ctypes.c_size_t * 4), ("fused", ctypes.c_int),
+                ("correct_tilt", ctypes.c_int), ("tilt_angle", ctypes.c_float),
+                ("tilt_pivot", ctypes.c_float), ("tilt_row0", ctypes.c_size_t),
+                ("tilt_rows", ctypes.c_size_t)]
 
 
 class FdkParams(ctypes.Structure):
@@ -85,7 +88,9 @@ class FdkParams(ctypes.Structure):
                 ("direction", ctypes.c_int), ("roi", ctypes.c_size_t * 4),
                 ("projection_roi3", ctypes.c_size_t), ("use_roi", ctypes.c_int),
                 ("matrix_roi", ctypes.c_size_t * 4), ("module", ctypes.c_int), ("fused", ctypes.c_int),
-                ("filter", ctypes.c_int), ("cosine_weight", ctypes.c_int)]
+                ("filter", ctypes.c_int), ("cosine_weight", ctypes.c_int),
+                ("correct_tilt", ctypes.c_int), ("tilt_angle", ctypes.c_float),
+                ("tilt_pivot", ctypes.c_float), ("tilt_rows", ctypes.c_size_t)]
 
 FDK_MODULES = {"FDKbp_single": 0, "FDKbp": 1}
 

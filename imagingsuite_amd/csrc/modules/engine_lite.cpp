@@ -79,6 +79,11 @@ ReconConfig make_config(const std::map<std::string, std::string> &kv)
     if (auto v = get("ob_count")) c.ProjectionInfo.nOBCount = std::stoul(*v);
     if (auto v = get("dc_count")) c.ProjectionInfo.nDCCount = std::stoul(*v);
     if (auto v = get("matrixroi")) parse_n(*v, c.MatrixInfo.roi, 4);
+    // ProjectionInfo's "dims", "correcttilt", "tiltangle", "tiltpivot" (ReconConfig.cpp:151,163-165)
+    if (auto v = get("dims")) parse_n(*v, c.ProjectionInfo.nDims, 2);
+    if (auto v = get("correcttilt")) c.ProjectionInfo.bCorrectTilt = (*v == "true");
+    if (auto v = get("tiltangle")) c.ProjectionInfo.fTiltAngle = std::stof(*v);
+    if (auto v = get("tiltpivot")) c.ProjectionInfo.fTiltPivotPosition = std::stof(*v);
     return c;
 }
 

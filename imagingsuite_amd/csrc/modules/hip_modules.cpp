@@ -72,6 +72,19 @@ bool optional_bool(std::map<std::string, std::string> &p, const std::string &key
     return (it == p.end() || it->second.empty()) ? dflt : string2bool(it->second);
 }
 
+// Axis-tilt settings of a block's plan: bCorrectTilt, fTiltAngle,
+// fTiltPivotPosition, and the projection height whose tan|angle| multiple
+// shrinks the circle mask (BackProjectorModuleBase.cpp:171-180).
+void set_tilt(const ReconConfig &c, const size_t *roi, int &on, float &angle, float &pivot, size_t &rows)
+{
+    on = c.ProjectionInfo.bCorrectTilt ? 1 : 0;
+    angle = c.ProjectionInfo.fTiltAngle;
+    pivot = c.ProjectionInfo.fTiltPivotPosition;
+    rows = c.ProjectionInfo.imagetype == ReconConfig::cProjections::ImageType_Proj_RepeatSinogram
+               ? roi[3]
+               : c.ProjectionInfo.nDims[1];
+}
+
 // Growable device buffer of floats.
 struct DevBuf {
     float *p = nullptr;
@@ -507,9 +520,6 @@ public:
     void SetROI(size_t *roi) override
     {
         ClearAll();
-        if (mConfig.ProjectionInfo.bCorrectTilt)
-            throw ReconException("Tilt correction is not supported by the HIP back-projector", __FILE__,
-                                 __LINE__);
         SizeU = roi[2] - roi[0];
         SizeV = mConfig.ProjectionInfo.imagetype == ReconConfig::cProjections::ImageType_Proj_RepeatSinogram
                     ? roi[3]
@@ -538,6 +548,8 @@ public:
         p.use_roi = mConfig.MatrixInfo.bUseROI ? 1 : 0;
         for (int i = 0; i < 4; ++i) p.roi[i] = mConfig.MatrixInfo.roi[i];
         p.fused = fused_ ? 1 : 0;
+        set_tilt(mConfig, roi, p.correct_tilt, p.tilt_angle, p.tilt_pivot, p.tilt_rows);
+        p.tilt_row0 = roi[1];
         MRM_CHECK(mr_bp_plan_create(&plan_, &p));
         dvol_.reserve(volume.Size());
         dxyz_.reserve(volume.Size());
@@ -755,9 +767,6 @@ public:
     void SetROI(size_t *roi) override
     {
         ClearAll();
-        if (mConfig.ProjectionInfo.bCorrectTilt)
-            throw ReconException("Tilt correction is not supported by the HIP FDK back-projector", __FILE__,
-                                 __LINE__);
         SizeU = roi[2] - roi[0];
         SizeV = mConfig.ProjectionInfo.imagetype == ReconConfig::cProjections::ImageType_Proj_RepeatSinogram
                     ? roi[3]
@@ -790,6 +799,7 @@ public:
         q.fused = fused_ ? 1 : 0;
         q.filter = filter_;
         q.cosine_weight = cosine_ ? 1 : 0;
+        set_tilt(mConfig, roi, q.correct_tilt, q.tilt_angle, q.tilt_pivot, q.tilt_rows);
         MRM_CHECK(mr_fdk_plan_create(&plan_, &q));
     }
 

@@ -195,7 +195,17 @@ void BackProjectorModuleBase::BuildCircleMask()
     const float cx = static_cast<float>(nx >> 1);
     const float cy = static_cast<float>(ny >> 1);
     mask.resize(ny);
-    const float R = cx - 1;
+    float R = cx - 1;
+    if (mConfig.ProjectionInfo.bCorrectTilt) {
+        // the tilted axis leaves the detector in the outer voxels: the radius
+        // shrinks by floor(tan|angle| * projection height)
+        const float slices = static_cast<float>(
+            mConfig.ProjectionInfo.imagetype == ReconConfig::cProjections::ImageType_Proj_RepeatSinogram
+                ? mConfig.ProjectionInfo.roi[3]
+                : mConfig.ProjectionInfo.nDims[1]);
+        R -= std::floor(std::tan(std::fabs(mConfig.ProjectionInfo.fTiltAngle) * 3.1415926535897932384626f / 180.0f) *
+                        slices);
+    }
     const float R2 = R * R;
     for (size_t i = 0; i < ny; ++i) {
         const float y = static_cast<float>(i) - cy;

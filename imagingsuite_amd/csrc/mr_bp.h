@@ -22,6 +22,10 @@ struct BpArgs {
     int ntx, nty, ntz;   // tiles along x, y and slice chunks
     int sw;              // stripe width (tiles) of the XCD-striped order
     int stage64;         // 1: stage with 64-bit addresses even when 32-bit offsets fit (tests)
+    // axis-tilt branch: 0 none, 1 MultiProjBPparallel, 2 MultiProjBP,
+    // 3 NearestNeighborBP; centeroffset and centerinc of the block
+    int tilt;
+    float tilt_co, tilt_ci;
 };
 
 // Uploads the tables of n_proj projections (angles, weights) into the plan
@@ -31,6 +35,9 @@ int bp_load_tables(mr_bp_plan *plan, size_t n_proj, const float *angles, const f
 // with the loaded tables into d_volume [width][width][slices]; no host sync.
 int bp_enqueue(mr_bp_plan *plan, const float *d_proj, float *d_volume, int accumulate, hipStream_t s);
 size_t bp_plan_slices(const mr_bp_plan *plan);
+// First detector row of the next bp_enqueue's block (the tilt terms' roi[1]);
+// no effect without tilt correction.
+void bp_set_tilt_row0(mr_bp_plan *plan, size_t row0);
 size_t bp_plan_width(const mr_bp_plan *plan);
 
 // sino[k][u][v] = proj[idx[k]][v][u] * wts[idx[k]] for the nk used

@@ -208,6 +208,100 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a)
     }
 }
 
+// Axis-tilt branches of the back-projectors (bCorrectTilt), direct gathers.
+// VAR 1, MultiProjBPparallel (MultiProjBPparallel.cpp:49-114):
+//   u = ((fStartU + fCos y) - centeroffset) - fSin x, n = (int)u, f = u - n;
+//   slice group g (4 slices) weights column n+1 by |f_g - g centerinc|, f_g
+//   being f with centerinc subtracted g times;
+// VAR 2, MultiProjBP (MultiProjBP.cpp:50-96): u = ((fStartU + fCos y) - fSin x)
+//   - centeroffset, group g weights column n+1 by |(u - n) - g centerinc|;
+// VAR 3, NearestNeighborBP (NNMultiProjBP.cpp:48-81): u as VAR 2, column n.
+// The reference reads columns n and n+1 for every group (its weights are not
+// a geometric tilt model) and this kernel does the same, in its float order.
+// g counts slice groups from the block's first slice, so a thread whose tile
+// starts at group g0 first subtracts centerinc g0 times, as the reference's
+// loop has by then.
+template <int ZT, int VAR>
+__global__ __launch_bounds__(256) void bp_tilt_kernel(BpArgs a)
+{
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int z0 = blockIdx.z * ZT;
+    if (x >= a.nu || y >= a.nu) return;
+    const int zn = min(ZT, a.nv - z0);
+    const int zl = max(0, min(ZT, a.nz4 - z0));
+    const int g0 = z0 >> 2;
+    float *out = a.vol + ((size_t)y * a.nu + x) * a.nv + z0;
+    float acc[ZT];
+#pragma unroll
+    for (int z = 0; z < ZT; ++z) acc[z] = (a.accumulate && z < zn) ? out[z] : 0.0f;
+    const int2 m = a.mask[y];
+    if (x >= m.x && x < m.y && zl > 0) {
+        const float yf = (float)(y + 1);
+        const float xf = (float)(x + 1);
+        const float co = a.tilt_co, ci = a.tilt_ci;
+        const int num2 = a.nu - 2;
+        const float *base = a.sino + z0;
+        for (int k = 0; k < a.nk; ++k) {
+            const float4 t = a.tab[k];
+            float u;
+            if (VAR == 1) {
+                const float lsu = (t.x + t.y * yf) - co;
+                u = lsu - t.z * xf;
+            } else {
+                const float lsu = t.x + t.y * yf;
+                u = (lsu - t.z * xf) - co;
+            }
+            const int n = (int)u;
+            if (n < 0 || n > num2) continue;
+            const float *A = base + ((size_t)k * a.nu + n) * a.vpad;
+            if (VAR == 3) {
+#pragma unroll
+                for (int g = 0; g < ZT / 4; ++g) {
+                    if (4 * g < zl) {
+                        const float4 va = *reinterpret_cast<const float4 *>(A + 4 * g);
+                        acc[4 * g + 0] = acc[4 * g + 0] + va.x;
+                        acc[4 * g + 1] = acc[4 * g + 1] + va.y;
+                        acc[4 * g + 2] = acc[4 * g + 2] + va.z;
+                        acc[4 * g + 3] = acc[4 * g + 3] + va.w;
+                    }
+                }
+                continue;
+            }
+            const float *B = A + a.bstride;
+            const float fr = u - (float)n;
+            float f = fr;
+            if (VAR == 1)
+                for (int j = 0; j < g0; ++j) f = f - ci;
+#pragma unroll
+            for (int g = 0; g < ZT / 4; ++g) {
+                if (4 * g < zl) {
+                    const float gz = (float)(g0 + g) * ci;
+                    const float wb = VAR == 1 ? fabsf(f - gz) : fabsf(fr - gz);
+                    const float wa = 1.0f - wb;
+                    const float4 va = *reinterpret_cast<const float4 *>(A + 4 * g);
+                    const float4 vb = *reinterpret_cast<const float4 *>(B + 4 * g);
+                    acc[4 * g + 0] = acc[4 * g + 0] + (va.x * wa + vb.x * wb);
+                    acc[4 * g + 1] = acc[4 * g + 1] + (va.y * wa + vb.y * wb);
+                    acc[4 * g + 2] = acc[4 * g + 2] + (va.z * wa + vb.z * wb);
+                    acc[4 * g + 3] = acc[4 * g + 3] + (va.w * wa + vb.w * wb);
+                }
+                if (VAR == 1) f = f - ci;
+            }
+        }
+    }
+    if ((a.nv & 3) == 0 && zn == ZT) {
+#pragma unroll
+        for (int g = 0; g < ZT / 4; ++g)
+            *reinterpret_cast<float4 *>(out + 4 * g) =
+                make_float4(acc[4 * g], acc[4 * g + 1], acc[4 * g + 2], acc[4 * g + 3]);
+    } else {
+#pragma unroll
+        for (int z = 0; z < ZT; ++z)
+            if (z < zn) out[z] = acc[z];
+    }
+}
+
 // LDS-staged variant.  A block owns a 16 x 16 voxel tile and ZT slices.  For
 // each chunk of KC projections it stages, per projection, the SPAN detector
 // columns the tile can reach (the tile diagonal spans at most 15*sqrt(2)+2
@@ -920,8 +1014,26 @@ static void lds_shape(int nu, int nv, int &zt, int &kc, int &pipe)
 // offset, which differs from the staged layout when slices % 4 != 0);
 // impl 1: LDS-staged tiles.  exact: the reference's float operation order
 // (otherwise fused multiply-adds).
+template <int ZT>
+static int launch_bp_tilt(const BpArgs &a, hipStream_t s)
+{
+    dim3 grid((unsigned)((a.nu + 15) / 16), (unsigned)((a.nu + 15) / 16), (unsigned)((a.nv + ZT - 1) / ZT));
+    ProfScope prof("bp_parallel_tilt", s);
+    if (a.tilt == 1) hipLaunchKernelGGL((bp_tilt_kernel<ZT, 1>), grid, dim3(256), 0, s, a);
+    else if (a.tilt == 2) hipLaunchKernelGGL((bp_tilt_kernel<ZT, 2>), grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((bp_tilt_kernel<ZT, 3>), grid, dim3(256), 0, s, a);
+    MR_HIP(hipGetLastError());
+    return MR_OK;
+}
+
 int run_bp(const BpArgs &a, int interp, int impl, int exact, hipStream_t s)
 {
+    if (a.tilt != 0) {
+        MR_REQUIRE(a.tilt >= 1 && a.tilt <= 3, "unknown tilt branch");
+        if (a.nv >= 16) return launch_bp_tilt<16>(a, s);
+        if (a.nv >= 8) return launch_bp_tilt<8>(a, s);
+        return launch_bp_tilt<4>(a, s);
+    }
     if (impl == 0) {
         if (a.nv >= 16) return interp == 0 ? launch_bp<16, 0>(a, s) : launch_bp<16, 1>(a, s);
         if (a.nv >= 8) return interp == 0 ? launch_bp<8, 0>(a, s) : launch_bp<8, 1>(a, s);
@@ -936,8 +1048,12 @@ int run_bp(const BpArgs &a, int interp, int impl, int exact, hipStream_t s)
 
 // The kernel run_bp launches for these settings, e.g.
 // "bp_lds_kernel<32,0,false,4,1> order=1 sw=4".
-std::string describe_bp(int nu, int nv, int interp, int impl, int exact)
+std::string describe_bp(int nu, int nv, int interp, int impl, int exact, int tilt)
 {
+    if (tilt != 0) {
+        const int zt = nv >= 16 ? 16 : (nv >= 8 ? 8 : 4);
+        return "bp_tilt_kernel<" + std::to_string(zt) + "," + std::to_string(tilt) + ">";
+    }
     if (impl == 0) {
         const int zt = nv >= 16 ? 16 : (nv >= 8 ? 8 : 4);
         return "bp_kernel<" + std::to_string(zt) + "," + std::to_string(interp) + ">";
@@ -980,6 +1096,8 @@ struct mr_bp_plan {
     hipEvent_t done_ev = nullptr;         // recorded after the last mr_bp_parallel's kernels
     hipStream_t side = nullptr;           // the adjoint's edge-bin kernel (default priority)
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    int tilt = 0;                         // BpArgs::tilt branch (0: no tilt correction)
+    size_t tilt_row0 = 0;                 // first detector row of the next block
 };
 
 namespace {
@@ -1042,8 +1160,16 @@ extern "C" int mr_bp_plan_create(mr_bp_plan **plan, const mr_bp_params *params)
     MR_REQUIRE(q.direction == 0 || q.direction == 1, "direction must be 0 (CW) or 1 (CCW)");
     MR_REQUIRE(q.buffer_size >= 0, "buffer_size must be >= 0");
     if (q.use_roi) MR_REQUIRE(q.roi[0] <= q.roi[2] && q.roi[1] <= q.roi[3], "matrix ROI x1<x0 or y1<y0");
+    MR_REQUIRE(q.correct_tilt == 0 || q.correct_tilt == 1, "correct_tilt must be 0 or 1");
+    if (q.correct_tilt)
+        MR_REQUIRE(std::isfinite(q.tilt_angle) && std::fabs(q.tilt_angle) < 90.0f && std::isfinite(q.tilt_pivot),
+                   "tilt angle must be finite and below 90 degrees, the pivot finite");
     auto *p = new mr_bp_plan();
     p->p = q;
+    if (q.correct_tilt) {
+        p->tilt = q.interp == MR_INTERP_NEAREST ? 3 : (q.multiprojbp ? 2 : 1);
+        p->tilt_row0 = q.tilt_row0;
+    }
     if (hipGetDevice(&p->device) != hipSuccess) p->device = 0;
     p->vpad = (q.slices + 3) & ~(size_t)3;
     // MultiProjBP's B-column offset only differs from the padded column when
@@ -1053,7 +1179,8 @@ extern "C" int mr_bp_plan_create(mr_bp_plan **plan, const mr_bp_params *params)
     // kernel stage through 64-bit addresses, the path that chunks too large
     // for 32-bit buffer offsets take (tested bit-identical to the default)
     if (const char *e = std::getenv("MR_BP_STAGE64")) p->stage64 = std::atoi(e) == 1 ? 1 : 0;
-    mr::circle_mask(q.width, q.width, q.use_roi != 0, q.roi, false, p->first, p->second);
+    mr::circle_mask(q.width, q.width, q.use_roi != 0, q.roi, false, p->first, p->second,
+                    q.correct_tilt ? mr::tilt_mask_shrink(q.tilt_angle, q.tilt_rows) : 0.0f);
     std::vector<int2> m(q.width);
     for (size_t y = 0; y < q.width; ++y) {
         // reference loop x = first+1 .. second (1-based) == [first, second) 0-based
@@ -1102,7 +1229,7 @@ extern "C" int mr_bp_plan_kernel(const mr_bp_plan *plan, char *name, size_t len)
     mr::clear_error();
     MR_REQUIRE(plan != nullptr && name != nullptr && len > 0, "NULL argument");
     const std::string d = mr::describe_bp((int)plan->p.width, (int)plan->p.slices, plan->p.interp, plan->impl,
-                                          plan->p.fused ? 0 : 1);
+                                          plan->p.fused ? 0 : 1, plan->tilt);
     std::snprintf(name, len, "%s", d.c_str());
     return MR_OK;
     MR_GUARD_END
@@ -1186,10 +1313,13 @@ int bp_enqueue(mr_bp_plan *plan, const float *d_proj, float *d_volume, int accum
     a.bstride = q.multiprojbp ? a.nz4 : (int)plan->vpad;
     a.accumulate = accumulate ? 1 : 0;
     a.stage64 = plan->stage64;
+    a.tilt = plan->tilt;
+    if (plan->tilt) mr::bp_tilt_terms(q.tilt_angle, q.tilt_pivot, plan->tilt_row0, a.tilt_co, a.tilt_ci);
     return mr::run_bp(a, q.interp, plan->impl, q.fused ? 0 : 1, s);
 }
 
 size_t bp_plan_slices(const mr_bp_plan *plan) { return plan->p.slices; }
+void bp_set_tilt_row0(mr_bp_plan *plan, size_t row0) { plan->tilt_row0 = row0; }
 size_t bp_plan_width(const mr_bp_plan *plan) { return plan->p.width; }
 
 }  // namespace mr
@@ -1294,6 +1424,10 @@ extern "C" int mr_bp_parallel_adjoint(mr_bp_plan *plan, const float *d_grad_volu
     MR_REQUIRE(n_proj == 0 || (angles && weights), "NULL angle/weight pointer");
     const mr_bp_params &q = plan->p;
     MR_REQUIRE(!(q.multiprojbp && (q.slices & 3)), "no adjoint for MultiProjBP's column offset with slices % 4 != 0");
+    if (q.correct_tilt) {
+        mr::set_error("no adjoint for the tilt-corrected back-projection");
+        return MR_ERR_UNSUPPORTED;
+    }
     // the kernel keeps every line's span in LDS (8 bytes per line)
     MR_REQUIRE(q.width <= 12288, "the back-projection adjoint supports widths up to 12288");
     hipStream_t s = (hipStream_t)stream;

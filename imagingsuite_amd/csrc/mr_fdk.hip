@@ -31,6 +31,7 @@ struct FdkArgs {
     float *vol;
     const float *ptab;  // 8 floats per projection: cx0 cx1 cx2 cy0 cy1 cy2 zc0 zc2
     const float *zt1;   // [np][nz] zip row term per projection and slice
+    const float *zt0;   // [np][nz] zip column term per slice (tilt correction only)
     const int2 *mask;   // (first, second]: reference loop i = first+1 .. second
     int nx, ny, nz, w, h, np;
     float ox, oy, sx, sy;
@@ -188,6 +189,7 @@ struct FdkdArgs {
     float *vol;
     const double *ptab;  // 8 doubles per projection: cx0 cx1 cx2 cy0 cy1 cy2 zc0 zc2
     const double *zt1;   // [np][nz]
+    const double *zt0;   // [np][nz] column term per slice (tilt correction only)
     const int2 *mask;
     int nx, ny, nz, w, h, np;
     float ox, oy, sx, sy;
@@ -262,6 +264,112 @@ __global__ __launch_bounds__(256) void fdkd_kernel(FdkdArgs a)
             a.vol[((size_t)(a.nz - 1 - (k0 + kk)) * a.ny + j) * a.nx + (a.nx - 1 - i)] = acc[kk];
 }
 
+// Axis-tilt correction (fdkbp_single.cpp:443-448, fdkbp.cpp:452-458): the
+// centre of rotation of slice k replaces the matrix's column offset, so the
+// zip column term, and with it the detector column, depends on the slice.
+// These kernels evaluate column and row per sample, in the reference's
+// order, with get_pixel_value_c's four bounds tests.
+template <int KT>
+__global__ __launch_bounds__(256) void fdk_tilt_kernel(FdkArgs a)
+{
+    int bx, by, bz;
+    if (!fdk_tile(a.order, a.ntx, a.nty, a.ntz, bx, by, bz)) return;
+    int i, j;
+    fdk_voxel(a.wx, a.bwx, bx, by, i, j);
+    const int k0 = bz * KT;
+    if (i >= a.nx || j >= a.ny) return;
+    const int kn = min(KT, a.nz - k0);
+    float acc[KT];
+#pragma unroll
+    for (int kk = 0; kk < KT; ++kk) acc[kk] = 0.0f;
+    const int2 m = a.mask[j];
+    if (i > m.x && i <= m.y) {
+        const float x = a.ox + (float)i * a.sx;
+        const float y = a.oy + (float)j * a.sy;
+        const float fw = (float)a.w, fh = (float)a.h;
+        for (int p = 0; p < a.np; ++p) {
+            const float *t = a.ptab + 8 * p;
+            const float xip0 = x * t[0], xip1 = x * t[1], xip2 = x * t[2];
+            const float yip0 = y * t[3], yip1 = y * t[4], yip2 = y * t[5];
+            const float acc22 = t[7] + yip2;
+            const float dw = 1.0f / (acc22 + xip2);
+            const float dw2 = dw * dw;
+            const float *img = a.img + (size_t)p * a.h * a.w;
+            const float *z0t = a.zt0 + (size_t)p * a.nz + k0;
+            const float *z1t = a.zt1 + (size_t)p * a.nz + k0;
+#pragma unroll
+            for (int kk = 0; kk < KT; ++kk) {
+                if (kk < kn) {
+                    const float acc20 = z0t[kk] + yip0;
+                    const float acc21 = z1t[kk] + yip1;
+                    float r = dw * (acc21 + xip1);
+                    float c = dw * (acc20 + xip0);
+                    r = r + 0.5f;
+                    c = c + 0.5f;
+                    float val = 0.0f;
+                    if (!(r < 0.0f) && !(r >= fh) && !(c < 0.0f) && !(c >= fw))
+                        val = img[(size_t)(int)r * a.w + (int)c];
+                    acc[kk] = acc[kk] + dw2 * val;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int kk = 0; kk < KT; ++kk)
+        if (kk < kn)
+            a.vol[((size_t)(a.nz - 1 - (k0 + kk)) * a.ny + j) * a.nx + (a.nx - 1 - i)] = acc[kk];
+}
+
+template <int KT>
+__global__ __launch_bounds__(256) void fdkd_tilt_kernel(FdkdArgs a)
+{
+    int bx, by, bz;
+    if (!fdk_tile(a.order, a.ntx, a.nty, a.ntz, bx, by, bz)) return;
+    int i, j;
+    fdk_voxel(a.wx, a.bwx, bx, by, i, j);
+    const int k0 = bz * KT;
+    if (i >= a.nx || j >= a.ny) return;
+    const int kn = min(KT, a.nz - k0);
+    float acc[KT];
+#pragma unroll
+    for (int kk = 0; kk < KT; ++kk) acc[kk] = 0.0f;
+    const int2 m = a.mask[j];
+    if (i > m.x && i <= m.y) {
+        const double x = (double)(a.ox + (float)i * a.sx);
+        const double y = (double)(a.oy + (float)j * a.sy);
+        const double fw = (double)a.w, fh = (double)a.h;
+        for (int p = 0; p < a.np; ++p) {
+            const double *t = a.ptab + 8 * p;
+            const double xip0 = x * t[0], xip1 = x * t[1], xip2 = x * t[2];
+            const double yip0 = y * t[3], yip1 = y * t[4], yip2 = y * t[5];
+            const double acc22 = t[7] + yip2;
+            const double dw = 1.0 / (acc22 + xip2);
+            const double dw2 = dw * dw;
+            const float *img = a.img + (size_t)p * a.h * a.w;
+            const double *z0t = a.zt0 + (size_t)p * a.nz + k0;
+            const double *z1t = a.zt1 + (size_t)p * a.nz + k0;
+#pragma unroll
+            for (int kk = 0; kk < KT; ++kk) {
+                if (kk < kn) {
+                    const double acc30 = (z0t[kk] + yip0) + xip0;
+                    const double acc31 = (z1t[kk] + yip1) + xip1;
+                    double r = acc31 * dw, c = acc30 * dw;
+                    r = r + 0.5;
+                    c = c + 0.5;
+                    float val = 0.0f;
+                    if (!(r < 0.0) && !(r >= fh) && !(c < 0.0) && !(c >= fw))
+                        val = img[(size_t)(int)r * a.w + (int)c];
+                    acc[kk] = (float)((double)acc[kk] + dw2 * (double)val);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int kk = 0; kk < KT; ++kk)
+        if (kk < kn)
+            a.vol[((size_t)(a.nz - 1 - (k0 + kk)) * a.ny + j) * a.nx + (a.nx - 1 - i)] = acc[kk];
+}
+
 // Thread layout of the FDK kernels: a wave covers 4 x 16 voxel columns (x by
 // y) and a block 2 x 2 waves.  Measured at 1024^3 x 720 (fdk_kernel<16,
 // false>), columns per wave along x: 64 -> 215 ms, 16 -> 217, 8 -> 163-168,
@@ -288,6 +396,9 @@ struct mr_fdk_plan {
     float *d_ptab = nullptr;
     float *d_zt1 = nullptr;
     float *d_w = nullptr;
+    float *d_zt0 = nullptr;     // tilt correction: per-slice column terms
+    double *d_zt0d = nullptr;
+    size_t cbd3 = 0;            // FDKbp's own window row 3 (its tilt position origin)
     size_t cap_np = 0;
 };
 
@@ -324,6 +435,11 @@ extern "C" int mr_fdk_plan_create(mr_fdk_plan **plan, const mr_fdk_params *param
     MR_REQUIRE(!(q.fused && q.module == MR_FDK_DOUBLE), "fused mode is defined for FDKbp_single only");
     MR_REQUIRE(q.filter == 0 || q.filter == 1, "filter must be 0 (reference ramp) or 1 (Shepp-Logan)");
     MR_REQUIRE(q.cosine_weight == 0 || q.cosine_weight == 1, "cosine_weight must be 0 or 1");
+    MR_REQUIRE(q.correct_tilt == 0 || q.correct_tilt == 1, "correct_tilt must be 0 or 1");
+    MR_REQUIRE(!(q.correct_tilt && q.fused), "fused mode has no tilt correction");
+    if (q.correct_tilt)
+        MR_REQUIRE(std::isfinite(q.tilt_angle) && std::fabs(q.tilt_angle) < 90.0f && std::isfinite(q.tilt_pivot),
+                   "tilt angle must be finite and below 90 degrees, the pivot finite");
     auto *p = new mr_fdk_plan();
     p->p = q;
     mr::FdkGeometry &g = p->g;
@@ -350,9 +466,11 @@ extern "C" int mr_fdk_plan_create(mr_fdk_plan **plan, const mr_fdk_params *param
         mr::fdk_cbct_roi(g, p->nx, cbd, true);
         p->icd[0] = (double)(q.ppoint[0] - (float)cbd[0]);
         p->icd[1] = (double)(q.ppoint[1] - (float)cbd[1]);
+        p->cbd3 = cbd[3];
     }
     std::vector<size_t> first, second;
-    mr::circle_mask(p->nx, p->nx, q.use_roi != 0, q.matrix_roi, true, first, second);
+    mr::circle_mask(p->nx, p->nx, q.use_roi != 0, q.matrix_roi, true, first, second,
+                    q.correct_tilt ? mr::tilt_mask_shrink(q.tilt_angle, q.tilt_rows) : 0.0f);
     std::vector<int2> m(p->nx);
     for (size_t y = 0; y < p->nx; ++y) { m[y].x = (int)first[y]; m[y].y = (int)second[y]; }
     // ramp_filter_tuned's spectral ramp with its roll-off (fdkbp_single.cpp:906-919)
@@ -406,7 +524,8 @@ extern "C" int mr_fdk_plan_destroy(mr_fdk_plan *plan)
     if (!plan) return MR_OK;
     for (void *ptr : {(void *)plan->d_mask, (void *)plan->d_hs, (void *)plan->d_tw,
                       (void *)plan->d_ptab, (void *)plan->d_zt1, (void *)plan->d_w,
-                      (void *)plan->d_ptabd, (void *)plan->d_zt1d})
+                      (void *)plan->d_ptabd, (void *)plan->d_zt1d, (void *)plan->d_zt0,
+                      (void *)plan->d_zt0d})
         if (ptr) (void)hipFree(ptr);
     delete plan;
     return MR_OK;
@@ -436,10 +555,11 @@ extern "C" int mr_fdk_bp(mr_fdk_plan *plan, float *d_proj, size_t n_proj, size_t
     hipStream_t s = (hipStream_t)stream;
     const size_t nx = plan->nx, nz = plan->nz, w = nx, h = win_rows;
     const bool dbl = plan->p.module == MR_FDK_DOUBLE;
+    const bool tilt = plan->p.correct_tilt != 0;
     if (n_proj > plan->cap_np) {
-        for (float **ptr : {&plan->d_ptab, &plan->d_zt1, &plan->d_w})
+        for (float **ptr : {&plan->d_ptab, &plan->d_zt1, &plan->d_w, &plan->d_zt0})
             if (*ptr) { (void)hipFree(*ptr); *ptr = nullptr; }
-        for (double **ptr : {&plan->d_ptabd, &plan->d_zt1d})
+        for (double **ptr : {&plan->d_ptabd, &plan->d_zt1d, &plan->d_zt0d})
             if (*ptr) { (void)hipFree(*ptr); *ptr = nullptr; }
         plan->cap_np = 0;
         MR_HIP(hipMalloc(&plan->d_ptab, sizeof(float) * 8 * n_proj));
@@ -447,12 +567,18 @@ extern "C" int mr_fdk_bp(mr_fdk_plan *plan, float *d_proj, size_t n_proj, size_t
         MR_HIP(hipMalloc(&plan->d_w, sizeof(float) * n_proj));
         MR_HIP(hipMalloc(&plan->d_ptabd, sizeof(double) * 8 * n_proj));
         MR_HIP(hipMalloc(&plan->d_zt1d, sizeof(double) * nz * n_proj));
+        if (tilt) {
+            MR_HIP(hipMalloc(&plan->d_zt0, sizeof(float) * nz * n_proj));
+            MR_HIP(hipMalloc(&plan->d_zt0d, sizeof(double) * nz * n_proj));
+        }
         plan->cap_np = n_proj;
     }
     // per-projection coefficients (fdkbp_single.cpp:420-456; fdkbp.cpp:430-465)
     const size_t npx = std::max<size_t>(n_proj, 1);
-    std::vector<float> ptab(dbl ? 0 : 8 * npx), zt1(dbl ? 0 : nz * npx);
-    std::vector<double> ptabd(dbl ? 8 * npx : 0), zt1d(dbl ? nz * npx : 0);
+    std::vector<float> ptab(dbl ? 0 : 8 * npx), zt1(dbl ? 0 : nz * npx), zt0(!dbl && tilt ? nz * npx : 0);
+    std::vector<double> ptabd(dbl ? 8 * npx : 0), zt1d(dbl ? nz * npx : 0), zt0d(dbl && tilt ? nz * npx : 0);
+    const mr_fdk_params &q = plan->p;
+    const double kDPi = 3.1415926535897932384626433832795028841971693993751;   // kipl dPi
     for (size_t ip = 0; ip < n_proj; ++ip) {
         if (!dbl) {
             const float *ic = plan->ic;
@@ -476,6 +602,15 @@ extern "C" int mr_fdk_bp(mr_fdk_plan *plan, float *d_proj, size_t n_proj, size_t
             for (size_t k = 0; k < nz; ++k) {
                 const float z = plan->origin[2] + (float)(long)k * plan->p.voxel[2];
                 zt1[ip * nz + k] = z * a1 + b1 + pm[7];
+                if (tilt) {
+                    // slice k's centre of rotation (tan in double, held in a
+                    // float) as the column offset (fdkbp_single.cpp:443-448)
+                    const float pos = (float)plan->cbct[3] - (float)(long)k - q.tilt_pivot;
+                    const float cor = (float)(std::tan((double)(-q.tilt_angle) * kDPi / 180.0) * (double)pos +
+                                              (double)q.center);
+                    const float pm3 = ((cor - (q.ppoint[0] - (float)q.roi[0])) * q.voxel[0]) / q.pixel[0];
+                    zt0[ip * nz + k] = z * (pm[2] + ic[0] * pm[10]) + ic[0] * pm[11] + pm3;
+                }
             }
         } else {
             const double *ic = plan->icd;
@@ -500,6 +635,14 @@ extern "C" int mr_fdk_bp(mr_fdk_plan *plan, float *d_proj, size_t n_proj, size_t
             for (size_t k = 0; k < nz; ++k) {
                 const double z = (double)(plan->origin[2] + (float)(long)k * plan->p.voxel[2]);
                 zt1d[ip * nz + k] = z * (pm[6] + ic[1] * pm[10]) + ic[1] * pm[11] + pm[7];
+                if (tilt) {
+                    // fdkbp.cpp:452-458: the same in double, from FDKbp's own window
+                    const double pos = (double)plan->cbd3 - (double)k - (double)q.tilt_pivot;
+                    const double cor = std::tan((double)(-q.tilt_angle) * kDPi / 180.0) * pos + (double)q.center;
+                    const double pm3 = ((cor - (double)(q.ppoint[0] - (float)q.roi[0])) * (double)q.voxel[0]) /
+                                       (double)q.pixel[0];
+                    zt0d[ip * nz + k] = z * (pm[2] + ic[0] * pm[10]) + ic[0] * pm[11] + pm3;
+                }
             }
         }
     }
@@ -513,9 +656,14 @@ extern "C" int mr_fdk_bp(mr_fdk_plan *plan, float *d_proj, size_t n_proj, size_t
     if (!dbl) {
         MR_HIP(hipMemcpyAsync(plan->d_ptab, ptab.data(), sizeof(float) * 8 * n_proj, hipMemcpyHostToDevice, s));
         MR_HIP(hipMemcpyAsync(plan->d_zt1, zt1.data(), sizeof(float) * nz * n_proj, hipMemcpyHostToDevice, s));
+        if (tilt)
+            MR_HIP(hipMemcpyAsync(plan->d_zt0, zt0.data(), sizeof(float) * nz * n_proj, hipMemcpyHostToDevice, s));
     } else {
         MR_HIP(hipMemcpyAsync(plan->d_ptabd, ptabd.data(), sizeof(double) * 8 * n_proj, hipMemcpyHostToDevice, s));
         MR_HIP(hipMemcpyAsync(plan->d_zt1d, zt1d.data(), sizeof(double) * nz * n_proj, hipMemcpyHostToDevice, s));
+        if (tilt)
+            MR_HIP(hipMemcpyAsync(plan->d_zt0d, zt0d.data(), sizeof(double) * nz * n_proj, hipMemcpyHostToDevice,
+                                  s));
     }
     MR_HIP(hipMemcpyAsync(plan->d_w, weights, sizeof(float) * n_proj, hipMemcpyHostToDevice, s));
     // weight, ramp filter, 1/P, (SOD/SDD)^2, then SOD/SDD (FDKbp_single) or
@@ -543,6 +691,7 @@ extern "C" int mr_fdk_bp(mr_fdk_plan *plan, float *d_proj, size_t n_proj, size_t
         MR_REQUIRE(w * h * 4 < (size_t)1 << 31, "FDK window larger than 2^29 samples");
         mr::FdkArgs a{};
         a.img = d_proj; a.vol = d_volume; a.ptab = plan->d_ptab; a.zt1 = plan->d_zt1; a.mask = plan->d_mask;
+        a.zt0 = plan->d_zt0;
         a.nx = (int)nx; a.ny = (int)nx; a.nz = (int)nz; a.w = (int)w; a.h = (int)h; a.np = (int)n_proj;
         a.ox = plan->origin[0]; a.oy = plan->origin[1]; a.sx = plan->p.voxel[0]; a.sy = plan->p.voxel[1];
         a.wx = mr::kFdkWaveX;
@@ -554,12 +703,14 @@ extern "C" int mr_fdk_bp(mr_fdk_plan *plan, float *d_proj, size_t n_proj, size_t
         a.order = 0;
         const dim3 g((unsigned)a.ntx, (unsigned)a.nty, (unsigned)a.ntz);
         mr::ProfScope prof("fdk_bp", s);
-        if (plan->p.fused) hipLaunchKernelGGL((mr::fdk_kernel<KT, true>), g, dim3(256), 0, s, a);
+        if (tilt) hipLaunchKernelGGL((mr::fdk_tilt_kernel<KT>), g, dim3(256), 0, s, a);
+        else if (plan->p.fused) hipLaunchKernelGGL((mr::fdk_kernel<KT, true>), g, dim3(256), 0, s, a);
         else hipLaunchKernelGGL((mr::fdk_kernel<KT, false>), g, dim3(256), 0, s, a);
     } else {
         MR_REQUIRE(w * h * 4 < (size_t)1 << 31, "FDK window larger than 2^29 samples");
         mr::FdkdArgs a{};
         a.img = d_proj; a.vol = d_volume; a.ptab = plan->d_ptabd; a.zt1 = plan->d_zt1d; a.mask = plan->d_mask;
+        a.zt0 = plan->d_zt0d;
         a.nx = (int)nx; a.ny = (int)nx; a.nz = (int)nz; a.w = (int)w; a.h = (int)h; a.np = (int)n_proj;
         a.ox = plan->origin[0]; a.oy = plan->origin[1]; a.sx = plan->p.voxel[0]; a.sy = plan->p.voxel[1];
         a.wx = mr::kFdkWaveX;
@@ -572,7 +723,8 @@ extern "C" int mr_fdk_bp(mr_fdk_plan *plan, float *d_proj, size_t n_proj, size_t
         const dim3 g = a.order ? dim3((unsigned)(8 * ((ntiles + 7) / 8)))
                                : dim3((unsigned)a.ntx, (unsigned)a.nty, (unsigned)a.ntz);
         mr::ProfScope prof("fdkd_bp", s);
-        hipLaunchKernelGGL(mr::fdkd_kernel<KT>, g, dim3(256), 0, s, a);
+        if (tilt) hipLaunchKernelGGL(mr::fdkd_tilt_kernel<KT>, g, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL(mr::fdkd_kernel<KT>, g, dim3(256), 0, s, a);
     }
     MR_HIP(hipGetLastError());
     MR_HIP(hipStreamSynchronize(s));

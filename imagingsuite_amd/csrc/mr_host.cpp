@@ -64,11 +64,12 @@ int build_pb_filter(size_t width, int filter_type, float cutoff, float order, bo
 }
 
 void circle_mask(size_t nx, size_t ny, bool use_roi, const size_t *roi, bool cone,
-                 std::vector<size_t> &first, std::vector<size_t> &second)
+                 std::vector<size_t> &first, std::vector<size_t> &second, float shrink)
 {
     const float cx = static_cast<float>(nx >> 1);
     const float cy = static_cast<float>(ny >> 1);
-    const float R = cx - 1;
+    float R = cx - 1;
+    R -= shrink;
     const float R2 = R * R;
     first.assign(ny, 0);
     second.assign(ny, 0);
@@ -95,6 +96,18 @@ void circle_mask(size_t nx, size_t ny, bool use_roi, const size_t *roi, bool con
             }
         }
     }
+}
+
+float tilt_mask_shrink(float tilt_angle, size_t rows)
+{
+    return std::floor(std::tan(std::fabs(tilt_angle) * kFPi / 180.0f) * static_cast<float>(rows));
+}
+
+void bp_tilt_terms(float tilt_angle, float tilt_pivot, size_t row0, float &centeroffset, float &centerinc)
+{
+    const float t = std::tan(tilt_angle * kFPi / 180.0f);
+    centeroffset = (static_cast<float>(row0) - tilt_pivot) * t;
+    centerinc = 4.0f * t;
 }
 
 std::vector<size_t> used_projections(size_t n_proj, size_t buffer_size)

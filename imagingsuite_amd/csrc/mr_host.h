@@ -18,10 +18,22 @@ int build_pb_filter(size_t width, int filter_type, float cutoff, float order, bo
                     std::vector<float> &H, std::vector<float> &taper);
 
 // BackProjectorModuleBase::BuildCircleMask
-// (frameworks/tomography/Framework/ReconFramework/src/BackProjectorModuleBase.cpp:153-218),
-// no tilt.  first/second are the reference's pair (second inclusive).
+// (frameworks/tomography/Framework/ReconFramework/src/BackProjectorModuleBase.cpp:153-218).
+// first/second are the reference's pair (second inclusive).  shrink is the
+// tilt correction's radius reduction (tilt_mask_shrink), 0 without tilt.
 void circle_mask(size_t nx, size_t ny, bool use_roi, const size_t *roi, bool cone,
-                 std::vector<size_t> &first, std::vector<size_t> &second);
+                 std::vector<size_t> &first, std::vector<size_t> &second, float shrink = 0.0f);
+
+// With bCorrectTilt the mask radius shrinks by floor(tan(|angle|) * rows)
+// (BackProjectorModuleBase.cpp:171-180; rows = ProjectionInfo.nDims[1], or
+// roi[3] for repeated sinograms), in float.
+float tilt_mask_shrink(float tilt_angle, size_t rows);
+
+// Tilt terms of the parallel-beam back-projectors' bCorrectTilt branches
+// (MultiProjBPparallel.cpp:68-75, MultiProjBP.cpp:63-64, NNMultiProjBP.cpp:61-62):
+// centeroffset = (row0 - pivot) tan(angle pi / 180), centerinc = 4 tan(...),
+// in float; row0 = ProjectionInfo.roi[1] of the block.
+void bp_tilt_terms(float tilt_angle, float tilt_pivot, size_t row0, float &centeroffset, float &centerinc);
 
 // Projections the reference back-projects (StdBackProjectorBase::Process,
 // frameworks/tomography/Backprojectors/StdBackProjectors/src/StdBackProjectorBase.cpp:104-114):

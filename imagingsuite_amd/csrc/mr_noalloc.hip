@@ -96,6 +96,10 @@ extern "C" int mr_bp_parallel_noalloc(const mr_bp_params *params, const float *d
     MR_REQUIRE(q.buffer_size >= 0, "buffer_size must be >= 0");
     if (q.use_roi) MR_REQUIRE(q.roi[0] <= q.roi[2] && q.roi[1] <= q.roi[3], "matrix ROI x1<x0 or y1<y0");
     MR_REQUIRE(n_proj <= 65535, "at most 65535 projections per call");
+    if (q.correct_tilt) {
+        mr::set_error("tilt correction runs through mr_bp_parallel (plan), not the no-allocation entry point");
+        return MR_ERR_UNSUPPORTED;
+    }
     const size_t vpad = (q.slices + 3) & ~size_t(3);
     const size_t proj_bytes = q.width * vpad * sizeof(float);
     const size_t sino_bytes = kScratchBytes - kTableBytes;

@@ -92,6 +92,7 @@ struct mr_fbp_stream {
     hipStream_t h2d = nullptr, cs = nullptr, d2h = nullptr;
     hipEvent_t ev_in[2] = {nullptr, nullptr}, ev_comp[2] = {nullptr, nullptr}, ev_out[2] = {nullptr, nullptr};
     int n_threads = 8;
+    size_t tilt_row0 = 0;   // detector row of the first slab (tilt correction)
 };
 
 namespace {
@@ -170,6 +171,7 @@ extern "C" int mr_fbp_stream_create(mr_fbp_stream **stream, const mr_bp_params *
     int rc = mr_filter_plan_create(&s->fplan, bp->width, filter_type, cutoff, order, use_bias, bias_weight,
                                    padding_doubler);
     mr_bp_params q = *bp;
+    s->tilt_row0 = bp->tilt_row0;
     if (rc == MR_OK) {
         q.slices = s->sb;
         rc = mr_bp_plan_create(&s->bp_full, &q);
@@ -289,6 +291,8 @@ int run_stream(mr_fbp_stream *s, const float *h_proj, const float *angles, const
         // kernels
         MR_HIP(hipStreamWaitEvent(s->cs, s->ev_in[i], 0));
         if ((rc = mr_filter_rows(s->fplan, s->d_in[i], np * nr, s->cs)) != MR_OK) return rc;
+        // each slab is a block of its own: the tilt terms use its first row
+        mr::bp_set_tilt_row0(plan, s->tilt_row0 + z0);
         if ((rc = mr::bp_enqueue(plan, s->d_in[i], s->d_vol, 0, s->cs)) != MR_OK) return rc;
         if (b >= 2) MR_HIP(hipStreamWaitEvent(s->cs, s->ev_out[i], 0));    // d_out[i] copied out
         {

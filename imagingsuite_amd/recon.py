@@ -140,6 +140,14 @@ class ProjectionFilterSingle:
     __del__ = close
 
 
+def _set_tilt(p, correct_tilt, tilt_angle, tilt_pivot, row0=None, tilt_rows=0):
+    p.correct_tilt = int(bool(correct_tilt))
+    p.tilt_angle, p.tilt_pivot = float(tilt_angle), float(tilt_pivot)
+    if row0 is not None:
+        p.tilt_row0 = int(row0)
+    p.tilt_rows = int(tilt_rows)
+
+
 class BackProjector:
     """Parallel-beam back-projector (MultiProjBPparallel, MultiProjBP, NearestNeighborBP).
 
@@ -148,10 +156,18 @@ class BackProjector:
     ProjectionBufferSize (16 reproduces the reference exactly, including its
     dropped projections; 0 back-projects every projection).  ``fused`` folds
     the interpolation into fused multiply-adds (float rounding differs from
-    the reference; otherwise results are bit-identical)."""
+    the reference; otherwise results are bit-identical).
+
+    ``correct_tilt`` runs the module's bCorrectTilt branch with
+    ``tilt_angle`` (degrees) and ``tilt_pivot`` (detector row); ``row0`` is
+    the block's first detector row (ProjectionInfo.roi[1]) and ``tilt_rows``
+    the projection height that shrinks the circle mask (ProjectionInfo.nDims[1],
+    default ``slices``)."""
 
     def __init__(self, name: str, width: int, slices: int, center: float, direction: int = 0,
-                 buffer_size: int = 16, roi: Optional[Sequence[int]] = None, fused: bool = False):
+                 buffer_size: int = 16, roi: Optional[Sequence[int]] = None, fused: bool = False,
+                 correct_tilt: bool = False, tilt_angle: float = 0.0, tilt_pivot: float = 0.0,
+                 row0: int = 0, tilt_rows: Optional[int] = None):
         if name not in BP_MODULES:
             raise MrError(_lib.MR_ERR_INVALID, f"unknown back-projector {name}")
         L = _lib.load()
@@ -161,6 +177,7 @@ class BackProjector:
         p.multiprojbp = BP_MODULES[name]["multiprojbp"]
         p.buffer_size = int(buffer_size)
         p.fused = int(bool(fused))
+        _set_tilt(p, correct_tilt, tilt_angle, tilt_pivot, row0, slices if tilt_rows is None else tilt_rows)
         p.use_roi = 0 if roi is None else 1
         if roi is not None:
             for i in range(4):
@@ -335,16 +352,23 @@ class FDKbp_single:
     ``filter="shepp-logan"`` and ``cosine_weight=True`` select the filter
     extension, which the reference does not have: the Shepp-Logan ramp
     |k|/(P/2) sinc(k/P) instead of ramp_filter_tuned's roll-off, and the
-    Feldkamp pre-weight SDD/sqrt(SDD^2+u^2+v^2) per detector pixel."""
+    Feldkamp pre-weight SDD/sqrt(SDD^2+u^2+v^2) per detector pixel.
+
+    ``correct_tilt`` gives slice k the centre of rotation
+    tan(-tilt_angle) (window row 3 - k - tilt_pivot) + center and shrinks the
+    mask by floor(tan|tilt_angle| tilt_rows), tilt_rows being
+    ProjectionInfo.nDims[1]."""
 
     module = "FDKbp_single"
 
     def __init__(self, sod: float, sdd: float, pixel, voxel, ppoint, center: float,
                  roi: Sequence[int], projection_roi3: int, direction: int = 0,
                  matrix_roi: Optional[Sequence[int]] = None, fused: bool = False,
-                 filter: str = "muhrec", cosine_weight: bool = False):
+                 filter: str = "muhrec", cosine_weight: bool = False, correct_tilt: bool = False,
+                 tilt_angle: float = 0.0, tilt_pivot: float = 0.0, tilt_rows: int = 0):
         L = _lib.load()
         p = FdkParams()
+        _set_tilt(p, correct_tilt, tilt_angle, tilt_pivot, None, tilt_rows)
         p.module = _lib.FDK_MODULES[self.module]
         p.fused = int(bool(fused))
         if filter not in ("muhrec", "shepp-logan"):
@@ -586,7 +610,9 @@ class FBPStream:
     def __init__(self, width: int, rows: int, n_proj: int, center: float, slice_block: int = 32,
                  backprojector: str = "MultiProjBPparallel", direction: int = 0, buffer_size: int = 16,
                  fused: bool = False, filtertype: str = "Hamming", cutoff: float = 0.5, order: float = 1.0,
-                 usebias: bool = True, biasweight: float = 0.1, paddingdoubler: int = 2):
+                 usebias: bool = True, biasweight: float = 0.1, paddingdoubler: int = 2,
+                 correct_tilt: bool = False, tilt_angle: float = 0.0, tilt_pivot: float = 0.0,
+                 row0: int = 0, tilt_rows: Optional[int] = None):
         if backprojector not in BP_MODULES:
             raise MrError(_lib.MR_ERR_INVALID, f"unknown back-projector {backprojector}")
         ft = _lib.FILTER_TYPES.get(filtertype.lower())
@@ -598,6 +624,9 @@ class FBPStream:
         p.multiprojbp = BP_MODULES[backprojector]["multiprojbp"]
         p.buffer_size = int(buffer_size)
         p.fused = int(bool(fused))
+        # row0: the first slab's detector row; every slab of slice_block rows
+        # is a block of its own, as the engine's SliceBlock loop makes it
+        _set_tilt(p, correct_tilt, tilt_angle, tilt_pivot, row0, rows if tilt_rows is None else tilt_rows)
         self.width, self.rows, self.n_proj = int(width), int(rows), int(n_proj)
         self._s = ctypes.c_void_p()
         check(_lib.load().mr_fbp_stream_create(ctypes.byref(self._s), ctypes.byref(p), self.rows, self.n_proj,

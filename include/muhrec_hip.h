@@ -116,6 +116,19 @@ typedef struct {
     int fused;           /* 0: the reference's float operation order per voxel (bit-identical
                             results); 1: interpolation folded into fused multiply-adds
                             (differs by float rounding only) */
+    /* Axis-tilt correction, the bCorrectTilt branches of the three back-projectors
+     * (MultiProjBPparallel.cpp:49-114, MultiProjBP.cpp:50-96, NNMultiProjBP.cpp:48-81)
+     * and the mask shrink of BuildCircleMask (BackProjectorModuleBase.cpp:171-180).
+     * mr_bp_parallel, mr_fbp_parallel and the mr_fbp_stream entry points honour it
+     * (always in the reference's float order: `fused` is ignored); the adjoint and
+     * the no-allocation entry point refuse it. */
+    int correct_tilt;    /* ProjectionInfo.bCorrectTilt */
+    float tilt_angle;    /* ProjectionInfo.fTiltAngle (degrees) */
+    float tilt_pivot;    /* ProjectionInfo.fTiltPivotPosition (detector row) */
+    size_t tilt_row0;    /* ProjectionInfo.roi[1] of this block: its first detector row
+                            (the stream entry points add each slab's offset) */
+    size_t tilt_rows;    /* ProjectionInfo.nDims[1] (roi[3] for repeated sinograms):
+                            the mask radius shrinks by floor(tan|angle| * tilt_rows) */
 } mr_bp_params;
 
 typedef struct mr_bp_plan mr_bp_plan;
@@ -287,6 +300,14 @@ typedef struct {
      * before the filter (u, v in mm from the piercing point). */
     int filter;
     int cosine_weight;
+    /* Axis-tilt correction (fdkbp_single.cpp:443-448, fdkbp.cpp:452-458): slice k's
+     * centre of rotation tan(-angle) (window row3 - k - pivot) + center replaces the
+     * projection matrix's column offset; the mask radius shrinks by
+     * floor(tan|angle| * tilt_rows) (BackProjectorModuleBase.cpp:171-180). */
+    int correct_tilt;       /* ProjectionInfo.bCorrectTilt */
+    float tilt_angle;       /* ProjectionInfo.fTiltAngle (degrees) */
+    float tilt_pivot;       /* ProjectionInfo.fTiltPivotPosition */
+    size_t tilt_rows;       /* ProjectionInfo.nDims[1] (roi[3] for repeated sinograms) */
 } mr_fdk_params;
 
 typedef struct mr_fdk_plan mr_fdk_plan;

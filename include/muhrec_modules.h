@@ -39,6 +39,8 @@ int GetModuleList(const char *application, void *listptr);
  *   voxelsize="vx vy vz", sod, sdd, ppoint="px py", roi="x0 y0 x1 y1",
  *   projection_roi3, slice_block, usematrixroi=true|false, matrixroi="x0 y0 x1 y1",
  *   ob_count, dc_count (nOBCount / nDCCount: FullLogNorm's reference counts),
+ *   dims="w h" (ProjectionInfo.nDims), correcttilt=true|false, tiltangle, tiltpivot
+ *   (the back-projectors' tilt branches; each slab's first row enters them),
  *   projection_at_a_time=false|reference|final: instead of the SliceBlock loop,
  *   ReconEngine::Run's projection-at-a-time path (ReconEngine.cpp:455-520):
  *   each projection goes through the pre-processing module's Process(2D) and

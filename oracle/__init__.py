@@ -63,7 +63,8 @@ class FdkGeom(ctypes.Structure):
                 ("pix", ctypes.c_float * 2), ("vox", ctypes.c_float * 3),
                 ("ppoint", ctypes.c_float * 2), ("center", ctypes.c_float),
                 ("direction", ctypes.c_int), ("roi", ctypes.c_uint64 * 4),
-                ("proj_roi3", ctypes.c_uint64)]
+                ("proj_roi3", ctypes.c_uint64), ("correct_tilt", ctypes.c_int),
+                ("tilt_angle", ctypes.c_float), ("tilt_pivot", ctypes.c_float)]
 
 
 def _setup(L):
@@ -78,6 +79,13 @@ def _setup(L):
     L.orc_filter_rows_f32.argtypes = [_fp, _sz, _sz, _sz, _fp, _fp, _sz]
     L.orc_filter_rows_f64.argtypes = [_fp, _dp, _sz, _sz, _sz, _dp, _dp, _sz]
     L.orc_circle_mask.argtypes = [_sz, _sz, _sp, ctypes.c_int, _sp, _sp]
+    L.orc_circle_mask_ex.argtypes = [_sz, _sz, _sp, ctypes.c_int, ctypes.c_float, _sp, _sp]
+    L.orc_tilt_mask_shrink.argtypes = [ctypes.c_float, _sz]
+    L.orc_tilt_mask_shrink.restype = ctypes.c_float
+    L.orc_bp_tilt_terms.argtypes = [ctypes.c_float, ctypes.c_float, _sz, _fp, _fp]
+    L.orc_bp_parallel_tilt_f32.argtypes = [_fp, _sz, _sz, _sz, _fp, _fp, ctypes.c_float, ctypes.c_int,
+                                           ctypes.c_int, _sz, ctypes.c_float, ctypes.c_float, _sz,
+                                           _sp, _sp, _fp]
     L.orc_used_projections.argtypes = [_sz, _sz, _sp]; L.orc_used_projections.restype = _sz
     L.orc_bp_tables_f32.argtypes = [_fp, _sz, ctypes.c_int, ctypes.c_float, _sz, _fp, _fp, _fp]
     L.orc_bp_parallel_f32.argtypes = [_fp, _sz, _sz, _sz, _fp, _fp, ctypes.c_float, ctypes.c_int,
@@ -237,14 +245,52 @@ def filter_rows_f64(rows: np.ndarray, ftype="ram-lak", cutoff=0.5, order=1.0, us
 # Mask and parallel-beam back-projection
 # --------------------------------------------------------------------------
 def circle_mask(nx: int, ny: Optional[int] = None, roi: Optional[Sequence[int]] = None,
-                cone: bool = False) -> Tuple[np.ndarray, np.ndarray]:
+                cone: bool = False, shrink: float = 0.0) -> Tuple[np.ndarray, np.ndarray]:
+    """BuildCircleMask; shrink is the tilt correction's radius reduction
+    (tilt_mask_shrink), 0 without tilt."""
     ny = nx if ny is None else ny
     first = np.zeros(ny, np.uintp); second = np.zeros(ny, np.uintp)
     r = None
     if roi is not None:
         r = np.asarray(roi, np.uintp)
-    lib().orc_circle_mask(nx, ny, _s(r) if r is not None else None, int(cone), _s(first), _s(second))
+    lib().orc_circle_mask_ex(nx, ny, _s(r) if r is not None else None, int(cone), float(shrink),
+                             _s(first), _s(second))
     return first.astype(np.int64), second.astype(np.int64)
+
+
+def tilt_mask_shrink(tilt_angle: float, slices: int) -> float:
+    return float(lib().orc_tilt_mask_shrink(tilt_angle, slices))
+
+
+def bp_tilt_terms(tilt_angle: float, tilt_pivot: float, row0: int) -> Tuple[float, float]:
+    """(centeroffset, centerinc) of the back-projectors' tilt branches."""
+    co = np.zeros(1, np.float32); ci = np.zeros(1, np.float32)
+    lib().orc_bp_tilt_terms(tilt_angle, tilt_pivot, row0, _f(co), _f(ci))
+    return float(co[0]), float(ci[0])
+
+
+TILT_VARIANTS = {"MultiProjBPparallel": 0, "MultiProjBP": 1, "NearestNeighborBP": 2}
+
+
+def bp_parallel_tilt_f32(proj: np.ndarray, angles, weights, center: float, direction: int = 0,
+                         variant: str = "MultiProjBPparallel", bufsize: int = 16,
+                         tilt_angle: float = 0.0, tilt_pivot: float = 0.0, row0: int = 0,
+                         tilt_rows: Optional[int] = None, mask=None) -> np.ndarray:
+    """The back-projectors' bCorrectTilt branches for a [np, nv, nu] stack.
+    The mask is BuildCircleMask shrunk by tilt_mask_shrink(tilt_angle,
+    tilt_rows) (tilt_rows defaults to nv) unless given as (first, second)."""
+    p = np.ascontiguousarray(proj, np.float32)
+    n_p, nv, nu = p.shape
+    a = np.ascontiguousarray(angles, np.float32); w = np.ascontiguousarray(weights, np.float32)
+    if mask is None:
+        rows = nv if tilt_rows is None else tilt_rows
+        mask = circle_mask(nu, shrink=tilt_mask_shrink(tilt_angle, rows))
+    f = np.ascontiguousarray(mask[0], np.uintp); s = np.ascontiguousarray(mask[1], np.uintp)
+    vol = np.zeros((nu, nu, nv), np.float32)
+    lib().orc_bp_parallel_tilt_f32(_f(p), nu, nv, n_p, _f(a), _f(w), center, direction,
+                                   TILT_VARIANTS[variant], bufsize, tilt_angle, tilt_pivot, row0,
+                                   _s(f), _s(s), _f(vol))
+    return vol
 
 
 def used_projections(np_: int, bufsize: int) -> np.ndarray:
@@ -321,8 +367,13 @@ def bp_parallel_adjoint_f64(grad_vol: np.ndarray, n_proj: int, angles, weights, 
 # --------------------------------------------------------------------------
 # FDK
 # --------------------------------------------------------------------------
-def fdk_geom(sod, sdd, pix, vox, ppoint, center, direction, roi, proj_roi3) -> FdkGeom:
+def fdk_geom(sod, sdd, pix, vox, ppoint, center, direction, roi, proj_roi3,
+             tilt: Optional[Tuple[float, float]] = None) -> FdkGeom:
+    """tilt = (fTiltAngle, fTiltPivotPosition) sets bCorrectTilt."""
     g = FdkGeom()
+    if tilt is not None:
+        g.correct_tilt = 1
+        g.tilt_angle, g.tilt_pivot = float(tilt[0]), float(tilt[1])
     g.sod = sod; g.sdd = sdd
     g.pix[0], g.pix[1] = pix
     g.vox[0], g.vox[1], g.vox[2] = vox
@@ -410,15 +461,20 @@ def fdkd_filter(img: np.ndarray) -> np.ndarray:
 
 
 def fdk_bp(proj: np.ndarray, angles, weights, g: FdkGeom, f64: bool = False,
-           module: str = "FDKbp_single") -> np.ndarray:
+           module: str = "FDKbp_single", tilt_rows: int = 0) -> np.ndarray:
     """FDKbp_single::Process(3D) (or FDKbp's with module="FDKbp") for one
     slab; proj is [np, h, w] (window rows).  Returns the XYZ volume
     [nz, ny, nx] after the final mirror/flip.  f64: the FDKbp_single algorithm
-    in double precision (FDKbp divides that by the projection count)."""
+    in double precision (FDKbp divides that by the projection count), without
+    tilt.  With g.correct_tilt the mask shrinks by
+    tilt_mask_shrink(g.tilt_angle, tilt_rows)."""
     p = np.ascontiguousarray(proj, np.float32)
     n_p, h, w = p.shape
     nx = int(g.roi[2] - g.roi[0]); nz = int(g.roi[3] - g.roi[1])
-    first, second = circle_mask(nx)
+    if f64 and g.correct_tilt:
+        raise ValueError("the FP64 FDK model has no tilt correction")
+    shrink = tilt_mask_shrink(g.tilt_angle, tilt_rows) if g.correct_tilt else 0.0
+    first, second = circle_mask(nx, shrink=shrink)
     f = first.astype(np.uintp); s = second.astype(np.uintp)
     if module == "FDKbp" and not f64:
         vol = np.zeros((nz, nx, nx), np.float32)

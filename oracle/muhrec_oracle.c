@@ -318,16 +318,18 @@ void orc_filter_rows_adjoint_f64(const double *g, double *out, size_t nrows, siz
 }
 
 /* ------------------------------------------------------------------------ */
-/* Circle mask, RF/src/BackProjectorModuleBase.cpp:153-218 (no tilt).        */
+/* Circle mask, RF/src/BackProjectorModuleBase.cpp:153-218.                 */
 /* roi == NULL: no MatrixInfo ROI.  roi = {x0,y0,x1,y1} inclusive,           */
-/* cone != 0 selects the cone-beam ROI clipping branch.                      */
+/* cone != 0 selects the cone-beam ROI clipping branch.  shrink: the radius  */
+/* reduction of the tilt correction (orc_tilt_mask_shrink; 0 without tilt).  */
 /* ------------------------------------------------------------------------ */
-void orc_circle_mask(size_t nSizeX, size_t nSizeY, const size_t *roi, int cone,
-                     size_t *first, size_t *second)
+void orc_circle_mask_ex(size_t nSizeX, size_t nSizeY, const size_t *roi, int cone, float shrink,
+                        size_t *first, size_t *second)
 {
     const float cx = (float)(nSizeX >> 1);
     const float cy = (float)(nSizeY >> 1);
-    const float R = cx - 1;
+    float R = cx - 1;
+    R -= shrink;
     const float R2 = R * R;
     for (size_t i = 0; i < nSizeY; ++i) {
         float y = (float)i - cy;
@@ -355,6 +357,21 @@ void orc_circle_mask(size_t nSizeX, size_t nSizeY, const size_t *roi, int cone,
             }
         }
     }
+}
+
+void orc_circle_mask(size_t nSizeX, size_t nSizeY, const size_t *roi, int cone,
+                     size_t *first, size_t *second)
+{
+    orc_circle_mask_ex(nSizeX, nSizeY, roi, cone, 0.0f, first, second);
+}
+
+/* BackProjectorModuleBase.cpp:171-180: with bCorrectTilt the mask radius
+ * shrinks by floor(tan(|fTiltAngle|) * slices), slices = ProjectionInfo.nDims[1]
+ * (roi[3] for repeated sinograms); float operations (the kipl headers bring
+ * "using namespace std", so fabs/tan/floor take their float overloads). */
+float orc_tilt_mask_shrink(float tilt_angle, size_t slices)
+{
+    return floorf(tanf(fabsf(tilt_angle) * ORC_FPI / 180.0f) * (float)slices);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -474,6 +491,102 @@ void orc_bp_parallel_f32(const float *proj, size_t nu, size_t nv, size_t np,
     free(buf); free(used); free(fsin); free(fcos); free(fsu);
 }
 
+/* Axis-tilt terms of the three parallel-beam back-projectors
+ * (MultiProjBPparallel.cpp:68-75, MultiProjBP.cpp:63-64, NNMultiProjBP.cpp:61-62):
+ * centeroffset = (roi[1] - fTiltPivotPosition) * tan(fTiltAngle pi / 180) and
+ * centerinc = 4 tan(fTiltAngle pi / 180), in float (tan of a float argument
+ * resolves to the float overload); row0 = ProjectionInfo.roi[1] of the block. */
+void orc_bp_tilt_terms(float tilt_angle, float tilt_pivot, size_t row0, float *centeroffset, float *centerinc)
+{
+    const float t = tanf(tilt_angle * ORC_FPI / 180.0f);
+    *centeroffset = ((float)row0 - tilt_pivot) * t;
+    *centerinc = 4.0f * t;
+}
+
+/* The bCorrectTilt branches of the back-projectors, as written in the
+ * reference (their weights are not a physical tilt model; they are
+ * reproduced as they are):
+ *   variant 0, MultiProjBPparallel (MultiProjBPparallel.cpp:49-114):
+ *     u = ((fStartU + fCos y) - centeroffset) - fSin x, n = (int)u, f = u - n;
+ *     slice group z (4 slices) weights B by |f_z - z centerinc|, where f_z is
+ *     f with centerinc subtracted z times (fPosU -= centerinc after each group);
+ *   variant 1, MultiProjBP (MultiProjBP.cpp:50-96):
+ *     u = ((fStartU + fCos y) - fSin x) - centeroffset, n = (int)u;
+ *     group z weights B by |(u - n) - z centerinc|;
+ *   variant 2, NearestNeighborBP (NNMultiProjBP.cpp:48-81): u as variant 1,
+ *     column n added unweighted.
+ * In every variant the two columns read are n and n + 1 (B at the padded
+ * column length for variant 0, 4*(nv/4) for variant 1) for all groups, and
+ * interpA = 1 - interpB.  Arguments as orc_bp_parallel_f32 (proj not yet
+ * weighted; vol ZXY, overwritten); the mask is the caller's (with the tilt
+ * shrink, orc_tilt_mask_shrink). */
+void orc_bp_parallel_tilt_f32(const float *proj, size_t nu, size_t nv, size_t np,
+                              const float *angles, const float *weights, float center, int direction,
+                              int variant, size_t bufsize, float tilt_angle, float tilt_pivot, size_t row0,
+                              const size_t *mfirst, const size_t *msecond, float *vol)
+{
+    const size_t vpad = (nv + 3) & ~(size_t)3;
+    const size_t sizeZ = nv / 4;
+    const size_t bstride = variant == 1 ? sizeZ * 4 : vpad;
+    const int sizeUm2 = (int)nu - 2;
+    float co, ci;
+    orc_bp_tilt_terms(tilt_angle, tilt_pivot, row0, &co, &ci);
+    size_t *used = (size_t *)malloc(sizeof(size_t) * (np + 1));
+    size_t nused = orc_used_projections(np, bufsize, used);
+    float *fsin = (float *)malloc(sizeof(float) * np);
+    float *fcos = (float *)malloc(sizeof(float) * np);
+    float *fsu = (float *)malloc(sizeof(float) * np);
+    orc_bp_tables_f32(angles, np, direction, center, nu / 2, fsin, fcos, fsu);
+    float *buf = (float *)calloc((nused * nu + 1) * vpad + 8, sizeof(float));
+    for (size_t k = 0; k < nused; ++k) {
+        const float *p = proj + used[k] * nu * nv;
+        float w = weights[used[k]];
+        for (size_t v = 0; v < nv; ++v)
+            for (size_t u = 0; u < nu; ++u)
+                buf[(k * nu + u) * vpad + v] = p[v * nu + u] * w;
+    }
+    memset(vol, 0, sizeof(float) * nu * nu * nv);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long y = 1; y <= (long)nu; ++y) {
+        const size_t x0 = mfirst[y - 1], x1 = msecond[y - 1];
+        for (size_t x = x0 + 1; x <= x1; ++x) {
+            float *col = vol + ((size_t)(y - 1) * nu + (x - 1)) * nv;
+            for (size_t k = 0; k < nused; ++k) {
+                const size_t i = used[k];
+                float u;
+                if (variant == 0) {
+                    const float lsu = fsu[i] + fcos[i] * (float)y - co;
+                    u = lsu - fsin[i] * (float)x;
+                } else {
+                    const float lsu = fsu[i] + fcos[i] * (float)y;
+                    u = lsu - fsin[i] * (float)x - co;
+                }
+                const int n = (int)u;
+                if ((n < 0) || (sizeUm2 < n)) continue;
+                const float *A = buf + (k * nu + (size_t)n) * vpad;
+                const float *B = A + bstride;
+                if (variant == 2) {
+                    for (size_t z = 0; z < 4 * sizeZ; ++z) col[z] = col[z] + A[z];
+                    continue;
+                }
+                float f = u - (float)n;
+                for (size_t z = 0; z < sizeZ; ++z) {
+                    const float wb = variant == 0 ? fabsf(f - (float)z * ci) : fabsf(u - (float)n - (float)z * ci);
+                    const float wa = 1.0f - wb;
+                    for (size_t c = 4 * z; c < 4 * z + 4; ++c) {
+                        float a = A[c] * wa;
+                        float b = B[c] * wb;
+                        float sm = a + b;
+                        col[c] = col[c] + sm;
+                    }
+                    f -= ci;
+                }
+            }
+        }
+    }
+    free(buf); free(used); free(fsin); free(fcos); free(fsu);
+}
+
 /* Adjoint of orc_bp_parallel_f32 with multiprojbp = 0 (MultiProjBPparallel,
  * NearestNeighborBP): the float positions n, w of the forward, sums in
  * double.  g: volume-shaped [nu(y)][nu(x)][nv]; out: [np][nv][nu], zero for
@@ -581,6 +694,9 @@ typedef struct {
     int direction;           /* 0 CW, 1 CCW */
     uint64_t roi[4];         /* ProjectionInfo.roi at back-projection time (slab) */
     uint64_t proj_roi3;      /* ProjectionInfo.projection_roi[3] */
+    int correct_tilt;        /* ProjectionInfo.bCorrectTilt */
+    float tilt_angle;        /* ProjectionInfo.fTiltAngle (degrees) */
+    float tilt_pivot;        /* ProjectionInfo.fTiltPivotPosition */
 } orc_fdk_geom;
 
 /* getProjMatrix, fdkbp_single.cpp:83-255 (tgt = 0, vup = z). */
@@ -806,6 +922,14 @@ void orc_fdk_bp_f32(const float *proj, size_t w, size_t h, size_t np,
         }
         for (size_t k = 0; k < nz; ++k) {
             float z = (float)(origin[2] + (float)k * sp[2]);
+            if (g->correct_tilt) {
+                /* fdkbp_single.cpp:443-448: the centre of rotation of slice k
+                 * (tan in double, the result held in a float) replaces pm[3] */
+                const float pos = (float)cb[3] - (float)k - g->tilt_pivot;
+                const float cor = (float)(tan((double)(-g->tilt_angle) * ORC_DPI / 180.0) * (double)pos +
+                                          (double)g->center);
+                pm[3] = ((cor - (g->ppoint[0] - (float)g->roi[0])) * g->vox[0]) / g->pix[0];
+            }
             zip[k * 3 + 0] = z * (pm[2] + ic[0] * pm[10]) + ic[0] * pm[11] + pm[3];
             zip[k * 3 + 1] = z * (pm[6] + ic[1] * pm[10]) + ic[1] * pm[11] + pm[7];
             zip[k * 3 + 2] = z * pm[10] + pm[11];
@@ -1087,6 +1211,12 @@ void orc_fdkd_bp(const float *proj, size_t w, size_t h, size_t np,
         }
         for (size_t k = 0; k < nz; ++k) {
             double z = (double)(origin[2] + (float)(long)k * sp[2]);
+            if (g->correct_tilt) {
+                /* fdkbp.cpp:452-458, in double, with FDKbp's own window row cb[3] */
+                const double pos = (double)cb[3] - (double)k - (double)g->tilt_pivot;
+                const double cor = tan((double)(-g->tilt_angle) * ORC_DPI / 180.0) * pos + (double)g->center;
+                pm[3] = ((cor - (double)(g->ppoint[0] - (float)g->roi[0])) * (double)g->vox[0]) / (double)g->pix[0];
+            }
             zip[k * 3 + 0] = z * (pm[2] + ic[0] * pm[10]) + ic[0] * pm[11] + pm[3];
             zip[k * 3 + 1] = z * (pm[6] + ic[1] * pm[10]) + ic[1] * pm[11] + pm[7];
             zip[k * 3 + 2] = z * pm[10] + pm[11];

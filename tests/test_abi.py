@@ -91,3 +91,25 @@ def test_batch_rejects_unknown_flags():
     assert lib.mr_batch_plan_create(ctypes.byref(plan), arr, 1, 4) == L.MR_ERR_INVALID
     assert not plan.value
     assert b"flags" in lib.mr_last_error()
+
+
+def test_tilt_refusals_without_device():
+    """The no-allocation entry point refuses a tilted configuration and the
+    FDK plan refuses fused mode with tilt, both before any device call (the
+    volume pointer is never dereferenced)."""
+    import imagingsuite_amd._lib as L
+    lib = L.load()
+    p = L.BpParams()
+    p.width, p.slices, p.interp, p.direction = 64, 8, 0, 0
+    p.correct_tilt, p.tilt_angle = 1, 2.0
+    rc = lib.mr_bp_parallel_noalloc(ctypes.byref(p), None, 0, None, None, ctypes.c_void_p(16), 0, None)
+    assert rc == L.MR_ERR_UNSUPPORTED
+    assert b"tilt" in lib.mr_last_error()
+    q = L.FdkParams()
+    q.sod, q.sdd = 500.0, 1000.0
+    q.pixel[0] = q.pixel[1] = 0.2
+    q.roi[0], q.roi[1], q.roi[2], q.roi[3] = 0, 0, 64, 32
+    q.fused, q.correct_tilt, q.tilt_angle = 1, 1, 2.0
+    plan = ctypes.c_void_p()
+    assert lib.mr_fdk_plan_create(ctypes.byref(plan), ctypes.byref(q)) == L.MR_ERR_INVALID
+    assert b"tilt" in lib.mr_last_error() and not plan.value

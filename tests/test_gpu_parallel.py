@@ -8,7 +8,8 @@ Tolerances
 * back-projection on identical filtered input: bit-exact (same float
   operation order per voxel as the reference).
 * end-to-end FBP: relRMSE <= 1e-4 of the reconstruction range against the
-  MuhRec-faithful float oracle (BASELINE tolerance); the error against the
+  MuhRec-faithful float oracle (the tolerance the workload is specified
+  with); the error against the
   FP64 oracle is printed.
 """
 import numpy as np

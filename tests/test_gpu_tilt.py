@@ -1,0 +1,148 @@
+"""GPU parity of the axis-tilt branches (ProjectionInfo.bCorrectTilt) against
+the oracle, whose tilt restatement is pinned in test_oracle_tilt.py.
+
+* back-projectors (MultiProjBPparallel, MultiProjBP, NearestNeighborBP):
+  bit-exact, including slice tiles that start past the first slice group
+  (the parallel branch's running offset), blocks away from the pivot, odd
+  slice counts and the buffer drop; the plan's mask equals the oracle's
+  shrunk circle mask;
+* block loops: the plugin driver (SetROI per SliceBlock slab, so each slab's
+  first detector row enters the tilt terms) and the streaming entry point
+  equal the direct path slab by slab, bit for bit, and the oracle within
+  relRMSE 1e-4 (the filter's float FFT);
+* FDKbp_single / FDKbp: relRMSE <= 1e-5 and max <= 1e-4 of the range
+  against the float oracle (the only difference is the filter's float FFT);
+* the adjoint refuses a tilted plan.
+"""
+import numpy as np
+import pytest
+import torch
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = ["MultiProjBPparallel", "MultiProjBP", "NearestNeighborBP"]
+
+
+def _gpu(x, dev):
+    return torch.from_numpy(np.ascontiguousarray(x, np.float32)).to(dev)
+
+
+@pytest.mark.parametrize("name", VARIANTS)
+@pytest.mark.parametrize("nv,bufsize,angle,pivot,row0", [(40, 16, 2.5, 10.0, 100), (10, 16, -4.0, 300.0, 0),
+                                                         (64, 0, 1.0, 0.0, 37)])
+def test_bp_tilt_bitexact(orc, dev, name, nv, bufsize, angle, pivot, row0):
+    from imagingsuite_amd.recon import BackProjector
+    rng = np.random.default_rng(nv + row0)
+    nu, n_p = 96, 45
+    proj = rng.normal(size=(n_p, nv, nu)).astype(np.float32)
+    ang = np.arange(n_p, dtype=np.float32) * np.float32(180.0 / n_p)
+    w = rng.uniform(0.5, 1.5, n_p).astype(np.float32)
+    center, rows = 47.5, 4 * nv
+    bp = BackProjector(name, nu, nv, center, direction=1, buffer_size=bufsize, correct_tilt=True,
+                       tilt_angle=angle, tilt_pivot=pivot, row0=row0, tilt_rows=rows)
+    assert bp.kernel.startswith("bp_tilt_kernel<")
+    shrink = orc.tilt_mask_shrink(angle, rows)
+    f, s = bp.mask
+    rf, rs = orc.circle_mask(nu, shrink=shrink)
+    assert np.array_equal(f, rf) and np.array_equal(s, rs)
+    vol = bp.process(_gpu(proj, dev), ang, w).cpu().numpy()
+    ref = orc.bp_parallel_tilt_f32(proj, ang, w, center, 1, variant=name, bufsize=bufsize, tilt_angle=angle,
+                                   tilt_pivot=pivot, row0=row0, tilt_rows=rows)
+    nbad = int(np.sum(vol != ref))
+    print(f"{name} nv={nv} buf={bufsize} tilt {angle} pivot {pivot} row0 {row0} (mask shrink {shrink}): "
+          f"{nbad} voxels differ, max {np.abs(vol - ref).max():.3e}")
+    assert nbad == 0
+
+
+FILTER = {"filtertype": "Hanning", "cutoff": 0.5, "order": 1, "usebias": "true", "biasweight": 0.1,
+          "paddingdoubler": 2}
+
+
+def _direct_blocks(dev, name, proj, angles, weights, center, sb, angle, pivot, rows):
+    """Filter + tilted back-projection block by block through the C ABI, each
+    block with its own first row; returns [z][y][x]."""
+    from imagingsuite_amd.recon import BackProjector, ProjectionFilterSingle, fbp_parallel
+    n_p, slices, nu = proj.shape
+    filt = ProjectionFilterSingle(nu, "Hanning")
+    out = np.zeros((slices, nu, nu), np.float32)
+    for r0 in range(0, slices, sb):
+        r1 = min(slices, r0 + sb)
+        bp = BackProjector(name, nu, r1 - r0, center, buffer_size=16, correct_tilt=True, tilt_angle=angle,
+                           tilt_pivot=pivot, row0=r0, tilt_rows=rows)
+        t = _gpu(proj[:, r0:r1], dev)
+        vol = torch.empty((nu, nu, r1 - r0), dtype=torch.float32, device=dev)
+        fbp_parallel(filt, bp, t, angles, weights, vol)
+        out[r0:r1] = vol.cpu().numpy().transpose(2, 0, 1)
+    return out
+
+
+@pytest.mark.parametrize("name", ["MultiProjBPparallel", "MultiProjBP"])
+def test_engine_and_stream_tilt(orc, dev, name):
+    from imagingsuite_amd import engine, scan
+    from imagingsuite_amd.recon import FBPStream
+    nu, n_p, slices, sb = 96, 90, 48, 16
+    center, angle, pivot = 48.5, 3.0, 20.0
+    angles, weights = scan.engine_angles_weights(n_p, 0.0, 178.0, 0.1)
+    rng = np.random.default_rng(23)
+    proj = (rng.random((n_p, slices, nu)) + 0.5).astype(np.float32)
+    direct = _direct_blocks(dev, name, proj, angles, weights, center, sb, angle, pivot, slices)
+    cfg = {"geometry": "parallel", "center": center, "roi": f"0 0 {nu} {slices}", "resolution": "0.1 0.1",
+           "dims": f"{nu} {slices}", "correcttilt": "true", "tiltangle": angle, "tiltpivot": pivot}
+    eng = engine.Engine(name, {"ProjectionBufferSize": 16, "SliceBlock": sb, "SubVolume": "1 1"}, cfg,
+                        "ProjectionFilterSingle", FILTER)
+    vol_e = eng.run(proj, angles, weights, slices)
+    st = FBPStream(nu, slices, n_p, center, slice_block=sb, backprojector=name, filtertype="Hanning",
+                   correct_tilt=True, tilt_angle=angle, tilt_pivot=pivot, row0=0, tilt_rows=slices)
+    vol_s = st.run(proj, angles, weights)
+    ne, ns = int(np.sum(vol_e != direct)), int(np.sum(vol_s != direct))
+    print(f"{name}: engine vs direct {ne}, stream vs direct {ns} differing voxels")
+    assert ne == 0 and ns == 0
+    f32 = orc.filter_rows_f32(proj.reshape(-1, nu), "hanning").reshape(proj.shape)
+    ref = np.zeros_like(direct)
+    for r0 in range(0, slices, sb):
+        ref[r0:r0 + sb] = orc.bp_parallel_tilt_f32(f32[:, r0:r0 + sb], angles, weights, center, 0, variant=name,
+                                                   bufsize=16, tilt_angle=angle, tilt_pivot=pivot, row0=r0,
+                                                   tilt_rows=slices).transpose(2, 0, 1)
+    e = rel_err(direct, ref)
+    print(f"{name}: tilted blocks vs float oracle relRMSE {e[0]:.2e} max {e[1]:.2e}")
+    assert e[0] <= 1e-4
+
+
+@pytest.mark.parametrize("cls", ["FDKbp_single", "FDKbp"])
+@pytest.mark.parametrize("slab,direction", [((0, 48, 128, 80), 0), ((0, 96, 128, 128), 1)])
+def test_fdk_tilt_matches_oracle(orc, dev, cls, slab, direction):
+    from imagingsuite_amd import recon
+    n, n_p = 128, 60
+    angle, pivot = 4.0, 70.0
+    kw = dict(sod=500.0, sdd=1000.0, pix=(0.2, 0.2), vox=(0.1, 0.1, 0.1), ppoint=(n / 2 + 0.3, n / 2 - 0.7),
+              center=n / 2 + 0.3, direction=direction, roi=slab, proj_roi3=n - 1)
+    g = orc.fdk_geom(**kw, tilt=(angle, pivot))
+    cb = orc.fdk_cbct_roi(g, n)
+    rng = np.random.default_rng(slab[1] + 7 * direction)
+    full = rng.random((n_p, n, n)).astype(np.float32)
+    win = np.ascontiguousarray(full[:, cb[1]:cb[3], :])
+    angles = np.arange(n_p, dtype=np.float32) * np.float32(360.0 / n_p)
+    weights = np.full(n_p, 1.0 / n_p, np.float32)
+    fdk = getattr(recon, cls)(500.0, 1000.0, (0.2, 0.2), (0.1, 0.1, 0.1), (n / 2 + 0.3, n / 2 - 0.7),
+                              n / 2 + 0.3, slab, n - 1, direction=direction, correct_tilt=True,
+                              tilt_angle=angle, tilt_pivot=pivot, tilt_rows=n)
+    vol = fdk.process(torch.from_numpy(win).to(dev), angles, weights).cpu().numpy()
+    ref = orc.fdk_bp(win, angles, weights, g, module=cls, tilt_rows=n)
+    plain = orc.fdk_bp(win, angles, weights, orc.fdk_geom(**kw), module=cls)
+    e = rel_err(vol, ref)
+    e0 = rel_err(vol, plain)
+    print(f"{cls} tilted slab {slab} dir {direction}: vs float oracle {e[0]:.2e}/{e[1]:.2e} "
+          f"(vs the untilted oracle {e0[0]:.2e})")
+    assert e[0] <= 1e-5 and e[1] <= 1e-4
+    assert e0[0] > 1e-3
+
+
+def test_tilt_adjoint_refused(dev):
+    from imagingsuite_amd import MrError
+    from imagingsuite_amd.recon import BackProjector
+    bp = BackProjector("MultiProjBPparallel", 32, 8, 16.0, correct_tilt=True, tilt_angle=1.0)
+    g = torch.zeros((32, 32, 8), dtype=torch.float32, device=dev)
+    with pytest.raises(MrError):
+        bp.adjoint(g, [0.0, 90.0], [1.0, 1.0])
