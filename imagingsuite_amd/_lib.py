@@ -42,6 +42,7 @@ EXPORTED_SYMBOLS = (
     "mr_fbp_stream_create", "mr_fbp_stream_run", "mr_fbp_stream_run_slabs", "mr_fbp_stream_destroy", "mr_host_alloc", "mr_host_free",
     "mr_copy_h2d", "mr_copy_d2h", "mr_volume_zxy_to_xyz",
     "mr_lognorm_reference", "mr_lognorm", "mr_lognorm_logdose", "mr_filter_rows_lognorm",
+    "mr_lognorm_lut", "mr_lognorm_lut_term",
     "mr_batch_plan_create", "mr_batch_plan_info", "mr_batch_run", "mr_batch_run_graph",
     "mr_batch_run_sequential", "mr_batch_plan_destroy",
 )
@@ -157,6 +158,8 @@ def load():
     L.mr_lognorm.argtypes = [vp, sz, sz, vp, vp, fp, f32, vp]
     L.mr_filter_rows_lognorm.argtypes = [vp, vp, sz, sz, vp, vp, fp, f32, vp]
     L.mr_lognorm_logdose.argtypes = [vp, sz, sz, vp, vp, fp, vp]
+    L.mr_lognorm_lut.argtypes = [vp, sz, sz, vp, vp, fp, f32, vp]
+    L.mr_lognorm_lut_term.argtypes = [vp, sz, sz, vp, vp, fp, vp]
     L.mr_batch_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(BatchStack), sz, i32]
     L.mr_batch_plan_info.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), szp,
                                      ctypes.POINTER(ctypes.c_double)]

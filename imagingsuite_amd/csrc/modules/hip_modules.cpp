@@ -3,7 +3,7 @@
 // Module names, parameter keys and the plugin quartet follow the reference
 // libraries they replace:
 //   ProjectionFilterSingle  frameworks/tomography/Preprocessing/StdPreprocModules/src/ProjectionFilter.cpp:78-375
-//   FullLogNorm             .../StdPreprocModules/src/NormPlugins.cpp:20-558 (non-LUT path)
+//   FullLogNorm             .../StdPreprocModules/src/NormPlugins.cpp:20-558 (log and LUT paths)
 //   MultiProjBPparallel, MultiProjBP, NearestNeighborBP
 //                           frameworks/tomography/Backprojectors/StdBackProjectors/src/StdBackProjectorBase.cpp:21-342
 //   FDKbp_single            frameworks/tomography/Backprojectors/FDKBackProjectors/src/fdkreconbase.cpp:20-330
@@ -313,7 +313,7 @@ private:
 };
 
 // -------------------------------------------------------------------------
-// FullLogNorm (NormPlugins.cpp:20-90, 232-558), non-LUT path.  Loading and
+// FullLogNorm (NormPlugins.cpp:20-90, 232-558), log and LUT paths.  Loading and
 // averaging the open-beam / dark images from files (NormBase::ReferenceLoader)
 // is I/O: the caller hands the averaged images and their doses to
 // SetReferenceImages, the public entry the reference's loader itself ends in.
@@ -333,9 +333,6 @@ public:
         nDCCount = config.ProjectionInfo.nDCCount;
         bUseNormROI = string2bool(GetStringParameter(parameters, "usenormregion"));
         bUseLUT = optional_bool(parameters, "uselut", false);
-        if (bUseLUT)
-            throw ReconException("FullLogNorm: the look-up-table path (uselut=true) is not provided by the HIP module",
-                                 __FILE__, __LINE__);
         auto it = parameters.find("referenceaverage");
         sReferenceAverage = it == parameters.end() ? "ImageWeightedAverage" : it->second;
         // optional, HIP-only: keep the normalised stack on the device for this
@@ -427,8 +424,10 @@ protected:
             throw ReconException("Flat field does not match projection", __FILE__, __LINE__);
         buf_.reserve(N);
         MRM_CHECK(mr_copy_h2d(buf_.p, img.GetDataPtr(), N * sizeof(float), nullptr));
-        MRM_CHECK(mr_lognorm_logdose(buf_.p, 1, N, nOBCount != 0 ? dflat_.p + row_off_ : nullptr,
-                                     nDCCount != 0 ? ddark_.p + row_off_ : nullptr, &cLogDose, nullptr));
+        const float *fl = nOBCount != 0 ? dflat_.p + row_off_ : nullptr;
+        const float *dk = nDCCount != 0 ? ddark_.p + row_off_ : nullptr;
+        if (bUseLUT) MRM_CHECK(mr_lognorm_lut_term(buf_.p, 1, N, fl, dk, &cLogDose, nullptr));
+        else MRM_CHECK(mr_lognorm_logdose(buf_.p, 1, N, fl, dk, &cLogDose, nullptr));
         MRM_CHECK(mr_copy_d2h(img.GetDataPtr(), buf_.p, N * sizeof(float), nullptr));
         return 0;
     }
@@ -450,9 +449,10 @@ protected:
         if (UpdateStatus(0.0f, m_sModuleName)) return 0;
         buf_.reserve(img.Size());
         MRM_CHECK(mr_copy_h2d(buf_.p, img.GetDataPtr(), img.Size() * sizeof(float), nullptr));
-        MRM_CHECK(mr_lognorm(buf_.p, nProj, N, nOBCount != 0 ? dflat_.p + row_off_ : nullptr,
-                             nDCCount != 0 ? ddark_.p + row_off_ : nullptr, bUseNormROI ? dose.data() : nullptr,
-                             fDarkDose, nullptr));
+        // the LUT path (NormPlugins.cpp:455-475) or the log path (:476-555)
+        MRM_CHECK((bUseLUT ? mr_lognorm_lut : mr_lognorm)(
+            buf_.p, nProj, N, nOBCount != 0 ? dflat_.p + row_off_ : nullptr,
+            nDCCount != 0 ? ddark_.p + row_off_ : nullptr, bUseNormROI ? dose.data() : nullptr, fDarkDose, nullptr));
         MRM_CHECK(mr_copy_d2h(img.GetDataPtr(), buf_.p, img.Size() * sizeof(float), nullptr));
         if (keep_) DeviceStack::get().publish(this, buf_.p, img.GetDataPtr(), img.Size());
         UpdateStatus(1.0f, m_sModuleName);

@@ -1,6 +1,6 @@
 // FullLogNorm on gfx950 (frameworks/tomography/Preprocessing/StdPreprocModules/
 // src/NormPlugins.cpp, FullLogNorm::SetReferenceImages :300-341 and
-// ProcessCore(3D) :411-558, the non-LUT path).
+// ProcessCore(2D/3D) :343-558, the non-LUT path and the LUT path).
 //
 //   reference:  flat_log = log((flat - dark) / flat_dose), 0 where <= 0
 //   projection: p = img - dark;  img = p <= 0 ? 0 : flat_log - log(p) + dose_log[j]
@@ -17,9 +17,80 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <vector>
 
 namespace mr {
+
+// FullLogNorm's look-up-table path (bUseLUT): kipl's LogLUT(1<<16, 0, 65535)
+// (NormPlugins.cpp:245; LUTCollection.cpp:11-36, LUTbase.cpp:17-24) has
+// slope 1 and intercept 0.5; entry 0 is log(1) and entry i is log(i), the
+// double log rounded to float.  The table is computed on the host and kept in
+// one array per GPU, written once per device before its first use.
+constexpr int kLogLut = 65536;
+__device__ __align__(256) float g_log_lut[kLogLut];
+
+// LUTbase::BasicLUT (LUTbase.cpp:60-78): below 0 entry 0, above 65535 the
+// last entry, else entry (size_t)(v * slope + 0.5) with slope 1.
+__device__ __forceinline__ float loglut_px(float v, const float *flat_log, const float *dark, size_t i, float t,
+                                           const float *lut)
+{
+    const float p = dark ? v - dark[i] : v;
+    float L;
+    if (p < 0.0f) L = lut[0];
+    else if (65535.0f < p) L = lut[kLogLut - 1];
+    else L = lut[(unsigned)(p + 0.5f)];
+    return flat_log ? flat_log[i] - L + t : -L + t;
+}
+
+// blockIdx.y = projection; t: per-projection additive term (NULL: 0).
+__global__ __launch_bounds__(256) void lognorm_lut_kernel(float *img, size_t n_pix, const float *flat_log,
+                                                          const float *dark, const float *term, const float *lut)
+{
+    const size_t j = blockIdx.y;
+    const float t = term ? term[j] : 0.0f;
+    float *p = img + j * n_pix;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    if ((n_pix & 3) == 0) {
+        float4 *p4 = reinterpret_cast<float4 *>(p);
+        for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n_pix / 4; q += stride) {
+            float4 v = p4[q];
+            v.x = loglut_px(v.x, flat_log, dark, 4 * q + 0, t, lut);
+            v.y = loglut_px(v.y, flat_log, dark, 4 * q + 1, t, lut);
+            v.z = loglut_px(v.z, flat_log, dark, 4 * q + 2, t, lut);
+            v.w = loglut_px(v.w, flat_log, dark, 4 * q + 3, t, lut);
+            p4[q] = v;
+        }
+    } else {
+        for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n_pix; i += stride)
+            p[i] = loglut_px(p[i], flat_log, dark, i, t, lut);
+    }
+}
+
+// Device address of this GPU's table, filled on the first call per device.
+static int log_lut_device(const float **out)
+{
+    static std::mutex mu;
+    static std::map<int, const float *> ready;
+    int dev = 0;
+    MR_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = ready.find(dev);
+    if (it != ready.end()) {
+        *out = it->second;
+        return MR_OK;
+    }
+    std::vector<float> h(kLogLut);
+    h[0] = (float)std::log(1.0);
+    for (int i = 1; i < kLogLut; ++i) h[i] = (float)std::log((double)(float)i);
+    void *ptr = nullptr;
+    MR_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(g_log_lut)));
+    MR_HIP(hipMemcpy(ptr, h.data(), sizeof(float) * kLogLut, hipMemcpyHostToDevice));
+    ready[dev] = static_cast<const float *>(ptr);
+    *out = static_cast<const float *>(ptr);
+    return MR_OK;
+}
 
 __global__ __launch_bounds__(256) void lognorm_reference_kernel(float *flat, const float *dark, size_t n, float dose)
 {
@@ -124,6 +195,79 @@ extern "C" int mr_lognorm(float *d_proj, size_t n_proj, size_t n_pix, const floa
         }
     }
     return run_lognorm(d_proj, n_proj, n_pix, d_flat_log, d_dark, dose ? dl.data() : nullptr, (hipStream_t)stream);
+    MR_GUARD_END
+}
+
+namespace {
+
+int run_lognorm_lut(float *d_proj, size_t n_proj, size_t n_pix, const float *d_flat_log, const float *d_dark,
+                    const float *term, hipStream_t s)
+{
+    const float *lut = nullptr;
+    int rc = mr::log_lut_device(&lut);
+    if (rc != MR_OK) return rc;
+    float *d_t = nullptr;
+    if (term) {
+        MR_HIP(hipMallocAsync((void **)&d_t, sizeof(float) * n_proj, s));
+        hipError_t e = hipMemcpyAsync(d_t, term, sizeof(float) * n_proj, hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) {
+            (void)hipFreeAsync(d_t, s);
+            return mr::hip_fail(e, "hipMemcpyAsync", __FILE__, __LINE__);
+        }
+    }
+    const size_t groups = (n_pix & 3) == 0 ? n_pix / 4 : n_pix;
+    const unsigned bx = (unsigned)std::max<size_t>(1, std::min<size_t>((groups + 255) / 256, 64));
+    hipError_t e;
+    {
+        mr::ProfScope prof("lognorm_lut", s);
+        hipLaunchKernelGGL(mr::lognorm_lut_kernel, dim3(bx, (unsigned)n_proj), dim3(256), 0, s, d_proj, n_pix,
+                           d_flat_log, d_dark, d_t, lut);
+        e = hipGetLastError();
+    }
+    if (d_t) (void)hipFreeAsync(d_t, s);
+    if (e != hipSuccess) return mr::hip_fail(e, "lognorm_lut_kernel", __FILE__, __LINE__);
+    MR_HIP(hipStreamSynchronize(s));   // the host term table must outlive the copy
+    return MR_OK;
+}
+
+}  // namespace
+
+extern "C" int mr_lognorm_lut(float *d_proj, size_t n_proj, size_t n_pix, const float *d_flat_log,
+                              const float *d_dark, const float *dose, float dark_dose, void *stream)
+{
+    MR_GUARD_BEGIN
+    mr::clear_error();
+    MR_REQUIRE(n_proj == 0 || (d_proj != nullptr && n_pix > 0), "NULL projection pointer or empty projection");
+    MR_REQUIRE(n_proj <= 65535, "at most 65535 projections per call");
+    if (n_proj == 0) return MR_OK;
+    // ProcessCore(3D) with bUseLUT (NormPlugins.cpp:418-428, 455-475):
+    // doselist[j] = log(max(dose[j] - fDarkDose, 1)) (0 without
+    // usenormregion); without open beams the term is doselist[j] - doselist[0]
+    std::vector<float> t(n_proj, 0.0f);
+    if (dose) {
+        for (size_t j = 0; j < n_proj; ++j) {
+            const float d = dose[j] - dark_dose;
+            t[j] = std::log(d < 1 ? 1.0f : d);
+            MR_REQUIRE(t[j] == t[j], "The projection dose is a NaN");
+        }
+    }
+    if (!d_flat_log) {
+        const float d0 = t[0];
+        for (size_t j = 0; j < n_proj; ++j) t[j] = t[j] - d0;
+    }
+    return run_lognorm_lut(d_proj, n_proj, n_pix, d_flat_log, d_dark, t.data(), (hipStream_t)stream);
+    MR_GUARD_END
+}
+
+extern "C" int mr_lognorm_lut_term(float *d_proj, size_t n_proj, size_t n_pix, const float *d_flat_log,
+                                   const float *d_dark, const float *term, void *stream)
+{
+    MR_GUARD_BEGIN
+    mr::clear_error();
+    MR_REQUIRE(n_proj == 0 || (d_proj != nullptr && n_pix > 0), "NULL projection pointer or empty projection");
+    MR_REQUIRE(n_proj <= 65535, "at most 65535 projections per call");
+    if (n_proj == 0) return MR_OK;
+    return run_lognorm_lut(d_proj, n_proj, n_pix, d_flat_log, d_dark, term, (hipStream_t)stream);
     MR_GUARD_END
 }
 

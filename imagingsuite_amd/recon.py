@@ -275,18 +275,21 @@ def fbp_parallel(filt: ProjectionFilterSingle, bp: BackProjector, proj, angles, 
 
 
 class FullLogNorm:
-    """FullLogNorm (StdPreprocModules/src/NormPlugins.cpp:300-341, 411-558),
-    non-LUT path, in place on device stacks.  ``flat`` / ``dark`` are the
-    averaged reference images (CUDA [rows, width] float32, or None when no
-    open-beam / dark-current images are configured); ``flat_dose`` and
-    ``dark_dose`` are the doses NormBase::ReferenceLoader computes."""
+    """FullLogNorm (StdPreprocModules/src/NormPlugins.cpp:300-341, 411-558)
+    in place on device stacks.  ``flat`` / ``dark`` are the averaged
+    reference images (CUDA [rows, width] float32, or None when no open-beam /
+    dark-current images are configured); ``flat_dose`` and ``dark_dose`` are
+    the doses NormBase::ReferenceLoader computes.  ``uselut`` selects the
+    look-up-table path (log of the count rounded to an integer in [0, 65535],
+    mr_lognorm_lut)."""
 
     def __init__(self, flat=None, dark=None, flat_dose: float = 1.0, dark_dose: float = 0.0,
-                 usenormregion: bool = True, stream=None):
+                 usenormregion: bool = True, uselut: bool = False, stream=None):
         _lib.load()
         self.dark = dark
         self.dark_dose = float(dark_dose)
         self.usenormregion = bool(usenormregion)
+        self.uselut = bool(uselut)
         self.flat_log = None
         if flat is not None:
             _require_cuda_f32(flat, 2, "flat")
@@ -313,21 +316,24 @@ class FullLogNorm:
                 raise MrError(_lib.MR_ERR_INVALID, "usenormregion needs the per-projection dose list")
             d = _f32arr(dose, n_proj)
         fp = ctypes.POINTER(ctypes.c_float)
-        check(_lib.load().mr_lognorm(_ptr(proj), n_proj, n_pix,
-                                     _ptr(self.flat_log) if self.flat_log is not None else None,
-                                     _ptr(self.dark) if self.dark is not None else None,
-                                     d.ctypes.data_as(fp) if d is not None else None,
-                                     self.dark_dose, _stream(stream)))
+        fn = _lib.load().mr_lognorm_lut if self.uselut else _lib.load().mr_lognorm
+        check(fn(_ptr(proj), n_proj, n_pix, _ptr(self.flat_log) if self.flat_log is not None else None,
+                 _ptr(self.dark) if self.dark is not None else None,
+                 d.ctypes.data_as(fp) if d is not None else None, self.dark_dose, _stream(stream)))
         return proj
 
     def process_and_filter(self, proj, filt: "ProjectionFilterSingle", dose=None, stream=None):
         """Normalisation and ``filt`` in one pass over ``proj`` (raw counts
         in, filtered -log projections out, in place): the same values as
-        ``process`` followed by ``filt.process`` (mr_filter_rows_lognorm)."""
+        ``process`` followed by ``filt.process`` (mr_filter_rows_lognorm).
+        The LUT path runs as the two steps."""
         _require_cuda_f32(proj, 3, "proj")
         n_proj, rows, width = proj.shape
         if width != filt.width:
             raise MrError(_lib.MR_ERR_INVALID, "row width does not match the configured ROI")
+        if self.uselut:
+            self.process(proj, dose, stream)
+            return filt.process(proj, stream)
         for ref in (self.flat_log, self.dark):
             if ref is not None and ref.numel() != rows * width:
                 raise MrError(_lib.MR_ERR_INVALID, "reference image does not match the projection size")

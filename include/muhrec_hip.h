@@ -202,6 +202,20 @@ int mr_lognorm(float *d_proj, size_t n_proj, size_t n_pix, const float *d_flat_l
  * (ProcessCore(2D), NormPlugins.cpp:343-357). */
 int mr_lognorm_logdose(float *d_proj, size_t n_proj, size_t n_pix, const float *d_flat_log, const float *d_dark,
                        const float *dose_log, void *stream);
+/* FullLogNorm's look-up-table path (uselut=true; NormPlugins.cpp:455-475 with
+ * kipl's LogLUT(65536, 0, 65535), LUTCollection.cpp:11-36, LUTbase.cpp:60-78):
+ * p = img - dark (or img); L = log(1) = 0 for p < 0, log(65535) for
+ * p > 65535, else log of the entry (size_t)(p + 0.5), entry 0 being log(1);
+ * img = flat_log - L + doselist[j], or -L + doselist[j] - doselist[0] without
+ * a flat (d_flat_log NULL), doselist as in mr_lognorm.  Unlike the non-LUT
+ * path there is no p <= 0 -> 0 case.  The table is computed on the host and
+ * held once per GPU. */
+int mr_lognorm_lut(float *d_proj, size_t n_proj, size_t n_pix, const float *d_flat_log, const float *d_dark,
+                   const float *dose, float dark_dose, void *stream);
+/* The same with each projection's additive term given (host, n_proj values, or
+ * NULL for 0): the 2-D path's cLogDose (NormPlugins.cpp:370-381). */
+int mr_lognorm_lut_term(float *d_proj, size_t n_proj, size_t n_pix, const float *d_flat_log, const float *d_dark,
+                        const float *term, void *stream);
 /* FullLogNorm fused into ProjectionFilterSingle's row load: d_proj
  * [n_proj][rows][width] raw counts become the filtered -log projections in
  * one pass (arguments as mr_lognorm, n_pix = rows * width); the same values

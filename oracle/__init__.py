@@ -112,6 +112,9 @@ def _setup(L):
                                               ctypes.c_int, _sz, _sp, _sp, _dp]
     L.orc_lognorm_flat.argtypes = [_fp, _fp, _sz, ctypes.c_float]
     L.orc_lognorm_f32.argtypes = [_fp, _sz, _sz, _fp, _fp, _fp, ctypes.c_float]
+    L.orc_loglut_build.argtypes = [_fp]
+    L.orc_lognorm_lut_f32.argtypes = [_fp, _sz, _sz, _fp, _fp, _fp]
+    L.orc_lognorm_lut_terms.argtypes = [_fp, _sz, ctypes.c_float, ctypes.c_int, _fp]
     L.orc_joseph_forward_f64.argtypes = [_dp, _sz, _sz, _sz, ctypes.c_double, ctypes.c_int, _dp, _sz, _dp]
     L.orc_joseph_adjoint_f64.argtypes = [_dp, _sz, _sz, _sz, ctypes.c_double, ctypes.c_int, _dp, _sz, _dp]
 
@@ -522,6 +525,37 @@ def lognorm(proj: np.ndarray, flat_log: Optional[np.ndarray], dark: Optional[np.
     dk = None if dark is None else np.ascontiguousarray(dark, np.float32)
     ds = None if dose is None else np.ascontiguousarray(dose, np.float32)
     lib().orc_lognorm_f32(_f(p), n_p, n_pix, _fopt(fl), _fopt(dk), _fopt(ds), dark_dose)
+    return p
+
+
+def loglut() -> np.ndarray:
+    """kipl's LogLUT(65536, 0, 65535) table as FullLogNorm builds it."""
+    t = np.zeros(65536, np.float32)
+    lib().orc_loglut_build(_f(t))
+    return t
+
+
+def lognorm_lut_terms(n_proj: int, dose: Optional[np.ndarray], dark_dose: float, has_flat: bool) -> np.ndarray:
+    t = np.zeros(n_proj, np.float32)
+    ds = None if dose is None else np.ascontiguousarray(dose, np.float32)
+    lib().orc_lognorm_lut_terms(_fopt(ds), n_proj, dark_dose, int(has_flat), _f(t))
+    return t
+
+
+def lognorm_lut(proj: np.ndarray, flat_log: Optional[np.ndarray], dark: Optional[np.ndarray],
+                dose: Optional[np.ndarray] = None, dark_dose: float = 0.0,
+                term: Optional[np.ndarray] = None) -> np.ndarray:
+    """FullLogNorm's LUT path on [n_proj, rows, width] (copy returned): the
+    3-D path's dose terms from ``dose``/``dark_dose``, or explicit ``term``
+    (the 2-D path's cLogDose)."""
+    p = np.ascontiguousarray(proj, np.float32).copy()
+    n_p = p.shape[0]
+    n_pix = p.size // n_p
+    fl = None if flat_log is None else np.ascontiguousarray(flat_log, np.float32)
+    dk = None if dark is None else np.ascontiguousarray(dark, np.float32)
+    t = lognorm_lut_terms(n_p, dose, dark_dose, fl is not None) if term is None \
+        else np.ascontiguousarray(term, np.float32)
+    lib().orc_lognorm_lut_f32(_f(p), n_p, n_pix, _fopt(fl), _fopt(dk), _f(t))
     return p
 
 

@@ -1294,6 +1294,68 @@ void orc_lognorm_f32(float *img, size_t n_proj, size_t n_pix, const float *flat_
     }
 }
 
+/* FullLogNorm's look-up-table path (bUseLUT).  The module's LUT is kipl's
+ * LogLUT(1<<16, 0, 65535) (NormPlugins.cpp:245, LUTCollection.cpp:11-36,
+ * LUTbase.cpp:17-24): slope (65536-1)/65535 = 1, intercept 0.5, entry 0 =
+ * log(1/slope) = 0, entry i = log(i) (x stepping from 1 by 1, exact in float;
+ * log of a float there takes the double overload, rounded to float).
+ * LUTbase::BasicLUT (:60-78): v < 0 -> entry 0, 65535 < v -> entry 65535,
+ * else entry (size_t)(v * 1 + 0.5). */
+void orc_loglut_build(float *lut /* 65536 */)
+{
+    lut[0] = (float)log(1.0);
+    for (size_t i = 1; i < 65536; ++i) lut[i] = (float)log((double)(float)i);
+}
+
+static float orc_loglut(const float *lut, float v)
+{
+    if (v < 0.0f) return lut[0];
+    if (65535.0f < v) return lut[65535];
+    return lut[(size_t)(v * 1.0f + 0.5f)];
+}
+
+/* ProcessCore(3D) with bUseLUT (NormPlugins.cpp:455-475) and ProcessCore(2D)
+ * (:370-381): p = img - dark (nDCCount != 0) or img, L = LUT(p) (no p <= 0
+ * test), img = flat_log - L + term[j], or -L + term[j] without a flat.
+ * term[j]: 3-D with open beams doselist[j]; 3-D without doselist[j] -
+ * doselist[0]; 2-D cLogDose (orc_lognorm_lut_terms makes the 3-D ones). */
+void orc_lognorm_lut_f32(float *img, size_t n_proj, size_t n_pix, const float *flat_log, const float *dark,
+                         const float *term)
+{
+    float *lut = (float *)malloc(sizeof(float) * 65536);
+    orc_loglut_build(lut);
+#pragma omp parallel for schedule(static)
+    for (long j = 0; j < (long)n_proj; ++j) {
+        const float t = term ? term[j] : 0.0f;
+        float *p = img + (size_t)j * n_pix;
+        for (size_t i = 0; i < n_pix; ++i) {
+            const float v = dark ? p[i] - dark[i] : p[i];
+            const float L = orc_loglut(lut, v);
+            p[i] = flat_log ? flat_log[i] - L + t : -L + t;
+        }
+    }
+    free(lut);
+}
+
+/* The 3-D path's per-projection terms (NormPlugins.cpp:418-428, 459, 470-471):
+ * doselist[j] = log(max(dose[j] - dark_dose, 1)) with usenormregion, else 0;
+ * without open beams the term is doselist[j] - doselist[0]. */
+void orc_lognorm_lut_terms(const float *dose, size_t n_proj, float dark_dose, int has_flat, float *term)
+{
+    for (size_t j = 0; j < n_proj; ++j) {
+        float dl = 0.0f;
+        if (dose) {
+            const float d = dose[j] - dark_dose;
+            dl = logf(d < 1 ? 1.0f : d);
+        }
+        term[j] = dl;
+    }
+    if (!has_flat) {
+        const float d0 = term[0];
+        for (size_t j = 0; j < n_proj; ++j) term[j] = term[j] - d0;
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Matched Joseph projector pair in double precision (the project's own     */
 /* operator; no reference counterpart).  Geometry of the parallel-beam BP:  */

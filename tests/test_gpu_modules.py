@@ -86,12 +86,14 @@ def test_engine_device_handoff(orc, dev):
     assert e[0] <= 1e-6
 
 
-def test_engine_lognorm_filter_chain(orc, dev):
+@pytest.mark.parametrize("uselut", [False, True])
+def test_engine_lognorm_filter_chain(orc, dev, uselut):
     """FullLogNorm -> ProjectionFilterSingle -> MultiProjBPparallel through
     the plugin quartet from raw counts, references and doses set through the
     driver: the same bits as FullLogNorm, the filter and the back-projector
     through the C ABI; each module hands its stack to the next on the device
-    (two hand-offs per slab), and the same bits without the hand-offs."""
+    (two hand-offs per slab), and the same bits without the hand-offs; for
+    FullLogNorm's log path and its look-up-table path."""
     from imagingsuite_amd import engine, scan
     from imagingsuite_amd.recon import BackProjector, FullLogNorm, ProjectionFilterSingle
     nu, n_p, slices, sb = 96, 90, 32, 16
@@ -111,7 +113,7 @@ def test_engine_lognorm_filter_chain(orc, dev):
     def run(handoff):
         eng = engine.Engine("MultiProjBPparallel", bpp, dict(cfg, device_handoff="true" if handoff else "false"),
                             ["FullLogNorm", "ProjectionFilterSingle"],
-                            [{"usenormregion": "true", "uselut": "false"}, FILTER])
+                            [{"usenormregion": "true", "uselut": "true" if uselut else "false"}, FILTER])
         eng.set_references(dark, flat, dark_dose, flat_dose)
         eng.set_doses(dose)
         n0 = engine.device_handoffs()
@@ -121,7 +123,8 @@ def test_engine_lognorm_filter_chain(orc, dev):
     v_on, taken_on = run(True)
     v_off, taken_off = run(False)
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
-    norm = FullLogNorm(t(flat), t(dark), flat_dose=flat_dose, dark_dose=dark_dose, usenormregion=True)
+    norm = FullLogNorm(t(flat), t(dark), flat_dose=flat_dose, dark_dose=dark_dose, usenormregion=True,
+                       uselut=uselut)
     filt = ProjectionFilterSingle(nu, "Hanning")
     bp = BackProjector("MultiProjBPparallel", nu, slices, 48.5, buffer_size=16)
     g = t(raw)
@@ -137,7 +140,7 @@ def test_engine_lognorm_filter_chain(orc, dev):
     # the same volume
     eng = engine.Engine("MultiProjBPparallel", bpp, dict(cfg, projection_at_a_time="final"),
                         ["FullLogNorm", "ProjectionFilterSingle"],
-                        [{"usenormregion": "true", "uselut": "false"}, FILTER])
+                        [{"usenormregion": "true", "uselut": "true" if uselut else "false"}, FILTER])
     eng.set_references(dark, flat, dark_dose, flat_dose)
     eng.set_doses(dose)
     v_2d = eng.run(raw, angles, weights, slices)

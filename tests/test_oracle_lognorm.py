@@ -1,4 +1,5 @@
-"""Oracle: FullLogNorm (NormPlugins.cpp:300-341, 411-558; non-LUT path).
+"""Oracle: FullLogNorm (NormPlugins.cpp:300-341, 411-558), log path and
+look-up-table path.
 
 What pins the restatement:
 * Beer-Lambert round trip: projections made as
@@ -8,6 +9,12 @@ What pins the restatement:
   with dose - dark_dose < 1 contributes log(1) = 0; without flat images the
   result is -log(I - dark) + log(dose); without usenormregion the dose term
   is 0; a non-positive flat becomes 0 in the prepared reference.
+* LUT path (uselut; kipl LogLUT(65536, 0, 65535), LUTbase::BasicLUT): the
+  table is log(i) rounded to float with entry 0 = log(1) = 0; a count p reads
+  entry (size_t)(p + 0.5) (round half up), below 0 entry 0, above 65535 the
+  last entry, with no p <= 0 -> 0 case; the result is flat_log - L + dose
+  term, the term without open beams being doselist[j] - doselist[0].  For
+  integer counts the LUT path equals the log path to one float ulp.
 """
 import numpy as np
 
@@ -66,3 +73,52 @@ def test_lognorm_dark_dose(orc):
     a = orc.lognorm(I, fl, dark, dose, 100.0)
     b = orc.lognorm(I, fl, dark, dose - 100.0, 0.0)
     np.testing.assert_array_equal(a, b)
+
+
+def test_loglut_table(orc):
+    t = orc.loglut()
+    i = np.arange(1, 65536, dtype=np.float64)
+    assert t[0] == 0.0
+    np.testing.assert_array_equal(t[1:], np.log(i).astype(np.float32))
+
+
+def test_lognorm_lut_rounding_and_clamps(orc):
+    v = np.array([-3.0, 0.0, 0.49, 0.5, 1.0, 2.49, 2.5, 7.5, 65534.6, 65535.0, 70000.0], np.float32)
+    idx = [0, 0, 0, 1, 1, 2, 3, 8, 65535, 65535, 65535]
+    L = np.array([0.0 if k <= 1 else np.log(k) for k in idx]).astype(np.float32)
+    I = v.reshape(1, 1, -1)
+    # no flat, no dark, no dose list: -L
+    np.testing.assert_array_equal(orc.lognorm_lut(I, None, None)[0, 0], -L)
+    flat_log = np.full((1, v.size), 0.75, np.float32)
+    dark = np.full((1, v.size), 0.0, np.float32)
+    got = orc.lognorm_lut(I, flat_log, dark, np.array([1001.0], np.float32), 1.0)
+    np.testing.assert_allclose(got[0, 0], 0.75 - L.astype(np.float64) + np.log(1000.0), atol=2e-6)
+    # the log path maps p <= 0 to 0, the LUT path to flat_log + term
+    assert orc.lognorm(I, flat_log, dark, None)[0, 0, 0] == 0.0
+    assert orc.lognorm_lut(I, flat_log, dark)[0, 0, 0] == np.float32(0.75)
+
+
+def test_lognorm_lut_dose_terms(orc):
+    dose = np.array([5.0, 10.0, 0.5], np.float32)
+    t = orc.lognorm_lut_terms(3, dose, 1.0, True)
+    np.testing.assert_allclose(t, [np.log(4.0), np.log(9.0), 0.0], atol=1e-6)
+    t0 = orc.lognorm_lut_terms(3, dose, 1.0, False)
+    np.testing.assert_allclose(t0, [0.0, np.log(9.0) - np.log(4.0), -np.log(4.0)], atol=1e-6)
+    assert np.all(orc.lognorm_lut_terms(3, None, 1.0, False) == 0.0)
+    I = np.full((3, 2, 4), 100.0, np.float32)
+    out = orc.lognorm_lut(I, None, None, dose, 1.0)
+    for j in range(3):
+        np.testing.assert_allclose(out[j], -np.log(100.0) + t0[j], atol=2e-6)
+
+
+def test_lognorm_lut_equals_log_path_on_integer_counts(orc):
+    rng = np.random.default_rng(9)
+    p, flat, dark, dose = _case(4, n_p=4, rows=5, cols=33)
+    I = rng.integers(1, 65536, p.shape).astype(np.float32)
+    dk = np.floor(dark).astype(np.float32)
+    I = np.maximum(I, dk[None] + 1)
+    fl = orc.lognorm_flat(flat, dk, 1000.0)
+    a = orc.lognorm_lut(I, fl, dk, dose, 10.0)
+    b = orc.lognorm(I, fl, dk, dose, 10.0)
+    ulp = np.spacing(np.maximum(np.abs(a), np.abs(b)).astype(np.float32))
+    assert np.all(np.abs(a - b) <= 2 * ulp + 1e-6)
