@@ -814,8 +814,10 @@ __global__ __launch_bounds__(256) void bp_adjoint_kernel(const float *__restrict
         // mask or past the window's run: weight 0, and a staged value is read)
         int j = p - sp.x;
         const bool out = j < 0 || j > min(sp.y - sp.x, W - 1);
-        w = out ? 0.0f : w;
-        j = out ? 0 : j;
+        // a zero-weight position is skipped: its lanes read nothing from LDS
+        // (the staged values are finite and the sums never -0, so the result
+        // is the same)
+        if (out || w == 0.0f) return;
         const float *v = &seg[buf][j][0];
 #pragma unroll
         for (int gq = 0; gq < G; ++gq) {
