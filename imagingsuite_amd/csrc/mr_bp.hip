@@ -320,10 +320,16 @@ constexpr int kBpSpan = 26;
 // PIPE = 1: the projection loop unrolled over the chunk and every slice group
 // computed (zeros staged beyond 4*(nv/4)), so the LDS reads of several groups
 // and projections are in flight together (about 80 registers, 6 waves).
-template <int ZT, int INTERP, bool EXACT, int KC, int PIPE>
+// TILT (with PIPE = 0 and EXACT): the back-projectors' bCorrectTilt branches
+// as in bp_tilt_kernel (1 MultiProjBPparallel, 2 MultiProjBP, 3
+// NearestNeighborBP): the detector position shifted by centeroffset and the
+// interpolation weight recomputed per slice group.
+template <int ZT, int INTERP, bool EXACT, int KC, int PIPE, int TILT = 0>
 __device__ __forceinline__ void bp_lds_tile(const BpArgs &a, int bx, int by, int bz,
                                             float (&sm)[KC][kBpSpan][ZT + 4], int (&sb0)[KC])
 {
+    static_assert(TILT == 0 || (PIPE == 0 && EXACT), "the tilt branches run on the reference-order, rolled path");
+    static_assert(TILT != 3 || INTERP == 1, "NearestNeighborBP's tilt branch is the nearest-neighbour form");
     constexpr int T = 16;
     constexpr int SPAN = kBpSpan;
     constexpr int RS = ZT + 4;
@@ -385,9 +391,20 @@ __device__ __forceinline__ void bp_lds_tile(const BpArgs &a, int bx, int by, int
                 const float4 t = a.tab[kb + threadIdx.x];
                 const float ya = (float)(y0 + 1), yb = (float)(y0 + T);
                 const float xa = (float)(x0 + 1), xb = (float)(x0 + T);
-                const float la = t.x + t.y * ya, lb = t.x + t.y * yb;
-                int n0 = min(min((int)(la - t.z * xa), (int)(la - t.z * xb)),
+                int n0;
+                if constexpr (TILT == 1) {
+                    const float la = (t.x + t.y * ya) - a.tilt_co, lb = (t.x + t.y * yb) - a.tilt_co;
+                    n0 = min(min((int)(la - t.z * xa), (int)(la - t.z * xb)),
                              min((int)(lb - t.z * xa), (int)(lb - t.z * xb)));
+                } else if constexpr (TILT >= 2) {
+                    const float la = t.x + t.y * ya, lb = t.x + t.y * yb, co = a.tilt_co;
+                    n0 = min(min((int)((la - t.z * xa) - co), (int)((la - t.z * xb) - co)),
+                             min((int)((lb - t.z * xa) - co), (int)((lb - t.z * xb) - co)));
+                } else {
+                    const float la = t.x + t.y * ya, lb = t.x + t.y * yb;
+                    n0 = min(min((int)(la - t.z * xa), (int)(la - t.z * xb)),
+                             min((int)(lb - t.z * xa), (int)(lb - t.z * xb)));
+                }
                 // one bin of margin: an interior voxel may round one ulp below a
                 // corner and truncate to the bin below
                 sb0[threadIdx.x] = max(n0 - 1, 0);
@@ -432,13 +449,48 @@ __device__ __forceinline__ void bp_lds_tile(const BpArgs &a, int bx, int by, int
             if (inside) {
                 for (int kk = 0; kk < kc; ++kk) {
                     const float4 t = tab_entry(a.tab, kb + kk);
-                    const float lsu = t.x + t.y * yf;
-                    const float u = lsu - t.z * xf;
+                    float u;
+                    if constexpr (TILT == 1) {
+                        const float lsu = (t.x + t.y * yf) - a.tilt_co;
+                        u = lsu - t.z * xf;
+                    } else if constexpr (TILT >= 2) {
+                        const float lsu = t.x + t.y * yf;
+                        u = (lsu - t.z * xf) - a.tilt_co;
+                    } else {
+                        const float lsu = t.x + t.y * yf;
+                        u = lsu - t.z * xf;
+                    }
                     const int n = (int)u;
                     if (n < 0 || n > num2) continue;
                     const int loc = min(max(n - sb0[kk], 0), SPAN - 2);
                     const float *A = &sm[kk][loc][0];
-                    if (INTERP == 0) {
+                    if constexpr (TILT == 1 || TILT == 2) {
+                        // per slice group g (from the block's first slice): weight
+                        // |f_g - g centerinc| (f_g: f less centerinc g times) or
+                        // |(u - n) - g centerinc|
+                        const float ci = a.tilt_ci;
+                        const int g0 = z0 >> 2;
+                        const float fr = u - (float)n;
+                        float f = fr;
+                        if constexpr (TILT == 1)
+                            for (int j = 0; j < g0; ++j) f = f - ci;
+                        const float *B = A + RS;
+#pragma unroll
+                        for (int g = 0; g < G; ++g) {
+                            const float gz = (float)(g0 + g) * ci;
+                            const float wb = TILT == 1 ? fabsf(f - gz) : fabsf(fr - gz);
+                            const float wa = 1.0f - wb;
+                            if (4 * g < zl) {
+                                const float4 va = *reinterpret_cast<const float4 *>(A + 4 * g);
+                                const float4 vb = *reinterpret_cast<const float4 *>(B + 4 * g);
+                                acc[4 * g + 0] = acc[4 * g + 0] + (va.x * wa + vb.x * wb);
+                                acc[4 * g + 1] = acc[4 * g + 1] + (va.y * wa + vb.y * wb);
+                                acc[4 * g + 2] = acc[4 * g + 2] + (va.z * wa + vb.z * wb);
+                                acc[4 * g + 3] = acc[4 * g + 3] + (va.w * wa + vb.w * wb);
+                            }
+                            if constexpr (TILT == 1) f = f - ci;
+                        }
+                    } else if (INTERP == 0) {
                         const float wb = fabsf(u - (float)n);
                         const float wa = 1.0f - wb;
                         const float *B = A + RS;
@@ -549,6 +601,17 @@ __global__ __launch_bounds__(256) void bp_lds_kernel(BpArgs a)
     int bx, by, bz;
     if (!tile_of_block(a, bx, by, bz)) return;
     bp_lds_tile<ZT, INTERP, EXACT, KC, PIPE>(a, bx, by, bz, sm, sb0);
+}
+
+// The tilt branches on the same tiles (reference order, rolled loop).
+template <int ZT, int INTERP, int KC, int TILT>
+__global__ __launch_bounds__(256) void bp_lds_tilt_kernel(BpArgs a)
+{
+    __shared__ float sm[KC][kBpSpan][ZT + 4];
+    __shared__ int sb0[KC];
+    int bx, by, bz;
+    if (!tile_of_block(a, bx, by, bz)) return;
+    bp_lds_tile<ZT, INTERP, true, KC, 0, TILT>(a, bx, by, bz, sm, sb0);
 }
 
 // Persistent batched back-projection: a grid sized to the device (every
@@ -989,6 +1052,19 @@ static int launch_bp_lds(BpArgs a, hipStream_t s)
     return MR_OK;
 }
 
+// Tilt branches on the LDS-staged tiles (the reference order, rolled loop).
+template <int ZT, int KC>
+static int launch_bp_lds_tilt(BpArgs a, hipStream_t s)
+{
+    const dim3 grid = tiled_grid(a, ZT);
+    ProfScope prof("bp_parallel_tilt", s);
+    if (a.tilt == 1) hipLaunchKernelGGL((bp_lds_tilt_kernel<ZT, 0, KC, 1>), grid, dim3(256), 0, s, a);
+    else if (a.tilt == 2) hipLaunchKernelGGL((bp_lds_tilt_kernel<ZT, 0, KC, 2>), grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((bp_lds_tilt_kernel<ZT, 1, KC, 3>), grid, dim3(256), 0, s, a);
+    MR_HIP(hipGetLastError());
+    return MR_OK;
+}
+
 template <int ZT, int KC, int PIPE>
 static int launch_bp_lds_mode(const BpArgs &a, int interp, int exact, hipStream_t s)
 {
@@ -1032,6 +1108,13 @@ int run_bp(const BpArgs &a, int interp, int impl, int exact, hipStream_t s)
 {
     if (a.tilt != 0) {
         MR_REQUIRE(a.tilt >= 1 && a.tilt <= 3, "unknown tilt branch");
+        if (impl == 1) {
+            int zt, kc, pipe;
+            lds_shape(a.nu, a.nv, zt, kc, pipe);
+            if (zt == 32) return launch_bp_lds_tilt<32, 4>(a, s);
+            if (zt == 16) return launch_bp_lds_tilt<16, 8>(a, s);
+            return launch_bp_lds_tilt<8, 16>(a, s);
+        }
         if (a.nv >= 16) return launch_bp_tilt<16>(a, s);
         if (a.nv >= 8) return launch_bp_tilt<8>(a, s);
         return launch_bp_tilt<4>(a, s);
@@ -1052,6 +1135,13 @@ int run_bp(const BpArgs &a, int interp, int impl, int exact, hipStream_t s)
 // "bp_lds_kernel<32,0,false,4,1> order=1 sw=4".
 std::string describe_bp(int nu, int nv, int interp, int impl, int exact, int tilt)
 {
+    if (tilt != 0 && impl == 1) {
+        int zt, kc, pipe;
+        lds_shape(nu, nv, zt, kc, pipe);
+        const long n = (long)((nu + 15) / 16) * ((nu + 15) / 16) * ((nv + zt - 1) / zt);
+        return "bp_lds_tilt_kernel<" + std::to_string(zt) + "," + (tilt == 3 ? "1" : "0") + "," + std::to_string(kc) +
+               "," + std::to_string(tilt) + ">" + (n < (1L << 30) ? " order=1 sw=4" : " order=0");
+    }
     if (tilt != 0) {
         const int zt = nv >= 16 ? 16 : (nv >= 8 ? 8 : 4);
         return "bp_tilt_kernel<" + std::to_string(zt) + "," + std::to_string(tilt) + ">";

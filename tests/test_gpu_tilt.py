@@ -2,10 +2,11 @@
 the oracle, whose tilt restatement is pinned in test_oracle_tilt.py.
 
 * back-projectors (MultiProjBPparallel, MultiProjBP, NearestNeighborBP):
-  bit-exact, including slice tiles that start past the first slice group
-  (the parallel branch's running offset), blocks away from the pivot, odd
-  slice counts and the buffer drop; the plan's mask equals the oracle's
-  shrunk circle mask;
+  bit-exact, on the LDS-staged tiles and (MultiProjBP with slices % 4 != 0)
+  the direct-gather kernel, including slice tiles that start past the first
+  slice group (the parallel branch's running offset), blocks away from the
+  pivot, odd slice counts and the buffer drop; the plan's mask equals the
+  oracle's shrunk circle mask;
 * block loops: the plugin driver (SetROI per SliceBlock slab, so each slab's
   first detector row enters the tilt terms) and the streaming entry point
   equal the direct path slab by slab, bit for bit, and the oracle within
@@ -42,7 +43,10 @@ def test_bp_tilt_bitexact(orc, dev, name, nv, bufsize, angle, pivot, row0):
     center, rows = 47.5, 4 * nv
     bp = BackProjector(name, nu, nv, center, direction=1, buffer_size=bufsize, correct_tilt=True,
                        tilt_angle=angle, tilt_pivot=pivot, row0=row0, tilt_rows=rows)
-    assert bp.kernel.startswith("bp_tilt_kernel<")
+    # LDS-staged tiles, except MultiProjBP with slices % 4 != 0 (its second
+    # column sits at another offset): direct gathers
+    direct = name == "MultiProjBP" and nv % 4 != 0
+    assert bp.kernel.startswith("bp_tilt_kernel<" if direct else "bp_lds_tilt_kernel<"), bp.kernel
     shrink = orc.tilt_mask_shrink(angle, rows)
     f, s = bp.mask
     rf, rs = orc.circle_mask(nu, shrink=shrink)
