@@ -830,10 +830,39 @@ __global__ __launch_bounds__(256) void bp_adjoint_kernel(const float *__restrict
     // group e % G).
     constexpr int NE = (W * G + 255) / 256;
     float4 pre[NE];
+    // When the whole volume lies within 2^31 bytes the loads go through a
+    // buffer resource over it with 32-bit byte offsets, whose per-element
+    // part (position j of the span, slice group gq) is computed once here:
+    // a line then costs one add per element instead of 64-bit address
+    // arithmetic.  Elements past the span get an offset past the range
+    // (zeros; put() does not store them).
+    const size_t vol_bytes = (size_t)nu * nu * nv * sizeof(float);
+    const bool off32 = vol_bytes < 0x7FFFFFF0u;
+    const unsigned stepb = rows ? (unsigned)nv * 4u : (unsigned)(off32 ? (size_t)nu * nv * 4 : 0);
+    unsigned st_off[NE];
+#pragma unroll
+    for (int i = 0; i < NE; ++i) {
+        const int e = tid + 256 * i;
+        const int j = e / G, gq = e - j * G;
+        st_off[i] = (unsigned)j * stepb + 4u * (unsigned)(z0 + 4 * gq);
+    }
+    const __amdgpu_buffer_rsrc_t vrs = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)g, (short)0, off32 ? (int)vol_bytes : 0x7FFFFFF0, 0x00020000);
     auto fetch = [&](int L) {
         const int2 sp = spans[L - 1];
         const int n_el = (min(sp.y - sp.x, W - 1) + 1) * G;
-        // voxel (position p, line L) at ((y - 1) nu + (x - 1)) nv
+        if (off32) {
+            // voxel (position p, line L) at ((y - 1) nu + (x - 1)) nv
+            const unsigned lineb = rows ? (unsigned)(L - 1) * (unsigned)nu * stepb : (unsigned)(L - 1) * (unsigned)nv * 4u;
+            const unsigned pb = lineb + (unsigned)(sp.x - 1) * stepb;
+#pragma unroll
+            for (int i = 0; i < NE; ++i) {
+                const int e = tid + 256 * i;
+                const unsigned off = e < n_el ? pb + st_off[i] : 0x80000000u;
+                pre[i] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(vrs, off, 0, 0));
+            }
+            return;
+        }
         const size_t base = rows ? (size_t)(L - 1) * nu * nv : (size_t)(L - 1) * nv;
         const size_t step = rows ? (size_t)nv : (size_t)nu * nv;
 #pragma unroll
