@@ -113,3 +113,36 @@ def test_tilt_refusals_without_device():
     plan = ctypes.c_void_p()
     assert lib.mr_fdk_plan_create(ctypes.byref(plan), ctypes.byref(q)) == L.MR_ERR_INVALID
     assert b"tilt" in lib.mr_last_error() and not plan.value
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """The ctypes mirrors of the parameter structs have the C header's size
+    and field offsets (gcc compiles a probe against include/muhrec_hip.h)."""
+    import shutil
+    import subprocess
+    import imagingsuite_amd._lib as L
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no C compiler")
+    structs = {"mr_bp_params": L.BpParams, "mr_fdk_params": L.FdkParams, "mr_batch_stack": L.BatchStack,
+               "mr_joseph_geom": L.JosephGeom}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "muhrec_hip.h"', "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append(f'printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    subprocess.run([gcc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for ln in out:
+        if ln:
+            cname, key, val = ln.split()
+            got[(cname, key)] = int(val)
+    for cname, cls in structs.items():
+        assert got[(cname, "sizeof")] == ctypes.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)
