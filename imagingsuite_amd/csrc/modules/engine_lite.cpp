@@ -6,10 +6,13 @@
 // (frameworks/tomography/Framework/ReconFramework/src/ReconEngine.cpp:531-574, 838-1185, 1314-1714).
 #include <dlfcn.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <exception>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "mr/module_api.h"
@@ -131,6 +134,9 @@ struct mre_engine {
     // wall time of the last run's phases, ms: slab copy-in, pre-processing
     // Process, back-projector SetROI, back-projector Process, GetSlice copy-out
     double phase_ms[5] = {0, 0, 0, 0, 0};
+    // the SliceBlock stack, kept between blocks and runs of the same shape so
+    // that its pages are allocated and zeroed once (held until destroy)
+    kipl::base::TImage<float, 3> stack;
 };
 
 namespace {
@@ -150,6 +156,35 @@ private:
     double *acc_;
     std::chrono::steady_clock::time_point t_;
 };
+
+// Runs f(t, nt) for t = 0..nt-1 on nt threads (t = 0 on the calling thread),
+// nt = the hardware threads, at most 16; rethrows the first exception raised.
+template <typename F>
+void on_threads(F f)
+{
+    const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::exception_ptr> err(nt);
+    auto run = [&](int t) {
+        try {
+            f(t, nt);
+        } catch (...) {
+            err[t] = std::current_exception();
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(run, t);
+    run(0);
+    for (auto &x : th) x.join();
+    for (auto &x : err)
+        if (x) std::rethrow_exception(x);
+}
+
+// [a, b) of n items for thread t of nt
+inline void share(size_t n, int t, int nt, size_t &a, size_t &b)
+{
+    a = n * (size_t)t / (size_t)nt;
+    b = n * (size_t)(t + 1) / (size_t)nt;
+}
 
 }  // namespace
 
@@ -340,9 +375,19 @@ extern "C" int mre_engine_run(mre_engine *e, const float *projections, size_t n_
             }
             const size_t nr = r1 - r0;
             size_t dims[3] = {width, nr, n_proj};
-            kipl::base::TImage<float, 3> img(dims);
-            for (size_t p = 0; p < n_proj; ++p)
-                std::memcpy(img.GetLinePtr(0, p), projections + (p * rows + r0) * width, sizeof(float) * nr * width);
+            // the block's stack (Resize keeps the buffer when the shape is
+            // unchanged; Clone detaches it from a module still holding the
+            // previous block), filled projection by projection on all threads
+            e->stack.Resize(dims);
+            e->stack.Clone();
+            kipl::base::TImage<float, 3> &img = e->stack;
+            on_threads([&](int t, int nt) {
+                size_t a, b;
+                share(n_proj, t, nt, a, b);
+                for (size_t p = a; p < b; ++p)
+                    std::memcpy(img.GetLinePtr(0, p), projections + (p * rows + r0) * width,
+                                sizeof(float) * nr * width);
+            });
             clock.lap(0);
             // ReconEngine::Process3D (ReconEngine.cpp:1204-1214): SetROI of every
             // pre-processing module for the block, then its Process
@@ -352,19 +397,29 @@ extern "C" int mre_engine_run(mre_engine *e, const float *projections, size_t n_
             for (auto *m : e->pres) m->Process(img, params);
             if (e->rescale_between != 1.0f) {
                 float *d = img.GetDataPtr();
-                for (size_t i = 0; i < img.Size(); ++i) d[i] *= e->rescale_between;
+                const float k = e->rescale_between;
+                on_threads([&](int t, int nt) {
+                    size_t a, b;
+                    share(img.Size(), t, nt, a, b);
+                    for (size_t i = a; i < b; ++i) d[i] *= k;
+                });
             }
             clock.lap(1);
             e->bp->SetROI(block);
             clock.lap(2);
             e->bp->Process(img, params);
             clock.lap(3);
+            // GetSlice only reads the module's volume: slices in parallel
             const size_t ns = block[3] - block[1];
-            for (size_t i = 0; i < ns; ++i) {
-                kipl::base::TImage<float, 2> slice = e->bp->GetSlice(i);
-                std::memcpy(volume + (block[1] - roi[1] + i) * width * width, slice.GetDataPtr(),
-                            sizeof(float) * std::min(slice.Size(), width * width));
-            }
+            on_threads([&](int t, int nt) {
+                size_t a, b;
+                share(ns, t, nt, a, b);
+                for (size_t i = a; i < b; ++i) {
+                    kipl::base::TImage<float, 2> slice = e->bp->GetSlice(i);
+                    std::memcpy(volume + (block[1] - roi[1] + i) * width * width, slice.GetDataPtr(),
+                                sizeof(float) * std::min(slice.Size(), width * width));
+                }
+            });
             clock.lap(4);
         }
     } catch (const std::exception &ex) {

@@ -25,6 +25,7 @@
 #include <mutex>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "mr/module_api.h"
@@ -32,6 +33,19 @@
 #include "muhrec_modules.h"
 
 namespace {
+
+// volume = 0.0f after the Resize of SetROI (StdBackProjectorBase.cpp:188),
+// on up to 16 threads: a 2048^2 x 256 volume is 4 GiB
+void zero_volume(kipl::base::TImage<float, 3> &v)
+{
+    float *d = v.GetDataPtr();
+    const size_t n = v.Size();
+    const size_t T = n < (size_t(1) << 22) ? 1 : std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < T; ++t) th.emplace_back([=] { std::fill(d + n * t / T, d + n * (t + 1) / T, 0.0f); });
+    std::fill(d, d + n / T, 0.0f);
+    for (auto &x : th) x.join();
+}
 
 void check_mr(int rc, const char *what, const char *file, int line)
 {
@@ -529,7 +543,7 @@ public:
         MatrixDims[1] = SizeU;
         MatrixDims[2] = SizeV;
         volume.Resize(MatrixDims);
-        volume = 0.0f;
+        zero_volume(volume);
         BuildCircleMask();
         if (plan_) mr_bp_plan_destroy(plan_);
         plan_ = nullptr;
@@ -776,7 +790,7 @@ public:
         MatrixDims[1] = SizeU;
         MatrixDims[2] = SizeV;
         volume.Resize(MatrixDims);
-        volume = 0.0f;
+        zero_volume(volume);
         BuildCircleMask();
         if (plan_) mr_fdk_plan_destroy(plan_);
         plan_ = nullptr;
