@@ -219,6 +219,8 @@ public:
     {
         DeviceStack::get().retract(this);
         drop_plan();
+        if (up_) (void)hipStreamDestroy(up_);
+        if (down_) (void)hipStreamDestroy(down_);
     }
 
     int Configure(ReconConfig config, std::map<std::string, std::string> parameters) override
@@ -258,7 +260,7 @@ protected:
     {
         if (img.Size() == 0) throw ReconException("Empty projection image", __FILE__, __LINE__);
         DeviceStack::get().retract(this);
-        filter_rows(img.GetDataPtr(), img.Size(1), img.Size(0), nullptr, nullptr);
+        filter_rows(img.GetDataPtr(), img.Size(1), img.Size(0));
         return 0;
     }
 
@@ -266,54 +268,101 @@ protected:
     // projections one by one and polls the interactor before each; here the
     // stack goes through the GPU in chunks of 64 projections with the poll
     // between chunks (an abort leaves the later projections unfiltered, as
-    // the reference's loop does).
+    // the reference's loop does).  The chunks are pipelined: while chunk k is
+    // uploaded and filtered on one stream, a second host thread downloads
+    // chunk k - 1 on another (two device buffers take turns unless the whole
+    // stack is kept on the device).
     int ProcessCore(kipl::base::TImage<float, 3> &img, std::map<std::string, std::string> &) override
     {
         if (img.Size() == 0) throw ReconException("Empty projection image", __FILE__, __LINE__);
         DeviceStack::get().retract(this);
+        if (img.Size(0) != nImageSize)
+            throw ReconException("Projection width does not match the configured ROI", __FILE__, __LINE__);
+        make_plan();
+        if (!up_) MRM_HIP(hipStreamCreateWithFlags(&up_, hipStreamNonBlocking));
+        if (!down_) MRM_HIP(hipStreamCreateWithFlags(&down_, hipStreamNonBlocking));
         // the previous module's device copy of this very stack (FullLogNorm
         // with keepondevice), else the host stack is copied in
         const float *src = DeviceStack::get().take(img.GetDataPtr(), img.Size());
         if (src) DeviceStack::get().count_taken();
         const size_t nProj = img.Size(2), per = img.Size(0) * img.Size(1);
-        if (keep_) stack_.reserve(img.Size());
         constexpr size_t kChunk = 64;
-        for (size_t p0 = 0; p0 < nProj; p0 += kChunk) {
-            if (UpdateStatus((float)p0 / nProj, m_sModuleName)) return 0;
-            const size_t n = std::min(kChunk, nProj - p0);
-            filter_rows(img.GetDataPtr() + p0 * per, img.Size(1) * n, img.Size(0),
-                        keep_ ? stack_.p + p0 * per : nullptr, src ? src + p0 * per : nullptr);
+        if (keep_) stack_.reserve(img.Size());
+        else
+            for (DevBuf &b : pipe_) b.reserve(std::min(kChunk, nProj) * per);
+        Download down;
+        for (size_t p0 = 0, k = 0; p0 < nProj; p0 += kChunk, ++k) {
+            if (UpdateStatus((float)p0 / nProj, m_sModuleName)) {
+                down.finish();
+                return 0;
+            }
+            const size_t n = std::min(kChunk, nProj - p0), bytes = n * per * sizeof(float);
+            float *rows = img.GetDataPtr() + p0 * per;
+            float *dev = keep_ ? stack_.p + p0 * per : pipe_[k & 1].p;
+            if (src) MRM_HIP(hipMemcpyAsync(dev, src + p0 * per, bytes, hipMemcpyDeviceToDevice, up_));
+            else MRM_CHECK(mr_copy_h2d(dev, rows, bytes, up_));
+            MRM_CHECK(mr_filter_rows(plan_, dev, img.Size(1) * n, up_));
+            MRM_HIP(hipStreamSynchronize(up_));
+            // chunk k - 1's download read the buffer chunk k + 1 fills
+            down.finish();
+            down.start(rows, dev, bytes, down_);
         }
+        down.finish();
         if (keep_) DeviceStack::get().publish(this, stack_.p, img.GetDataPtr(), img.Size());
         UpdateStatus(1.0f, m_sModuleName);
         return 0;
     }
 
 private:
+    // one device-to-host copy on a thread of its own; finish() joins it and
+    // raises its error (the destructor only joins, for an exception thrown
+    // meanwhile)
+    struct Download {
+        std::thread t;
+        int rc = MR_OK;
+        std::string err;
+        void start(float *h, const float *d, size_t bytes, hipStream_t s)
+        {
+            t = std::thread([=] {
+                rc = mr_copy_d2h(h, d, bytes, s);
+                if (rc != MR_OK) err = mr_last_error();
+            });
+        }
+        void finish()
+        {
+            if (t.joinable()) t.join();
+            if (rc != MR_OK) throw ReconException("mr_copy_d2h: " + err, __FILE__, __LINE__);
+        }
+        ~Download()
+        {
+            if (t.joinable()) t.join();
+        }
+    };
+
+    void make_plan()
+    {
+        if (!plan_)
+            MRM_CHECK(mr_filter_plan_create(&plan_, nImageSize, m_FilterType, m_fCutOff, m_fOrder,
+                                            m_bUseBias ? 1 : 0, m_fBiasWeight, m_nPaddingDoubler));
+    }
+
     void drop_plan()
     {
         if (plan_) mr_filter_plan_destroy(plan_);
         plan_ = nullptr;
     }
 
-    // filters the host rows through the device buffer dev (buf_ when null);
-    // the input comes from the device copy src when given, else from rows
-    void filter_rows(float *rows, size_t n_rows, size_t width, float *dev, const float *src)
+    // filters one projection's host rows through the device buffer buf_
+    void filter_rows(float *rows, size_t n_rows, size_t width)
     {
         if (width != nImageSize)
             throw ReconException("Projection width does not match the configured ROI", __FILE__, __LINE__);
-        if (!plan_)
-            MRM_CHECK(mr_filter_plan_create(&plan_, nImageSize, m_FilterType, m_fCutOff, m_fOrder,
-                                            m_bUseBias ? 1 : 0, m_fBiasWeight, m_nPaddingDoubler));
+        make_plan();
         const size_t n = n_rows * width;
-        if (!dev) {
-            buf_.reserve(n);
-            dev = buf_.p;
-        }
-        if (src) MRM_HIP(hipMemcpy(dev, src, n * sizeof(float), hipMemcpyDeviceToDevice));
-        else MRM_CHECK(mr_copy_h2d(dev, rows, n * sizeof(float), nullptr));
-        MRM_CHECK(mr_filter_rows(plan_, dev, n_rows, nullptr));
-        MRM_CHECK(mr_copy_d2h(rows, dev, n * sizeof(float), nullptr));
+        buf_.reserve(n);
+        MRM_CHECK(mr_copy_h2d(buf_.p, rows, n * sizeof(float), nullptr));
+        MRM_CHECK(mr_filter_rows(plan_, buf_.p, n_rows, nullptr));
+        MRM_CHECK(mr_copy_d2h(rows, buf_.p, n * sizeof(float), nullptr));
     }
 
     int m_FilterType = 3;  // Hamming
@@ -323,7 +372,8 @@ private:
     size_t m_nPaddingDoubler = 2, nImageSize = 0;
     bool keep_ = false;
     mr_filter_plan *plan_ = nullptr;
-    DevBuf buf_, stack_;
+    DevBuf buf_, stack_, pipe_[2];
+    hipStream_t up_ = nullptr, down_ = nullptr;   // chunk uploads + filtering, chunk downloads
 };
 
 // -------------------------------------------------------------------------

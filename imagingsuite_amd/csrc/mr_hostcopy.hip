@@ -6,8 +6,10 @@
 // (BackProjectorModuleBase::GetSlice is not virtual,
 // frameworks/tomography/Framework/ReconFramework/src/BackProjectorModuleBase.cpp:62-124).
 // A plain hipMemcpy of pageable memory runs at a fraction of the link's
-// rate; here the bytes go through two pinned staging buffers per device:
-// host threads fill (or drain) one while the DMA engine moves the other.
+// rate; here the bytes go through two pinned staging buffers per device and
+// direction: host threads fill (or drain) one while the DMA engine moves the
+// other, and an upload and a download on two streams run at once (the link
+// is full duplex).
 // Host memory from mr_host_alloc is copied by DMA directly.
 #include "mr_common.h"
 
@@ -33,7 +35,7 @@ struct Staging {
     bool used[2] = {false, false};
 };
 
-Staging g_staging[kMaxDevices];
+Staging g_staging[2][kMaxDevices];   // [0]: host to device, [1]: device to host
 
 int copy_threads()
 {
@@ -59,12 +61,12 @@ void parallel_memcpy(char *dst, const char *src, size_t n)
     for (auto &x : th) x.join();
 }
 
-int staging(Staging **out)
+int staging(int dir, Staging **out)
 {
     int dev = 0;
     MR_HIP(hipGetDevice(&dev));
     MR_REQUIRE(dev >= 0 && dev < kMaxDevices, "device index beyond the staging table");
-    Staging &s = g_staging[dev];
+    Staging &s = g_staging[dir][dev];
     if (!s.buf[0]) {
         for (int i = 0; i < 2; ++i) {
             MR_HIP(hipHostMalloc((void **)&s.buf[i], kChunk, hipHostMallocDefault));
@@ -112,7 +114,7 @@ extern "C" int mr_copy_h2d(void *d_dst, const void *h_src, size_t bytes, void *s
         return MR_OK;
     }
     mr::Staging *st = nullptr;
-    int rc = mr::staging(&st);
+    int rc = mr::staging(0, &st);
     if (rc != MR_OK) return rc;
     std::lock_guard<std::mutex> lk(st->mu);
     const char *src = static_cast<const char *>(h_src);
@@ -145,7 +147,7 @@ extern "C" int mr_copy_d2h(void *h_dst, const void *d_src, size_t bytes, void *s
         return MR_OK;
     }
     mr::Staging *st = nullptr;
-    int rc = mr::staging(&st);
+    int rc = mr::staging(1, &st);
     if (rc != MR_OK) return rc;
     std::lock_guard<std::mutex> lk(st->mu);
     const char *src = static_cast<const char *>(d_src);

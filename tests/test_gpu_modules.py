@@ -17,13 +17,15 @@ FILTER = {"filtertype": "Hanning", "cutoff": 0.5, "order": 1, "usebias": "true",
           "paddingdoubler": 2}
 
 
-@pytest.mark.parametrize("bp_name,slices,sb", [("MultiProjBPparallel", 64, 32),
-                                               ("MultiProjBP", 40, 16),
-                                               ("NearestNeighborBP", 24, 32)])
-def test_engine_parallel_matches_direct_and_oracle(orc, dev, bp_name, slices, sb):
+# n_p > 128: the filter module's 64-projection chunks reuse its two device
+# buffers while the previous chunk is still being downloaded
+@pytest.mark.parametrize("bp_name,slices,sb,n_p", [("MultiProjBPparallel", 64, 32, 120),
+                                                   ("MultiProjBP", 40, 16, 300),
+                                                   ("NearestNeighborBP", 24, 32, 200)])
+def test_engine_parallel_matches_direct_and_oracle(orc, dev, bp_name, slices, sb, n_p):
     from imagingsuite_amd import engine, scan
     from imagingsuite_amd.recon import BackProjector, ProjectionFilterSingle, fbp_parallel
-    nu, n_p = 128, 120
+    nu = 128
     center = 64.5
     angles, weights = scan.engine_angles_weights(n_p, 0.0, 178.5, 0.1)
     rng = np.random.default_rng(slices)
@@ -57,7 +59,7 @@ def test_engine_device_handoff(orc, dev):
     (keepondevice): bit-identical to copying the host stack again, taken once
     per slab, and not taken when the host stack changed in between."""
     from imagingsuite_amd import engine, scan
-    nu, n_p, slices, sb = 96, 90, 48, 16
+    nu, n_p, slices, sb = 96, 200, 48, 16   # 4 filter chunks
     angles, weights = scan.engine_angles_weights(n_p, 0.0, 178.0, 0.1)
     rng = np.random.default_rng(11)
     proj = (rng.random((n_p, slices, nu)) + 0.5).astype(np.float32)
@@ -96,7 +98,7 @@ def test_engine_lognorm_filter_chain(orc, dev, uselut):
     FullLogNorm's log path and its look-up-table path."""
     from imagingsuite_amd import engine, scan
     from imagingsuite_amd.recon import BackProjector, FullLogNorm, ProjectionFilterSingle
-    nu, n_p, slices, sb = 96, 90, 32, 16
+    nu, n_p, slices, sb = 96, 150, 32, 16   # 3 filter chunks
     angles, weights = scan.engine_angles_weights(n_p, 0.0, 178.0, 0.1)
     rng = np.random.default_rng(21)
     att = rng.uniform(0.0, 2.0, (n_p, slices, nu)).astype(np.float32)
