@@ -556,6 +556,7 @@ public:
     ~HipStdBackProjector() override
     {
         if (plan_) mr_bp_plan_destroy(plan_);
+        if (up_) (void)hipStreamDestroy(up_);
     }
 
     int Configure(ReconConfig config, std::map<std::string, std::string> parameters) override
@@ -686,9 +687,10 @@ public:
         // the filter's device copy of this very stack, else a copy of the host stack
         const float *src = DeviceStack::get().take(projections.GetDataPtr(), nProj * psz);
         if (src) DeviceStack::get().count_taken();
-        if (!src) {
+        const bool upload = src == nullptr;
+        if (upload) {
             dproj_.reserve(nProj * psz);
-            MRM_CHECK(mr_copy_h2d(dproj_.p, projections.GetDataPtr(), nProj * psz * sizeof(float), nullptr));
+            if (!up_) MRM_HIP(hipStreamCreateWithFlags(&up_, hipStreamNonBlocking));
             src = dproj_.p;
         }
         // launches of whole buffers, so that the drop pattern restarts where
@@ -702,6 +704,12 @@ public:
                 return 0;
             }
             const size_t n = std::min(chunk, nProj - p0);
+            // the host stack goes up chunk by chunk on a stream of its own
+            // (the copy returns once its bytes are on the device), so chunk
+            // k + 1's upload overlaps chunk k's back-projection
+            if (upload)
+                MRM_CHECK(mr_copy_h2d(dproj_.p + p0 * psz, projections.GetLinePtr(0, p0), n * psz * sizeof(float),
+                                      up_));
             MRM_CHECK(mr_bp_parallel(plan_, src + p0 * psz, n, angles.data() + p0, weights.data() + p0,
                                      dvol_.p, 1, nullptr));
         }
@@ -780,6 +788,7 @@ private:
     std::vector<float> pending_, pending_angle_, pending_weight_;
     mr_bp_plan *plan_ = nullptr;
     DevBuf dproj_, dvol_, dxyz_;
+    hipStream_t up_ = nullptr;   // chunk uploads of a host stack in Process(3D)
 };
 
 // -------------------------------------------------------------------------
