@@ -119,6 +119,31 @@ struct DevBuf {
     ~DevBuf() { release(); }
 };
 
+// one device-to-host copy on a thread of its own; finish() joins it and
+// raises its error (the destructor only joins, for an exception thrown
+// meanwhile)
+struct Download {
+    std::thread t;
+    int rc = MR_OK;
+    std::string err;
+    void start(float *h, const float *d, size_t bytes, hipStream_t s)
+    {
+        t = std::thread([=] {
+            rc = mr_copy_d2h(h, d, bytes, s);
+            if (rc != MR_OK) err = mr_last_error();
+        });
+    }
+    void finish()
+    {
+        if (t.joinable()) t.join();
+        if (rc != MR_OK) throw ReconException("mr_copy_d2h: " + err, __FILE__, __LINE__);
+    }
+    ~Download()
+    {
+        if (t.joinable()) t.join();
+    }
+};
+
 // Device hand-off of a filtered stack.  MuhRec's engine passes the
 // projection TImage its pre-processing modules changed in place straight on
 // to the back-projector's Process(3D)
@@ -314,31 +339,6 @@ protected:
     }
 
 private:
-    // one device-to-host copy on a thread of its own; finish() joins it and
-    // raises its error (the destructor only joins, for an exception thrown
-    // meanwhile)
-    struct Download {
-        std::thread t;
-        int rc = MR_OK;
-        std::string err;
-        void start(float *h, const float *d, size_t bytes, hipStream_t s)
-        {
-            t = std::thread([=] {
-                rc = mr_copy_d2h(h, d, bytes, s);
-                if (rc != MR_OK) err = mr_last_error();
-            });
-        }
-        void finish()
-        {
-            if (t.joinable()) t.join();
-            if (rc != MR_OK) throw ReconException("mr_copy_d2h: " + err, __FILE__, __LINE__);
-        }
-        ~Download()
-        {
-            if (t.joinable()) t.join();
-        }
-    };
-
     void make_plan()
     {
         if (!plan_)
@@ -388,7 +388,12 @@ public:
         : PreprocModuleBase("FullLogNorm", interactor)
     {
     }
-    ~HipFullLogNorm() override { DeviceStack::get().retract(this); }
+    ~HipFullLogNorm() override
+    {
+        DeviceStack::get().retract(this);
+        if (up_) (void)hipStreamDestroy(up_);
+        if (down_) (void)hipStreamDestroy(down_);
+    }
 
     int Configure(ReconConfig config, std::map<std::string, std::string> parameters) override
     {
@@ -512,12 +517,25 @@ protected:
         DeviceStack::get().retract(this);
         if (UpdateStatus(0.0f, m_sModuleName)) return 0;
         buf_.reserve(img.Size());
-        MRM_CHECK(mr_copy_h2d(buf_.p, img.GetDataPtr(), img.Size() * sizeof(float), nullptr));
-        // the LUT path (NormPlugins.cpp:455-475) or the log path (:476-555)
-        MRM_CHECK((bUseLUT ? mr_lognorm_lut : mr_lognorm)(
-            buf_.p, nProj, N, nOBCount != 0 ? dflat_.p + row_off_ : nullptr,
-            nDCCount != 0 ? ddark_.p + row_off_ : nullptr, bUseNormROI ? dose.data() : nullptr, fDarkDose, nullptr));
-        MRM_CHECK(mr_copy_d2h(img.GetDataPtr(), buf_.p, img.Size() * sizeof(float), nullptr));
+        if (!up_) MRM_HIP(hipStreamCreateWithFlags(&up_, hipStreamNonBlocking));
+        if (!down_) MRM_HIP(hipStreamCreateWithFlags(&down_, hipStreamNonBlocking));
+        // chunks of 64 projections: chunk k is uploaded and normalised on one
+        // stream while a second host thread downloads chunk k - 1 on another
+        constexpr size_t kChunk = 64;
+        Download down;
+        for (size_t p0 = 0; p0 < nProj; p0 += kChunk) {
+            const size_t n = std::min(kChunk, nProj - p0), bytes = n * N * sizeof(float);
+            float *d = buf_.p + p0 * N, *h = img.GetDataPtr() + p0 * N;
+            MRM_CHECK(mr_copy_h2d(d, h, bytes, up_));
+            // the LUT path (NormPlugins.cpp:455-475) or the log path (:476-555)
+            MRM_CHECK((bUseLUT ? mr_lognorm_lut : mr_lognorm)(
+                d, n, N, nOBCount != 0 ? dflat_.p + row_off_ : nullptr, nDCCount != 0 ? ddark_.p + row_off_ : nullptr,
+                bUseNormROI ? dose.data() + p0 : nullptr, fDarkDose, up_));
+            MRM_HIP(hipStreamSynchronize(up_));
+            down.finish();
+            down.start(h, d, bytes, down_);
+        }
+        down.finish();
         if (keep_) DeviceStack::get().publish(this, buf_.p, img.GetDataPtr(), img.Size());
         UpdateStatus(1.0f, m_sModuleName);
         return 0;
@@ -532,6 +550,7 @@ private:
     std::string sReferenceAverage = "ImageWeightedAverage";
     float fDarkDose = 0.0f, fFlatDose = 1.0f;
     DevBuf dflat_, ddark_, buf_;
+    hipStream_t up_ = nullptr, down_ = nullptr;   // chunk uploads + normalisation, chunk downloads
 };
 
 // -------------------------------------------------------------------------

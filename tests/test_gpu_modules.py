@@ -98,7 +98,7 @@ def test_engine_lognorm_filter_chain(orc, dev, uselut):
     FullLogNorm's log path and its look-up-table path."""
     from imagingsuite_amd import engine, scan
     from imagingsuite_amd.recon import BackProjector, FullLogNorm, ProjectionFilterSingle
-    nu, n_p, slices, sb = 96, 150, 32, 16   # 3 filter chunks
+    nu, n_p, slices, sb = 96, 150, 32, 16   # 3 chunks in FullLogNorm and the filter
     angles, weights = scan.engine_angles_weights(n_p, 0.0, 178.0, 0.1)
     rng = np.random.default_rng(21)
     att = rng.uniform(0.0, 2.0, (n_p, slices, nu)).astype(np.float32)
