@@ -130,6 +130,7 @@ struct mre_engine {
     // final projection
     int projection_at_a_time = 0;
     AbortAfter *abort_bp = nullptr;   // bp_abort_after: the back-projector's interactor
+    AbortAfter *abort_filter = nullptr;   // filter_abort_after: ProjectionFilterSingle's interactor
     float rescale_between = 1.0f;     // stack multiplied by this between the two modules
     // wall time of the last run's phases, ms: slab copy-in, pre-processing
     // Process, back-projector SetROI, back-projector Process, GetSlice copy-out
@@ -200,6 +201,7 @@ extern "C" int mre_engine_destroy(mre_engine *e)
     }
     if (e->so) dlclose(e->so);
     delete e->abort_bp;
+    delete e->abort_filter;
     delete e;
     return 0;
 }
@@ -268,8 +270,12 @@ extern "C" int mre_engine_create(mre_engine **engine, const char *module_so, con
         }
         const auto rb = kvc.find("rescale_between");
         if (rb != kvc.end()) e->rescale_between = std::stof(rb->second);
+        const auto fa = kvc.find("filter_abort_after");
+        if (fa != kvc.end()) e->abort_filter = new AbortAfter(std::stoi(fa->second));
         for (size_t i = 0; i < e->pre_names.size(); ++i) {
-            auto *m = static_cast<PreprocModuleBase *>(e->get("muhrec", e->pre_names[i].c_str(), nullptr));
+            kipl::interactors::InteractionBase *ia =
+                e->pre_names[i] == "ProjectionFilterSingle" ? e->abort_filter : nullptr;
+            auto *m = static_cast<PreprocModuleBase *>(e->get("muhrec", e->pre_names[i].c_str(), ia));
             if (!m) throw ReconException("Failed to create module " + e->pre_names[i], __FILE__, __LINE__);
             e->pres.push_back(m);
             m->Configure(e->cfg, e->pre_params[i]);

@@ -50,6 +50,9 @@ int GetModuleList(const char *application, void *listptr);
  *   sets it on the final projection, as StdBackProjectorBase::Process(3D) does;
  *   bp_abort_after=k: the back-projector gets an interactor whose k-th
  *   progress report asks for an abort (the GUI's cancel, at a known point);
+ *   filter_abort_after=k: the same for ProjectionFilterSingle (the driver
+ *   then goes on with the stack as the filter left it, where MuhRec's engine
+ *   would stop: the volume shows which projections were filtered);
  *   device_handoff=true|false (default true): FullLogNorm before
  *   ProjectionFilterSingle, and ProjectionFilterSingle last before a
  *   parallel-beam back-projector, all of the same library, get

@@ -272,3 +272,36 @@ def test_backprojector_cancel_between_launches(orc, dev, abort_after, n_done):
     bp = BackProjector("MultiProjBPparallel", nu, nv, center, buffer_size=16)
     direct = bp.process(t[:n_done].contiguous(), angles[:n_done], weights[:n_done]).cpu().numpy().transpose(2, 0, 1)
     assert np.array_equal(vol, direct)
+
+
+@pytest.mark.parametrize("abort_after,n_filtered", [(1, 0), (3, 128), (5, 256)])
+def test_filter_cancel_between_chunks(orc, dev, abort_after, n_filtered):
+    """ProjectionFilterSingle polls its interactor before each chunk of 64
+    projections while the previous chunk is still being downloaded; an abort
+    at the k-th poll leaves the projections of the chunks before it filtered
+    (their downloads finished) and the later ones as they were, as the
+    reference's projection loop leaves them.  The driver back-projects the
+    stack as the filter left it, so the volume equals that of the stack
+    filtered up to projection n_filtered."""
+    from imagingsuite_amd import engine, scan
+    from imagingsuite_amd.recon import BackProjector, ProjectionFilterSingle
+    nu, nv, n_p = 64, 8, 300
+    center = 32.5
+    rng = np.random.default_rng(40 + abort_after)
+    proj = (rng.random((n_p, nv, nu)) + 0.5).astype(np.float32)
+    angles, weights = scan.engine_angles_weights(n_p, 0.0, 179.4, 0.1)
+    cfg = {"geometry": "parallel", "center": center, "roi": f"0 0 {nu} {nv}", "resolution": "0.1 0.1",
+           "filter_abort_after": abort_after, "device_handoff": "false"}
+    eng = engine.Engine("MultiProjBPparallel", {"ProjectionBufferSize": 16, "SliceBlock": nv, "SubVolume": "1 1"},
+                        cfg, "ProjectionFilterSingle", FILTER)
+    vol = eng.run(proj, angles, weights, nv)
+    eng.close()
+    print(f"filter abort at poll {abort_after}: {n_filtered} projections filtered", flush=True)
+    t = torch.from_numpy(proj).to(dev)
+    if n_filtered:
+        head = t[:n_filtered].contiguous()
+        ProjectionFilterSingle(nu, "Hanning").process(head)
+        t[:n_filtered] = head
+    bp = BackProjector("MultiProjBPparallel", nu, nv, center, buffer_size=16)
+    direct = bp.process(t, angles, weights).cpu().numpy().transpose(2, 0, 1)
+    assert np.array_equal(vol, direct)
