@@ -22,6 +22,8 @@ with its own kernel time and roofline fraction:
   "adjoint"           config 5 (Joseph A and A^T, FBP and FBP^T at 1024^2 x 64 x 900)
   "plugin_path"       config 2 through the MuhRec plugin boundary (block driver +
                       module quartet; host stack in, host slices out)
+  "tilt_correction"   config 2's back-projection with the axis-tilt correction
+                      (the three back-projectors, 0.5 degrees)
 
 Other configurations (reported by their own JSON line):
   --config 1  parallel FBP 256^2 x 64, 180 projections, Ram-Lak
@@ -344,6 +346,8 @@ def bench_parallel(args):
         r["pcie_inclusive"] = stream_leg(pristine, angles, weights, cfg, center, buf, nu, nv, n_p, updates)
     if args.config == 2 and not args.no_legs:
         r["lognorm_filter"] = lognorm_leg(pristine, filt, max(2, min(args.steps, 5)))
+        r["tilt_correction"] = tilt_leg(pristine, filt, angles, weights, center, buf, max(2, min(args.steps, 5)),
+                                        nu * nu * nv * n_p)
     host = None
     if not args.no_cpu_baseline or (args.config == 2 and not args.no_legs):
         host = pristine.cpu().numpy()
@@ -364,6 +368,49 @@ def bench_parallel(args):
         r["cpu_baseline"] = cpu_baseline(host[:args.cpu_sample_proj, :args.cpu_sample_slices], angles, weights, cfg,
                                          min(args.cpu_sample_proj, n_p), min(args.cpu_sample_slices, nv))
     emit(r)
+
+
+def tilt_leg(pristine, filt, angles, weights, center, buf, steps, updates):
+    """Config 2's back-projection with the axis-tilt correction
+    (ProjectionInfo.bCorrectTilt; MultiProjBPparallel.cpp:49-114,
+    MultiProjBP.cpp:50-96, NNMultiProjBP.cpp:48-81), 0.5 degrees about the
+    middle row, for the three back-projectors on the filtered stack (the
+    reference's float order; the tilt branches have no fused form): wall
+    time per call and the kernel's own time from the library's events."""
+    import torch
+    from imagingsuite_amd import _lib
+    from imagingsuite_amd.recon import BackProjector
+    log("[bench] tilt-correction leg ...")
+    n_p, nv, nu = pristine.shape
+    work = pristine.clone()
+    filt.process(work)
+    vol = torch.empty((nu, nu, nv), dtype=torch.float32, device=pristine.device)
+    angle, pivot = 0.5, nv / 2.0
+    out = {"tilt_angle": angle, "tilt_pivot": pivot}
+    for name in ("MultiProjBPparallel", "MultiProjBP", "NearestNeighborBP"):
+        bp = BackProjector(name, nu, nv, center, direction=0, buffer_size=buf, correct_tilt=True,
+                           tilt_angle=angle, tilt_pivot=pivot, row0=0, tilt_rows=nv)
+        bp.process(work, angles, weights, vol)
+        torch.cuda.synchronize()
+        _lib.profile(True)
+        _lib.profile_reset()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            bp.process(work, angles, weights, vol)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        k_ms, k_n = _lib.profile_query("bp_parallel_tilt")
+        _lib.profile(False)
+        assert torch.isfinite(vol).all().item()
+        out[name] = {"ms": round(ms, 3), "GUPS": round(updates / (ms / 1e3) / 1e9, 2), "kernel": bp.kernel,
+                     "kernel_ms": round(k_ms / max(k_n, 1), 3)}
+        log(f"[bench] tilt {name}: {ms:.1f} ms ({bp.kernel})")
+        del bp
+    del work, vol
+    torch.cuda.empty_cache()
+    out["note"] = (f"mean of {steps} calls after one warm-up on the filtered config-2 stack; "
+                   "bit-identical to the oracle's tilt restatement (tests/test_gpu_tilt.py)")
+    return out
 
 
 def lognorm_leg(pristine, filt, steps):
