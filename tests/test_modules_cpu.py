@@ -91,3 +91,16 @@ def test_modules_header_symbols_exported():
         lib = eng if n.startswith("mre_") else mods
         assert hasattr(lib, n), n
     assert engine.device_handoffs() >= 0
+
+
+def test_abort_after_options_parse_without_device():
+    """bp_abort_after / filter_abort_after give the back-projector and
+    ProjectionFilterSingle an interactor; a value that is not an integer is
+    refused when the engine is made."""
+    for key in ("bp_abort_after", "filter_abort_after"):
+        e = engine.Engine("MultiProjBPparallel", BP_PARAMS, dict(CFG, **{key: 2}), "ProjectionFilterSingle",
+                          FILTER_PARAMS)
+        e.close()
+        with pytest.raises(engine.EngineError):
+            engine.Engine("MultiProjBPparallel", BP_PARAMS, dict(CFG, **{key: "soon"}), "ProjectionFilterSingle",
+                          FILTER_PARAMS)
