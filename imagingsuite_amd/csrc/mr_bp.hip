@@ -472,8 +472,15 @@ __device__ __forceinline__ void bp_lds_tile(const BpArgs &a, int bx, int by, int
                         const int g0 = z0 >> 2;
                         const float fr = u - (float)n;
                         float f = fr;
-                        if constexpr (TILT == 1)
-                            for (int j = 0; j < g0; ++j) f = f - ci;
+                        if constexpr (TILT == 1) {
+                            // the same subtractions in the same order, eight per trip
+                            int j = 0;
+                            for (; j + 8 <= g0; j += 8) {
+#pragma unroll
+                                for (int e = 0; e < 8; ++e) f = f - ci;
+                            }
+                            for (; j < g0; ++j) f = f - ci;
+                        }
                         const float *B = A + RS;
 #pragma unroll
                         for (int g = 0; g < G; ++g) {
