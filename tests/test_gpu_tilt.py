@@ -31,8 +31,10 @@ def _gpu(x, dev):
 
 
 @pytest.mark.parametrize("name", VARIANTS)
+# 256 slices: slice tiles that start up to 56 groups in, so MultiProjBPparallel's
+# running offset is caught up over many subtractions before the tile's groups
 @pytest.mark.parametrize("nv,bufsize,angle,pivot,row0", [(40, 16, 2.5, 10.0, 100), (10, 16, -4.0, 300.0, 0),
-                                                         (64, 0, 1.0, 0.0, 37)])
+                                                         (64, 0, 1.0, 0.0, 37), (256, 16, 1.5, 120.0, 5)])
 def test_bp_tilt_bitexact(orc, dev, name, nv, bufsize, angle, pivot, row0):
     from imagingsuite_amd.recon import BackProjector
     rng = np.random.default_rng(nv + row0)
